@@ -1,0 +1,158 @@
+/*
+ * lineax_amd — C ABI of the MI355X (gfx950) solver kernels behind the
+ * lineax-compatible host layer in lineax_amd/.
+ *
+ * Conventions shared by every entry point:
+ *   - `stream` is a hipStream_t passed as void* (0 = the legacy default
+ *     stream).  Every call is asynchronous and stream-ordered: it enqueues
+ *     kernels and returns; it never synchronises and never allocates device
+ *     memory.  Scratch memory is caller-provided (`*_workspace_size`).
+ *   - Matrices are row-major.  Batched arguments carry an element stride
+ *     between consecutive systems; a stride of 0 broadcasts one operand
+ *     (e.g. one factorisation reused across many right-hand sides).
+ *   - `dtype` is LXA_F32 or LXA_F64 and applies to every floating operand
+ *     unless an argument says otherwise.
+ *   - The return value is LXA_OK or an error code; lxa_last_error() gives a
+ *     message for the calling thread.  Numerical outcomes (zero pivots,
+ *     non-positive pivots, refinement status) are never errors: they are
+ *     written per system to device `info` arrays and mapped to lineax's
+ *     RESULTS codes by the host.
+ *
+ * Reference citations are to patrick-kidger/lineax (lineax/...).
+ */
+#ifndef LINEAX_AMD_H_
+#define LINEAX_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LXA_OK 0
+#define LXA_EINVAL 1       /* invalid argument (shape, stride, dtype, size) */
+#define LXA_EHIP 2         /* a HIP runtime call or kernel launch failed    */
+#define LXA_EUNSUPPORTED 3 /* valid request outside what is implemented     */
+
+#define LXA_F32 0
+#define LXA_F64 1
+
+/* RESULTS codes, in lineax's declaration order (lineax/_solution.py:52-68). */
+#define LXA_RESULT_SUCCESSFUL 0
+#define LXA_RESULT_MAX_STEPS_REACHED 1
+#define LXA_RESULT_SINGULAR 2
+#define LXA_RESULT_BREAKDOWN 3
+#define LXA_RESULT_STAGNATION 4
+#define LXA_RESULT_CONLIM 5
+#define LXA_RESULT_NONFINITE_INPUT 6
+
+const char* lxa_version(void);
+const char* lxa_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * LU (lineax/_solver/lu.py)
+ * ---------------------------------------------------------------------- */
+
+/* getrf with partial pivoting on `batch` matrices of order n.
+ * Replaces jax.scipy.linalg.lu_factor(operator.as_matrix()) at
+ * lineax/_solver/lu.py:53.  `a` is overwritten with the unit lower factor
+ * (strictly below the diagonal) and the upper factor.  piv[k] (0-based) is
+ * the row interchanged with row k at step k, as jax.scipy's lu_factor
+ * returns.  info[i] = k+1 for the first exactly-zero pivot of system i, else
+ * 0 (the factorisation still completes; the solve then yields inf/NaN,
+ * which lineax maps to RESULTS.singular at lineax/_solve.py:104-113). */
+int lxa_lu_factor(void* stream, int dtype, int64_t batch, int64_t n, void* a,
+                  int64_t a_batch_stride, int32_t* piv, int32_t* info);
+
+/* getrs: solves A x = b (trans = 0) or A^T x = b (trans = 1) in place in b
+ * ([batch, n], stride b_batch_stride).  Replaces jax.scipy.linalg.lu_solve
+ * at lineax/_solver/lu.py:64 (trans flag from lu.py:60-61, flipped by
+ * LU.transpose at lu.py:68-81). */
+int lxa_lu_solve(void* stream, int dtype, int64_t batch, int64_t n,
+                 const void* lu, int64_t lu_batch_stride, const int32_t* piv,
+                 int64_t piv_batch_stride, void* b, int64_t b_batch_stride,
+                 int trans);
+
+/* ------------------------------------------------------------------------
+ * Cholesky (lineax/_solver/cholesky.py)
+ * ---------------------------------------------------------------------- */
+
+/* potrf, upper (A = U^T U), on `batch` matrices of order n.  Replaces
+ * jax.scipy.linalg.cho_factor(matrix) at lineax/_solver/cholesky.py:60.
+ * Only the upper triangle of A is read.  negate != 0 factors -A (the NSD
+ * branch, cholesky.py:58-59).  On success the upper triangle holds U and the
+ * strict lower triangle zeros; on a non-positive (or NaN) pivot the whole
+ * matrix is set to NaN and info[i] = k+1, matching JAX's NaN-filled
+ * factor so that the solve reports RESULTS.singular. */
+int lxa_chol_factor(void* stream, int dtype, int64_t batch, int64_t n, void* a,
+                    int64_t a_batch_stride, int negate, int32_t* info);
+
+/* potrs: solves U^T U x = b in place in b.  Replaces
+ * jax.scipy.linalg.cho_solve at lineax/_solver/cholesky.py:74; negate != 0
+ * negates the solution (cholesky.py:75-76). */
+int lxa_chol_solve(void* stream, int dtype, int64_t batch, int64_t n,
+                   const void* u, int64_t u_batch_stride, void* b,
+                   int64_t b_batch_stride, int negate);
+
+/* Uneven batch of Cholesky solves: system i has order n[i] in [1, 256], its
+ * float32 matrix at a + a_offset[i] (row-major, leading dimension n[i];
+ * only the upper triangle is read) and its float32 right-hand side at
+ * b + b_offset[i]; its solution is written at x + b_offset[i] in x_dtype.
+ * Systems are taken in the order given by order[0..count) (a permutation or
+ * a subset of 0..count-1 of length `count`).
+ *
+ * flags & LXA_UNEVEN_PERSISTENT: one persistent launch sized to the device;
+ *   workgroups pull systems from a device work queue in `order` (callers
+ *   sort it largest-first for load balance).  Otherwise one workgroup per
+ *   listed system.
+ * max_refine > 0: after the FP32 factorisation and solve, FP64 residuals
+ *   r = b - A x are formed and x (kept in FP64) is corrected with FP32
+ *   solves until the normwise backward error
+ *   eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol, at most max_refine
+ *   residuals.  x_dtype must then be LXA_F64.
+ *
+ * Outputs per system: info (0 ok, k+1 non-positive pivot -> x is NaN, -1
+ * order outside [1, max_n] -> system skipped),
+ * iters (residuals formed), eta (last backward error, -1 if none formed).
+ * iters and eta may be NULL.  Semantically each system is
+ * lx.linear_solve(lx.MatrixLinearOperator(A_i, positive_semidefinite_tag),
+ * b_i, lx.Cholesky()) (lineax/_solver/cholesky.py:43-78), evaluated for the
+ * exact float64 value of the float32 data when max_refine > 0. */
+#define LXA_UNEVEN_PERSISTENT 1
+int lxa_chol_uneven_workspace_size(int64_t count, size_t* bytes);
+int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n,
+                          const int32_t* n, const int64_t* a_offset,
+                          const float* a, const int64_t* b_offset,
+                          const float* b, void* x, int x_dtype,
+                          const int32_t* order, int flags, int max_refine,
+                          double tol, int32_t* info, int32_t* iters,
+                          double* eta, void* workspace,
+                          size_t workspace_bytes);
+
+/* ------------------------------------------------------------------------
+ * Operators and result mapping
+ * ---------------------------------------------------------------------- */
+
+/* y = A x (trans = 0, A is [m, n]) or y = A^T x (trans = 1), batched.
+ * Replaces jnp.matmul(self.matrix, vector, precision=HIGHEST) in
+ * MatrixLinearOperator.mv, lineax/_operator.py:265-269. */
+int lxa_gemv(void* stream, int dtype, int64_t batch, int64_t m, int64_t n,
+             const void* a, int64_t a_batch_stride, const void* x,
+             int64_t x_batch_stride, void* y, int64_t y_batch_stride,
+             int trans);
+
+/* The RESULTS post-processing of lineax/_solve.py:104-123, per system:
+ * result[i] == successful and any non-finite entry in solution row i
+ * -> singular; then result[i] == singular and any non-finite entry in
+ * vector row i -> nonfinite_input.  solution/vector are [rows, len]. */
+int lxa_solve_results(void* stream, int dtype, int64_t rows, int64_t len,
+                      const void* solution, int64_t solution_row_stride,
+                      const void* vector, int64_t vector_row_stride,
+                      int32_t* result);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LINEAX_AMD_H_ */

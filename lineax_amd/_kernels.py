@@ -1,0 +1,351 @@
+"""ctypes boundary to the gfx950 kernel library (``include/lineax_amd.h``).
+
+Every wrapper checks devices, dtypes, shapes and strides on the host before
+a launch, passes torch's current HIP stream, and raises ``RuntimeError`` with
+the library's message on a non-zero status.  There is no CPU fallback: a
+missing library or a CPU tensor is an error.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import pathlib
+import threading
+
+import torch
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "_lib" / "liblineax_amd.so"
+
+F32 = 0
+F64 = 1
+UNEVEN_PERSISTENT = 1
+
+_lib = None
+_lock = threading.Lock()
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+_szp = ctypes.POINTER(ctypes.c_size_t)
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGNATURES = {
+    "lxa_lu_factor": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _vp],
+    "lxa_lu_solve": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int],
+    "lxa_chol_factor": [_vp, _int, _i64, _i64, _vp, _i64, _int, _vp],
+    "lxa_chol_solve": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _i64, _int],
+    "lxa_chol_uneven_workspace_size": [_i64, _szp],
+    "lxa_chol_uneven_solve": [
+        _vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _int, _int, _dbl,
+        _vp, _vp, _vp, _vp, ctypes.c_size_t,
+    ],
+    "lxa_gemv": [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int],
+    "lxa_solve_results": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp],
+    "lxa_version": [],
+    "lxa_last_error": [],
+}
+_RESTYPES = {"lxa_version": ctypes.c_char_p, "lxa_last_error": ctypes.c_char_p}
+
+
+def declared_symbols() -> list[str]:
+    """Names of every entry point this module binds."""
+    return sorted(_SIGNATURES)
+
+
+def load() -> ctypes.CDLL:
+    """Loads the kernel library (once); raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"lineax_amd kernel library not found at {LIB_PATH}; build it with "
+                    "`python -c 'import __graft_entry__ as g; g.build()'`"
+                )
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, argtypes in _SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.argtypes = argtypes
+                fn.restype = _RESTYPES.get(name, ctypes.c_int)
+            _lib = lib
+    return _lib
+
+
+def version() -> str:
+    return load().lxa_version().decode()
+
+
+def _check(rc: int, name: str) -> None:
+    if rc != 0:
+        msg = load().lxa_last_error().decode()
+        raise RuntimeError(f"{name} failed (status {rc}): {msg}")
+
+
+def _dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float32:
+        return F32
+    if dtype == torch.float64:
+        return F64
+    raise TypeError(f"lineax_amd kernels support float32 and float64, got {dtype}")
+
+
+def _gpu(t: torch.Tensor, what: str) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a torch.Tensor, got {type(t)}")
+    if not t.is_cuda:
+        raise ValueError(
+            f"{what} is on {t.device}; lineax_amd solves run on the GPU (HIP) only"
+        )
+
+
+def _stream(t: torch.Tensor) -> int:
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _flat_batch(t: torch.Tensor, core: int) -> tuple[torch.Tensor, int, int]:
+    """Views t as [batch, *core dims] with a uniform batch stride.
+
+    Returns (tensor with contiguous core, batch, batch_stride_in_elements).
+    A batch of identical (broadcast, stride-0) leading dims gives stride 0.
+    """
+    lead = t.shape[: t.dim() - core]
+    batch = math.prod(lead)
+    inner = t.shape[t.dim() - core:]
+    inner_contig = True
+    expect = 1
+    for size, stride in zip(reversed(inner), reversed(t.stride()[t.dim() - core:])):
+        if size != 1 and stride != expect:
+            inner_contig = False
+        expect *= size
+    if inner_contig:
+        lead_strides = t.stride()[: t.dim() - core]
+        if all(s == 0 or sz == 1 for s, sz in zip(lead_strides, lead)):
+            return t, batch, 0
+        try:
+            v = t.view(batch, *inner) if batch > 0 else t.reshape(0, *inner)
+            return v, batch, (v.stride(0) if batch > 1 else math.prod(inner))
+        except RuntimeError:
+            pass
+    v = t.reshape(batch, *inner).contiguous()
+    return v, batch, math.prod(inner)
+
+
+# ---------------------------------------------------------------------------
+# LU
+# ---------------------------------------------------------------------------
+
+
+def lu_factor(matrix: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """getrf over leading batch dims: returns (lu, piv int32, info int32)."""
+    _gpu(matrix, "matrix")
+    *lead, n, n2 = matrix.shape
+    if n != n2:
+        raise ValueError(f"lu_factor needs square matrices, got {tuple(matrix.shape)}")
+    dt = _dtype_code(matrix.dtype)
+    lu = matrix.contiguous().clone()
+    batch = math.prod(lead)
+    piv = torch.empty((*lead, n), dtype=torch.int32, device=matrix.device)
+    info = torch.empty(tuple(lead), dtype=torch.int32, device=matrix.device)
+    if batch == 0 or n == 0:
+        return lu, piv, info
+    lib = load()
+    _check(
+        lib.lxa_lu_factor(_stream(lu), dt, batch, n, _ptr(lu), n * n, _ptr(piv), _ptr(info)),
+        "lxa_lu_factor",
+    )
+    return lu, piv, info
+
+
+def lu_solve(lu: torch.Tensor, piv: torch.Tensor, b: torch.Tensor, trans: int) -> torch.Tensor:
+    """getrs: x with A x = b (trans 0) or A^T x = b (trans 1); batch dims broadcast."""
+    _gpu(lu, "lu")
+    _gpu(b, "vector")
+    n = lu.shape[-1]
+    if b.shape[-1] != n:
+        raise ValueError(f"vector of size {b.shape[-1]} does not match operator of order {n}")
+    if b.dtype != lu.dtype:
+        raise TypeError(f"vector dtype {b.dtype} differs from factor dtype {lu.dtype}")
+    dt = _dtype_code(lu.dtype)
+    lead = torch.broadcast_shapes(lu.shape[:-2], piv.shape[:-1], b.shape[:-1])
+    x = b.expand(*lead, n).contiguous().clone()
+    batch = math.prod(lead)
+    if batch == 0 or n == 0:
+        return x
+    luv, lb, lus = _flat_batch(lu.expand(*lead, n, n), 2)
+    pvv, pb, pvs = _flat_batch(piv.to(torch.int32).expand(*lead, n), 1)
+    lib = load()
+    _check(
+        lib.lxa_lu_solve(_stream(x), dt, batch, n, _ptr(luv), lus, _ptr(pvv), pvs, _ptr(x), n, int(trans)),
+        "lxa_lu_solve",
+    )
+    return x
+
+
+# ---------------------------------------------------------------------------
+# Cholesky
+# ---------------------------------------------------------------------------
+
+
+def chol_factor(matrix: torch.Tensor, negate: bool = False) -> tuple[torch.Tensor, torch.Tensor]:
+    """potrf (upper) over leading batch dims: returns (U with zeros below, info)."""
+    _gpu(matrix, "matrix")
+    *lead, n, n2 = matrix.shape
+    if n != n2:
+        raise ValueError(f"chol_factor needs square matrices, got {tuple(matrix.shape)}")
+    dt = _dtype_code(matrix.dtype)
+    u = matrix.contiguous().clone()
+    info = torch.empty(tuple(lead), dtype=torch.int32, device=matrix.device)
+    batch = math.prod(lead)
+    if batch == 0 or n == 0:
+        return u, info
+    lib = load()
+    _check(
+        lib.lxa_chol_factor(_stream(u), dt, batch, n, _ptr(u), n * n, int(bool(negate)), _ptr(info)),
+        "lxa_chol_factor",
+    )
+    return u, info
+
+
+def chol_solve(u: torch.Tensor, b: torch.Tensor, negate: bool = False) -> torch.Tensor:
+    """potrs with U from chol_factor; batch dims broadcast."""
+    _gpu(u, "factor")
+    _gpu(b, "vector")
+    n = u.shape[-1]
+    if b.shape[-1] != n:
+        raise ValueError(f"vector of size {b.shape[-1]} does not match operator of order {n}")
+    if b.dtype != u.dtype:
+        raise TypeError(f"vector dtype {b.dtype} differs from factor dtype {u.dtype}")
+    dt = _dtype_code(u.dtype)
+    lead = torch.broadcast_shapes(u.shape[:-2], b.shape[:-1])
+    x = b.expand(*lead, n).contiguous().clone()
+    batch = math.prod(lead)
+    if batch == 0 or n == 0:
+        return x
+    uv, _, us = _flat_batch(u.expand(*lead, n, n), 2)
+    lib = load()
+    _check(
+        lib.lxa_chol_solve(_stream(x), dt, batch, n, _ptr(uv), us, _ptr(x), n, int(bool(negate))),
+        "lxa_chol_solve",
+    )
+    return x
+
+
+def chol_uneven_solve(
+    n: torch.Tensor,
+    a_offset: torch.Tensor,
+    a: torch.Tensor,
+    b_offset: torch.Tensor,
+    b: torch.Tensor,
+    x: torch.Tensor,
+    order: torch.Tensor,
+    *,
+    max_n: int,
+    persistent: bool,
+    max_refine: int,
+    tol: float,
+    info: torch.Tensor,
+    iters: torch.Tensor | None = None,
+    eta: torch.Tensor | None = None,
+    workspace: torch.Tensor | None = None,
+) -> None:
+    """Raw launch of lxa_chol_uneven_solve (see the header for semantics).
+
+    Arguments are device tensors already laid out; ``order`` may list a
+    subset of the systems (one size bucket, for instance).
+    """
+    for t, what in ((n, "n"), (a_offset, "a_offset"), (a, "a"), (b_offset, "b_offset"), (b, "b"),
+                    (x, "x"), (order, "order"), (info, "info")):
+        _gpu(t, what)
+    if n.dtype != torch.int32 or order.dtype != torch.int32 or info.dtype != torch.int32:
+        raise TypeError("n, order and info must be int32")
+    if a_offset.dtype != torch.int64 or b_offset.dtype != torch.int64:
+        raise TypeError("offsets must be int64")
+    if a.dtype != torch.float32 or b.dtype != torch.float32:
+        raise TypeError("uneven batches hold float32 matrices and right-hand sides")
+    xdt = _dtype_code(x.dtype)
+    count = order.numel()
+    lib = load()
+    ws_ptr, ws_bytes = None, 0
+    if persistent:
+        need = ctypes.c_size_t(0)
+        _check(lib.lxa_chol_uneven_workspace_size(count, ctypes.byref(need)), "lxa_chol_uneven_workspace_size")
+        if workspace is None or workspace.numel() * workspace.element_size() < need.value:
+            workspace = torch.empty(int(need.value), dtype=torch.uint8, device=a.device)
+        ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    _check(
+        lib.lxa_chol_uneven_solve(
+            _stream(a), count, int(max_n), _ptr(n), _ptr(a_offset), _ptr(a), _ptr(b_offset), _ptr(b),
+            _ptr(x), xdt, _ptr(order), UNEVEN_PERSISTENT if persistent else 0, int(max_refine), float(tol),
+            _ptr(info), _ptr(iters), _ptr(eta), ws_ptr, ws_bytes,
+        ),
+        "lxa_chol_uneven_solve",
+    )
+
+
+# ---------------------------------------------------------------------------
+# Operators and results
+# ---------------------------------------------------------------------------
+
+
+def gemv(matrix: torch.Tensor, x: torch.Tensor, trans: bool = False) -> torch.Tensor:
+    """y = A x or A^T x over broadcast leading batch dims."""
+    _gpu(matrix, "matrix")
+    _gpu(x, "vector")
+    if x.dtype != matrix.dtype:
+        raise TypeError(f"vector dtype {x.dtype} differs from matrix dtype {matrix.dtype}")
+    dt = _dtype_code(matrix.dtype)
+    m, n = matrix.shape[-2:]
+    kin, kout = (m, n) if trans else (n, m)
+    if x.shape[-1] != kin:
+        raise ValueError(f"vector of size {x.shape[-1]} does not match operator input size {kin}")
+    lead = torch.broadcast_shapes(matrix.shape[:-2], x.shape[:-1])
+    batch = math.prod(lead)
+    y = torch.empty((*lead, kout), dtype=matrix.dtype, device=matrix.device)
+    if batch == 0 or kout == 0:
+        return y
+    if kin == 0:
+        return y.zero_()
+    av, _, as_ = _flat_batch(matrix.expand(*lead, m, n), 2)
+    xv, _, xs = _flat_batch(x.expand(*lead, kin), 1)
+    lib = load()
+    _check(
+        lib.lxa_gemv(_stream(y), dt, batch, m, n, _ptr(av), as_, _ptr(xv), xs, _ptr(y), kout, int(bool(trans))),
+        "lxa_gemv",
+    )
+    return y
+
+
+def solve_results(solution: torch.Tensor, vector: torch.Tensor, result: torch.Tensor) -> torch.Tensor:
+    """RESULTS post-processing (lineax/_solve.py:104-123) on [*batch, len] rows.
+
+    ``result`` (int32, batch shape) is updated in place and returned.
+    """
+    _gpu(solution, "solution")
+    _gpu(vector, "vector")
+    _gpu(result, "result")
+    if result.dtype != torch.int32 or not result.is_contiguous():
+        raise TypeError("result must be a contiguous int32 tensor")
+    if solution.dtype != vector.dtype:
+        vector = vector.to(solution.dtype)
+    dt = _dtype_code(solution.dtype)
+    rows = result.numel()
+    length = solution.numel() // rows if rows else 0
+    sol = solution.reshape(rows, -1).contiguous()
+    vec = vector.expand_as(solution).reshape(rows, -1).contiguous()
+    lib = load()
+    _check(
+        lib.lxa_solve_results(_stream(result), dt, rows, length, _ptr(sol), sol.stride(0), _ptr(vec),
+                              vec.stride(0), _ptr(result)),
+        "lxa_solve_results",
+    )
+    return result

@@ -1,0 +1,13 @@
+from .cg import CG
+from .cholesky import Cholesky
+from .diagonal import Diagonal
+from .gmres import GMRES
+from .lu import LU
+from .qr import QR
+from .triangular import Triangular
+from .unsupported import LSMR, SVD, BiCGStab, Normal, Tridiagonal
+
+__all__ = [
+    "BiCGStab", "CG", "Cholesky", "Diagonal", "GMRES", "LSMR", "LU", "Normal", "QR", "SVD", "Triangular",
+    "Tridiagonal",
+]

@@ -1,0 +1,782 @@
+// Cholesky factorisation (potrf, upper: A = U^T U) and solve (potrs).
+//
+// Three tiers, chosen by order n:
+//  * n <= 64: one lane per row in registers, 64/LPS systems per wave; the
+//    scaled pivot row is broadcast through a double-buffered LDS row.
+//  * n <= 256 (float) / 181 (double): one 256-thread workgroup per system
+//    with the upper triangle resident in LDS in row-packed form (131.6 KB
+//    at n = 256); right-looking factorisation, triangular solves by one
+//    wave with the vector in registers.  The same device code drives the
+//    uneven-batch launch, which adds FP64 iterative refinement.
+//  * larger n: one workgroup per system in global memory (correctness path).
+//
+// Failure semantics follow jax.scipy.linalg.cho_factor as lineax uses it
+// (lineax/_solver/cholesky.py:60): a non-positive or NaN pivot makes the
+// whole factor NaN, so the later solve yields NaN and lineax reports
+// RESULTS.singular (lineax/_solve.py:104-113).
+#include "lxa_common.h"
+
+namespace lxa {
+namespace {
+
+template <int N>
+struct CholLanes {
+  static constexpr int value = N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
+};
+
+// ---------------------------------------------------------------------------
+// Tier 1: n <= 64, lane per row.  Orders n < N are padded with an identity
+// block, which factors to itself and leaves the real block unchanged.
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__(64) void chol_factor_small_kernel(int64_t batch, int n, T* __restrict__ a,
+                                                               int64_t a_stride, int negate,
+                                                               int32_t* __restrict__ info) {
+  constexpr int LPS = CholLanes<N>::value;
+  constexpr int SPW = 64 / LPS;
+  __shared__ T prow[2][SPW][N];
+  const int lane = threadIdx.x;
+  const int g = lane / LPS;
+  const int r = lane % LPS;
+  const int64_t sys = static_cast<int64_t>(blockIdx.x) * SPW + g;
+  const bool valid = sys < batch;
+  const bool real_row = valid && r < n;
+  T* A = a + sys * a_stride;
+
+  T row[N];
+  if (real_row) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      T v = T(0);
+      if (j >= r && j < n) v = A[static_cast<int64_t>(r) * n + j];  // upper triangle only
+      row[j] = negate ? -v : v;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) row[j] = (j == r) ? T(1) : T(0);
+  }
+
+  int fail = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (r == k) {
+      const T s = sqrt(row[k]);  // NaN for a negative pivot
+#pragma unroll
+      for (int j = k; j < N; ++j) prow[k & 1][g][j] = (j == k) ? s : row[j] / s;
+    }
+    __syncthreads();
+    const T ukk = prow[k & 1][g][k];
+    if (k < n && !(ukk > T(0)) && fail == 0) fail = k + 1;
+    if (r == k) {
+#pragma unroll
+      for (int j = k; j < N; ++j) row[j] = prow[k & 1][g][j];
+    } else if (r > k && r < N) {
+      const T ukr = prow[k & 1][g][r];
+#pragma unroll
+      for (int j = k + 1; j < N; ++j) row[j] -= ukr * prow[k & 1][g][j];
+    }
+  }
+
+  if (real_row) {
+    const T nan = quiet_nan<T>();
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (j < n) A[static_cast<int64_t>(r) * n + j] = fail ? nan : (j < r ? T(0) : row[j]);
+    }
+  }
+  if (valid && r == 0) info[sys] = fail;
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(64) void chol_solve_small_kernel(int64_t batch, int n, const T* __restrict__ u,
+                                                              int64_t u_stride, T* __restrict__ b,
+                                                              int64_t b_stride, int negate) {
+  constexpr int LPS = CholLanes<N>::value;
+  const int lane = threadIdx.x;
+  const int g = lane / LPS;
+  const int r = lane % LPS;
+  const int64_t sys = static_cast<int64_t>(blockIdx.x) * (64 / LPS) + g;
+  const bool real_row = sys < batch && r < n;
+  const T* U = u + sys * u_stride;
+  T* B = b + sys * b_stride;
+
+  T m[N];
+  // Forward U^T y = b with lane c holding column c of U.
+  if (real_row) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) m[i] = (i <= r) ? U[static_cast<int64_t>(i) * n + r] : T(0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) m[i] = (i == r) ? T(1) : T(0);
+  }
+  T y = real_row ? B[r] : T(0);
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const T t = y / m[j];
+    const T yj = __shfl(t, j, LPS);
+    if (r > j) {
+      y -= m[j] * yj;
+    } else if (r == j) {
+      y = yj;
+    }
+  }
+  // Backward U x = y with lane r holding row r of U.
+  if (real_row) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) m[j] = (j >= r && j < n) ? U[static_cast<int64_t>(r) * n + j] : T(0);
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) m[j] = (j == r) ? T(1) : T(0);
+  }
+#pragma unroll
+  for (int j = N - 1; j >= 0; --j) {
+    const T t = y / m[j];
+    const T xj = __shfl(t, j, LPS);
+    if (r < j) {
+      y -= m[j] * xj;
+    } else if (r == j) {
+      y = xj;
+    }
+  }
+  if (real_row) B[r] = negate ? -y : y;
+}
+
+// ---------------------------------------------------------------------------
+// Tier 2: one 256-thread workgroup per system, upper triangle in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kWG = 256;
+constexpr int kLdsMaxN = 256;
+
+__device__ __forceinline__ float bcast_lane(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double bcast_lane(double v, int lane) {
+  const int64_t bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits & 0xffffffff), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), lane);
+  return __longlong_as_double((static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo));
+}
+
+// Loads the upper triangle of A (row-major, leading dimension lda) into the
+// row-packed LDS triangle L.  Wave w takes rows w, w+4, ...; lanes cover the
+// columns, and four rows are in flight per wave.
+template <typename T, typename TA>
+__device__ void lds_load_upper(int n, const TA* __restrict__ A, int64_t lda, T* L, bool negate) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (int i0 = w; i0 < n; i0 += 16) {
+    TA v[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 4 * u;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int j = l + 64 * c;
+        v[u][c] = (i < n && j >= i && j < n) ? A[static_cast<int64_t>(i) * lda + j] : TA(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 4 * u;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int j = l + 64 * c;
+        if (i < n && j >= i && j < n) {
+          const T x = static_cast<T>(v[u][c]);
+          L[packed_upper(n, i, j)] = negate ? -x : x;
+        }
+      }
+    }
+  }
+}
+
+// Right-looking factorisation of the packed triangle in place.  Returns 0 or
+// the 1-based index of the first non-positive pivot (uniform over the block).
+template <typename T>
+__device__ int lds_chol_factor(int n, T* L) {
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  for (int k = 0; k < n; ++k) {
+    const int rk = packed_upper(n, k, k);
+    const T akk = L[rk];
+    if (!(akk > T(0))) return k + 1;
+    const T ukk = sqrt(akk);
+    for (int j = k + 1 + tid; j < n; j += kWG) L[rk + (j - k)] /= ukk;
+    __syncthreads();
+    if (tid == 0) L[rk] = ukk;  // nobody reads L[k][k] until the next barrier
+    const int m = n - k - 1;
+    for (int ii = ty; ii < m; ii += 16) {
+      const int i = k + 1 + ii;
+      const T uki = L[rk + (i - k)];
+      const int ri = packed_upper(n, i, i);
+      int jj = ii + ((tx - ii) & 15);  // first jj >= ii with jj == tx (mod 16)
+      for (; jj < m; jj += 16) {
+        const int j = k + 1 + jj;
+        L[ri + (j - i)] -= uki * L[rk + (j - k)];
+      }
+    }
+    __syncthreads();
+  }
+  return 0;
+}
+
+// Solves U^T U x = v in place, v distributed over one wave: lane l holds
+// v[l + 64 c].  Must be called by a whole wave with uniform control flow.
+template <typename T>
+__device__ void wave_chol_solve(int n, const T* L, T (&v)[4]) {
+  const int l = threadIdx.x & 63;
+  // U^T y = v, column oriented: y_k = v_k / U_kk, then v_j -= U_kj y_k.
+#pragma unroll
+  for (int c0 = 0; c0 < 4; ++c0) {
+    const int kend = min(64, n - 64 * c0);
+    for (int kk = 0; kk < kend; ++kk) {
+      const int k = 64 * c0 + kk;
+      const int rk = packed_upper(n, k, k);
+      const T yk = bcast_lane(v[c0], kk) / L[rk];
+      if (l == kk) v[c0] = yk;
+#pragma unroll
+      for (int c = c0; c < 4; ++c) {
+        const int j = l + 64 * c;
+        if (j > k && j < n) v[c] -= L[rk + (j - k)] * yk;
+      }
+    }
+  }
+  // U x = y, column oriented: x_k = y_k / U_kk, then y_i -= U_ik x_k.
+#pragma unroll
+  for (int c0 = 3; c0 >= 0; --c0) {
+    const int kend = min(64, n - 64 * c0);
+    for (int kk = kend - 1; kk >= 0; --kk) {
+      const int k = 64 * c0 + kk;
+      const T xk = bcast_lane(v[c0], kk) / L[packed_upper(n, k, k)];
+      if (l == kk) v[c0] = xk;
+#pragma unroll
+      for (int c = 0; c <= c0; ++c) {
+        const int i = l + 64 * c;
+        if (i < k) v[c] -= L[packed_upper(n, i, k)] * xk;
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void chol_factor_lds_kernel(int n, T* __restrict__ a, int64_t a_stride,
+                                                              int negate, int32_t* __restrict__ info) {
+  extern __shared__ unsigned char lxa_smem[];
+  T* L = reinterpret_cast<T*>(lxa_smem);
+  T* A = a + static_cast<int64_t>(blockIdx.x) * a_stride;
+  lds_load_upper<T, T>(n, A, n, L, negate != 0);
+  __syncthreads();
+  const int fail = lds_chol_factor<T>(n, L);
+  __syncthreads();
+  const T nan = quiet_nan<T>();
+  const int64_t nn = static_cast<int64_t>(n) * n;
+  for (int64_t t = threadIdx.x; t < nn; t += kWG) {
+    const int i = static_cast<int>(t / n), j = static_cast<int>(t % n);
+    A[t] = fail ? nan : (j < i ? T(0) : L[packed_upper(n, i, j)]);
+  }
+  if (threadIdx.x == 0) info[blockIdx.x] = fail;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void chol_solve_lds_kernel(int n, const T* __restrict__ u, int64_t u_stride,
+                                                             T* __restrict__ b, int64_t b_stride, int negate) {
+  extern __shared__ unsigned char lxa_smem[];
+  T* L = reinterpret_cast<T*>(lxa_smem);
+  const T* U = u + static_cast<int64_t>(blockIdx.x) * u_stride;
+  T* B = b + static_cast<int64_t>(blockIdx.x) * b_stride;
+  lds_load_upper<T, T>(n, U, n, L, false);
+  __syncthreads();
+  if ((threadIdx.x >> 6) == 0) {
+    const int l = threadIdx.x;
+    T v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = (l + 64 * c < n) ? B[l + 64 * c] : T(0);
+    wave_chol_solve<T>(n, L, v);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (l + 64 * c < n) B[l + 64 * c] = negate ? -v[c] : v[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tier 3: any n, one workgroup per system in global memory.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kWG) void chol_factor_generic_kernel(int n, T* __restrict__ a, int64_t a_stride,
+                                                                  int negate, int32_t* __restrict__ info) {
+  T* A = a + static_cast<int64_t>(blockIdx.x) * a_stride;
+  const int tid = threadIdx.x;
+  const int64_t ld = n;
+  if (negate) {
+    for (int64_t t = tid; t < ld * ld; t += kWG) {
+      const int64_t i = t / ld, j = t % ld;
+      if (j >= i) A[t] = -A[t];
+    }
+    __syncthreads();
+  }
+  int fail = 0;
+  for (int k = 0; k < n; ++k) {
+    const T akk = A[k * ld + k];
+    if (!(akk > T(0))) {
+      fail = k + 1;
+      break;
+    }
+    const T ukk = sqrt(akk);
+    for (int j = k + 1 + tid; j < n; j += kWG) A[k * ld + j] /= ukk;
+    __syncthreads();
+    if (tid == 0) A[k * ld + k] = ukk;
+    const int64_t m = n - k - 1;
+    for (int64_t t = tid; t < m * m; t += kWG) {
+      const int64_t i = k + 1 + t / m, j = k + 1 + t % m;
+      if (j >= i) A[i * ld + j] -= A[k * ld + i] * A[k * ld + j];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  const T nan = quiet_nan<T>();
+  for (int64_t t = tid; t < ld * ld; t += kWG) {
+    const int64_t i = t / ld, j = t % ld;
+    if (fail) {
+      A[t] = nan;
+    } else if (j < i) {
+      A[t] = T(0);
+    }
+  }
+  if (tid == 0) info[blockIdx.x] = fail;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void chol_solve_generic_kernel(int n, const T* __restrict__ u, int64_t u_stride,
+                                                                 T* __restrict__ b, int64_t b_stride, int negate) {
+  extern __shared__ unsigned char lxa_smem[];
+  T* v = reinterpret_cast<T*>(lxa_smem);
+  const T* U = u + static_cast<int64_t>(blockIdx.x) * u_stride;
+  T* B = b + static_cast<int64_t>(blockIdx.x) * b_stride;
+  const int tid = threadIdx.x;
+  const int64_t ld = n;
+  for (int i = tid; i < n; i += kWG) v[i] = B[i];
+  __syncthreads();
+  // U^T y = b: y_j final when read at step j; kept in B.
+  for (int j = 0; j < n; ++j) {
+    const T yj = v[j] / U[j * ld + j];
+    for (int i = j + 1 + tid; i < n; i += kWG) v[i] -= U[j * ld + i] * yj;
+    if (tid == 0) B[j] = yj;
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += kWG) v[i] = B[i];
+  __syncthreads();
+  for (int j = n - 1; j >= 0; --j) {
+    const T xj = v[j] / U[j * ld + j];
+    for (int i = tid; i < j; i += kWG) v[i] -= U[i * ld + j] * xj;
+    if (tid == 0) B[j] = negate ? -xj : xj;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Uneven batch: fused load / factor / solve / FP64 refinement per system.
+// ---------------------------------------------------------------------------
+
+// LDS carve-up for one system of order <= max_n.
+struct UnevenLds {
+  double* x;        // [max_n] solution in FP64
+  double* r;        // [max_n] residual
+  double* rowpart;  // [max_n] (T x)_j part of the residual
+  double* rowabs;   // [max_n] row abs sums of the upper triangle
+  double* acc;      // [4][max_n] per-wave partial (T^T x)_i
+  double* red;      // [8] block reduction scratch
+  float* L;         // packed upper triangle
+  int* item;        // work-queue slot
+};
+
+__device__ UnevenLds carve_uneven(unsigned char* base, int max_n, bool refine) {
+  UnevenLds s;
+  double* d = reinterpret_cast<double*>(base);
+  s.x = d;
+  d += max_n;
+  if (refine) {
+    s.r = d;
+    d += max_n;
+    s.rowpart = d;
+    d += max_n;
+    s.rowabs = d;
+    d += max_n;
+    s.acc = d;
+    d += 4 * max_n;
+  } else {
+    s.r = s.rowpart = s.rowabs = s.acc = nullptr;
+  }
+  s.red = d;
+  d += 8;
+  s.item = reinterpret_cast<int*>(d);
+  s.L = reinterpret_cast<float*>(s.item + 4);
+  return s;
+}
+
+__host__ __device__ inline size_t uneven_lds_bytes(int max_n, bool refine) {
+  const size_t doubles = static_cast<size_t>(max_n) * (refine ? 8 : 1) + 8;
+  const size_t packed = static_cast<size_t>(max_n) * (max_n + 1) / 2;
+  return doubles * sizeof(double) + 4 * sizeof(int) + packed * sizeof(float);
+}
+
+__device__ double block_max(double v, double* red) {
+  v = wave_max(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  const double out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  __syncthreads();
+  return out;
+}
+
+// r = b - S x with S the symmetric matrix whose upper triangle is A's;
+// optionally rowabs[j] = sum_i |S_ji|.  Each upper-triangle element is read
+// once: row j contributes sum_{i>=j} A_ji x_i to r_j (wave reduction) and
+// A_ji x_j to r_i for i > j (lane accumulators).
+__device__ void residual_sym_upper(int n, const float* __restrict__ A, const float* __restrict__ b,
+                                   const UnevenLds& s, bool want_abs) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double aab[4] = {0.0, 0.0, 0.0, 0.0};
+  double xi[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) xi[c] = (l + 64 * c < n) ? s.x[l + 64 * c] : 0.0;
+  for (int j0 = w; j0 < n; j0 += 16) {
+    // Four rows per wave in flight: rows j0, j0+4, j0+8, j0+12.
+    float av[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + 4 * u;
+      const float* Aj = A + static_cast<int64_t>(j) * n;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = l + 64 * c;
+        av[u][c] = (j < n && i >= j && i < n) ? Aj[i] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + 4 * u;
+      if (j >= n) break;
+      const double xj = s.x[j];
+      double dot = 0.0, rab = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = l + 64 * c;
+        const double aji = static_cast<double>(av[u][c]);  // zero outside the upper triangle
+        dot = fma(aji, xi[c], dot);
+        rab += fabs(aji);
+        if (i > j) {
+          acc[c] = fma(aji, xj, acc[c]);
+          aab[c] += fabs(aji);
+        }
+      }
+      dot = wave_sum(dot);
+      if (want_abs) rab = wave_sum(rab);
+      if (l == 0) {
+        s.rowpart[j] = dot;
+        if (want_abs) s.rowabs[j] = rab;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int i = l + 64 * c;
+    if (i < n) s.acc[w * n + i] = acc[c];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kWG) {
+    const double tx = s.rowpart[i] + ((s.acc[i] + s.acc[n + i]) + (s.acc[2 * n + i] + s.acc[3 * n + i]));
+    s.r[i] = static_cast<double>(b[i]) - tx;
+  }
+  if (want_abs) {
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = l + 64 * c;
+      if (i < n) s.acc[w * n + i] = aab[c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kWG)
+      s.rowabs[i] += (s.acc[i] + s.acc[n + i]) + (s.acc[2 * n + i] + s.acc[3 * n + i]);
+  }
+  __syncthreads();
+}
+
+template <bool kRefine, typename TX>
+__global__ __launch_bounds__(kWG) void chol_uneven_kernel(int64_t count, int max_n, const int32_t* __restrict__ ns,
+                                                          const int64_t* __restrict__ a_off,
+                                                          const float* __restrict__ a,
+                                                          const int64_t* __restrict__ b_off,
+                                                          const float* __restrict__ b, TX* __restrict__ x,
+                                                          const int32_t* __restrict__ order, int persistent,
+                                                          unsigned int* __restrict__ queue, int max_refine,
+                                                          double tol, int32_t* __restrict__ info,
+                                                          int32_t* __restrict__ iters, double* __restrict__ eta) {
+  extern __shared__ unsigned char lxa_smem[];
+  const UnevenLds s = carve_uneven(lxa_smem, max_n, kRefine);
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  int64_t next = blockIdx.x;
+  for (;;) {
+    if (persistent) {
+      if (tid == 0) s.item[0] = static_cast<int>(atomicAdd(queue, 1u));
+      __syncthreads();
+      next = s.item[0];
+      __syncthreads();
+    }
+    if (next >= count) break;
+    const int64_t sys = order[next];
+    if (!persistent) next = count;  // one system per workgroup
+    const int n = ns[sys];
+    if (n < 1 || n > max_n) {  // order outside the LDS sizing: reject, touch nothing
+      if (tid == 0) {
+        info[sys] = -1;
+        if (iters) iters[sys] = 0;
+        if (eta) eta[sys] = -1.0;
+      }
+      __syncthreads();
+      continue;
+    }
+    const float* A = a + a_off[sys];
+    const float* B = b + b_off[sys];
+    TX* X = x + b_off[sys];
+
+    lds_load_upper<float, float>(n, A, n, s.L, false);
+    __syncthreads();
+    const int fail = lds_chol_factor<float>(n, s.L);
+    __syncthreads();
+    if (fail) {
+      for (int i = tid; i < n; i += kWG) X[i] = static_cast<TX>(quiet_nan<double>());
+      if (tid == 0) {
+        info[sys] = fail;
+        if (iters) iters[sys] = 0;
+        if (eta) eta[sys] = -1.0;
+      }
+      __syncthreads();
+      continue;
+    }
+    if (w == 0) {
+      float v[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] = (l + 64 * c < n) ? B[l + 64 * c] : 0.0f;
+      wave_chol_solve<float>(n, s.L, v);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (l + 64 * c < n) s.x[l + 64 * c] = static_cast<double>(v[c]);
+    }
+    __syncthreads();
+
+    int steps = 0;
+    double last_eta = -1.0;
+    if constexpr (kRefine) {
+      double bnorm = 0.0;
+      for (int i = tid; i < n; i += kWG) bnorm = fmax(bnorm, fabs(static_cast<double>(B[i])));
+      bnorm = block_max(bnorm, s.red);
+      double anorm = 0.0;
+      for (int it = 0; it < max_refine; ++it) {
+        residual_sym_upper(n, A, B, s, it == 0);
+        double rmax = 0.0, xmax = 0.0, amax = 0.0;
+        for (int i = tid; i < n; i += kWG) {
+          rmax = fmax(rmax, fabs(s.r[i]));
+          xmax = fmax(xmax, fabs(s.x[i]));
+          if (it == 0) amax = fmax(amax, s.rowabs[i]);
+        }
+        rmax = block_max(rmax, s.red);
+        xmax = block_max(xmax, s.red);
+        if (it == 0) anorm = block_max(amax, s.red);
+        const double denom = anorm * xmax + bnorm;
+        last_eta = denom > 0.0 ? rmax / denom : (rmax > 0.0 ? INFINITY : 0.0);
+        ++steps;
+        if (!(last_eta > tol)) break;  // also stops on NaN
+        if (w == 0) {
+          float v[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) v[c] = (l + 64 * c < n) ? static_cast<float>(s.r[l + 64 * c]) : 0.0f;
+          wave_chol_solve<float>(n, s.L, v);
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            if (l + 64 * c < n) s.x[l + 64 * c] += static_cast<double>(v[c]);
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = tid; i < n; i += kWG) X[i] = static_cast<TX>(s.x[i]);
+    if (tid == 0) {
+      info[sys] = 0;
+      if (iters) iters[sys] = steps;
+      if (eta) eta[sys] = last_eta;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void zero_u32_kernel(unsigned int* p) { *p = 0u; }
+
+// Kernels using more than 64 KiB of dynamic LDS must say so before launch.
+template <typename K>
+int allow_lds(K kernel, size_t bytes) {
+  if (bytes <= 64 * 1024) return LXA_OK;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+  return e == hipSuccess ? LXA_OK : hip_error(e, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
+}
+
+template <typename T, int N>
+void launch_chol_small(hipStream_t s, int64_t batch, int n, T* a, int64_t stride, int negate, int32_t* info) {
+  constexpr int SPW = 64 / CholLanes<N>::value;
+  const int64_t blocks = (batch + SPW - 1) / SPW;
+  hipLaunchKernelGGL((chol_factor_small_kernel<T, N>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, batch,
+                     n, a, stride, negate, info);
+}
+
+template <typename T, int N>
+void launch_chol_solve_small(hipStream_t s, int64_t batch, int n, const T* u, int64_t us, T* b, int64_t bs,
+                             int negate) {
+  constexpr int SPW = 64 / CholLanes<N>::value;
+  const int64_t blocks = (batch + SPW - 1) / SPW;
+  hipLaunchKernelGGL((chol_solve_small_kernel<T, N>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, batch,
+                     n, u, us, b, bs, negate);
+}
+
+template <typename T>
+int lds_max_n() {
+  return sizeof(T) == 4 ? 256 : 181;  // packed triangle <= 131.6 KB
+}
+
+template <typename T>
+int chol_factor_t(hipStream_t s, int64_t batch, int64_t n64, T* a, int64_t stride, int negate, int32_t* info) {
+  const int n = static_cast<int>(n64);
+  if (n <= 8) launch_chol_small<T, 8>(s, batch, n, a, stride, negate, info);
+  else if (n <= 16) launch_chol_small<T, 16>(s, batch, n, a, stride, negate, info);
+  else if (n <= 32) launch_chol_small<T, 32>(s, batch, n, a, stride, negate, info);
+  else if (n <= 64) launch_chol_small<T, 64>(s, batch, n, a, stride, negate, info);
+  else if (n <= lds_max_n<T>()) {
+    const size_t shmem = static_cast<size_t>(n) * (n + 1) / 2 * sizeof(T);
+    if (int rc = allow_lds(chol_factor_lds_kernel<T>, shmem)) return rc;
+    hipLaunchKernelGGL((chol_factor_lds_kernel<T>), dim3(static_cast<unsigned>(batch)), dim3(kWG), shmem, s, n, a,
+                       stride, negate, info);
+  } else {
+    hipLaunchKernelGGL((chol_factor_generic_kernel<T>), dim3(static_cast<unsigned>(batch)), dim3(kWG), 0, s, n, a,
+                       stride, negate, info);
+  }
+  LXA_LAUNCH_CHECK("lxa_chol_factor launch");
+  return LXA_OK;
+}
+
+template <typename T>
+int chol_solve_t(hipStream_t s, int64_t batch, int64_t n64, const T* u, int64_t us, T* b, int64_t bs, int negate) {
+  const int n = static_cast<int>(n64);
+  if (n <= 8) launch_chol_solve_small<T, 8>(s, batch, n, u, us, b, bs, negate);
+  else if (n <= 16) launch_chol_solve_small<T, 16>(s, batch, n, u, us, b, bs, negate);
+  else if (n <= 32) launch_chol_solve_small<T, 32>(s, batch, n, u, us, b, bs, negate);
+  else if (n <= 64) launch_chol_solve_small<T, 64>(s, batch, n, u, us, b, bs, negate);
+  else if (n <= lds_max_n<T>()) {
+    const size_t shmem = static_cast<size_t>(n) * (n + 1) / 2 * sizeof(T);
+    if (int rc = allow_lds(chol_solve_lds_kernel<T>, shmem)) return rc;
+    hipLaunchKernelGGL((chol_solve_lds_kernel<T>), dim3(static_cast<unsigned>(batch)), dim3(kWG), shmem, s, n, u, us,
+                       b, bs, negate);
+  } else {
+    const size_t shmem = static_cast<size_t>(n) * sizeof(T);
+    if (shmem > 160 * 1024) return set_error(LXA_EUNSUPPORTED, "lxa_chol_solve: n=%d too large", n);
+    if (int rc = allow_lds(chol_solve_generic_kernel<T>, shmem)) return rc;
+    hipLaunchKernelGGL((chol_solve_generic_kernel<T>), dim3(static_cast<unsigned>(batch)), dim3(kWG), shmem, s, n, u,
+                       us, b, bs, negate);
+  }
+  LXA_LAUNCH_CHECK("lxa_chol_solve launch");
+  return LXA_OK;
+}
+
+constexpr int64_t kMaxGrid = 2147483647;
+
+}  // namespace
+}  // namespace lxa
+
+extern "C" int lxa_chol_factor(void* stream, int dtype, int64_t batch, int64_t n, void* a, int64_t a_batch_stride,
+                               int negate, int32_t* info) {
+  LXA_REQUIRE(batch >= 0 && batch <= lxa::kMaxGrid, "lxa_chol_factor: bad batch %lld", (long long)batch);
+  LXA_REQUIRE(n >= 1 && n <= 65536, "lxa_chol_factor: bad n %lld", (long long)n);
+  LXA_REQUIRE(a && info, "lxa_chol_factor: null pointer");
+  LXA_REQUIRE(a_batch_stride >= n * n || batch <= 1, "lxa_chol_factor: a_batch_stride < n*n");
+  if (batch == 0) return LXA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == LXA_F32)
+    return lxa::chol_factor_t<float>(s, batch, n, static_cast<float*>(a), a_batch_stride, negate, info);
+  if (dtype == LXA_F64)
+    return lxa::chol_factor_t<double>(s, batch, n, static_cast<double*>(a), a_batch_stride, negate, info);
+  return lxa::set_error(LXA_EINVAL, "lxa_chol_factor: bad dtype %d", dtype);
+}
+
+extern "C" int lxa_chol_solve(void* stream, int dtype, int64_t batch, int64_t n, const void* u,
+                              int64_t u_batch_stride, void* b, int64_t b_batch_stride, int negate) {
+  LXA_REQUIRE(batch >= 0 && batch <= lxa::kMaxGrid, "lxa_chol_solve: bad batch %lld", (long long)batch);
+  LXA_REQUIRE(n >= 1 && n <= 65536, "lxa_chol_solve: bad n %lld", (long long)n);
+  LXA_REQUIRE(u && b, "lxa_chol_solve: null pointer");
+  LXA_REQUIRE(u_batch_stride >= 0 && b_batch_stride >= 0, "lxa_chol_solve: negative stride");
+  LXA_REQUIRE(b_batch_stride >= n || batch <= 1, "lxa_chol_solve: b_batch_stride < n");
+  if (batch == 0) return LXA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == LXA_F32)
+    return lxa::chol_solve_t<float>(s, batch, n, static_cast<const float*>(u), u_batch_stride,
+                                    static_cast<float*>(b), b_batch_stride, negate);
+  if (dtype == LXA_F64)
+    return lxa::chol_solve_t<double>(s, batch, n, static_cast<const double*>(u), u_batch_stride,
+                                     static_cast<double*>(b), b_batch_stride, negate);
+  return lxa::set_error(LXA_EINVAL, "lxa_chol_solve: bad dtype %d", dtype);
+}
+
+extern "C" int lxa_chol_uneven_workspace_size(int64_t count, size_t* bytes) {
+  LXA_REQUIRE(count >= 0 && bytes, "lxa_chol_uneven_workspace_size: bad argument");
+  *bytes = 256;  // work-queue counter
+  return LXA_OK;
+}
+
+extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, const int32_t* n,
+                                     const int64_t* a_offset, const float* a, const int64_t* b_offset,
+                                     const float* b, void* x, int x_dtype, const int32_t* order, int flags,
+                                     int max_refine, double tol, int32_t* info, int32_t* iters, double* eta,
+                                     void* workspace, size_t workspace_bytes) {
+  LXA_REQUIRE(count >= 0 && count <= lxa::kMaxGrid, "lxa_chol_uneven_solve: bad count");
+  LXA_REQUIRE(max_n >= 1 && max_n <= lxa::kLdsMaxN, "lxa_chol_uneven_solve: max_n must be in [1, %d]",
+              lxa::kLdsMaxN);
+  LXA_REQUIRE(n && a_offset && a && b_offset && b && x && order && info, "lxa_chol_uneven_solve: null pointer");
+  LXA_REQUIRE(x_dtype == LXA_F32 || x_dtype == LXA_F64, "lxa_chol_uneven_solve: bad x_dtype");
+  LXA_REQUIRE(max_refine >= 0 && max_refine <= 64, "lxa_chol_uneven_solve: max_refine in [0, 64]");
+  LXA_REQUIRE(max_refine == 0 || x_dtype == LXA_F64, "lxa_chol_uneven_solve: refinement needs x_dtype F64");
+  const bool persistent = (flags & LXA_UNEVEN_PERSISTENT) != 0;
+  LXA_REQUIRE(!persistent || (workspace && workspace_bytes >= 4), "lxa_chol_uneven_solve: workspace required");
+  if (count == 0) return LXA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool refine = max_refine > 0;
+  const size_t shmem = lxa::uneven_lds_bytes(max_n, refine);
+  LXA_REQUIRE(shmem <= 160 * 1024, "lxa_chol_uneven_solve: LDS need %zu exceeds 160 KiB", shmem);
+  unsigned int* queue = static_cast<unsigned int*>(workspace);
+  int64_t grid = count;
+  if (persistent) {
+    hipLaunchKernelGGL(lxa::zero_u32_kernel, dim3(1), dim3(1), 0, s, queue);
+    LXA_LAUNCH_CHECK("lxa_chol_uneven_solve queue reset");
+    const int per_cu = static_cast<int>((160 * 1024) / shmem);
+    grid = static_cast<int64_t>(lxa::num_cus()) * (per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu));
+    if (grid > count) grid = count;
+  }
+  const int pers = persistent ? 1 : 0;
+  int rc = LXA_OK;
+  if (refine) rc = lxa::allow_lds(lxa::chol_uneven_kernel<true, double>, shmem);
+  else if (x_dtype == LXA_F64) rc = lxa::allow_lds(lxa::chol_uneven_kernel<false, double>, shmem);
+  else rc = lxa::allow_lds(lxa::chol_uneven_kernel<false, float>, shmem);
+  if (rc) return rc;
+  if (refine) {
+    hipLaunchKernelGGL((lxa::chol_uneven_kernel<true, double>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kWG),
+                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<double*>(x), order, pers,
+                       queue, max_refine, tol, info, iters, eta);
+  } else if (x_dtype == LXA_F64) {
+    hipLaunchKernelGGL((lxa::chol_uneven_kernel<false, double>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kWG),
+                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<double*>(x), order, pers,
+                       queue, 0, tol, info, iters, eta);
+  } else {
+    hipLaunchKernelGGL((lxa::chol_uneven_kernel<false, float>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kWG),
+                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<float*>(x), order, pers,
+                       queue, 0, tol, info, iters, eta);
+  }
+  LXA_LAUNCH_CHECK("lxa_chol_uneven_solve launch");
+  return LXA_OK;
+}

@@ -1,0 +1,75 @@
+// Shared helpers for the lineax_amd HIP kernels (gfx950 / CDNA4, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/lineax_amd.h"
+
+namespace lxa {
+
+// Records a printf-style message in the thread-local last-error slot that
+// lxa_last_error() returns. Always returns `code`.
+int set_error(int code, const char* fmt, ...);
+
+// Converts a HIP error into LXA_EHIP with the HIP error string attached.
+int hip_error(hipError_t e, const char* what);
+
+// Number of compute units of the current device (cached per device).
+int num_cus();
+
+constexpr int kWave = 64;
+
+template <typename T>
+__device__ __forceinline__ T quiet_nan();
+template <>
+__device__ __forceinline__ float quiet_nan<float>() {
+  return __builtin_nanf("");
+}
+template <>
+__device__ __forceinline__ double quiet_nan<double>() {
+  return __builtin_nan("");
+}
+
+// Orders LDS traffic between lanes of one wavefront.  LDS instructions of a
+// wave execute in issue order; this only stops the compiler from moving
+// loads/stores across the point.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Index of element (i, j), j >= i, in a row-packed upper triangle of order n.
+__device__ __forceinline__ int packed_upper(int n, int i, int j) {
+  return i * n - ((i * (i - 1)) >> 1) + (j - i);
+}
+
+}  // namespace lxa
+
+#define LXA_REQUIRE(cond, ...)                                   \
+  do {                                                           \
+    if (!(cond)) return ::lxa::set_error(LXA_EINVAL, __VA_ARGS__); \
+  } while (0)
+
+#define LXA_LAUNCH_CHECK(what)                                   \
+  do {                                                           \
+    hipError_t lxa_e_ = hipGetLastError();                       \
+    if (lxa_e_ != hipSuccess) return ::lxa::hip_error(lxa_e_, what); \
+  } while (0)

@@ -1,0 +1,427 @@
+"""ORACLE — test infrastructure only.
+
+A CPU restatement of lineax's solver hot path in float64, used to check the
+GPU kernels.  Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU
+baseline leg may import it; the product path (``lineax_amd``) never does.
+
+Dense factorisations call the C restatement of the LAPACK routines that
+lineax reaches through JAX (``oracle/lxo_dense.c``, built by
+``oracle/Makefile``).  The Krylov solvers are restated here line by line
+from the reference:
+
+  - ``linear_solve_results``: lineax/_solve.py:104-123
+  - ``lu_factor``/``lu_solve``: lineax/_solver/lu.py:43-66
+  - ``cho_factor``/``cho_solve``: lineax/_solver/cholesky.py:43-78
+  - ``qr_factor``/``qr_solve``: lineax/_solver/qr.py:55-94
+  - ``cg``: lineax/_solver/cg.py:114-227
+  - ``gmres``: lineax/_solver/gmres.py:106-413
+  - ``resolve_rcond``: lineax/_misc.py:30-38
+  - ``max_norm``/``two_norm``: lineax/_norm.py:59-139
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liblxo.so"
+
+SUCCESSFUL, MAX_STEPS_REACHED, SINGULAR, BREAKDOWN, STAGNATION, CONLIM, NONFINITE_INPUT = range(7)
+
+_lib = None
+
+
+def build() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ctypes.CDLL(str(LIB))
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int32)
+        L.lxo_getrf.argtypes = [ctypes.c_int, dp, ctypes.c_int, ip]
+        L.lxo_getrf.restype = ctypes.c_int
+        L.lxo_getrs.argtypes = [ctypes.c_int, dp, ctypes.c_int, ip, dp, ctypes.c_int]
+        L.lxo_potrf_upper.argtypes = [ctypes.c_int, dp, ctypes.c_int]
+        L.lxo_potrf_upper.restype = ctypes.c_int
+        L.lxo_potrs_upper.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp]
+        L.lxo_geqrf.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp]
+        L.lxo_ormqr_vec.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, dp, ctypes.c_int]
+        L.lxo_trsv_upper.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int]
+        L.lxo_chol_uneven.argtypes = [
+            ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+        ]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _f64c(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+# ---------------------------------------------------------------------------
+# Dense factorisations (single systems; batches loop in Python)
+# ---------------------------------------------------------------------------
+
+
+def lu_factor(a):
+    """getrf: (lu, piv, info) with 0-based interchanges (jax.scipy convention)."""
+    lu = _f64c(a).copy()
+    n = lu.shape[0]
+    piv = np.zeros(n, dtype=np.int32)
+    info = lib().lxo_getrf(n, _dp(lu), n, _ip(piv))
+    return lu, piv, info
+
+
+def lu_solve(lu, piv, b, trans=0):
+    x = _f64c(b).copy()
+    lib().lxo_getrs(lu.shape[0], _dp(_f64c(lu)), lu.shape[0], _ip(np.ascontiguousarray(piv, np.int32)), _dp(x), trans)
+    return x
+
+
+def cho_factor(a, negate=False):
+    """potrf 'U' of A (or -A): (U with zeros below, info); NaN factor on failure."""
+    u = _f64c(a).copy()
+    if negate:
+        u = -u
+    n = u.shape[0]
+    info = lib().lxo_potrf_upper(n, _dp(u), n)
+    if info:
+        return np.full_like(u, np.nan), info
+    return np.triu(u), 0
+
+
+def cho_solve(u, b, negate=False):
+    x = _f64c(b).copy()
+    lib().lxo_potrs_upper(u.shape[0], _dp(_f64c(u)), u.shape[0], _dp(x))
+    return -x if negate else x
+
+
+def qr_factor(a):
+    """lineax QR.init (qr.py:55-65): geqrf of A, or of A^T when A is wide.
+
+    Returns (a_raw [max(m,n), min(m,n)] with R above and reflectors below,
+    taus, transposed).
+    """
+    a = _f64c(a)
+    m, n = a.shape
+    transpose = n > m
+    if transpose:
+        a = np.ascontiguousarray(a.T)
+        m, n = n, m
+    h = a.copy()
+    tau = np.zeros(min(m, n))
+    lib().lxo_geqrf(m, n, _dp(h), n, _dp(tau))
+    return h, tau, transpose
+
+
+def qr_solve(h, tau, transpose, b):
+    """lineax QR.compute (qr.py:67-94).
+
+    Not transposed: least squares x = R^{-1} (Q^T b)[:n].
+    Transposed (operator was wide): minimum norm x = Q (R^{-T} b ; 0).
+    """
+    n_full, n_min = h.shape
+    h = np.ascontiguousarray(h)
+    if transpose:
+        y = _f64c(b).copy()
+        lib().lxo_trsv_upper(n_min, _dp(h), n_min, _dp(y), 1)
+        c = np.concatenate([y, np.zeros(n_full - n_min)])
+        lib().lxo_ormqr_vec(n_full, n_min, _dp(h), n_min, _dp(np.ascontiguousarray(tau)), _dp(c), 0)
+        return c
+    c = _f64c(b).copy()
+    lib().lxo_ormqr_vec(n_full, n_min, _dp(h), n_min, _dp(np.ascontiguousarray(tau)), _dp(c), 1)
+    x = np.ascontiguousarray(c[:n_min])
+    lib().lxo_trsv_upper(n_min, _dp(h), n_min, _dp(x), 0)
+    return x
+
+
+def chol_uneven(ns, a_off, a32, b_off, b32, threads=1):
+    """float64 Cholesky solves of an uneven batch held as float32 (see lxo_dense.c)."""
+    ns = np.ascontiguousarray(ns, np.int32)
+    a_off = np.ascontiguousarray(a_off, np.int64)
+    b_off = np.ascontiguousarray(b_off, np.int64)
+    a32 = np.ascontiguousarray(a32, np.float32)
+    b32 = np.ascontiguousarray(b32, np.float32)
+    x = np.zeros(b32.shape[0], np.float64)
+    info = np.zeros(ns.shape[0], np.int32)
+    lib().lxo_chol_uneven(
+        ns.shape[0], ns.ctypes.data, a_off.ctypes.data, a32.ctypes.data, b_off.ctypes.data, b32.ctypes.data,
+        x.ctypes.data, info.ctypes.data, int(threads),
+    )
+    return x, info
+
+
+# ---------------------------------------------------------------------------
+# linear_solve semantics
+# ---------------------------------------------------------------------------
+
+
+def linear_solve_results(solution, vector, result):
+    """lineax/_solve.py:104-123 for one system."""
+    if result == SUCCESSFUL and not np.all(np.isfinite(solution)):
+        result = SINGULAR
+    if result == SINGULAR and not np.all(np.isfinite(vector)):
+        result = NONFINITE_INPUT
+    return result
+
+
+def solve_dense(kind, a, b, *, negate=False):
+    """linear_solve(MatrixLinearOperator(a), b, LU()/Cholesky()/QR()) -> (x, result)."""
+    if kind == "lu":
+        lu, piv, _ = lu_factor(a)
+        x = lu_solve(lu, piv, b)
+    elif kind == "cholesky":
+        u, _ = cho_factor(a, negate=negate)
+        x = cho_solve(u, b, negate=negate)
+    elif kind == "qr":
+        h, tau, t = qr_factor(a)
+        x = qr_solve(h, tau, t, b)
+    else:
+        raise ValueError(kind)
+    return x, linear_solve_results(x, b, SUCCESSFUL)
+
+
+def normwise_backward_error(a, x, b):
+    """eta = |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) (Rigal-Gaches), in float64."""
+    a = np.asarray(a, np.float64)
+    x = np.asarray(x, np.float64)
+    b = np.asarray(b, np.float64)
+    r = b - a @ x
+    denom = np.abs(a).sum(axis=-1).max() * np.abs(x).max() + np.abs(b).max()
+    return float(np.abs(r).max() / denom) if denom > 0 else float(np.abs(r).max())
+
+
+def relative_error(x, x_ref):
+    x = np.asarray(x, np.float64)
+    x_ref = np.asarray(x_ref, np.float64)
+    d = np.abs(x_ref).max()
+    return float(np.abs(x - x_ref).max() / d) if d > 0 else float(np.abs(x - x_ref).max())
+
+
+# ---------------------------------------------------------------------------
+# Norms and rcond
+# ---------------------------------------------------------------------------
+
+
+def max_norm(x):
+    x = np.asarray(x)
+    return np.max(np.abs(x)) if x.size else 0.0
+
+
+def two_norm(x):
+    x = np.asarray(x)
+    if x.size == 1:
+        return float(np.abs(x.reshape(())))
+    return float(np.sqrt(np.dot(x.ravel(), x.ravel())))
+
+
+def resolve_rcond(n, m, dtype):
+    return 2 * np.finfo(dtype).eps * max(n, m)
+
+
+# ---------------------------------------------------------------------------
+# CG (lineax/_solver/cg.py:114-227)
+# ---------------------------------------------------------------------------
+
+
+def cg(mv, b, rtol, atol, *, max_steps=None, stabilise_every=10, y0=None, precond=None, norm=max_norm,
+       dtype=np.float64, negate=False, return_trace=False):
+    """Returns (y, result, num_steps).  ``mv`` is the operator's matvec.
+
+    ``negate``: the operator is NSD; CG runs on -A and negates the solution
+    (cg.py:100-101, 224-225).
+    """
+    b = np.asarray(b, dtype)
+    op = (lambda v: -mv(v)) if negate else mv
+    M = precond if precond is not None else (lambda v: v)
+    size = b.size
+    steps_cap = 10 * size if max_steps is None else max_steps
+    y = np.zeros_like(b) if y0 is None else np.asarray(y0, dtype).copy()
+    r = b - op(y)
+    p = M(r)
+    gamma = np.dot(p, r)
+    rcond = resolve_rcond(size, size, dtype)
+    diff = np.full_like(y, np.inf)
+    has_scale = not (atol == 0 and rtol == 0)
+    b_scale = atol + rtol * np.abs(b)
+    step = 0
+    trace = []
+
+    def not_converged(r, diff, y):
+        if not has_scale:
+            return True
+        y_scale = atol + rtol * np.abs(y)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            n1 = norm(r / b_scale)
+            n2 = norm(diff / y_scale)
+        return bool(n1 > 1) or bool(n2 > 1)
+
+    while bool(gamma > 0) and step < steps_cap and not_converged(r, diff, y):
+        q = op(p)
+        ip = np.dot(q, p)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            alpha = gamma / ip
+        if not bool(abs(ip) > 100 * rcond * abs(gamma)):
+            alpha = np.nan
+        diff = alpha * p
+        y = y + diff
+        step += 1
+        if stabilise_every == 1:
+            r = b - op(y)
+        elif stabilise_every is None:
+            r = r - alpha * q
+        elif step % stabilise_every == 0:
+            r = b - op(y)
+        else:
+            r = r - alpha * q
+        z = M(r)
+        gamma_prev = gamma
+        gamma = np.dot(z, r)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            beta = gamma / gamma_prev
+        p = z + beta * p
+        if return_trace:
+            trace.append(float(np.sqrt(np.dot(r, r))))
+
+    if max_steps is None:
+        result = SINGULAR if step == steps_cap else SUCCESSFUL
+    elif has_scale:
+        result = MAX_STEPS_REACHED if step == steps_cap else SUCCESSFUL
+    else:
+        result = SUCCESSFUL
+    if negate:
+        y = -y
+    result = linear_solve_results(y, b, result)
+    if return_trace:
+        return y, result, step, trace
+    return y, result, step
+
+
+# ---------------------------------------------------------------------------
+# GMRES (lineax/_solver/gmres.py:106-413)
+# ---------------------------------------------------------------------------
+
+
+def _normalise(x, eps=None):
+    nrm = two_norm(x)
+    if eps is None:
+        eps = np.finfo(np.asarray(x).dtype).eps
+    breakdown = nrm < eps
+    safe = np.inf if breakdown else nrm
+    return x / safe, nrm, breakdown
+
+
+def _hessenberg_lstsq(coeff_mat, r_norm):
+    """z = linear_solve(MatrixLinearOperator(coeff_mat.T), beta e1, QR(), throw=False)."""
+    h = coeff_mat.T  # (restart + 1, restart)
+    beta = np.zeros(h.shape[0])
+    beta[0] = r_norm
+    raw, tau, transposed = qr_factor(h)
+    return qr_solve(raw, tau, transposed, beta)
+
+
+def gmres(mv, b, rtol, atol, *, max_steps=None, restart=20, stagnation_iters=20, y0=None, precond=None,
+          norm=max_norm, dtype=np.float64):
+    """Returns (y, result, num_steps) with lineax's restart/breakdown/stagnation rules."""
+    b = np.asarray(b, dtype)
+    M = precond if precond is not None else (lambda v: v)
+    has_scale = not (atol == 0 and rtol == 0)
+    b_scale = atol + rtol * np.abs(b)
+    size = b.size
+    steps_cap = 10 * size if max_steps is None else max_steps
+    restart = min(restart, size)
+    eps = np.finfo(dtype).eps
+
+    def not_converged(r, diff, y):
+        if not has_scale:
+            return True
+        y_scale = atol + rtol * np.abs(y)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            n1 = norm(r / b_scale)
+            n2 = norm(diff / y_scale)
+        return bool(n1 > 1) or bool(n2 > 1)
+
+    def cycle(y, r, first_pass):
+        if first_pass:
+            y_new, diff, breakdown = y, np.full_like(y, np.inf), False
+        else:
+            r_normalised, r_norm, breakdown = _normalise(r)
+            basis = np.zeros((size, restart + 1), dtype)
+            basis[:, 0] = r_normalised
+            coeff = np.eye(restart, restart + 1, dtype=dtype)
+            j = 0
+            while j < restart and not breakdown:
+                w = M(mv(basis[:, j]))
+                step_norm = two_norm(w)
+                proj = basis.T @ w
+                w_new = w - basis @ proj
+                w_normed, w_norm, breakdown = _normalise(w_new, eps=step_norm * eps)
+                basis[:, j + 1] = w_normed
+                proj[j + 1] = w_norm
+                coeff[j, :] = proj
+                j += 1
+            z = _hessenberg_lstsq(coeff, r_norm)
+            diff = basis[:, :-1] @ z
+            y_new = y + diff
+        r_new = M(b - mv(y_new))
+        return y_new, r_new, breakdown, diff
+
+    y = np.zeros_like(b) if y0 is None else np.asarray(y0, dtype).copy()
+    r = np.zeros_like(b)
+    breakdown = False
+    deferred = False
+    diff = np.full_like(y, np.inf)
+    r_min = np.inf
+    step = 0
+    stag = 0
+
+    def cond():
+        out = (not deferred) and stag < stagnation_iters
+        out = out and not_converged(r, diff, y)
+        out = out and step < steps_cap
+        return out or step == 0
+
+    while cond():
+        y_new, r_new, breakdown_new, diff_new = cycle(y, r, step == 0)
+        r_new_norm = norm(r_new)
+        if (r_new_norm - r_min) < 0:
+            stag = 0
+        else:
+            stag += 1
+        r_min = min(r_new_norm, r_min)
+        deferred = breakdown
+        breakdown = breakdown_new
+        y, r, diff = y_new, r_new, diff_new
+        step += 1
+
+    if max_steps is None:
+        result = SINGULAR if step == steps_cap else SUCCESSFUL
+    elif has_scale:
+        result = MAX_STEPS_REACHED if step == steps_cap else SUCCESSFUL
+    else:
+        result = SUCCESSFUL
+    if stag >= stagnation_iters:
+        result = STAGNATION
+    if deferred and not_converged(r, diff, y):
+        result = BREAKDOWN
+    result = linear_solve_results(y, b, result)
+    return y, result, step

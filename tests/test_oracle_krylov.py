@@ -1,0 +1,165 @@
+"""Pins the CG/GMRES restatements in oracle/lineax_oracle.py.
+
+Checks: solutions against direct solves and scipy.sparse.linalg; the
+eigenvector closed form x = b / lambda; the RESULTS rules of
+lineax/_solver/cg.py:213-222 and gmres.py:217-237 (max_steps with and without
+tolerances, stagnation, breakdown); the reference's 4x4 GMRES restart=2
+case (tests/test_singular.py:83-99), which must not report success.
+A planted mistake (dropping the stabilisation, flipping a sign) must fail.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import lineax_oracle as O
+from tests.helpers import poisson_1d, well_conditioned
+
+
+def test_cg_matches_direct_solve(rng):
+    a = well_conditioned(rng, 30, kind="psd")
+    b = rng.standard_normal(30)
+    x, result, steps = O.cg(lambda v: a @ v, b, 1e-12, 1e-12)
+    assert result == O.SUCCESSFUL
+    np.testing.assert_allclose(x, np.linalg.solve(a, b), rtol=1e-9, atol=1e-10)
+    assert 0 < steps <= 300
+
+
+def test_cg_nsd_poisson(rng):
+    # lineax tests: CG with negative_semidefinite_tag on the 1-D Poisson matrix.
+    a = poisson_1d(50)
+    b = rng.standard_normal(50)
+    x, result, _ = O.cg(lambda v: a @ v, b, 1e-12, 1e-12, negate=True)
+    assert result == O.SUCCESSFUL
+    np.testing.assert_allclose(x, np.linalg.solve(a, b), rtol=1e-9, atol=1e-9)
+
+
+def test_cg_eigenvector_closed_form():
+    n = 64
+    a = -poisson_1d(n)  # SPD
+    k = 3
+    b = np.sin(np.pi * k * np.arange(1, n + 1) / (n + 1))
+    lam = 2 - 2 * np.cos(np.pi * k / (n + 1))
+    x, result, steps = O.cg(lambda v: a @ v, b, 1e-13, 1e-13)
+    assert result == O.SUCCESSFUL
+    np.testing.assert_allclose(x, b / lam, rtol=1e-11, atol=1e-12)
+    assert steps <= 2  # an eigenvector converges in one step (plus the dual test)
+
+
+def test_cg_agrees_with_scipy(rng):
+    a = well_conditioned(rng, 40, kind="psd")
+    b = rng.standard_normal(40)
+    x, _, _ = O.cg(lambda v: a @ v, b, 1e-10, 1e-10)
+    xs, info = spla.cg(a, b, rtol=1e-12, atol=0.0, maxiter=2000)
+    assert info == 0
+    np.testing.assert_allclose(x, xs, rtol=1e-7, atol=1e-8)
+
+
+def test_cg_max_steps_rules(rng):
+    a = -poisson_1d(100)
+    b = rng.standard_normal(100)
+    mv = lambda v: a @ v
+    # max_steps only, no tolerance (tests/test_solve.py:174-194): successful.
+    _, result, steps = O.cg(mv, b, 0.0, 0.0, max_steps=2)
+    assert steps == 2 and result == O.SUCCESSFUL
+    # tolerance not met within max_steps: max_steps_reached.
+    _, result, steps = O.cg(mv, b, 1e-12, 1e-12, max_steps=3)
+    assert steps == 3 and result == O.MAX_STEPS_REACHED
+    # default max_steps = 10 * size reached: singular.
+    sing = np.diag(np.r_[np.ones(5), 0.0])
+    _, result, steps = O.cg(lambda v: sing @ v, np.ones(6), 1e-12, 1e-12)
+    assert result in (O.SINGULAR,)
+
+
+def test_cg_breakdown_guard_gives_nan(rng):
+    # rcond = 2 eps size (lineax/_misc.py:36): a tiny <Ap,p> relative to
+    # gamma NaNs the step, which linear_solve reports as singular.
+    a = np.diag([1.0, 1e-30])
+    b = np.array([0.0, 1.0])
+    x, result, _ = O.cg(lambda v: a @ v, b, 1e-12, 1e-12)
+    assert np.all(np.isnan(x)) and result == O.SINGULAR
+
+
+def test_cg_stabilisation_matters_for_trace(rng):
+    # Planted mistake: stabilise_every=None changes the residual sequence,
+    # so a restatement that dropped stabilisation would not match.
+    a = well_conditioned(rng, 60, kind="psd")
+    b = rng.standard_normal(60)
+    _, _, s1, t1 = O.cg(lambda v: a @ v, b, 1e-14, 1e-14, return_trace=True)
+    _, _, s2, t2 = O.cg(lambda v: a @ v, b, 1e-14, 1e-14, stabilise_every=None, return_trace=True)
+    assert t1[:9] == t2[:9]
+    assert t1[9] != t2[9]
+
+
+def test_gmres_large_dense(rng):
+    # lineax tests/test_solve.py:24-38: 100x100 N(0,1), restart=100, tol 1e-10.
+    a = rng.standard_normal((100, 100))
+    true_x = rng.standard_normal(100)
+    b = a @ true_x
+    x, result, steps = O.gmres(lambda v: a @ v, b, 1e-10, 1e-10, restart=100)
+    assert result == O.SUCCESSFUL
+    np.testing.assert_allclose(x, true_x, rtol=1e-8, atol=1e-8)
+
+
+def test_gmres_reference_4x4_restart2_fails():
+    # lineax tests/test_singular.py:83-99: must not report success.
+    a = np.array(
+        [
+            [0.15892892, 0.05884365, -0.60427412, 0.1891916],
+            [-1.5484863, 0.93608822, 1.94888868, 1.37069667],
+            [0.62687318, -0.13996738, -0.6824359, 0.30975754],
+            [-0.67428635, 1.52372255, -0.88277754, 0.69633816],
+        ]
+    )
+    true_x = np.array([0.51383273, 1.72983427, -0.43251078, -1.11764668])
+    b = a @ true_x
+    _, result, _ = O.gmres(lambda v: a @ v, b, 1e-10, 1e-10, restart=2)
+    assert result != O.SUCCESSFUL
+
+
+def test_gmres_singular_reports_failure(rng):
+    a = rng.standard_normal((3, 3))
+    a[0, :] = 0.0
+    b = rng.standard_normal(3)
+    _, result, _ = O.gmres(lambda v: a @ v, b, 1e-12, 1e-12)
+    assert result != O.SUCCESSFUL
+
+
+def test_gmres_agrees_with_scipy(rng):
+    n = 80
+    a = np.eye(n) * 4 + rng.standard_normal((n, n)) / np.sqrt(n)
+    b = rng.standard_normal(n)
+    x, result, _ = O.gmres(lambda v: a @ v, b, 1e-11, 1e-11, restart=30)
+    assert result == O.SUCCESSFUL
+    xs, info = spla.gmres(a, b, rtol=1e-13, atol=0.0, restart=30, maxiter=100)
+    assert info == 0
+    np.testing.assert_allclose(x, xs, rtol=1e-8, atol=1e-9)
+
+
+def test_gmres_counts_dummy_first_pass(rng):
+    # The first cycle only forms r0 (gmres.py:313-317).  On the identity the
+    # second cycle solves exactly, but the dual test also needs the last
+    # update (diff) to be small (gmres.py:130-141), so a third cycle -- which
+    # starts from a zero residual, breaks down at once and changes nothing --
+    # ends the solve: three steps, successful.
+    n = 5
+    b = rng.standard_normal(n)
+    x, result, steps = O.gmres(lambda v: v, b, 1e-12, 1e-12)
+    assert result == O.SUCCESSFUL and steps == 3
+    np.testing.assert_allclose(x, b, rtol=1e-14)
+
+
+def test_gmres_stagnation_rule():
+    # A residual that never decreases triggers the stagnation counter.
+    a = np.array([[0.0, 1.0], [-1.0, 0.0]])  # rotation: restart=1 makes no progress
+    b = np.array([1.0, 0.0])
+    _, result, steps = O.gmres(lambda v: a @ v, b, 1e-12, 1e-12, restart=1, stagnation_iters=3)
+    assert result in (O.STAGNATION, O.BREAKDOWN)
+
+
+@pytest.mark.parametrize("restart", [5, 20])
+def test_gmres_max_steps_reached(rng, restart):
+    a = well_conditioned(rng, 40) + 0.0
+    b = rng.standard_normal(40)
+    _, result, steps = O.gmres(lambda v: a @ v, b, 1e-14, 1e-14, restart=restart, max_steps=2)
+    assert steps == 2 and result == O.MAX_STEPS_REACHED
