@@ -151,6 +151,122 @@ int lxa_solve_results(void* stream, int dtype, int64_t rows, int64_t len,
                       const void* vector, int64_t vector_row_stride,
                       int32_t* result);
 
+
+/* ------------------------------------------------------------------------
+ * Grid operators (matrix-free FunctionLinearOperator.mv of the Krylov
+ * workloads, lineax/_operator.py:664-732)
+ * ---------------------------------------------------------------------- */
+
+/* y = A x for the five-point stencil with constant coefficients
+ * coeffs = {center, west (j-1), east (j+1), south (i-1), north (i+1)} on an
+ * nx x ny grid (j fastest) with zero Dirichlet boundary, for `rows` grid
+ * functions.  in/out dtypes may differ (FP64 accumulation). */
+int lxa_stencil5_apply(void* stream, int in_dtype, int out_dtype, int64_t rows, int64_t nx, int64_t ny,
+                       const double* coeffs, const void* x, int64_t x_row_stride, void* y,
+                       int64_t y_row_stride);
+
+/* ------------------------------------------------------------------------
+ * Krylov building blocks (lineax/_solver/cg.py, lineax/_solver/gmres.py)
+ *
+ * A batch of right-hand sides is `rows` vectors of length n, row-major.
+ * Each row has its own device state; finished rows are frozen (lineax under
+ * jax.vmap).  The operator's matvec is applied by the caller between these
+ * calls (it can be any operator: the GEMV kernel, a stencil kernel, or user
+ * code).  Reductions are FP64, two-phase and deterministic.  `ydtype` is
+ * the dtype of y and b; `vdtype` that of the working vectors.  Supported
+ * pairs (y, vectors): (F64, F64), (F32, F32), (F64, F32).
+ * ---------------------------------------------------------------------- */
+
+typedef struct {
+  double gamma;     /* CG: <z, r>                                       */
+  double alpha;     /* CG: step length (NaN when the breakdown guard fires) */
+  double beta;      /* CG: gamma_new / gamma                            */
+  double ip;        /* CG: <A p, p>                                     */
+  double norm_r;    /* solver norm of r / (atol + rtol |b|)             */
+  double norm_diff; /* solver norm of diff / (atol + rtol |y|)          */
+  int32_t step;     /* CG iterations / GMRES cycles incl. the first pass */
+  int32_t active;   /* 1 while the row's loop condition holds           */
+  int32_t result;   /* RESULTS code, written by lxa_*_result            */
+  int32_t pad;
+} lxa_krylov_row; /* 64 bytes */
+
+typedef struct {
+  double r_norm;    /* |r|_2 at the start of the cycle                  */
+  double step_norm; /* |A v_j|_2 of the current Arnoldi step             */
+  double scale;     /* 1/|.| of the last normalised vector (0 on breakdown) */
+  double r_min;     /* smallest residual norm seen (stagnation test)     */
+  int32_t inner;    /* Arnoldi steps done in this cycle                  */
+  int32_t columns;  /* basis columns written in this cycle               */
+  int32_t breakdown;       /* breakdown in this cycle                    */
+  int32_t inner_done;      /* inner loop finished for this row           */
+  int32_t carry_breakdown; /* breakdown of the last cycle                */
+  int32_t deferred;        /* breakdown of the cycle before (gmres.py:145-158) */
+  int32_t stagnation;      /* cycles without a decrease of the residual  */
+  int32_t pad;
+} lxa_gmres_row; /* 64 bytes */
+
+typedef struct {
+  double rtol, atol;
+  double rcond;        /* CG guard: 2 eps(dtype) size (lineax/_misc.py:30-38) */
+  double eps;          /* GMRES breakdown scale: eps(dtype)                   */
+  int64_t max_steps;   /* 10 * size when max_steps_given == 0                  */
+  int32_t max_steps_given;
+  int32_t norm_kind;   /* 0 max_norm (default), 1 two_norm, 2 rms_norm        */
+  int32_t restart;     /* GMRES: min(restart, size), at most 128              */
+  int32_t stagnation_iters;
+} lxa_krylov_params;
+
+/* Bytes of FP64 partial-sum scratch for a batch (GMRES: pass its restart). */
+int lxa_krylov_partials_size(int64_t rows, int64_t n, int restart, size_t* bytes);
+
+/* CG, one iteration = alpha, update (or update + residual on stabilising
+ * steps), [preconditioner gamma], finish.  Replaces the lax.while_loop of
+ * lineax/_solver/cg.py:162-211; the result mapping is cg.py:213-222. */
+int lxa_cg_begin(void* stream, int vdtype, int64_t rows, int64_t n, const void* z, const void* r,
+                 const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
+int lxa_cg_alpha(void* stream, int vdtype, int64_t rows, int64_t n, const void* q, const void* p,
+                 const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
+int lxa_cg_update(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* p, const void* q,
+                  void* y, void* r, const void* b, int stable, int fuse_gamma, const lxa_krylov_params* prm,
+                  lxa_krylov_row* state, double* partials);
+int lxa_cg_residual(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* b, const void* t,
+                    void* r, int fuse_gamma, const lxa_krylov_params* prm, lxa_krylov_row* state,
+                    double* partials);
+int lxa_cg_precond_gamma(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* b,
+                         const void* z, const void* r, const lxa_krylov_params* prm, lxa_krylov_row* state,
+                         double* partials);
+int lxa_cg_finish(void* stream, int vdtype, int64_t rows, int64_t n, const void* z, void* p,
+                  const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
+int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state);
+
+/* GMRES: init, then per restart cycle: cycle_begin (normalise r into basis
+ * column 0), for j < restart: [caller: w = M A v_j] arnoldi(j), then update
+ * (Hessenberg least squares by Householder QR, y += V z), [caller:
+ * t = A y], residual (r = b - t, then optionally the caller preconditions
+ * r and calls residual again with write = 0 for the norms), cycle_end.  The
+ * first cycle is lineax's dummy pass: skip cycle_begin/arnoldi/update and
+ * call cycle_end with first_pass = 1.  Replaces gmres.py:143-320 and the
+ * result mapping gmres.py:217-237.  Basis: [rows][restart + 1][n];
+ * hessenberg: [rows][restart][restart + 1] FP64; proj: [rows][restart + 1];
+ * z: [rows][restart]. */
+int lxa_gmres_init(void* stream, int64_t rows, lxa_krylov_row* state, lxa_gmres_row* gstate);
+int lxa_gmres_cycle_begin(void* stream, int vdtype, int64_t rows, int64_t n, const void* r, void* basis,
+                          const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                          double* hessenberg, double* partials);
+int lxa_gmres_arnoldi(void* stream, int vdtype, int64_t rows, int64_t n, int j, void* basis, const void* w,
+                      const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                      double* hessenberg, double* proj, double* partials);
+int lxa_gmres_update(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* basis, void* y,
+                     const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                     const double* hessenberg, double* z, double* partials);
+int lxa_gmres_residual(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* b,
+                       const void* t, void* r, int write, const lxa_krylov_params* prm, lxa_krylov_row* state,
+                       double* partials);
+int lxa_gmres_cycle_end(void* stream, int64_t rows, int64_t n, int first_pass, const lxa_krylov_params* prm,
+                        lxa_krylov_row* state, lxa_gmres_row* gstate, double* partials);
+int lxa_gmres_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state,
+                     const lxa_gmres_row* gstate);
+
 #ifdef __cplusplus
 }
 #endif

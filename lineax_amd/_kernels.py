@@ -45,6 +45,22 @@ _SIGNATURES = {
     ],
     "lxa_gemv": [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int],
     "lxa_solve_results": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp],
+    "lxa_stencil5_apply": [_vp, _int, _int, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64],
+    "lxa_krylov_partials_size": [_i64, _i64, _int, _szp],
+    "lxa_cg_begin": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "lxa_cg_alpha": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "lxa_cg_update": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp],
+    "lxa_cg_residual": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _int, _vp, _vp, _vp],
+    "lxa_cg_precond_gamma": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lxa_cg_finish": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "lxa_cg_result": [_vp, _i64, _vp, _vp],
+    "lxa_gmres_init": [_vp, _i64, _vp, _vp],
+    "lxa_gmres_cycle_begin": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lxa_gmres_arnoldi": [_vp, _int, _i64, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lxa_gmres_update": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lxa_gmres_residual": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _int, _vp, _vp, _vp],
+    "lxa_gmres_cycle_end": [_vp, _i64, _i64, _int, _vp, _vp, _vp, _vp],
+    "lxa_gmres_result": [_vp, _i64, _vp, _vp, _vp],
     "lxa_version": [],
     "lxa_last_error": [],
 }
@@ -349,3 +365,105 @@ def solve_results(solution: torch.Tensor, vector: torch.Tensor, result: torch.Te
         "lxa_solve_results",
     )
     return result
+
+
+# ---------------------------------------------------------------------------
+# Grid stencils
+# ---------------------------------------------------------------------------
+
+
+def stencil5(x: torch.Tensor, nx: int, ny: int, coeffs, out_dtype=None) -> torch.Tensor:
+    """Five-point stencil on grid functions: x is [..., nx, ny] or [..., nx*ny]."""
+    _gpu(x, "vector")
+    out_dtype = out_dtype or x.dtype
+    size = nx * ny
+    if x.numel() % size:
+        raise ValueError(f"vector of shape {tuple(x.shape)} is not a batch of {nx}x{ny} grids")
+    rows = x.numel() // size
+    xv = x.reshape(rows, size)
+    if not xv.is_contiguous():
+        xv = xv.contiguous()
+    y = torch.empty((rows, size), dtype=out_dtype, device=x.device)
+    if rows == 0:
+        return y.reshape(x.shape)
+    c = (ctypes.c_double * 5)(*[float(v) for v in coeffs])
+    _check(
+        load().lxa_stencil5_apply(_stream(x), _dtype_code(x.dtype), _dtype_code(out_dtype), rows, nx, ny,
+                                  ctypes.cast(c, ctypes.c_void_p), _ptr(xv), size, _ptr(y), size),
+        "lxa_stencil5_apply",
+    )
+    return y.reshape(x.shape)
+
+
+# ---------------------------------------------------------------------------
+# Krylov building blocks
+# ---------------------------------------------------------------------------
+
+
+class KrylovParams(ctypes.Structure):
+    _fields_ = [
+        ("rtol", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("rcond", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("max_steps", ctypes.c_int64),
+        ("max_steps_given", ctypes.c_int32),
+        ("norm_kind", ctypes.c_int32),
+        ("restart", ctypes.c_int32),
+        ("stagnation_iters", ctypes.c_int32),
+    ]
+
+
+ROW_BYTES = 64  # lxa_krylov_row and lxa_gmres_row
+
+
+class RowState:
+    """Device array of per-row Krylov state (lxa_krylov_row / lxa_gmres_row)."""
+
+    def __init__(self, rows: int, device):
+        self.raw = torch.zeros((rows, ROW_BYTES), dtype=torch.uint8, device=device)
+        self.f64 = self.raw.view(torch.float64)  # [rows, 8]
+        self.i32 = self.raw.view(torch.int32)  # [rows, 16]
+
+    @property
+    def ptr(self):
+        return self.raw.data_ptr()
+
+    # lxa_krylov_row fields
+    def step(self):
+        return self.i32[:, 12]
+
+    def active(self):
+        return self.i32[:, 13]
+
+    def result(self):
+        return self.i32[:, 14]
+
+
+def partials_buffer(rows: int, n: int, restart: int, device) -> torch.Tensor:
+    need = ctypes.c_size_t(0)
+    _check(load().lxa_krylov_partials_size(rows, n, restart, ctypes.byref(need)), "lxa_krylov_partials_size")
+    return torch.empty(max(8, int(need.value) // 8), dtype=torch.float64, device=device)
+
+
+def call(name: str, *args) -> None:
+    """Invokes a Krylov entry point; tensors are passed as device pointers."""
+    conv = []
+    for a in args:
+        if isinstance(a, torch.Tensor):
+            conv.append(a.data_ptr())
+        elif isinstance(a, RowState):
+            conv.append(a.ptr)
+        elif isinstance(a, KrylovParams):
+            conv.append(ctypes.cast(ctypes.pointer(a), ctypes.c_void_p))
+        else:
+            conv.append(a)
+    _check(getattr(load(), name)(*conv), name)
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    return _dtype_code(dtype)
+
+
+def current_stream(t: torch.Tensor) -> int:
+    return _stream(t)

@@ -165,6 +165,7 @@ class _MatVec(torch.autograd.Function):
     def setup_context(ctx, inputs, output):
         a, x = inputs
         ctx.save_for_backward(a, x)
+        ctx.save_for_forward(a, x)
 
     @staticmethod
     def backward(ctx, g):
@@ -934,3 +935,22 @@ def conj(operator):
 def diagonal(operator) -> torch.Tensor:
     """Diagonal of the operator's matrix."""
     return torch.diagonal(operator.as_matrix(), dim1=-2, dim2=-1)
+
+
+def mv_as(operator, v: torch.Tensor, struct, out_dtype: torch.dtype) -> torch.Tensor:
+    """operator.mv on a flat [rows, n] tensor, returned flat in ``out_dtype``.
+
+    Operators with an ``mv_as`` method (the stencil operators) compute in
+    the requested storage dtype directly; negation and tag wrappers pass it
+    through; any other operator is called on its own dtype with casts.
+    """
+    rows = v.shape[0]
+    if isinstance(operator, TaggedLinearOperator):
+        return mv_as(operator.operator, v, struct, out_dtype)
+    if isinstance(operator, NegLinearOperator):
+        return -mv_as(operator.operator, v, struct, out_dtype)
+    direct = getattr(operator, "mv_as", None)
+    if direct is not None and isinstance(struct, TensorStruct):
+        return direct(v.reshape(rows, *struct.shape), out_dtype).reshape(rows, -1)
+    out = operator.mv(unravel(v, struct, (rows,)))
+    return ravel(out, operator.out_structure(), (rows,)).to(out_dtype)

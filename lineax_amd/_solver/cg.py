@@ -1,12 +1,176 @@
-"""CG solver (lineax/_solver/cg.py) — kernels pending."""
+"""CG solver (lineax/_solver/cg.py) driving the device Krylov kernels."""
 
 from __future__ import annotations
 
-from .unsupported import _Unsupported
+import torch
+
+from .. import _kernels as K
+from .._misc import resolve_rcond
+from .._norm import max_norm, rms_norm, two_norm
+from .._operator import (
+    IdentityLinearOperator,
+    conj,
+    mv_as,
+    is_negative_semidefinite,
+    is_positive_semidefinite,
+    linearise,
+)
+from .._solve import AbstractLinearSolver
+from .._structure import batch_shape_of, ravel, structure_equal, unravel
+from .misc import preconditioner_and_y0
+
+NORM_KINDS = {max_norm: 0, two_norm: 1, rms_norm: 2}
+# Iterations between host checks of the per-row `active` flags: grows from
+# 1 to this, so short solves stop promptly and long ones sync rarely.
+MAX_CHECK_BLOCK = 32
 
 
-class CG(_Unsupported):
-    _reference = "lineax/_solver/cg.py"
+def norm_kind(norm) -> int:
+    try:
+        return NORM_KINDS[norm]
+    except KeyError:
+        raise NotImplementedError(
+            "lineax_amd Krylov solvers support norm=max_norm, two_norm or rms_norm"
+        ) from None
+
+
+class FlatOperator:
+    """An operator's matvec on flat [rows, n] tensors, in a chosen output dtype."""
+
+    def __init__(self, operator, struct, batch):
+        self.op = operator
+        self.struct = struct
+        self.identity = isinstance(operator, IdentityLinearOperator)
+
+    def __call__(self, v: torch.Tensor, out_dtype: torch.dtype) -> torch.Tensor:
+        rows = v.shape[0]
+        if self.identity:
+            return v if v.dtype == out_dtype else v.to(out_dtype)
+        return mv_as(self.op, v, self.struct, out_dtype)
+
+
+class CG(AbstractLinearSolver):
+    """Conjugate gradients for positive or negative definite operators.
+
+    Semantics of lineax/_solver/cg.py:48-250: stabilised residual every
+    ``stabilise_every`` steps, the dual stopping test in both the residual and
+    the iterate, the ``rcond`` breakdown guard, max_steps = 10 * size by
+    default and the RESULTS mapping.  ``working_dtype`` (not in lineax)
+    stores r, p and A p in a narrower dtype than the vector's: with a float64
+    right-hand side and ``working_dtype=torch.float32`` the iterate, the
+    stabilising residuals and all inner products stay float64.
+    """
+
+    def __init__(self, rtol, atol, norm=max_norm, stabilise_every: int | None = 10, max_steps: int | None = None,
+                 working_dtype: torch.dtype | None = None):
+        if isinstance(rtol, (int, float)) and rtol < 0:
+            raise ValueError("Tolerances must be non-negative.")
+        if isinstance(atol, (int, float)) and atol < 0:
+            raise ValueError("Tolerances must be non-negative.")
+        if atol == 0 and rtol == 0 and max_steps is None:
+            raise ValueError("Must specify `rtol`, `atol`, or `max_steps` (or some combination of all three).")
+        self.rtol = rtol
+        self.atol = atol
+        self.norm = norm
+        self.stabilise_every = stabilise_every
+        self.max_steps = max_steps
+        self.working_dtype = working_dtype
+
+    def init(self, operator, options):
+        is_nsd = is_negative_semidefinite(operator)
+        if not structure_equal(operator.in_structure(), operator.out_structure()):
+            raise ValueError("`CG()` may only be used for linear solves with square matrices.")
+        if not (is_positive_semidefinite(operator) or is_nsd):
+            raise ValueError("`CG()` may only be used for positive or negative definite linear operators")
+        if is_nsd:
+            operator = -operator
+        return linearise(operator), is_nsd
+
+    def compute(self, state, vector, options):
+        operator, is_nsd = state
+        preconditioner, y0 = preconditioner_and_y0(operator, vector, options)
+        if not is_positive_semidefinite(preconditioner):
+            raise ValueError("The preconditioner must be positive definite.")
+        struct = operator.in_structure()
+        batch = batch_shape_of(vector, struct)
+        b = ravel(vector, struct, batch)
+        size = b.shape[-1]
+        rows = b.numel() // size if size else 0
+        b = b.reshape(rows, size).contiguous()
+        ydt = b.dtype
+        vdt = self.working_dtype or ydt
+        dev = b.device
+        max_steps = 10 * size if self.max_steps is None else int(self.max_steps)
+        prm = K.KrylovParams(
+            rtol=float(self.rtol), atol=float(self.atol), rcond=resolve_rcond(None, size, size, ydt), eps=0.0,
+            max_steps=max_steps, max_steps_given=int(self.max_steps is not None), norm_kind=norm_kind(self.norm),
+            restart=0, stagnation_iters=0,
+        )
+        A = FlatOperator(operator, struct, batch)
+        M = FlatOperator(preconditioner, struct, batch)
+        precond = not M.identity
+        yd, vd = K.dtype_code(ydt), K.dtype_code(vdt)
+        st = K.RowState(rows, dev)
+        part = K.partials_buffer(rows, size, 0, dev)
+        stream = K.current_stream(b)
+
+        y = ravel(y0, struct, batch).reshape(rows, size).to(ydt).contiguous().clone()
+        r = torch.empty((rows, size), dtype=vdt, device=dev)
+        st.active().fill_(1)
+        t = A(y, ydt)
+        K.call("lxa_cg_residual", stream, yd, vd, rows, size, b, t, r, 0, prm, st, part)
+        z = M(r, vdt) if precond else r
+        p = z.clone()
+        K.call("lxa_cg_begin", stream, vd, rows, size, z, r, prm, st, part)
+
+        it = 0
+        block = 1
+        while True:
+            for _ in range(block):
+                q = A(p, vdt)
+                K.call("lxa_cg_alpha", stream, vd, rows, size, q, p, prm, st, part)
+                se = self.stabilise_every
+                stable = se == 1 or (se is not None and (it + 1) % se == 0)
+                fuse = (not precond) and (not stable)
+                K.call("lxa_cg_update", stream, yd, vd, rows, size, p, q, y, r, b, int(stable), int(fuse), prm, st,
+                       part)
+                if stable:
+                    t = A(y, ydt)
+                    K.call("lxa_cg_residual", stream, yd, vd, rows, size, b, t, r, int(not precond), prm, st, part)
+                if precond:
+                    z = M(r, vdt)
+                    K.call("lxa_cg_precond_gamma", stream, yd, vd, rows, size, b, z, r, prm, st, part)
+                else:
+                    z = r
+                K.call("lxa_cg_finish", stream, vd, rows, size, z, p, prm, st, part)
+                it += 1
+                if it >= max_steps:
+                    break
+            if it >= max_steps or not bool(st.active().any()):
+                break
+            block = min(block * 2, MAX_CHECK_BLOCK)
+        K.call("lxa_cg_result", stream, rows, prm, st)
+
+        if is_nsd:
+            y = -y
+        solution = unravel(y.reshape(*batch, size), struct, batch)
+        result = st.result().clone().reshape(batch)
+        stats = {"num_steps": st.step().clone().reshape(batch), "max_steps": self.max_steps}
+        return solution, result, stats
+
+    def transpose(self, state, options):
+        transpose_options = {}
+        if "preconditioner" in options:
+            transpose_options["preconditioner"] = options["preconditioner"].transpose()
+        psd_op, is_nsd = state
+        return (psd_op.transpose(), is_nsd), transpose_options
+
+    def conj(self, state, options):
+        conj_options = {}
+        if "preconditioner" in options:
+            conj_options["preconditioner"] = conj(options["preconditioner"])
+        psd_op, is_nsd = state
+        return (conj(psd_op), is_nsd), conj_options
 
     def assume_full_rank(self):
         return True

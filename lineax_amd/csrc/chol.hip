@@ -6,8 +6,8 @@
 //  * n <= 256 (float) / 181 (double): one 256-thread workgroup per system
 //    with the upper triangle resident in LDS in row-packed form (131.6 KB
 //    at n = 256); right-looking factorisation, triangular solves by one
-//    wave with the vector in registers.  The same device code drives the
-//    uneven-batch launch, which adds FP64 iterative refinement.
+//    wave with the vector in registers.  (Uneven batches have their own
+//    register-blocked kernel in chol_uneven.hip.)
 //  * larger n: one workgroup per system in global memory (correctness path).
 //
 // Failure semantics follow jax.scipy.linalg.cho_factor as lineax uses it
@@ -145,7 +145,6 @@ __global__ __launch_bounds__(64) void chol_solve_small_kernel(int64_t batch, int
 // Tier 2: one 256-thread workgroup per system, upper triangle in LDS.
 // ---------------------------------------------------------------------------
 constexpr int kWG = 256;
-constexpr int kLdsMaxN = 256;
 
 __device__ __forceinline__ float bcast_lane(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
@@ -372,246 +371,6 @@ __global__ __launch_bounds__(kWG) void chol_solve_generic_kernel(int n, const T*
   }
 }
 
-// ---------------------------------------------------------------------------
-// Uneven batch: fused load / factor / solve / FP64 refinement per system.
-// ---------------------------------------------------------------------------
-
-// LDS carve-up for one system of order <= max_n.
-struct UnevenLds {
-  double* x;        // [max_n] solution in FP64
-  double* r;        // [max_n] residual
-  double* rowpart;  // [max_n] (T x)_j part of the residual
-  double* rowabs;   // [max_n] row abs sums of the upper triangle
-  double* acc;      // [4][max_n] per-wave partial (T^T x)_i
-  double* red;      // [8] block reduction scratch
-  float* L;         // packed upper triangle
-  int* item;        // work-queue slot
-};
-
-__device__ UnevenLds carve_uneven(unsigned char* base, int max_n, bool refine) {
-  UnevenLds s;
-  double* d = reinterpret_cast<double*>(base);
-  s.x = d;
-  d += max_n;
-  if (refine) {
-    s.r = d;
-    d += max_n;
-    s.rowpart = d;
-    d += max_n;
-    s.rowabs = d;
-    d += max_n;
-    s.acc = d;
-    d += 4 * max_n;
-  } else {
-    s.r = s.rowpart = s.rowabs = s.acc = nullptr;
-  }
-  s.red = d;
-  d += 8;
-  s.item = reinterpret_cast<int*>(d);
-  s.L = reinterpret_cast<float*>(s.item + 4);
-  return s;
-}
-
-__host__ __device__ inline size_t uneven_lds_bytes(int max_n, bool refine) {
-  const size_t doubles = static_cast<size_t>(max_n) * (refine ? 8 : 1) + 8;
-  const size_t packed = static_cast<size_t>(max_n) * (max_n + 1) / 2;
-  return doubles * sizeof(double) + 4 * sizeof(int) + packed * sizeof(float);
-}
-
-__device__ double block_max(double v, double* red) {
-  v = wave_max(v);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[w] = v;
-  __syncthreads();
-  const double out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-  __syncthreads();
-  return out;
-}
-
-// r = b - S x with S the symmetric matrix whose upper triangle is A's;
-// optionally rowabs[j] = sum_i |S_ji|.  Each upper-triangle element is read
-// once: row j contributes sum_{i>=j} A_ji x_i to r_j (wave reduction) and
-// A_ji x_j to r_i for i > j (lane accumulators).
-__device__ void residual_sym_upper(int n, const float* __restrict__ A, const float* __restrict__ b,
-                                   const UnevenLds& s, bool want_abs) {
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  double aab[4] = {0.0, 0.0, 0.0, 0.0};
-  double xi[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) xi[c] = (l + 64 * c < n) ? s.x[l + 64 * c] : 0.0;
-  for (int j0 = w; j0 < n; j0 += 16) {
-    // Four rows per wave in flight: rows j0, j0+4, j0+8, j0+12.
-    float av[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int j = j0 + 4 * u;
-      const float* Aj = A + static_cast<int64_t>(j) * n;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int i = l + 64 * c;
-        av[u][c] = (j < n && i >= j && i < n) ? Aj[i] : 0.0f;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int j = j0 + 4 * u;
-      if (j >= n) break;
-      const double xj = s.x[j];
-      double dot = 0.0, rab = 0.0;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int i = l + 64 * c;
-        const double aji = static_cast<double>(av[u][c]);  // zero outside the upper triangle
-        dot = fma(aji, xi[c], dot);
-        rab += fabs(aji);
-        if (i > j) {
-          acc[c] = fma(aji, xj, acc[c]);
-          aab[c] += fabs(aji);
-        }
-      }
-      dot = wave_sum(dot);
-      if (want_abs) rab = wave_sum(rab);
-      if (l == 0) {
-        s.rowpart[j] = dot;
-        if (want_abs) s.rowabs[j] = rab;
-      }
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int i = l + 64 * c;
-    if (i < n) s.acc[w * n + i] = acc[c];
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += kWG) {
-    const double tx = s.rowpart[i] + ((s.acc[i] + s.acc[n + i]) + (s.acc[2 * n + i] + s.acc[3 * n + i]));
-    s.r[i] = static_cast<double>(b[i]) - tx;
-  }
-  if (want_abs) {
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int i = l + 64 * c;
-      if (i < n) s.acc[w * n + i] = aab[c];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += kWG)
-      s.rowabs[i] += (s.acc[i] + s.acc[n + i]) + (s.acc[2 * n + i] + s.acc[3 * n + i]);
-  }
-  __syncthreads();
-}
-
-template <bool kRefine, typename TX>
-__global__ __launch_bounds__(kWG) void chol_uneven_kernel(int64_t count, int max_n, const int32_t* __restrict__ ns,
-                                                          const int64_t* __restrict__ a_off,
-                                                          const float* __restrict__ a,
-                                                          const int64_t* __restrict__ b_off,
-                                                          const float* __restrict__ b, TX* __restrict__ x,
-                                                          const int32_t* __restrict__ order, int persistent,
-                                                          unsigned int* __restrict__ queue, int max_refine,
-                                                          double tol, int32_t* __restrict__ info,
-                                                          int32_t* __restrict__ iters, double* __restrict__ eta) {
-  extern __shared__ unsigned char lxa_smem[];
-  const UnevenLds s = carve_uneven(lxa_smem, max_n, kRefine);
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  int64_t next = blockIdx.x;
-  for (;;) {
-    if (persistent) {
-      if (tid == 0) s.item[0] = static_cast<int>(atomicAdd(queue, 1u));
-      __syncthreads();
-      next = s.item[0];
-      __syncthreads();
-    }
-    if (next >= count) break;
-    const int64_t sys = order[next];
-    if (!persistent) next = count;  // one system per workgroup
-    const int n = ns[sys];
-    if (n < 1 || n > max_n) {  // order outside the LDS sizing: reject, touch nothing
-      if (tid == 0) {
-        info[sys] = -1;
-        if (iters) iters[sys] = 0;
-        if (eta) eta[sys] = -1.0;
-      }
-      __syncthreads();
-      continue;
-    }
-    const float* A = a + a_off[sys];
-    const float* B = b + b_off[sys];
-    TX* X = x + b_off[sys];
-
-    lds_load_upper<float, float>(n, A, n, s.L, false);
-    __syncthreads();
-    const int fail = lds_chol_factor<float>(n, s.L);
-    __syncthreads();
-    if (fail) {
-      for (int i = tid; i < n; i += kWG) X[i] = static_cast<TX>(quiet_nan<double>());
-      if (tid == 0) {
-        info[sys] = fail;
-        if (iters) iters[sys] = 0;
-        if (eta) eta[sys] = -1.0;
-      }
-      __syncthreads();
-      continue;
-    }
-    if (w == 0) {
-      float v[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) v[c] = (l + 64 * c < n) ? B[l + 64 * c] : 0.0f;
-      wave_chol_solve<float>(n, s.L, v);
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (l + 64 * c < n) s.x[l + 64 * c] = static_cast<double>(v[c]);
-    }
-    __syncthreads();
-
-    int steps = 0;
-    double last_eta = -1.0;
-    if constexpr (kRefine) {
-      double bnorm = 0.0;
-      for (int i = tid; i < n; i += kWG) bnorm = fmax(bnorm, fabs(static_cast<double>(B[i])));
-      bnorm = block_max(bnorm, s.red);
-      double anorm = 0.0;
-      for (int it = 0; it < max_refine; ++it) {
-        residual_sym_upper(n, A, B, s, it == 0);
-        double rmax = 0.0, xmax = 0.0, amax = 0.0;
-        for (int i = tid; i < n; i += kWG) {
-          rmax = fmax(rmax, fabs(s.r[i]));
-          xmax = fmax(xmax, fabs(s.x[i]));
-          if (it == 0) amax = fmax(amax, s.rowabs[i]);
-        }
-        rmax = block_max(rmax, s.red);
-        xmax = block_max(xmax, s.red);
-        if (it == 0) anorm = block_max(amax, s.red);
-        const double denom = anorm * xmax + bnorm;
-        last_eta = denom > 0.0 ? rmax / denom : (rmax > 0.0 ? INFINITY : 0.0);
-        ++steps;
-        if (!(last_eta > tol)) break;  // also stops on NaN
-        if (w == 0) {
-          float v[4];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) v[c] = (l + 64 * c < n) ? static_cast<float>(s.r[l + 64 * c]) : 0.0f;
-          wave_chol_solve<float>(n, s.L, v);
-#pragma unroll
-          for (int c = 0; c < 4; ++c)
-            if (l + 64 * c < n) s.x[l + 64 * c] += static_cast<double>(v[c]);
-        }
-        __syncthreads();
-      }
-    }
-    for (int i = tid; i < n; i += kWG) X[i] = static_cast<TX>(s.x[i]);
-    if (tid == 0) {
-      info[sys] = 0;
-      if (iters) iters[sys] = steps;
-      if (eta) eta[sys] = last_eta;
-    }
-    __syncthreads();
-  }
-}
-
-__global__ void zero_u32_kernel(unsigned int* p) { *p = 0u; }
-
 // Kernels using more than 64 KiB of dynamic LDS must say so before launch.
 template <typename K>
 int allow_lds(K kernel, size_t bytes) {
@@ -722,61 +481,4 @@ extern "C" int lxa_chol_solve(void* stream, int dtype, int64_t batch, int64_t n,
     return lxa::chol_solve_t<double>(s, batch, n, static_cast<const double*>(u), u_batch_stride,
                                      static_cast<double*>(b), b_batch_stride, negate);
   return lxa::set_error(LXA_EINVAL, "lxa_chol_solve: bad dtype %d", dtype);
-}
-
-extern "C" int lxa_chol_uneven_workspace_size(int64_t count, size_t* bytes) {
-  LXA_REQUIRE(count >= 0 && bytes, "lxa_chol_uneven_workspace_size: bad argument");
-  *bytes = 256;  // work-queue counter
-  return LXA_OK;
-}
-
-extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, const int32_t* n,
-                                     const int64_t* a_offset, const float* a, const int64_t* b_offset,
-                                     const float* b, void* x, int x_dtype, const int32_t* order, int flags,
-                                     int max_refine, double tol, int32_t* info, int32_t* iters, double* eta,
-                                     void* workspace, size_t workspace_bytes) {
-  LXA_REQUIRE(count >= 0 && count <= lxa::kMaxGrid, "lxa_chol_uneven_solve: bad count");
-  LXA_REQUIRE(max_n >= 1 && max_n <= lxa::kLdsMaxN, "lxa_chol_uneven_solve: max_n must be in [1, %d]",
-              lxa::kLdsMaxN);
-  LXA_REQUIRE(n && a_offset && a && b_offset && b && x && order && info, "lxa_chol_uneven_solve: null pointer");
-  LXA_REQUIRE(x_dtype == LXA_F32 || x_dtype == LXA_F64, "lxa_chol_uneven_solve: bad x_dtype");
-  LXA_REQUIRE(max_refine >= 0 && max_refine <= 64, "lxa_chol_uneven_solve: max_refine in [0, 64]");
-  LXA_REQUIRE(max_refine == 0 || x_dtype == LXA_F64, "lxa_chol_uneven_solve: refinement needs x_dtype F64");
-  const bool persistent = (flags & LXA_UNEVEN_PERSISTENT) != 0;
-  LXA_REQUIRE(!persistent || (workspace && workspace_bytes >= 4), "lxa_chol_uneven_solve: workspace required");
-  if (count == 0) return LXA_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const bool refine = max_refine > 0;
-  const size_t shmem = lxa::uneven_lds_bytes(max_n, refine);
-  LXA_REQUIRE(shmem <= 160 * 1024, "lxa_chol_uneven_solve: LDS need %zu exceeds 160 KiB", shmem);
-  unsigned int* queue = static_cast<unsigned int*>(workspace);
-  int64_t grid = count;
-  if (persistent) {
-    hipLaunchKernelGGL(lxa::zero_u32_kernel, dim3(1), dim3(1), 0, s, queue);
-    LXA_LAUNCH_CHECK("lxa_chol_uneven_solve queue reset");
-    const int per_cu = static_cast<int>((160 * 1024) / shmem);
-    grid = static_cast<int64_t>(lxa::num_cus()) * (per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu));
-    if (grid > count) grid = count;
-  }
-  const int pers = persistent ? 1 : 0;
-  int rc = LXA_OK;
-  if (refine) rc = lxa::allow_lds(lxa::chol_uneven_kernel<true, double>, shmem);
-  else if (x_dtype == LXA_F64) rc = lxa::allow_lds(lxa::chol_uneven_kernel<false, double>, shmem);
-  else rc = lxa::allow_lds(lxa::chol_uneven_kernel<false, float>, shmem);
-  if (rc) return rc;
-  if (refine) {
-    hipLaunchKernelGGL((lxa::chol_uneven_kernel<true, double>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kWG),
-                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<double*>(x), order, pers,
-                       queue, max_refine, tol, info, iters, eta);
-  } else if (x_dtype == LXA_F64) {
-    hipLaunchKernelGGL((lxa::chol_uneven_kernel<false, double>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kWG),
-                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<double*>(x), order, pers,
-                       queue, 0, tol, info, iters, eta);
-  } else {
-    hipLaunchKernelGGL((lxa::chol_uneven_kernel<false, float>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kWG),
-                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<float*>(x), order, pers,
-                       queue, 0, tol, info, iters, eta);
-  }
-  LXA_LAUNCH_CHECK("lxa_chol_uneven_solve launch");
-  return LXA_OK;
 }

@@ -1,0 +1,507 @@
+// Uneven batch of Cholesky solves (orders 1..256) in one launch.
+//
+// One 256-thread workgroup (a 16 x 16 thread grid, four waves) owns one
+// system at a time.  The trailing matrix lives in registers, distributed
+// cyclically: thread (ty, tx) holds A(ty + 16a, tx + 16b) for a <= b, i.e.
+// the upper triangle in 136 registers.  The factorisation runs in 16-row
+// panels:
+//   (a) the owners copy panel rows K into an LDS triangle T (row-packed,
+//       order N16 = n rounded up to 16; padding rows are the identity);
+//   (b) wave 0 factors the 16x16 diagonal block in registers (lane q
+//       holds column q; broadcasts by readlane) and stores U_KK in T; then
+//       every wave solves U_KK^T U_K,J = A_K,J for its own 64-column chunk
+//       of the panel (U_KK read by LDS broadcast) and writes the result to
+//       T and to a transposed, bank-conflict-free copy PT[j][p];
+//   (c) all threads apply the rank-16 update to their register slots below
+//       the panel, reading U values from PT two at a time.
+// Three barriers per panel.  T ends up holding U, which one wave uses for the
+// triangular solves (vector in registers, reciprocal diagonal in LDS).
+// With refinement, residuals r = b - A x are formed in FP64 from A's upper
+// triangle in global memory, and x (FP64, LDS) is corrected with FP32
+// solves until the normwise backward error meets the tolerance.
+//
+// Work arrives either from a device queue (persistent launch: workgroups
+// pull the next system; callers order the queue largest first) or one
+// system per workgroup.
+#include "lxa_common.h"
+
+namespace lxa {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kMaxN = 256;
+constexpr int kPTStride = 18;  // 16 panel values + 2 pad: b64 reads hit distinct banks
+constexpr size_t kUnionBytes = 256 * kPTStride * sizeof(float);  // >= (8 * 256 + 8) doubles
+constexpr size_t kRinvBytes = 256 * sizeof(float);
+constexpr size_t kItemBytes = 16;
+
+__host__ __device__ inline int round16(int n) { return (n + 15) & ~15; }
+
+__host__ __device__ inline size_t uneven_lds_bytes(int max_n) {
+  const size_t n16 = static_cast<size_t>(round16(max_n));
+  return kUnionBytes + kRinvBytes + kItemBytes + n16 * (n16 + 1) / 2 * sizeof(float);
+}
+
+__device__ __forceinline__ float bcast(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ int pku(int n, int i, int j) { return i * n - ((i * (i - 1)) >> 1) + (j - i); }
+
+struct Lds {
+  float* PT;         // [256][18] transposed panel (factorisation phase)
+  double* x;         // [256] FP64 solution      } refinement phase,
+  double* r;         // [256] residual           } aliasing PT
+  double* rowpart;   // [256]                    }
+  double* rowabs;    // [256]                    }
+  double* acc;       // [4][256]                 }
+  double* red;       // [8]                      }
+  float* rinv;       // [256] 1 / U_kk
+  int* item;         // work-queue slot
+  float* T;          // packed U, order N16
+};
+
+__device__ Lds carve(unsigned char* base) {
+  Lds s;
+  s.PT = reinterpret_cast<float*>(base);
+  double* d = reinterpret_cast<double*>(base);
+  s.x = d;
+  s.r = d + 256;
+  s.rowpart = d + 512;
+  s.rowabs = d + 768;
+  s.acc = d + 1024;
+  s.red = d + 2048;
+  s.rinv = reinterpret_cast<float*>(base + kUnionBytes);
+  s.item = reinterpret_cast<int*>(base + kUnionBytes + kRinvBytes);
+  s.T = reinterpret_cast<float*>(base + kUnionBytes + kRinvBytes + kItemBytes);
+  return s;
+}
+
+// Factorises the system held in R (see the file comment).  Returns 0 or the
+// 1-based index of the first non-positive pivot; uniform over the block.
+__device__ int factor_registers(float (&R)[16][16], int n, const Lds& s) {
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4, w = tid >> 6, l = tid & 63;
+  const int N16 = round16(n);
+  const int nk = N16 >> 4;
+  float* T = s.T;
+  float* PT = s.PT;
+  for (int K = 0; K < nk; ++K) {
+    const int k0 = 16 * K;
+    // (a) panel rows to T
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+      if (a == K) {
+#pragma unroll
+        for (int b = a; b < 16; ++b) {
+          if (b < nk) {
+            const int i = ty + 16 * a, j = tx + 16 * b;
+            if (j >= i) T[pku(N16, i, j)] = R[a][b];
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    // (b1) wave 0 factors the diagonal block in registers, lane q holding
+    // column q (broadcasts by readlane), and stores U_KK and 1/diag to LDS.
+    if (w == 0) {
+      const int q = l & 15;
+      float c[16];
+#pragma unroll
+      for (int p = 0; p < 16; ++p) c[p] = (p <= q) ? T[pku(N16, k0 + p, k0 + q)] : 0.0f;
+      int dfail = 0;
+#pragma unroll
+      for (int st = 0; st < 16; ++st) {
+        const float dss = bcast(c[st], st);
+        if (!(dss > 0.0f) && dfail == 0) dfail = st + 1;
+        const float d = sqrtf(dss);
+        c[st] = (q == st) ? d : (q > st ? c[st] / d : c[st]);
+#pragma unroll
+        for (int p = st + 1; p < 16; ++p) {
+          const float usp = bcast(c[st], p);
+          c[p] = (q >= p) ? c[p] - usp * c[st] : c[p];
+        }
+      }
+      if (l < 16) {
+#pragma unroll
+        for (int p = 0; p < 16; ++p)
+          if (p <= q) T[pku(N16, k0 + p, k0 + q)] = c[p];
+        s.rinv[k0 + q] = 1.0f / c[q];
+      }
+      if (l == 0) s.item[1] = dfail ? k0 + dfail : 0;  // padding rows never fail
+    }
+    __syncthreads();
+    if (s.item[1]) return s.item[1];
+
+    // (b2) block row for this wave's 64 columns: U_KK^T u = a, with U_KK
+    // read by broadcast from LDS.  Lanes without a column of their own work
+    // on the last column, which always exists, and do not store.
+    {
+      const int j = 64 * w + l;
+      const bool jvalid = j >= k0 + 16 && j < N16;
+      const int jc = jvalid ? j : N16 - 1;
+      float u[16];
+#pragma unroll
+      for (int p = 0; p < 16; ++p) u[p] = T[pku(N16, k0 + p, jc)];
+#pragma unroll
+      for (int p = 0; p < 16; ++p) {
+#pragma unroll
+        for (int st = 0; st < p; ++st) u[p] -= T[pku(N16, k0 + st, k0 + p)] * u[st];
+        u[p] = u[p] / T[pku(N16, k0 + p, k0 + p)];
+      }
+      if (jvalid) {
+#pragma unroll
+        for (int p = 0; p < 16; ++p) T[pku(N16, k0 + p, j)] = u[p];
+        float2* dst = reinterpret_cast<float2*>(PT + j * kPTStride);
+#pragma unroll
+        for (int p = 0; p < 16; p += 2) dst[p >> 1] = make_float2(u[p], u[p + 1]);
+      }
+    }
+    __syncthreads();
+
+    // (c) rank-16 update of the register slots below the panel.  The p loop
+    // stays rolled: unrolled, the per-panel code would outgrow the
+    // instruction cache.
+#pragma unroll 1
+    for (int pc = 0; pc < 16; pc += 2) {
+      float2 uj[16];
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        uj[b] = make_float2(0.0f, 0.0f);
+        if (b > K && b < nk) uj[b] = *reinterpret_cast<const float2*>(PT + (tx + 16 * b) * kPTStride + pc);
+      }
+#pragma unroll
+      for (int a = 0; a < 16; ++a) {
+        if (a > K && a < nk) {
+          const float2 ui = *reinterpret_cast<const float2*>(PT + (ty + 16 * a) * kPTStride + pc);
+#pragma unroll
+          for (int b = a; b < 16; ++b) {
+            if (b < nk) {
+              R[a][b] -= ui.x * uj[b].x;
+              R[a][b] -= ui.y * uj[b].y;
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  return 0;
+}
+
+// U^T U x = v for v held by one wave (lane l: v[l + 64c]); uses k < n only.
+__device__ void wave_solve(int n, const Lds& s, float (&v)[4]) {
+  const int l = threadIdx.x & 63;
+  const int N16 = round16(n);
+  const float* T = s.T;
+#pragma unroll
+  for (int c0 = 0; c0 < 4; ++c0) {
+    const int kend = min(64, n - 64 * c0);
+    for (int kk = 0; kk < kend; ++kk) {
+      const int k = 64 * c0 + kk;
+      const int rk = pku(N16, k, k);
+      const float yk = bcast(v[c0], kk) * s.rinv[k];
+      if (l == kk) v[c0] = yk;
+#pragma unroll
+      for (int c = c0; c < 4; ++c) {
+        const int jj = l + 64 * c;
+        if (jj > k && jj < n) v[c] -= T[rk + (jj - k)] * yk;
+      }
+    }
+  }
+#pragma unroll
+  for (int c0 = 3; c0 >= 0; --c0) {
+    const int kend = min(64, n - 64 * c0);
+    for (int kk = kend - 1; kk >= 0; --kk) {
+      const int k = 64 * c0 + kk;
+      const float xk = bcast(v[c0], kk) * s.rinv[k];
+      if (l == kk) v[c0] = xk;
+#pragma unroll
+      for (int c = 0; c <= c0; ++c) {
+        const int i = l + 64 * c;
+        if (i < k) v[c] -= T[pku(N16, i, k)] * xk;
+      }
+    }
+  }
+}
+
+__device__ double block_max(double v, double* red) {
+  v = wave_max(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  const double out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  __syncthreads();
+  return out;
+}
+
+// r = b - S x, S the symmetric matrix of A's upper triangle; optionally the
+// row sums of |S|.  Every upper-triangle element is read once from global
+// memory: row j gives sum_{i>=j} A_ji x_i to r_j (wave reduction) and
+// A_ji x_j to r_i for i > j (lane accumulators).
+__device__ void residual_sym_upper(int n, const float* __restrict__ A, const float* __restrict__ b, const Lds& s,
+                                   bool want_abs) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double aab[4] = {0.0, 0.0, 0.0, 0.0};
+  double xi[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) xi[c] = (l + 64 * c < n) ? s.x[l + 64 * c] : 0.0;
+  for (int j0 = w; j0 < n; j0 += 32) {
+    float av[8][4];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + 4 * u;
+      const float* Aj = A + static_cast<int64_t>(j) * n;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = l + 64 * c;
+        av[u][c] = (j < n && i >= j && i < n) ? Aj[i] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + 4 * u;
+      if (j >= n) break;
+      const double xj = s.x[j];
+      double dot = 0.0, rab = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = l + 64 * c;
+        const double aji = static_cast<double>(av[u][c]);
+        dot = fma(aji, xi[c], dot);
+        rab += fabs(aji);
+        if (i > j) {
+          acc[c] = fma(aji, xj, acc[c]);
+          aab[c] += fabs(aji);
+        }
+      }
+      dot = wave_sum(dot);
+      if (want_abs) rab = wave_sum(rab);
+      if (l == 0) {
+        s.rowpart[j] = dot;
+        if (want_abs) s.rowabs[j] = rab;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int i = l + 64 * c;
+    if (i < n) s.acc[w * 256 + i] = acc[c];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kT) {
+    const double tx = s.rowpart[i] + ((s.acc[i] + s.acc[256 + i]) + (s.acc[512 + i] + s.acc[768 + i]));
+    s.r[i] = static_cast<double>(b[i]) - tx;
+  }
+  if (want_abs) {
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = l + 64 * c;
+      if (i < n) s.acc[w * 256 + i] = aab[c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kT)
+      s.rowabs[i] += (s.acc[i] + s.acc[256 + i]) + (s.acc[512 + i] + s.acc[768 + i]);
+  }
+  __syncthreads();
+}
+
+template <bool kRefine, typename TX>
+__global__ __launch_bounds__(kT) void chol_uneven_kernel(int64_t count, int max_n, const int32_t* __restrict__ ns,
+                                                         const int64_t* __restrict__ a_off,
+                                                         const float* __restrict__ a,
+                                                         const int64_t* __restrict__ b_off,
+                                                         const float* __restrict__ b, TX* __restrict__ x,
+                                                         const int32_t* __restrict__ order, int persistent,
+                                                         unsigned int* __restrict__ queue, int max_refine, double tol,
+                                                         int32_t* __restrict__ info, int32_t* __restrict__ iters,
+                                                         double* __restrict__ eta) {
+  extern __shared__ __align__(16) unsigned char lxa_smem[];
+  const Lds s = carve(lxa_smem);
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int tx = tid & 15, ty = tid >> 4;
+  int64_t next = blockIdx.x;
+  for (;;) {
+    if (persistent) {
+      if (tid == 0) s.item[0] = static_cast<int>(atomicAdd(queue, 1u));
+      __syncthreads();
+      next = s.item[0];
+      __syncthreads();
+    }
+    if (next >= count) break;
+    const int64_t sys = order[next];
+    if (!persistent) next = count;
+    const int n = ns[sys];
+    if (n < 1 || n > max_n) {
+      if (tid == 0) {
+        info[sys] = -1;
+        if (iters) iters[sys] = 0;
+        if (eta) eta[sys] = -1.0;
+      }
+      __syncthreads();
+      continue;
+    }
+    const float* A = a + a_off[sys];
+    const float* B = b + b_off[sys];
+    TX* X = x + b_off[sys];
+    const int nk = round16(n) >> 4;
+
+    float R[16][16];
+#pragma unroll
+    for (int aa = 0; aa < 16; ++aa) {
+#pragma unroll
+      for (int bb = aa; bb < 16; ++bb) {
+        R[aa][bb] = 0.0f;
+        if (aa < nk && bb < nk) {
+          const int i = ty + 16 * aa, j = tx + 16 * bb;
+          if (i < n && j < n) {
+            if (j >= i) R[aa][bb] = A[static_cast<int64_t>(i) * n + j];
+          } else if (i == j) {
+            R[aa][bb] = 1.0f;
+          }
+        }
+      }
+    }
+
+    const int fail = factor_registers(R, n, s);
+    if (fail) {
+      __syncthreads();
+      for (int i = tid; i < n; i += kT) X[i] = static_cast<TX>(quiet_nan<double>());
+      if (tid == 0) {
+        info[sys] = fail;
+        if (iters) iters[sys] = 0;
+        if (eta) eta[sys] = -1.0;
+      }
+      __syncthreads();
+      continue;
+    }
+    if (w == 0) {
+      float v[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] = (l + 64 * c < n) ? B[l + 64 * c] : 0.0f;
+      wave_solve(n, s, v);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (l + 64 * c < n) s.x[l + 64 * c] = static_cast<double>(v[c]);
+    }
+    __syncthreads();
+
+    int steps = 0;
+    double last_eta = -1.0;
+    if constexpr (kRefine) {
+      double bnorm = 0.0;
+      for (int i = tid; i < n; i += kT) bnorm = fmax(bnorm, fabs(static_cast<double>(B[i])));
+      bnorm = block_max(bnorm, s.red);
+      double anorm = 0.0;
+      for (int it = 0; it < max_refine; ++it) {
+        residual_sym_upper(n, A, B, s, it == 0);
+        double rmax = 0.0, xmax = 0.0, amax = 0.0;
+        for (int i = tid; i < n; i += kT) {
+          rmax = fmax(rmax, fabs(s.r[i]));
+          xmax = fmax(xmax, fabs(s.x[i]));
+          if (it == 0) amax = fmax(amax, s.rowabs[i]);
+        }
+        rmax = block_max(rmax, s.red);
+        xmax = block_max(xmax, s.red);
+        if (it == 0) anorm = block_max(amax, s.red);
+        const double denom = anorm * xmax + bnorm;
+        last_eta = denom > 0.0 ? rmax / denom : (rmax > 0.0 ? INFINITY : 0.0);
+        ++steps;
+        if (!(last_eta > tol)) break;  // also stops on NaN
+        if (w == 0) {
+          float v[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) v[c] = (l + 64 * c < n) ? static_cast<float>(s.r[l + 64 * c]) : 0.0f;
+          wave_solve(n, s, v);
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            if (l + 64 * c < n) s.x[l + 64 * c] += static_cast<double>(v[c]);
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = tid; i < n; i += kT) X[i] = static_cast<TX>(s.x[i]);
+    if (tid == 0) {
+      info[sys] = 0;
+      if (iters) iters[sys] = steps;
+      if (eta) eta[sys] = last_eta;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void zero_u32_kernel(unsigned int* p) { *p = 0u; }
+
+template <typename K>
+int allow_lds(K kernel, size_t bytes) {
+  if (bytes <= 64 * 1024) return LXA_OK;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+  return e == hipSuccess ? LXA_OK : hip_error(e, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
+}
+
+constexpr int64_t kMaxGrid = 2147483647;
+
+}  // namespace
+}  // namespace lxa
+
+extern "C" int lxa_chol_uneven_workspace_size(int64_t count, size_t* bytes) {
+  LXA_REQUIRE(count >= 0 && bytes, "lxa_chol_uneven_workspace_size: bad argument");
+  *bytes = 256;  // work-queue counter
+  return LXA_OK;
+}
+
+extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, const int32_t* n,
+                                     const int64_t* a_offset, const float* a, const int64_t* b_offset,
+                                     const float* b, void* x, int x_dtype, const int32_t* order, int flags,
+                                     int max_refine, double tol, int32_t* info, int32_t* iters, double* eta,
+                                     void* workspace, size_t workspace_bytes) {
+  LXA_REQUIRE(count >= 0 && count <= lxa::kMaxGrid, "lxa_chol_uneven_solve: bad count");
+  LXA_REQUIRE(max_n >= 1 && max_n <= lxa::kMaxN, "lxa_chol_uneven_solve: max_n must be in [1, %d]", lxa::kMaxN);
+  LXA_REQUIRE(n && a_offset && a && b_offset && b && x && order && info, "lxa_chol_uneven_solve: null pointer");
+  LXA_REQUIRE(x_dtype == LXA_F32 || x_dtype == LXA_F64, "lxa_chol_uneven_solve: bad x_dtype");
+  LXA_REQUIRE(max_refine >= 0 && max_refine <= 64, "lxa_chol_uneven_solve: max_refine in [0, 64]");
+  LXA_REQUIRE(max_refine == 0 || x_dtype == LXA_F64, "lxa_chol_uneven_solve: refinement needs x_dtype F64");
+  const bool persistent = (flags & LXA_UNEVEN_PERSISTENT) != 0;
+  LXA_REQUIRE(!persistent || (workspace && workspace_bytes >= 4), "lxa_chol_uneven_solve: workspace required");
+  if (count == 0) return LXA_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool refine = max_refine > 0;
+  const size_t shmem = lxa::uneven_lds_bytes(max_n);
+  LXA_REQUIRE(shmem <= 160 * 1024, "lxa_chol_uneven_solve: LDS need %zu exceeds 160 KiB", shmem);
+  unsigned int* queue = static_cast<unsigned int*>(workspace);
+  int64_t grid = count;
+  if (persistent) {
+    hipLaunchKernelGGL(lxa::zero_u32_kernel, dim3(1), dim3(1), 0, s, queue);
+    LXA_LAUNCH_CHECK("lxa_chol_uneven_solve queue reset");
+    int per_cu = static_cast<int>((160 * 1024) / shmem);
+    per_cu = per_cu < 1 ? 1 : (per_cu > 2 ? 2 : per_cu);  // register file: at most two 256-thread groups
+    grid = static_cast<int64_t>(lxa::num_cus()) * per_cu;
+    if (grid > count) grid = count;
+  }
+  const int pers = persistent ? 1 : 0;
+  int rc = LXA_OK;
+  if (refine) rc = lxa::allow_lds(lxa::chol_uneven_kernel<true, double>, shmem);
+  else if (x_dtype == LXA_F64) rc = lxa::allow_lds(lxa::chol_uneven_kernel<false, double>, shmem);
+  else rc = lxa::allow_lds(lxa::chol_uneven_kernel<false, float>, shmem);
+  if (rc) return rc;
+  if (refine) {
+    hipLaunchKernelGGL((lxa::chol_uneven_kernel<true, double>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kT),
+                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<double*>(x), order, pers,
+                       queue, max_refine, tol, info, iters, eta);
+  } else if (x_dtype == LXA_F64) {
+    hipLaunchKernelGGL((lxa::chol_uneven_kernel<false, double>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kT),
+                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<double*>(x), order, pers,
+                       queue, 0, tol, info, iters, eta);
+  } else {
+    hipLaunchKernelGGL((lxa::chol_uneven_kernel<false, float>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kT),
+                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<float*>(x), order, pers,
+                       queue, 0, tol, info, iters, eta);
+  }
+  LXA_LAUNCH_CHECK("lxa_chol_uneven_solve launch");
+  return LXA_OK;
+}

@@ -1,0 +1,1052 @@
+// Building blocks of lineax's CG and GMRES on the device.
+//
+// A batch of right-hand sides is a set of rows [rows, n].  Each row carries
+// its own scalar state (lxa_krylov_row, include/lineax_amd.h) so rows
+// converge, break down or stagnate independently, exactly as lineax under
+// jax.vmap (finished rows are frozen).  Every reduction accumulates in
+// FP64 in two deterministic phases: per-block partials, then a per-row
+// finalize in a fixed order.  NaN propagates through the max-norms as it
+// does through jnp.max, so a NaN residual ends a solve the way it ends
+// lineax's while loop.
+//
+// Vector storage: TY for the iterate y and right-hand side b, TV for the
+// working vectors (CG: r, p, q, z; GMRES: basis, w, r).  <double, double>
+// and <float, float> follow lineax in that dtype; <double, float> keeps y
+// and b in FP64 with FP32 working vectors.
+#include "lxa_common.h"
+
+namespace lxa {
+namespace {
+
+constexpr int kB = 256;
+constexpr int kMaxRestart = 128;
+
+__device__ __forceinline__ double nanmax(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a > b ? a : b;
+}
+
+__device__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  const double out = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return out;
+}
+
+__device__ double wave_nanmax(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ double block_nanmax(double v, double* red) {
+  v = wave_nanmax(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  const double out = nanmax(nanmax(red[0], red[1]), nanmax(red[2], red[3]));
+  __syncthreads();
+  return out;
+}
+
+// Norm of a scaled vector: kind 0 max |.|, 1 two-norm, 2 rms.  Per-element
+// contribution and how partials combine.
+__device__ __forceinline__ double norm_elem(int kind, double v) { return kind == 0 ? fabs(v) : v * v; }
+__device__ __forceinline__ double norm_block(int kind, double v, double* red) {
+  return kind == 0 ? block_nanmax(v, red) : block_sum(v, red);
+}
+__device__ __forceinline__ double norm_accum(int kind, double a, double b) { return kind == 0 ? nanmax(a, b) : a + b; }
+__device__ __forceinline__ double norm_final(int kind, double v, int64_t n) {
+  if (kind == 0) return v;
+  if (kind == 1) return sqrt(v);
+  return n > 0 ? sqrt(v / static_cast<double>(n)) : 0.0;
+}
+
+// Row-block geometry: blockIdx.y is the row, blockIdx.x the chunk.
+struct Chunk {
+  int64_t begin, end;
+};
+__device__ __forceinline__ Chunk chunk_of(int64_t n) {
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  Chunk c;
+  c.begin = static_cast<int64_t>(blockIdx.x) * per;
+  c.end = c.begin + per < n ? c.begin + per : n;
+  return c;
+}
+
+__device__ __forceinline__ bool row_active(const lxa_krylov_row* st) { return st[blockIdx.y].active != 0; }
+
+// ---------------------------------------------------------------------------
+// Generic reductions
+// ---------------------------------------------------------------------------
+
+// part[row * pstride + poff + blk] = sum x*y over the chunk.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kB) void dot_partial_kernel(int64_t n, const TA* __restrict__ x, const TB* __restrict__ y,
+                                                         const lxa_krylov_row* __restrict__ st, int gate,
+                                                         double* __restrict__ part, int64_t pstride, int64_t poff) {
+  __shared__ double red[4];
+  if (gate && !row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const TA* xr = x + row * n;
+  const TB* yr = y + row * n;
+  double acc = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB)
+    acc = fma(static_cast<double>(xr[i]), static_cast<double>(yr[i]), acc);
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[row * pstride + poff + blockIdx.x] = acc;
+}
+
+__device__ double sum_partials(const double* p, int nblk) {
+  double s = 0.0;
+  for (int k = 0; k < nblk; ++k) s += p[k];
+  return s;
+}
+
+__device__ double max_partials(const double* p, int nblk) {
+  double s = 0.0;
+  for (int k = 0; k < nblk; ++k) s = nanmax(s, p[k]);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// CG (lineax/_solver/cg.py:114-227)
+// ---------------------------------------------------------------------------
+
+struct CgParams {
+  double rtol, atol, rcond;
+  int64_t max_steps;
+  int has_scale, norm_kind, max_steps_given;
+};
+
+// gamma = <z, r>; active = gamma > 0 & max_steps > 0 (the first test of
+// cond_fun: diff = inf makes the tolerance test true).  cg.py:128-167.
+__global__ void cg_begin_kernel(int64_t rows, int nblk, const double* __restrict__ part, CgParams prm,
+                                lxa_krylov_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  lxa_krylov_row s = {};
+  s.gamma = sum_partials(part + row * nblk, nblk);
+  s.step = 0;
+  s.norm_diff = INFINITY;
+  s.active = (s.gamma > 0.0) && (prm.max_steps > 0);
+  s.result = 0;
+  st[row] = s;
+}
+
+// alpha = gamma / <Ap, p>, NaN when |<Ap, p>| <= 100 rcond |gamma| (cg.py:172-178).
+__global__ void cg_alpha_kernel(int64_t rows, int nblk, const double* __restrict__ part, CgParams prm,
+                                lxa_krylov_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  const double ip = sum_partials(part + row * nblk, nblk);
+  const double gamma = st[row].gamma;
+  double alpha = gamma / ip;
+  if (!(fabs(ip) > 100.0 * prm.rcond * fabs(gamma))) alpha = __builtin_nan("");
+  st[row].alpha = alpha;
+  st[row].ip = ip;
+}
+
+// y += alpha p; unless `stable`, r -= alpha q.  Partials: [0] |diff / y_scale|
+// norm contribution, and when fuse_gamma (identity preconditioner, not a
+// stable step) [1] sum r^2 and [2] |r / b_scale| contribution.
+template <typename TY, typename TV>
+__global__ __launch_bounds__(kB) void cg_update_kernel(int64_t n, const TV* __restrict__ p, const TV* __restrict__ q,
+                                                       TY* __restrict__ y, TV* __restrict__ r,
+                                                       const TY* __restrict__ b, CgParams prm, int stable,
+                                                       int fuse_gamma, const lxa_krylov_row* __restrict__ st,
+                                                       double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const double alpha = st[row].alpha;
+  const int kind = prm.norm_kind;
+  double n2 = 0.0, rr = 0.0, n1 = 0.0;
+  const int64_t o = row * n;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double d = alpha * static_cast<double>(p[o + i]);
+    const double yn = static_cast<double>(y[o + i]) + d;
+    y[o + i] = static_cast<TY>(yn);
+    if (prm.has_scale) n2 = norm_accum(kind, n2, norm_elem(kind, d / (prm.atol + prm.rtol * fabs(yn))));
+    if (!stable) {
+      const double rn = static_cast<double>(r[o + i]) - alpha * static_cast<double>(q[o + i]);
+      const TV rs = static_cast<TV>(rn);
+      r[o + i] = rs;
+      if (fuse_gamma) {
+        const double rv = static_cast<double>(rs);
+        rr = fma(rv, rv, rr);
+        if (prm.has_scale)
+          n1 = norm_accum(kind, n1,
+                          norm_elem(kind, rv / (prm.atol + prm.rtol * fabs(static_cast<double>(b[o + i])))));
+      }
+    }
+  }
+  n2 = norm_block(kind, n2, red);
+  if (fuse_gamma) {
+    rr = block_sum(rr, red);
+    n1 = norm_block(kind, n1, red);
+  }
+  if (threadIdx.x == 0) {
+    const int64_t base = (row * 3) * gridDim.x + blockIdx.x;
+    part[base] = n2;
+    if (fuse_gamma) {
+      part[base + gridDim.x] = rr;
+      part[base + 2 * gridDim.x] = n1;
+    }
+  }
+}
+
+// r = b - t with t = A y in the vector dtype TY (stable step, cg.py:187-188);
+// partials as cg_update [1],[2] when fuse_gamma.
+template <typename TY, typename TV>
+__global__ __launch_bounds__(kB) void cg_residual_kernel(int64_t n, const TY* __restrict__ b, const TY* __restrict__ t,
+                                                         TV* __restrict__ r, CgParams prm, int fuse_gamma,
+                                                         const lxa_krylov_row* __restrict__ st,
+                                                         double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const int kind = prm.norm_kind;
+  double rr = 0.0, n1 = 0.0;
+  const int64_t o = row * n;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double bv = static_cast<double>(b[o + i]);
+    const TV rs = static_cast<TV>(bv - static_cast<double>(t[o + i]));
+    r[o + i] = rs;
+    if (fuse_gamma) {
+      const double rv = static_cast<double>(rs);
+      rr = fma(rv, rv, rr);
+      if (prm.has_scale) n1 = norm_accum(kind, n1, norm_elem(kind, rv / (prm.atol + prm.rtol * fabs(bv))));
+    }
+  }
+  if (fuse_gamma) {
+    rr = block_sum(rr, red);
+    n1 = norm_block(kind, n1, red);
+    if (threadIdx.x == 0) {
+      const int64_t base = (row * 3) * gridDim.x + blockIdx.x;
+      part[base + gridDim.x] = rr;
+      part[base + 2 * gridDim.x] = n1;
+    }
+  }
+}
+
+// |r / b_scale| partials only (preconditioned solves), into part[1] and [2]:
+// [1] receives <z, r> from a separate dot, [2] the norm contribution.
+template <typename TY, typename TV>
+__global__ __launch_bounds__(kB) void cg_rnorm_kernel(int64_t n, const TY* __restrict__ b, const TV* __restrict__ r,
+                                                      CgParams prm, const lxa_krylov_row* __restrict__ st,
+                                                      double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const int kind = prm.norm_kind;
+  double n1 = 0.0;
+  const int64_t o = row * n;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB)
+    n1 = norm_accum(kind, n1,
+                    norm_elem(kind, static_cast<double>(r[o + i]) /
+                                        (prm.atol + prm.rtol * fabs(static_cast<double>(b[o + i])))));
+  n1 = norm_block(kind, n1, red);
+  if (threadIdx.x == 0) part[(row * 3 + 2) * gridDim.x + blockIdx.x] = n1;
+}
+
+// gamma' = <z, r>, beta = gamma'/gamma, step += 1, then cond_fun
+// (cg.py:162-167, 202-206).  part layout [row][3][nblk].
+__global__ void cg_finish_kernel(int64_t rows, int nblk, int64_t n, const double* __restrict__ part, CgParams prm,
+                                 lxa_krylov_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  const double* pr = part + row * 3 * nblk;
+  const int kind = prm.norm_kind;
+  double n2 = 0.0, rr = 0.0, n1 = 0.0;
+  for (int k = 0; k < nblk; ++k) {
+    n2 = norm_accum(kind, n2, pr[k]);
+    rr += pr[nblk + k];
+    n1 = norm_accum(kind, n1, pr[2 * nblk + k]);
+  }
+  lxa_krylov_row s = st[row];
+  const double gamma_prev = s.gamma;
+  s.gamma = rr;
+  s.beta = rr / gamma_prev;
+  s.step += 1;
+  s.norm_r = norm_final(kind, n1, n);
+  s.norm_diff = norm_final(kind, n2, n);
+  bool not_converged = true;
+  if (prm.has_scale) not_converged = (s.norm_r > 1.0) || (s.norm_diff > 1.0);
+  s.active = (s.gamma > 0.0) && (s.step < prm.max_steps) && not_converged;
+  st[row] = s;
+}
+
+// p = z + beta p (cg.py:206), rows that are still active.
+template <typename TV>
+__global__ __launch_bounds__(kB) void cg_direction_kernel(int64_t n, const TV* __restrict__ z, TV* __restrict__ p,
+                                                          const lxa_krylov_row* __restrict__ st) {
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const double beta = st[row].beta;
+  const int64_t o = row * n;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB)
+    p[o + i] = static_cast<TV>(static_cast<double>(z[o + i]) + beta * static_cast<double>(p[o + i]));
+}
+
+// RESULTS (cg.py:213-222): num_steps == max_steps -> singular (max_steps not
+// given) or max_steps_reached (given, with tolerances).
+__global__ void cg_result_kernel(int64_t rows, CgParams prm, lxa_krylov_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  int res = LXA_RESULT_SUCCESSFUL;
+  const bool capped = st[row].step == prm.max_steps;
+  if (!prm.max_steps_given) {
+    if (capped) res = LXA_RESULT_SINGULAR;
+  } else if (prm.has_scale) {
+    if (capped) res = LXA_RESULT_MAX_STEPS_REACHED;
+  }
+  st[row].result = res;
+}
+
+// ---------------------------------------------------------------------------
+// GMRES (lineax/_solver/gmres.py:106-413)
+// ---------------------------------------------------------------------------
+
+struct GmParams {
+  double rtol, atol, eps;
+  int64_t max_steps;
+  int has_scale, norm_kind, max_steps_given, restart, stagnation_iters;
+};
+
+// part[row][blk] = sum x^2 (TV storage).
+template <typename TV>
+__global__ __launch_bounds__(kB) void sumsq_partial_kernel(int64_t n, const TV* __restrict__ x, int64_t row_stride,
+                                                           const lxa_krylov_row* __restrict__ st,
+                                                           double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const TV* xr = x + row * row_stride;
+  double acc = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double v = static_cast<double>(xr[i]);
+    acc = fma(v, v, acc);
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = acc;
+}
+
+// Start of a cycle (gmres.py:255-268): r_norm = |r|_2, initial breakdown
+// if r_norm < eps; the Hessenberg buffer becomes eye(m, m+1); inner step 0.
+__global__ void gm_cycle_begin_kernel(int64_t rows, int nblk, int64_t n, const double* __restrict__ part,
+                                      GmParams prm, double* __restrict__ H, lxa_krylov_row* __restrict__ st,
+                                      lxa_gmres_row* __restrict__ gs) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  const int m = prm.restart;
+  const double ss = sum_partials(part + row * nblk, nblk);
+  // two_norm of a single element is |x| (lineax/_norm.py:74-80): sqrt(x^2) equals it.
+  const double nrm = sqrt(ss);
+  lxa_gmres_row g = gs[row];
+  g.r_norm = nrm;
+  g.breakdown = nrm < prm.eps ? 1 : 0;
+  g.scale = g.breakdown ? 0.0 : 1.0 / nrm;
+  g.inner = 0;
+  g.columns = 1;
+  g.inner_done = g.breakdown;
+  gs[row] = g;
+  double* h = H + row * static_cast<int64_t>(m) * (m + 1);
+  for (int i = 0; i < m; ++i)
+    for (int k = 0; k <= m; ++k) h[i * (m + 1) + k] = (i == k) ? 1.0 : 0.0;
+}
+
+// dst = src * scale(row): normalisation of the starting vector / a new basis
+// vector (gmres.py:401-413).  `which` 0 uses r_norm's scale, 1 the new
+// column's scale.
+template <typename TS, typename TD>
+__global__ __launch_bounds__(kB) void gm_scale_kernel(int64_t n, const TS* __restrict__ src, int64_t src_stride,
+                                                      TD* __restrict__ dst, int64_t dst_stride,
+                                                      const lxa_krylov_row* __restrict__ st,
+                                                      const lxa_gmres_row* __restrict__ gs, int gate_inner) {
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  if (gate_inner && gs[row].inner_done) return;
+  const Chunk c = chunk_of(n);
+  const double sc = gs[row].scale;
+  const TS* s = src + row * src_stride;
+  TD* d = dst + row * dst_stride;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) d[i] = static_cast<TD>(static_cast<double>(s[i]) * sc);
+}
+
+// h_k = <v_k, w> for k <= j and |w|^2 (gmres.py:355-363).
+// part[row][k][blk], k = 0..j for the dots, k = m + 1 for |w|^2.
+template <typename TV>
+__global__ __launch_bounds__(kB) void gm_project_kernel(int64_t n, const TV* __restrict__ V, const TV* __restrict__ w,
+                                                        int j, int m, const lxa_krylov_row* __restrict__ st,
+                                                        const lxa_gmres_row* __restrict__ gs, double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  if (gs[row].inner_done) return;
+  const Chunk c = chunk_of(n);
+  const TV* wr = w + row * n;
+  const TV* Vr = V + row * static_cast<int64_t>(m + 1) * n;
+  const int64_t pbase = row * static_cast<int64_t>(m + 2) * gridDim.x + blockIdx.x;
+  double ww = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double v = static_cast<double>(wr[i]);
+    ww = fma(v, v, ww);
+  }
+  ww = block_sum(ww, red);
+  if (threadIdx.x == 0) part[pbase + static_cast<int64_t>(m + 1) * gridDim.x] = ww;
+  for (int k = 0; k <= j; ++k) {
+    const TV* vk = Vr + static_cast<int64_t>(k) * n;
+    double acc = 0.0;
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB)
+      acc = fma(static_cast<double>(vk[i]), static_cast<double>(wr[i]), acc);
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) part[pbase + static_cast<int64_t>(k) * gridDim.x] = acc;
+  }
+}
+
+// Finalize the projections: proj[row][k] = h_k, and step_norm = |w|_2.
+__global__ void gm_project_finish_kernel(int64_t rows, int nblk, int j, int m, const double* __restrict__ part,
+                                         double* __restrict__ proj, const lxa_krylov_row* __restrict__ st,
+                                         lxa_gmres_row* __restrict__ gs) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active || gs[row].inner_done) return;
+  const double* pr = part + row * static_cast<int64_t>(m + 2) * nblk;
+  double* h = proj + row * (m + 1);
+  for (int k = 0; k <= m; ++k) h[k] = k <= j ? sum_partials(pr + static_cast<int64_t>(k) * nblk, nblk) : 0.0;
+  gs[row].step_norm = sqrt(sum_partials(pr + static_cast<int64_t>(m + 1) * nblk, nblk));
+}
+
+// w' = w - sum_{k<=j} h_k v_k, written to basis column j+1 (unnormalised);
+// partial |w'|^2 (gmres.py:364-366).
+template <typename TV>
+__global__ __launch_bounds__(kB) void gm_orthogonalise_kernel(int64_t n, TV* __restrict__ V, const TV* __restrict__ w,
+                                                              const double* __restrict__ proj, int j, int m,
+                                                              const lxa_krylov_row* __restrict__ st,
+                                                              const lxa_gmres_row* __restrict__ gs,
+                                                              double* __restrict__ part) {
+  __shared__ double red[4];
+  __shared__ double hs[kMaxRestart + 1];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  if (gs[row].inner_done) return;
+  const Chunk c = chunk_of(n);
+  const TV* wr = w + row * n;
+  TV* Vr = V + row * static_cast<int64_t>(m + 1) * n;
+  for (int k = threadIdx.x; k <= j; k += kB) hs[k] = proj[row * (m + 1) + k];
+  __syncthreads();
+  TV* out = Vr + static_cast<int64_t>(j + 1) * n;
+  double acc = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    double v = static_cast<double>(wr[i]);
+    for (int k = 0; k <= j; ++k) v -= hs[k] * static_cast<double>(Vr[static_cast<int64_t>(k) * n + i]);
+    const TV vs = static_cast<TV>(v);
+    out[i] = vs;
+    const double vv = static_cast<double>(vs);
+    acc = fma(vv, vv, acc);
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = acc;
+}
+
+// |w'|, breakdown test |w'| < |w| eps, Hessenberg row j (gmres.py:367-398),
+// scale for the new column, inner loop bookkeeping (gmres.py:269-271).
+__global__ void gm_arnoldi_finish_kernel(int64_t rows, int nblk, int j, int m, const double* __restrict__ part,
+                                         double* __restrict__ proj, GmParams prm, double* __restrict__ H,
+                                         const lxa_krylov_row* __restrict__ st, lxa_gmres_row* __restrict__ gs) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active || gs[row].inner_done) return;
+  lxa_gmres_row g = gs[row];
+  const double nrm = sqrt(sum_partials(part + row * nblk, nblk));
+  const double eps = g.step_norm * prm.eps;
+  const int bd = nrm < eps;
+  g.scale = bd ? 0.0 : 1.0 / nrm;
+  double* h = proj + row * (m + 1);
+  h[j + 1] = nrm;
+  double* Hr = H + row * static_cast<int64_t>(m) * (m + 1);
+  for (int k = 0; k <= m; ++k) Hr[j * (m + 1) + k] = h[k];
+  g.breakdown = bd;
+  g.inner = j + 1;
+  g.columns = j + 2;
+  g.inner_done = (g.inner >= m) || bd;
+  gs[row] = g;
+}
+
+// Least squares min |H^T z - r_norm e1| by Householder QR, one workgroup per
+// row: the nested linear_solve(MatrixLinearOperator(coeff_mat.T), beta_vec,
+// QR()) of gmres.py:295-303 (geqrf, Q^T b, back substitution).
+__global__ __launch_bounds__(64) void gm_lstsq_kernel(int64_t rows, int m, const double* __restrict__ H,
+                                                      double* __restrict__ z, const lxa_krylov_row* __restrict__ st,
+                                                      const lxa_gmres_row* __restrict__ gs) {
+  extern __shared__ double sm[];
+  const int64_t row = blockIdx.x;
+  if (!st[row].active) return;
+  const int M = m + 1;  // rows of A = H^T
+  double* A = sm;       // [M][m] row-major
+  double* c = sm + M * m;
+  double* tau = c + M;
+  const double* Hr = H + row * static_cast<int64_t>(m) * (m + 1);
+  const int l = threadIdx.x;
+  for (int t = l; t < M * m; t += 64) {
+    const int i = t / m, k = t % m;
+    A[i * m + k] = Hr[k * (m + 1) + i];  // A = coeff_mat^T
+  }
+  for (int i = l; i < M; i += 64) c[i] = (i == 0) ? gs[row].r_norm : 0.0;
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {
+    // dlarfg on column k, rows k..M-1
+    double part = 0.0;
+    for (int i = k + 1 + l; i < M; i += 64) part = fma(A[i * m + k], A[i * m + k], part);
+    part = wave_sum(part);
+    const double alpha = A[k * m + k];
+    const double xnorm = sqrt(part);
+    double tk = 0.0, beta = alpha, scal = 0.0;
+    if (xnorm != 0.0) {
+      beta = -copysign(hypot(alpha, xnorm), alpha);
+      tk = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    __syncthreads();
+    for (int i = k + 1 + l; i < M; i += 64) A[i * m + k] *= (xnorm != 0.0 ? scal : 1.0);
+    if (l == 0) {
+      A[k * m + k] = beta;
+      tau[k] = tk;
+    }
+    __syncthreads();
+    // apply H_k to the remaining columns and to c
+    for (int col = k + 1 + l; col <= m; col += 64) {
+      // col == m means the right-hand side c
+      double wsum = (col == m) ? c[k] : A[k * m + col];
+      for (int i = k + 1; i < M; ++i) wsum += A[i * m + k] * ((col == m) ? c[i] : A[i * m + col]);
+      wsum *= tk;
+      if (col == m) {
+        c[k] -= wsum;
+        for (int i = k + 1; i < M; ++i) c[i] -= A[i * m + k] * wsum;
+      } else {
+        A[k * m + col] -= wsum;
+        for (int i = k + 1; i < M; ++i) A[i * m + col] -= A[i * m + k] * wsum;
+      }
+    }
+    __syncthreads();
+  }
+  if (l == 0) {
+    double* zr = z + row * m;
+    for (int i = m - 1; i >= 0; --i) {
+      double s = c[i];
+      for (int k = i + 1; k < m; ++k) s -= A[i * m + k] * zr[k];
+      zr[i] = s / A[i * m + i];
+    }
+  }
+}
+
+// diff = sum_k z_k v_k over the written columns; y += diff; partial
+// |diff / y_scale| (gmres.py:304-310, 130-141).
+template <typename TY, typename TV>
+__global__ __launch_bounds__(kB) void gm_update_kernel(int64_t n, const TV* __restrict__ V, const double* __restrict__ z,
+                                                       int m, TY* __restrict__ y, GmParams prm,
+                                                       const lxa_krylov_row* __restrict__ st,
+                                                       const lxa_gmres_row* __restrict__ gs,
+                                                       double* __restrict__ part) {
+  __shared__ double red[4];
+  __shared__ double zs[kMaxRestart];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const int cols = gs[row].columns < m ? gs[row].columns : m;
+  const Chunk c = chunk_of(n);
+  const TV* Vr = V + row * static_cast<int64_t>(m + 1) * n;
+  for (int k = threadIdx.x; k < m; k += kB) zs[k] = z[row * m + k];
+  __syncthreads();
+  const int kind = prm.norm_kind;
+  double n2 = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    double d = 0.0;
+    for (int k = 0; k < cols; ++k) d = fma(zs[k], static_cast<double>(Vr[static_cast<int64_t>(k) * n + i]), d);
+    const double yn = static_cast<double>(y[row * n + i]) + d;
+    y[row * n + i] = static_cast<TY>(yn);
+    if (prm.has_scale) n2 = norm_accum(kind, n2, norm_elem(kind, d / (prm.atol + prm.rtol * fabs(yn))));
+  }
+  n2 = norm_block(kind, n2, red);
+  if (threadIdx.x == 0) part[(row * 3) * gridDim.x + blockIdx.x] = n2;
+}
+
+// r = b - t (then the caller may precondition); partials: [1] norm(r) in
+// the solver's norm (stagnation, gmres.py:175), [2] |r / b_scale|.
+template <typename TY, typename TV, typename TR>
+__global__ __launch_bounds__(kB) void gm_residual_kernel(int64_t n, const TY* __restrict__ b, const TV* __restrict__ t,
+                                                         TR* __restrict__ r, GmParams prm, int write,
+                                                         const lxa_krylov_row* __restrict__ st,
+                                                         double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const int kind = prm.norm_kind;
+  double nr = 0.0, n1 = 0.0;
+  const int64_t o = row * n;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double bv = static_cast<double>(b[o + i]);
+    double rv;
+    if (write) {
+      rv = static_cast<double>(static_cast<TR>(bv - static_cast<double>(t[o + i])));
+      r[o + i] = static_cast<TR>(rv);
+    } else {
+      rv = static_cast<double>(r[o + i]);  // r already holds M (b - A y)
+    }
+    nr = norm_accum(kind, nr, norm_elem(kind, rv));
+    if (prm.has_scale) n1 = norm_accum(kind, n1, norm_elem(kind, rv / (prm.atol + prm.rtol * fabs(bv))));
+  }
+  nr = norm_block(kind, nr, red);
+  n1 = norm_block(kind, n1, red);
+  if (threadIdx.x == 0) {
+    const int64_t base = (row * 3) * gridDim.x + blockIdx.x;
+    part[base + gridDim.x] = nr;
+    part[base + 2 * gridDim.x] = n1;
+  }
+}
+
+// End of a restart cycle (gmres.py:143-190): stagnation counter, deferred
+// breakdown, step += 1, then cond_fun.  first_pass: diff = inf.
+__global__ void gm_cycle_end_kernel(int64_t rows, int nblk, int64_t n, const double* __restrict__ part, GmParams prm,
+                                    int first_pass, lxa_krylov_row* __restrict__ st, lxa_gmres_row* __restrict__ gs) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  const int kind = prm.norm_kind;
+  const double* pr = part + row * 3 * nblk;
+  double n2 = 0.0, nr = 0.0, n1 = 0.0;
+  for (int k = 0; k < nblk; ++k) {
+    if (!first_pass) n2 = norm_accum(kind, n2, pr[k]);
+    nr = norm_accum(kind, nr, pr[nblk + k]);
+    n1 = norm_accum(kind, n1, pr[2 * nblk + k]);
+  }
+  lxa_krylov_row s = st[row];
+  lxa_gmres_row g = gs[row];
+  const double r_new_norm = norm_final(kind, nr, n);
+  if ((r_new_norm - g.r_min) < 0.0) {
+    g.stagnation = 0;
+  } else {
+    g.stagnation += 1;
+  }
+  // jnp.minimum propagates NaN
+  g.r_min = (r_new_norm != r_new_norm || g.r_min != g.r_min) ? __builtin_nan("") : fmin(r_new_norm, g.r_min);
+  const int new_breakdown = first_pass ? 0 : g.breakdown;
+  g.deferred = g.carry_breakdown;
+  g.carry_breakdown = new_breakdown;
+  s.step += 1;
+  s.norm_r = norm_final(kind, n1, n);
+  s.norm_diff = first_pass ? INFINITY : norm_final(kind, n2, n);
+  bool not_converged = true;
+  if (prm.has_scale) not_converged = (s.norm_r > 1.0) || (s.norm_diff > 1.0);
+  const bool cont = (!g.deferred) && (g.stagnation < prm.stagnation_iters) && not_converged &&
+                    (s.step < prm.max_steps);
+  s.active = cont ? 1 : 0;
+  g.inner_done = 0;
+  st[row] = s;
+  gs[row] = g;
+}
+
+// RESULTS (gmres.py:217-237).
+__global__ void gm_result_kernel(int64_t rows, GmParams prm, lxa_krylov_row* __restrict__ st,
+                                 const lxa_gmres_row* __restrict__ gs) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  const lxa_krylov_row s = st[row];
+  const lxa_gmres_row g = gs[row];
+  int res = LXA_RESULT_SUCCESSFUL;
+  const bool capped = s.step == prm.max_steps;
+  if (!prm.max_steps_given) {
+    if (capped) res = LXA_RESULT_SINGULAR;
+  } else if (prm.has_scale) {
+    if (capped) res = LXA_RESULT_MAX_STEPS_REACHED;
+  }
+  if (g.stagnation >= prm.stagnation_iters) res = LXA_RESULT_STAGNATION;
+  bool not_converged = true;
+  if (prm.has_scale) not_converged = (s.norm_r > 1.0) || (s.norm_diff > 1.0);
+  if (g.deferred && not_converged) res = LXA_RESULT_BREAKDOWN;
+  st[row].result = res;
+}
+
+__global__ void gm_init_kernel(int64_t rows, lxa_krylov_row* __restrict__ st, lxa_gmres_row* __restrict__ gs) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  lxa_krylov_row s = {};
+  s.active = 1;  // the dummy first pass always runs (cond_fun's `step == 0`)
+  s.norm_diff = INFINITY;
+  st[row] = s;
+  lxa_gmres_row g = {};
+  g.r_min = INFINITY;
+  gs[row] = g;
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+
+int vtype(int code, int* out) {
+  if (code != LXA_F32 && code != LXA_F64) return set_error(LXA_EINVAL, "bad dtype %d", code);
+  *out = code;
+  return LXA_OK;
+}
+
+int nblocks_for(int64_t n, int64_t rows) {
+  int64_t target = static_cast<int64_t>(num_cus()) * 8;
+  int64_t per_row = (target + rows - 1) / (rows > 0 ? rows : 1);
+  int64_t byn = (n + kB * 4 - 1) / (kB * 4);
+  int64_t nb = per_row < byn ? per_row : byn;
+  if (nb < 1) nb = 1;
+  if (nb > 1024) nb = 1024;
+  return static_cast<int>(nb);
+}
+
+dim3 grid2(int nblk, int64_t rows) { return dim3(static_cast<unsigned>(nblk), static_cast<unsigned>(rows)); }
+dim3 grid_rows(int64_t rows) { return dim3(static_cast<unsigned>((rows + 63) / 64)); }
+
+#define LXA_DISPATCH_YV(ydt, vdt, ...)                                              \
+  do {                                                                              \
+    if ((ydt) == LXA_F64 && (vdt) == LXA_F64) {                                     \
+      using TY = double;                                                            \
+      using TV = double;                                                            \
+      __VA_ARGS__;                                                                  \
+    } else if ((ydt) == LXA_F64 && (vdt) == LXA_F32) {                              \
+      using TY = double;                                                            \
+      using TV = float;                                                             \
+      __VA_ARGS__;                                                                  \
+    } else if ((ydt) == LXA_F32 && (vdt) == LXA_F32) {                              \
+      using TY = float;                                                             \
+      using TV = float;                                                             \
+      __VA_ARGS__;                                                                  \
+    } else {                                                                        \
+      return ::lxa::set_error(LXA_EINVAL, "unsupported dtype pair (y %d, vectors %d)", ydt, vdt); \
+    }                                                                               \
+  } while (0)
+
+}  // namespace
+}  // namespace lxa
+
+using lxa::CgParams;
+using lxa::GmParams;
+
+static CgParams cg_params(const lxa_krylov_params* p) {
+  CgParams c;
+  c.rtol = p->rtol;
+  c.atol = p->atol;
+  c.rcond = p->rcond;
+  c.max_steps = p->max_steps;
+  c.has_scale = !(p->rtol == 0.0 && p->atol == 0.0);
+  c.norm_kind = p->norm_kind;
+  c.max_steps_given = p->max_steps_given;
+  return c;
+}
+
+static GmParams gm_params(const lxa_krylov_params* p) {
+  GmParams g;
+  g.rtol = p->rtol;
+  g.atol = p->atol;
+  g.eps = p->eps;
+  g.max_steps = p->max_steps;
+  g.has_scale = !(p->rtol == 0.0 && p->atol == 0.0);
+  g.norm_kind = p->norm_kind;
+  g.max_steps_given = p->max_steps_given;
+  g.restart = p->restart;
+  g.stagnation_iters = p->stagnation_iters;
+  return g;
+}
+
+extern "C" int lxa_krylov_partials_size(int64_t rows, int64_t n, int restart, size_t* bytes) {
+  LXA_REQUIRE(rows >= 1 && n >= 1 && restart >= 0 && bytes, "lxa_krylov_partials_size: bad argument");
+  const int nb = lxa::nblocks_for(n, rows);
+  const int64_t q = restart + 2 > 4 ? restart + 2 : 4;
+  *bytes = static_cast<size_t>(rows) * static_cast<size_t>(q) * static_cast<size_t>(nb) * sizeof(double);
+  return LXA_OK;
+}
+
+static int check_common(int64_t rows, int64_t n, const lxa_krylov_params* prm, const void* st, const void* part) {
+  LXA_REQUIRE(rows >= 1 && rows <= 65535, "Krylov: rows must be in [1, 65535]");
+  LXA_REQUIRE(n >= 1, "Krylov: n must be positive");
+  LXA_REQUIRE(prm && st && part, "Krylov: null pointer");
+  LXA_REQUIRE(prm->norm_kind >= 0 && prm->norm_kind <= 2, "Krylov: bad norm_kind");
+  return LXA_OK;
+}
+
+template <typename TV>
+static void launch_dot(hipStream_t s, int64_t rows, int64_t n, int nb, const void* x, const void* y,
+                       const lxa_krylov_row* st, int gate, double* part, int64_t pstride, int64_t poff) {
+  hipLaunchKernelGGL((lxa::dot_partial_kernel<TV, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                     static_cast<const TV*>(x), static_cast<const TV*>(y), st, gate, part, pstride, poff);
+}
+
+// ----------------------------------------------------------------------------- CG
+
+extern "C" int lxa_cg_begin(void* stream, int vdtype, int64_t rows, int64_t n, const void* z, const void* r,
+                            const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(z && r, "lxa_cg_begin: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  if (vdtype == LXA_F64) launch_dot<double>(s, rows, n, nb, z, r, state, 0, partials, nb, 0);
+  else if (vdtype == LXA_F32) launch_dot<float>(s, rows, n, nb, z, r, state, 0, partials, nb, 0);
+  else return lxa::set_error(LXA_EINVAL, "lxa_cg_begin: bad dtype");
+  hipLaunchKernelGGL(lxa::cg_begin_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials, cg_params(prm),
+                     state);
+  LXA_LAUNCH_CHECK("lxa_cg_begin");
+  return LXA_OK;
+}
+
+extern "C" int lxa_cg_alpha(void* stream, int vdtype, int64_t rows, int64_t n, const void* q, const void* p,
+                            const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(q && p, "lxa_cg_alpha: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  if (vdtype == LXA_F64) launch_dot<double>(s, rows, n, nb, q, p, state, 1, partials, nb, 0);
+  else if (vdtype == LXA_F32) launch_dot<float>(s, rows, n, nb, q, p, state, 1, partials, nb, 0);
+  else return lxa::set_error(LXA_EINVAL, "lxa_cg_alpha: bad dtype");
+  hipLaunchKernelGGL(lxa::cg_alpha_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials, cg_params(prm),
+                     state);
+  LXA_LAUNCH_CHECK("lxa_cg_alpha");
+  return LXA_OK;
+}
+
+extern "C" int lxa_cg_update(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* p,
+                             const void* q, void* y, void* r, const void* b, int stable, int fuse_gamma,
+                             const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(p && y && r && b && (stable || q), "lxa_cg_update: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  const CgParams c = cg_params(prm);
+  LXA_DISPATCH_YV(ydtype, vdtype,
+                  hipLaunchKernelGGL((lxa::cg_update_kernel<TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                                     static_cast<const TV*>(p), static_cast<const TV*>(q), static_cast<TY*>(y),
+                                     static_cast<TV*>(r), static_cast<const TY*>(b), c, stable, fuse_gamma, state,
+                                     partials));
+  LXA_LAUNCH_CHECK("lxa_cg_update");
+  return LXA_OK;
+}
+
+extern "C" int lxa_cg_residual(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* b,
+                               const void* t, void* r, int fuse_gamma, const lxa_krylov_params* prm,
+                               lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(b && t && r, "lxa_cg_residual: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  const CgParams c = cg_params(prm);
+  LXA_DISPATCH_YV(ydtype, vdtype,
+                  hipLaunchKernelGGL((lxa::cg_residual_kernel<TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                                     static_cast<const TY*>(b), static_cast<const TY*>(t), static_cast<TV*>(r), c,
+                                     fuse_gamma, state, partials));
+  LXA_LAUNCH_CHECK("lxa_cg_residual");
+  return LXA_OK;
+}
+
+extern "C" int lxa_cg_precond_gamma(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* b,
+                                    const void* z, const void* r, const lxa_krylov_params* prm,
+                                    lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(b && z && r, "lxa_cg_precond_gamma: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  const CgParams c = cg_params(prm);
+  // <z, r> into slot 1 and |r / b_scale| into slot 2 of the [row][3][nb] layout.
+  LXA_DISPATCH_YV(ydtype, vdtype, {
+    launch_dot<TV>(s, rows, n, nb, z, r, state, 1, partials, 3 * nb, nb);
+    hipLaunchKernelGGL((lxa::cg_rnorm_kernel<TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const TY*>(b), static_cast<const TV*>(r), c, state, partials);
+  });
+  LXA_LAUNCH_CHECK("lxa_cg_precond_gamma");
+  return LXA_OK;
+}
+
+extern "C" int lxa_cg_finish(void* stream, int vdtype, int64_t rows, int64_t n, const void* z, void* p,
+                             const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(z && p, "lxa_cg_finish: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  const CgParams c = cg_params(prm);
+  hipLaunchKernelGGL(lxa::cg_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, n, partials, c, state);
+  if (vdtype == LXA_F64)
+    hipLaunchKernelGGL((lxa::cg_direction_kernel<double>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const double*>(z), static_cast<double*>(p), state);
+  else if (vdtype == LXA_F32)
+    hipLaunchKernelGGL((lxa::cg_direction_kernel<float>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const float*>(z), static_cast<float*>(p), state);
+  else
+    return lxa::set_error(LXA_EINVAL, "lxa_cg_finish: bad dtype");
+  LXA_LAUNCH_CHECK("lxa_cg_finish");
+  return LXA_OK;
+}
+
+extern "C" int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state) {
+  LXA_REQUIRE(rows >= 1 && prm && state, "lxa_cg_result: bad argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(lxa::cg_result_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, cg_params(prm), state);
+  LXA_LAUNCH_CHECK("lxa_cg_result");
+  return LXA_OK;
+}
+
+// ----------------------------------------------------------------------------- GMRES
+
+static int check_gmres(int64_t rows, int64_t n, const lxa_krylov_params* prm, const void* st, const void* gs,
+                       const void* part) {
+  if (int rc = check_common(rows, n, prm, st, part)) return rc;
+  LXA_REQUIRE(gs, "GMRES: null row state");
+  LXA_REQUIRE(prm->restart >= 1 && prm->restart <= lxa::kMaxRestart, "GMRES: restart must be in [1, %d]",
+              lxa::kMaxRestart);
+  return LXA_OK;
+}
+
+extern "C" int lxa_gmres_init(void* stream, int64_t rows, lxa_krylov_row* state, lxa_gmres_row* gstate) {
+  LXA_REQUIRE(rows >= 1 && state && gstate, "lxa_gmres_init: bad argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(lxa::gm_init_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, state, gstate);
+  LXA_LAUNCH_CHECK("lxa_gmres_init");
+  return LXA_OK;
+}
+
+extern "C" int lxa_gmres_cycle_begin(void* stream, int vdtype, int64_t rows, int64_t n, const void* r, void* basis,
+                                     const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                                     double* hessenberg, double* partials) {
+  if (int rc = check_gmres(rows, n, prm, state, gstate, partials)) return rc;
+  LXA_REQUIRE(r && basis && hessenberg, "lxa_gmres_cycle_begin: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  const int m = prm->restart;
+  const int64_t vstride = static_cast<int64_t>(m + 1) * n;
+  if (vdtype == LXA_F64) {
+    hipLaunchKernelGGL((lxa::sumsq_partial_kernel<double>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const double*>(r), n, state, partials);
+  } else if (vdtype == LXA_F32) {
+    hipLaunchKernelGGL((lxa::sumsq_partial_kernel<float>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const float*>(r), n, state, partials);
+  } else {
+    return lxa::set_error(LXA_EINVAL, "lxa_gmres_cycle_begin: bad dtype");
+  }
+  hipLaunchKernelGGL(lxa::gm_cycle_begin_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, n, partials,
+                     gm_params(prm), hessenberg, state, gstate);
+  if (vdtype == LXA_F64)
+    hipLaunchKernelGGL((lxa::gm_scale_kernel<double, double>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const double*>(r), n, static_cast<double*>(basis), vstride, state, gstate, 0);
+  else
+    hipLaunchKernelGGL((lxa::gm_scale_kernel<float, float>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const float*>(r), n, static_cast<float*>(basis), vstride, state, gstate, 0);
+  LXA_LAUNCH_CHECK("lxa_gmres_cycle_begin");
+  return LXA_OK;
+}
+
+extern "C" int lxa_gmres_arnoldi(void* stream, int vdtype, int64_t rows, int64_t n, int j, void* basis, const void* w,
+                                 const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                                 double* hessenberg, double* proj, double* partials) {
+  if (int rc = check_gmres(rows, n, prm, state, gstate, partials)) return rc;
+  LXA_REQUIRE(basis && w && hessenberg && proj, "lxa_gmres_arnoldi: null pointer");
+  const int m = prm->restart;
+  LXA_REQUIRE(j >= 0 && j < m, "lxa_gmres_arnoldi: step %d outside [0, %d)", j, m);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  const GmParams g = gm_params(prm);
+  const int64_t vstride = static_cast<int64_t>(m + 1) * n;
+  if (vdtype == LXA_F64) {
+    using TV = double;
+    hipLaunchKernelGGL((lxa::gm_project_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const TV*>(basis), static_cast<const TV*>(w), j, m, state, gstate, partials);
+    hipLaunchKernelGGL(lxa::gm_project_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, j, m, partials,
+                       proj, state, gstate);
+    hipLaunchKernelGGL((lxa::gm_orthogonalise_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<TV*>(basis), static_cast<const TV*>(w), proj, j, m, state, gstate, partials);
+    hipLaunchKernelGGL(lxa::gm_arnoldi_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, j, m, partials,
+                       proj, g, hessenberg, state, gstate);
+    TV* col = static_cast<TV*>(basis) + static_cast<int64_t>(j + 1) * n;
+    hipLaunchKernelGGL((lxa::gm_scale_kernel<TV, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n, col, vstride,
+                       col, vstride, state, gstate, 0);
+  } else if (vdtype == LXA_F32) {
+    using TV = float;
+    hipLaunchKernelGGL((lxa::gm_project_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const TV*>(basis), static_cast<const TV*>(w), j, m, state, gstate, partials);
+    hipLaunchKernelGGL(lxa::gm_project_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, j, m, partials,
+                       proj, state, gstate);
+    hipLaunchKernelGGL((lxa::gm_orthogonalise_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<TV*>(basis), static_cast<const TV*>(w), proj, j, m, state, gstate, partials);
+    hipLaunchKernelGGL(lxa::gm_arnoldi_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, j, m, partials,
+                       proj, g, hessenberg, state, gstate);
+    TV* col = static_cast<TV*>(basis) + static_cast<int64_t>(j + 1) * n;
+    hipLaunchKernelGGL((lxa::gm_scale_kernel<TV, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n, col, vstride,
+                       col, vstride, state, gstate, 0);
+  } else {
+    return lxa::set_error(LXA_EINVAL, "lxa_gmres_arnoldi: bad dtype");
+  }
+  LXA_LAUNCH_CHECK("lxa_gmres_arnoldi");
+  return LXA_OK;
+}
+
+extern "C" int lxa_gmres_update(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* basis,
+                                void* y, const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                                const double* hessenberg, double* z, double* partials) {
+  if (int rc = check_gmres(rows, n, prm, state, gstate, partials)) return rc;
+  LXA_REQUIRE(basis && y && hessenberg && z, "lxa_gmres_update: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  const int m = prm->restart;
+  const size_t lds = (static_cast<size_t>(m + 1) * m + (m + 1) + m) * sizeof(double);
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(lxa::gm_lstsq_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return lxa::hip_error(e, "hipFuncSetAttribute(gm_lstsq_kernel)");
+  }
+  hipLaunchKernelGGL(lxa::gm_lstsq_kernel, dim3(static_cast<unsigned>(rows)), dim3(64), lds, s, rows, m, hessenberg,
+                     z, state, gstate);
+  const GmParams g = gm_params(prm);
+  LXA_DISPATCH_YV(ydtype, vdtype,
+                  hipLaunchKernelGGL((lxa::gm_update_kernel<TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                                     static_cast<const TV*>(basis), z, m, static_cast<TY*>(y), g, state, gstate,
+                                     partials));
+  LXA_LAUNCH_CHECK("lxa_gmres_update");
+  return LXA_OK;
+}
+
+extern "C" int lxa_gmres_residual(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* b,
+                                  const void* t, void* r, int write, const lxa_krylov_params* prm,
+                                  lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(b && r && (t || !write), "lxa_gmres_residual: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  const GmParams g = gm_params(prm);
+  LXA_DISPATCH_YV(ydtype, vdtype,
+                  hipLaunchKernelGGL((lxa::gm_residual_kernel<TY, TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0,
+                                     s, n, static_cast<const TY*>(b), static_cast<const TY*>(t),
+                                     static_cast<TV*>(r), g, write, state, partials));
+  LXA_LAUNCH_CHECK("lxa_gmres_residual");
+  return LXA_OK;
+}
+
+extern "C" int lxa_gmres_cycle_end(void* stream, int64_t rows, int64_t n, int first_pass,
+                                   const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                                   double* partials) {
+  if (int rc = check_gmres(rows, n, prm, state, gstate, partials)) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  hipLaunchKernelGGL(lxa::gm_cycle_end_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, n, partials,
+                     gm_params(prm), first_pass, state, gstate);
+  LXA_LAUNCH_CHECK("lxa_gmres_cycle_end");
+  return LXA_OK;
+}
+
+extern "C" int lxa_gmres_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state,
+                                const lxa_gmres_row* gstate) {
+  LXA_REQUIRE(rows >= 1 && prm && state && gstate, "lxa_gmres_result: bad argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(lxa::gm_result_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, gm_params(prm), state,
+                     gstate);
+  LXA_LAUNCH_CHECK("lxa_gmres_result");
+  return LXA_OK;
+}

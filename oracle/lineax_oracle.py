@@ -425,3 +425,34 @@ def gmres(mv, b, rtol, atol, *, max_steps=None, restart=20, stagnation_iters=20,
         result = BREAKDOWN
     result = linear_solve_results(y, b, result)
     return y, result, step
+
+
+# ---------------------------------------------------------------------------
+# Grid operators of the Krylov workloads (fixture definitions)
+# ---------------------------------------------------------------------------
+
+
+def stencil_apply(x, coeffs):
+    """Five-point stencil on a (nx, ny) grid, zero Dirichlet boundary;
+    coeffs = (center, west j-1, east j+1, south i-1, north i+1)."""
+    c, w, e, s, n = coeffs
+    x = np.asarray(x, np.float64)
+    xp = np.pad(x, ((1, 1), (1, 1)))
+    return (c * x + w * xp[1:-1, :-2] + e * xp[1:-1, 2:] + s * xp[:-2, 1:-1] + n * xp[2:, 1:-1])
+
+
+def stencil_matrix(nx, ny, coeffs):
+    """The same operator as a scipy.sparse CSR matrix (row-major grid order)."""
+    import scipy.sparse as sp
+
+    c, w, e, s, n = coeffs
+    N = nx * ny
+    idx = np.arange(N).reshape(nx, ny)
+    rows, cols, vals = [idx.ravel()], [idx.ravel()], [np.full(N, c)]
+    for di, dj, v in ((0, -1, w), (0, 1, e), (-1, 0, s), (1, 0, n)):
+        src = idx[max(0, -di): nx - max(0, di), max(0, -dj): ny - max(0, dj)]
+        dst = idx[max(0, di): nx + min(0, di), max(0, dj): ny + min(0, dj)]
+        rows.append(src.ravel())
+        cols.append(dst.ravel())
+        vals.append(np.full(src.size, v))
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))

@@ -1,0 +1,167 @@
+"""GPU parity of CG and GMRES against the oracle's restatement of lineax.
+
+Same inputs, float64: solutions agree with the oracle to the solver
+tolerance (x 100), RESULTS codes agree, and the number of steps agrees to
+within one (reduction order differs).  Float32 and mixed working precision
+are checked against the float64 answer within the stopping tolerance.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import lineax_amd as lx
+from lineax_amd.stencil import convection_diffusion2d, poisson2d
+from oracle import lineax_oracle as O
+from tests.helpers import poisson_1d, well_conditioned
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, device, dtype=torch.float64):
+    return torch.as_tensor(np.asarray(x), dtype=dtype, device=device)
+
+
+@pytest.mark.parametrize("n", [1, 2, 30, 100])
+def test_cg_dense_psd(device, rng, n):
+    a = well_conditioned(rng, n, kind="psd")
+    b = rng.standard_normal(n)
+    op = lx.MatrixLinearOperator(_t(a, device), lx.positive_semidefinite_tag)
+    sol = lx.linear_solve(op, _t(b, device), lx.CG(rtol=1e-10, atol=1e-10))
+    x_ref, res_ref, steps_ref = O.cg(lambda v: a @ v, b, 1e-10, 1e-10)
+    assert int(sol.result) == res_ref == 0
+    np.testing.assert_allclose(sol.value.cpu().numpy(), x_ref, rtol=1e-8, atol=1e-9)
+    assert abs(int(sol.stats["num_steps"]) - steps_ref) <= 1
+
+
+def test_cg_nsd_poisson_batched(device, rng):
+    a = poisson_1d(64)
+    b = rng.standard_normal((3, 64))
+    op = lx.MatrixLinearOperator(_t(a, device), (lx.negative_semidefinite_tag, lx.symmetric_tag))
+    sol = lx.linear_solve(op, _t(b, device), lx.CG(rtol=1e-10, atol=1e-10))
+    for i in range(3):
+        x_ref, res_ref, steps_ref = O.cg(lambda v: a @ v, b[i], 1e-10, 1e-10, negate=True)
+        np.testing.assert_allclose(sol.value[i].cpu().numpy(), x_ref, rtol=1e-7, atol=1e-8)
+        assert int(sol.result[i]) == res_ref
+        assert abs(int(sol.stats["num_steps"][i]) - steps_ref) <= 1
+
+
+def test_cg_max_steps_rules(device, rng):
+    a = -poisson_1d(100)
+    b = rng.standard_normal(100)
+    op = lx.MatrixLinearOperator(_t(a, device), lx.positive_semidefinite_tag)
+    s = lx.linear_solve(op, _t(b, device), lx.CG(0.0, 0.0, max_steps=2), throw=False)
+    assert int(s.stats["num_steps"]) == 2 and int(s.result) == 0
+    s = lx.linear_solve(op, _t(b, device), lx.CG(1e-12, 1e-12, max_steps=3), throw=False)
+    assert int(s.stats["num_steps"]) == 3 and int(s.result) == lx.RESULTS.max_steps_reached
+
+
+def test_cg_breakdown_guard_singular(device):
+    a = np.diag([1.0, 1e-30])
+    op = lx.MatrixLinearOperator(_t(a, device), lx.positive_semidefinite_tag)
+    s = lx.linear_solve(op, _t([0.0, 1.0], device), lx.CG(1e-12, 1e-12), throw=False)
+    assert int(s.result) == lx.RESULTS.singular
+
+
+def test_cg_poisson_stencil_vs_oracle(device, rng):
+    n = 24
+    op = poisson2d(n, device=device)
+    b = rng.standard_normal((n, n))
+    coeffs = (4.0, -1.0, -1.0, -1.0, -1.0)
+    sol = lx.linear_solve(op, _t(b, device), lx.CG(rtol=1e-8, atol=1e-8))
+    x_ref, res_ref, steps_ref = O.cg(lambda v: O.stencil_apply(v.reshape(n, n), coeffs).ravel(), b.ravel(), 1e-8,
+                                     1e-8)
+    assert int(sol.result) == res_ref == 0
+    assert abs(int(sol.stats["num_steps"]) - steps_ref) <= 1
+    np.testing.assert_allclose(sol.value.cpu().numpy().ravel(), x_ref, rtol=1e-6, atol=1e-7)
+
+
+def test_cg_mixed_working_precision(device, rng):
+    n = 32
+    op = poisson2d(n, device=device)
+    b = rng.standard_normal((2, n, n))
+    A = O.stencil_matrix(n, n, (4.0, -1.0, -1.0, -1.0, -1.0))
+    sol = lx.linear_solve(op, _t(b, device), lx.CG(rtol=1e-6, atol=1e-6, working_dtype=torch.float32))
+    assert sol.value.dtype == torch.float64
+    assert torch.all(sol.result == 0)
+    for i in range(2):
+        x = sol.value[i].cpu().numpy().ravel()
+        r = b[i].ravel() - A @ x
+        assert np.max(np.abs(r) / (1e-6 + 1e-6 * np.abs(b[i].ravel()))) <= 1.0
+
+
+def test_gmres_dense_vs_oracle(device, rng):
+    # lineax tests/test_solve.py:24-38 (100 x 100 N(0,1), restart 100)
+    a = rng.standard_normal((100, 100))
+    true_x = rng.standard_normal(100)
+    b = a @ true_x
+    sol = lx.linear_solve(lx.MatrixLinearOperator(_t(a, device)), _t(b, device),
+                          lx.GMRES(rtol=1e-10, atol=1e-10, restart=100))
+    np.testing.assert_allclose(sol.value.cpu().numpy(), true_x, rtol=1e-8, atol=1e-8)
+    x_ref, res_ref, steps_ref = O.gmres(lambda v: a @ v, b, 1e-10, 1e-10, restart=100)
+    assert int(sol.result) == res_ref == 0
+    assert abs(int(sol.stats["num_steps"]) - steps_ref) <= 1
+
+
+def test_gmres_reference_4x4_fails(device):
+    a = np.array(
+        [
+            [0.15892892, 0.05884365, -0.60427412, 0.1891916],
+            [-1.5484863, 0.93608822, 1.94888868, 1.37069667],
+            [0.62687318, -0.13996738, -0.6824359, 0.30975754],
+            [-0.67428635, 1.52372255, -0.88277754, 0.69633816],
+        ]
+    )
+    true_x = np.array([0.51383273, 1.72983427, -0.43251078, -1.11764668])
+    b = a @ true_x
+    sol = lx.linear_solve(lx.MatrixLinearOperator(_t(a, device)), _t(b, device),
+                          lx.GMRES(rtol=1e-10, atol=1e-10, restart=2), throw=False)
+    _, res_ref, steps_ref = O.gmres(lambda v: a @ v, b, 1e-10, 1e-10, restart=2)
+    assert int(sol.result) != lx.RESULTS.successful
+    assert int(sol.result) == res_ref
+    assert abs(int(sol.stats["num_steps"]) - steps_ref) <= 1
+
+
+@pytest.mark.parametrize("restart", [5, 20])
+def test_gmres_convdiff_stencil(device, rng, restart):
+    n = 16
+    op = convection_diffusion2d(n, device=device)
+    coeffs = op.coeffs
+    b = rng.standard_normal((2, n, n))
+    sol = lx.linear_solve(op, _t(b, device), lx.GMRES(rtol=1e-8, atol=1e-8, restart=restart), throw=False)
+    for i in range(2):
+        x_ref, res_ref, steps_ref = O.gmres(lambda v: O.stencil_apply(v.reshape(n, n), coeffs).ravel(),
+                                            b[i].ravel(), 1e-8, 1e-8, restart=restart)
+        assert int(sol.result[i]) == res_ref
+        assert abs(int(sol.stats["num_steps"][i]) - steps_ref) <= 1
+        np.testing.assert_allclose(sol.value[i].cpu().numpy().ravel(), x_ref, rtol=1e-6, atol=1e-7)
+
+
+def test_gmres_identity_three_steps(device, rng):
+    b = rng.standard_normal(5)
+    op = lx.MatrixLinearOperator(_t(np.eye(5), device))
+    sol = lx.linear_solve(op, _t(b, device), lx.GMRES(1e-12, 1e-12))
+    assert int(sol.stats["num_steps"]) == 3
+    np.testing.assert_allclose(sol.value.cpu().numpy(), b, rtol=1e-14)
+
+
+def test_stencil_apply_matches_oracle(device, rng):
+    op = convection_diffusion2d(13, device=device)
+    x = rng.standard_normal((2, 13, 13))
+    y = op.mv(_t(x, device)).cpu().numpy()
+    for i in range(2):
+        np.testing.assert_allclose(y[i], O.stencil_apply(x[i], op.coeffs), rtol=1e-13, atol=1e-13)
+    yt = op.T.mv(_t(x, device)).cpu().numpy()
+    A = O.stencil_matrix(13, 13, op.coeffs)
+    np.testing.assert_allclose(yt[0].ravel(), A.T @ x[0].ravel(), rtol=1e-13, atol=1e-13)
+
+
+def test_cg_grad_matches_closed_form(device, rng):
+    a = well_conditioned(rng, 8, kind="psd")
+    b = rng.standard_normal(8)
+    g = rng.standard_normal(8)
+    B = _t(b, device).requires_grad_(True)
+    op = lx.MatrixLinearOperator(_t(a, device), lx.positive_semidefinite_tag)
+    x = lx.linear_solve(op, B, lx.CG(rtol=1e-12, atol=1e-12)).value
+    (x * _t(g, device)).sum().backward()
+    np.testing.assert_allclose(B.grad.cpu().numpy(), np.linalg.solve(a.T, g), rtol=1e-8, atol=1e-9)
