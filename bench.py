@@ -125,7 +125,20 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 512 MiB device-to-device copy 5 times (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(128 << 20, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": src.numel() * 4, "launches": 5}))
+        return
 
     if args.gpus != 1:
         print(

@@ -267,6 +267,104 @@ int lxa_gmres_cycle_end(void* stream, int64_t rows, int64_t n, int first_pass, c
 int lxa_gmres_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state,
                      const lxa_gmres_row* gstate);
 
+/* ---------------------------------------------------------------------------
+ * Matrix-core update C <- C - A B (float32 storage, row-major), the
+ * Schur-complement update of the blocked LU (the trailing-matrix GEMM inside
+ * LAPACK getrf that jax.scipy.linalg.lu_factor reaches from
+ * lineax/_solver/lu.py:53).  mode selects the product precision: float32,
+ * or bfloat16 / float16 inputs rounded to nearest, FP32 accumulation.
+ * ------------------------------------------------------------------------- */
+#define LXA_GEMM_F32 0
+#define LXA_GEMM_BF16 1
+#define LXA_GEMM_F16 2
+int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_t k, const float* a, int64_t lda,
+                 const float* b, int64_t ldb, float* c, int64_t ldc);
+
+/* ---------------------------------------------------------------------------
+ * Large dense LU (one system of order n in the thousands), the factor and
+ * solve behind lineax/_solver/lu.py:53 (lu_factor) and :64 (lu_solve) for
+ * large operators.  The system is randomised by a two-sided depth-2 random
+ * butterfly transform A' = U^T A V (lxa_rbt_matrix; n divisible by 4: pad
+ * with an identity block), A' = L U is factored without pivoting, and
+ * solutions are refined against A by the caller (GMRES preconditioned with
+ * these factors, FP64 residuals from lxa_gemv_f32a).
+ *
+ * lxa_lu_large_factor: blocked LU in place (L unit lower below the diagonal,
+ *   U on and above); linv / uinv receive the inverses of the 128 x 128
+ *   diagonal blocks of L and U (ceil(n/128) blocks each, row-major, dense);
+ *   mode (LXA_GEMM_*) is the precision of the outer Schur-complement
+ *   products (panels are float32).  info[0] (zeroed by the caller) receives
+ *   1 + the index of the first exactly-zero pivot.
+ * lxa_lu_large_trsv: kind 0: L y = x, 1: U y = x, 2: U^T y = x, 3: L^T y = x
+ *   for nrhs <= 8 vectors x[r * ldx + i] in place; inv is linv (kinds 0, 3)
+ *   or uinv (kinds 1, 2); sync is caller scratch of ceil(n/128) + 2 int32
+ *   (cleared by the call).  One persistent launch.
+ * lxa_rbt_matrix: A <- U^T A V with du, dv of 2n diagonal entries each
+ *   (outer level d[0..n), inner level d[n..2n)).
+ * lxa_rbt_vectors: form 1 applies U^T (or V^T) to nrhs vectors, form 0
+ *   applies V (or U) with the given diagonals.
+ * lxa_gemv_f32a: y = A x (b == NULL) or y = b - A x with float32 A [m, k],
+ *   x / b / y float32 or float64 (nrhs vectors, row strides ld*), FP64
+ *   accumulation: the refinement residual and the GMRES products.
+ * ------------------------------------------------------------------------- */
+int lxa_lu_large_block(void);
+int lxa_lu_large_factor(void* stream, int mode, int64_t n, float* a, int64_t lda, float* linv, float* uinv,
+                        int32_t* info);
+int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, const float* lu, int64_t lda, const float* inv,
+                      float* x, int64_t ldx, int32_t* sync);
+int lxa_rbt_matrix(void* stream, int64_t n, float* a, int64_t lda, const float* du, const float* dv);
+int lxa_rbt_vectors(void* stream, int dtype, int64_t n, int nrhs, void* x, int64_t ldx, const float* d, int form);
+int lxa_gemv_f32a(void* stream, int64_t m, int64_t k, int nrhs, const float* a, int64_t lda, int x_dtype,
+                  const void* x, int64_t ldx, const void* b, int64_t ldb, int y_dtype, void* y, int64_t ldy);
+
+/* ---------------------------------------------------------------------------
+ * Householder QR over an uneven batch, and triangular solves.
+ *
+ * Replaces the arithmetic of lineax/_solver/qr.py: `jnp.linalg.qr(mode="raw")`
+ * (qr.py:62, LAPACK geqrf), `jll.ormqr` (qr.py:89, 90) and
+ * `jsp.linalg.solve_triangular` (qr.py:83, 91; lineax/_solver/triangular.py).
+ *
+ * Each system is described by one lxa_qr_desc in device memory.  The matrix
+ * is row-major m x n at a + a_offset with row stride lda.  After
+ * lxa_qr_factor it holds R on and above the diagonal and the Householder
+ * vectors below it (unit leading entry implied), as geqrf's raw output does
+ * (transposed to row-major); tau receives min(m, n) scalars at tau_offset
+ * and tblk the compact-WY blocks: for panel p (columns [p*nb, p*nb + nb)),
+ * an upper-triangular nb x nb T_p at t_offset + p*nb*nb, row-major, with
+ * H_{p*nb} ... H_{p*nb+nb-1} = I - V_p T_p V_p^T.  nb = lxa_qr_block(dtype).
+ * w_offset addresses lxa_qr_work_elems() elements of scratch per system.
+ * Right-hand sides for lxa_qr_apply / lxa_trsv: nrhs vectors per system at
+ * b + b_offset + r*ldb.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int64_t a_offset;
+  int64_t tau_offset;
+  int64_t t_offset;
+  int64_t w_offset;
+  int64_t b_offset;
+  int32_t m, n, lda, ldb;
+} lxa_qr_desc; /* 56 bytes */
+
+/* Panel width of the blocked factorisation: 16 for float32, 8 for float64. */
+int lxa_qr_block(int dtype);
+/* Scratch elements one system of order m x n needs in lxa_qr_factor. */
+int lxa_qr_work_elems(int dtype, int64_t m, int64_t n, int64_t* elems);
+/* geqrf of every system (qr.py:62).  max_m / max_n / max_k bound m, n and
+ * min(m, n) over the batch (they size the launch grids). */
+int lxa_qr_factor(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, int max_m, int max_n,
+                  int max_k, void* a, void* tau, void* tblk, void* work);
+/* b <- Q^T b (trans = 1, ormqr with transpose=True, qr.py:89) or b <- Q b
+ * (trans = 0, the min-norm branch's ormqr, qr.py:86) for vectors of length m. */
+int lxa_qr_apply(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, int max_m, int max_k,
+                 const void* a, const void* tblk, int nrhs, void* b, int trans);
+/* In-place triangular solve with the leading k x k triangle of a, k =
+ * min(m, n) (solve_triangular, qr.py:83, 91; triangular.py:81-83):
+ * lower = 0 reads the upper triangle, 1 the lower; trans = 1 solves with the
+ * transpose; unit_diag = 1 takes the diagonal as ones.  The first k entries
+ * of each right-hand side are overwritten.  Supports k <= 8192. */
+int lxa_trsv(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, int max_k, const void* a, int nrhs,
+             void* b, int lower, int trans, int unit_diag);
+
 #ifdef __cplusplus
 }
 #endif

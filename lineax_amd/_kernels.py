@@ -13,6 +13,7 @@ import math
 import pathlib
 import threading
 
+import numpy as np
 import torch
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "_lib" / "liblineax_amd.so"
@@ -61,6 +62,11 @@ _SIGNATURES = {
     "lxa_gmres_residual": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _int, _vp, _vp, _vp],
     "lxa_gmres_cycle_end": [_vp, _i64, _i64, _int, _vp, _vp, _vp, _vp],
     "lxa_gmres_result": [_vp, _i64, _vp, _vp, _vp],
+    "lxa_qr_block": [_int],
+    "lxa_qr_work_elems": [_int, _i64, _i64, _i64p],
+    "lxa_qr_factor": [_vp, _int, _i64, _vp, _int, _int, _int, _vp, _vp, _vp, _vp],
+    "lxa_qr_apply": [_vp, _int, _i64, _vp, _int, _int, _vp, _vp, _int, _vp, _int],
+    "lxa_trsv": [_vp, _int, _i64, _vp, _int, _vp, _int, _vp, _int, _int, _int],
     "lxa_version": [],
     "lxa_last_error": [],
 }
@@ -467,3 +473,182 @@ def dtype_code(dtype: torch.dtype) -> int:
 
 def current_stream(t: torch.Tensor) -> int:
     return _stream(t)
+
+
+# ---------------------------------------------------------------------------
+# Householder QR and triangular solves
+# ---------------------------------------------------------------------------
+
+# lxa_qr_desc (56 bytes): offsets in elements, then m, n, lda, ldb.
+QR_DESC = np.dtype(
+    [("a_offset", "<i8"), ("tau_offset", "<i8"), ("t_offset", "<i8"), ("w_offset", "<i8"), ("b_offset", "<i8"),
+     ("m", "<i4"), ("n", "<i4"), ("lda", "<i4"), ("ldb", "<i4")]
+)
+assert QR_DESC.itemsize == 56
+
+
+def qr_block(dtype: torch.dtype) -> int:
+    """Panel width of the blocked QR (columns per compact-WY block)."""
+    return int(load().lxa_qr_block(_dtype_code(dtype)))
+
+
+def qr_work_elems(dtype: torch.dtype, m: int, n: int) -> int:
+    out = ctypes.c_int64(0)
+    _check(load().lxa_qr_work_elems(_dtype_code(dtype), int(m), int(n), ctypes.byref(out)), "lxa_qr_work_elems")
+    return int(out.value)
+
+
+def descs_to_device(descs: np.ndarray, device) -> torch.Tensor:
+    """Copies a QR_DESC array to device memory (uint8 tensor)."""
+    if descs.dtype != QR_DESC:
+        raise TypeError("descriptor array must have dtype QR_DESC")
+    raw = torch.from_numpy(np.ascontiguousarray(descs).view(np.uint8).reshape(-1).copy())
+    return raw.to(device)
+
+
+def qr_factor_raw(descs: torch.Tensor, count: int, max_m: int, max_n: int, max_k: int, a: torch.Tensor,
+                  tau: torch.Tensor, tblk: torch.Tensor, work: torch.Tensor) -> None:
+    """lxa_qr_factor on caller-laid-out storage (uneven batches)."""
+    for t, what in ((descs, "descs"), (a, "a"), (tau, "tau"), (tblk, "tblk"), (work, "work")):
+        _gpu(t, what)
+    if not (a.dtype == tau.dtype == tblk.dtype == work.dtype):
+        raise TypeError("a, tau, tblk and work must share one dtype")
+    _check(
+        load().lxa_qr_factor(_stream(a), _dtype_code(a.dtype), int(count), _ptr(descs), int(max_m), int(max_n),
+                             int(max_k), _ptr(a), _ptr(tau), _ptr(tblk), _ptr(work)),
+        "lxa_qr_factor",
+    )
+
+
+def qr_apply_raw(descs: torch.Tensor, count: int, max_m: int, max_k: int, a: torch.Tensor, tblk: torch.Tensor,
+                 nrhs: int, b: torch.Tensor, trans: bool) -> None:
+    for t, what in ((descs, "descs"), (a, "a"), (tblk, "tblk"), (b, "b")):
+        _gpu(t, what)
+    if not (a.dtype == tblk.dtype == b.dtype):
+        raise TypeError("a, tblk and b must share one dtype")
+    _check(
+        load().lxa_qr_apply(_stream(b), _dtype_code(a.dtype), int(count), _ptr(descs), int(max_m), int(max_k),
+                            _ptr(a), _ptr(tblk), int(nrhs), _ptr(b), int(bool(trans))),
+        "lxa_qr_apply",
+    )
+
+
+def trsv_raw(descs: torch.Tensor, count: int, max_k: int, a: torch.Tensor, nrhs: int, b: torch.Tensor, *,
+             lower: bool, trans: bool, unit_diag: bool) -> None:
+    for t, what in ((descs, "descs"), (a, "a"), (b, "b")):
+        _gpu(t, what)
+    if a.dtype != b.dtype:
+        raise TypeError("a and b must share one dtype")
+    _check(
+        load().lxa_trsv(_stream(b), _dtype_code(a.dtype), int(count), _ptr(descs), int(max_k), _ptr(a), int(nrhs),
+                        _ptr(b), int(bool(lower)), int(bool(trans)), int(bool(unit_diag))),
+        "lxa_trsv",
+    )
+
+
+def qr_factor(matrix: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """geqrf over leading batch dims of a [*batch, m, n] matrix.
+
+    Returns (a, tau, tblk): a holds R on and above the diagonal and the
+    Householder vectors below it (row-major, as geqrf's raw output
+    transposed), tau is [*batch, min(m, n)], tblk is [*batch, panels, nb, nb]
+    (the compact-WY T factors the apply kernel uses).
+    """
+    _gpu(matrix, "matrix")
+    *lead, m, n = matrix.shape
+    dt = matrix.dtype
+    _dtype_code(dt)
+    nb = qr_block(dt)
+    k = min(m, n)
+    panels = -(-k // nb)
+    a = matrix.contiguous().clone()
+    batch = math.prod(lead)
+    tau = torch.zeros((*lead, k), dtype=dt, device=matrix.device)
+    tblk = torch.zeros((*lead, panels, nb, nb), dtype=dt, device=matrix.device)
+    if batch == 0 or k == 0:
+        return a, tau, tblk
+    welems = qr_work_elems(dt, m, n)
+    work = torch.empty(max(1, batch * welems), dtype=dt, device=matrix.device)
+    d = np.zeros(batch, dtype=QR_DESC)
+    idx = np.arange(batch, dtype=np.int64)
+    d["a_offset"] = idx * m * n
+    d["tau_offset"] = idx * k
+    d["t_offset"] = idx * panels * nb * nb
+    d["w_offset"] = idx * welems
+    d["m"], d["n"], d["lda"] = m, n, n
+    qr_factor_raw(descs_to_device(d, matrix.device), batch, m, n, k, a, tau, tblk, work)
+    return a, tau, tblk
+
+
+def _solve_descs(lead, op_lead, m, n, nb, length, device):
+    count = math.prod(lead)
+    k = min(m, n)
+    panels = -(-k // nb)
+    op_count = math.prod(op_lead)
+    opi = np.broadcast_to(np.arange(op_count, dtype=np.int64).reshape(op_lead), lead).reshape(-1)
+    d = np.zeros(count, dtype=QR_DESC)
+    d["a_offset"] = opi * m * n
+    d["tau_offset"] = opi * k
+    d["t_offset"] = opi * panels * nb * nb
+    d["b_offset"] = np.arange(count, dtype=np.int64) * length
+    d["m"], d["n"], d["lda"], d["ldb"] = m, n, n, length
+    return descs_to_device(d, device), count
+
+
+def qr_solve(a: torch.Tensor, tblk: torch.Tensor, b: torch.Tensor, transpose: bool) -> torch.Tensor:
+    """Solves with the factors of qr_factor (a is [*batch, M, N], M >= N).
+
+    transpose = False: least squares min |A x - b| (b of length M, x of
+    length N): x = R^{-1} (Q^T b)[:N] (lineax/_solver/qr.py:87-92).
+    transpose = True: minimum-norm solution of A^T x = b (b of length N, x of
+    length M): x = Q [R^{-T} b; 0] (qr.py:79-86).
+    """
+    _gpu(a, "factor")
+    _gpu(b, "vector")
+    M, N = a.shape[-2:]
+    if M < N:
+        raise ValueError("qr_solve expects the factor of a tall (or square) matrix")
+    if b.dtype != a.dtype:
+        raise TypeError(f"vector dtype {b.dtype} differs from factor dtype {a.dtype}")
+    length_in = N if transpose else M
+    if b.shape[-1] != length_in:
+        raise ValueError(f"vector of size {b.shape[-1]} does not match operator (expected {length_in})")
+    op_lead = tuple(a.shape[:-2])
+    lead = tuple(torch.broadcast_shapes(op_lead, b.shape[:-1]))
+    nb = qr_block(a.dtype)
+    descs, count = _solve_descs(lead, op_lead, M, N, nb, M, a.device)
+    buf = torch.zeros((*lead, M), dtype=a.dtype, device=a.device)
+    if count == 0:
+        return buf[..., : (M if transpose else N)]
+    if N == 0:
+        return buf if transpose else buf[..., :0]
+    if transpose:
+        buf[..., :N] = b
+        trsv_raw(descs, count, N, a.contiguous(), 1, buf, lower=False, trans=True, unit_diag=False)
+        qr_apply_raw(descs, count, M, N, a.contiguous(), tblk.contiguous(), 1, buf, trans=False)
+        return buf
+    buf.copy_(b.expand(*lead, M))
+    qr_apply_raw(descs, count, M, N, a.contiguous(), tblk.contiguous(), 1, buf, trans=True)
+    trsv_raw(descs, count, N, a.contiguous(), 1, buf, lower=False, trans=False, unit_diag=False)
+    return buf[..., :N].contiguous()
+
+
+def tri_solve(matrix: torch.Tensor, b: torch.Tensor, *, lower: bool, trans: bool, unit_diag: bool) -> torch.Tensor:
+    """solve_triangular over broadcast batch dims (matrix [*batch, n, n])."""
+    _gpu(matrix, "matrix")
+    _gpu(b, "vector")
+    n = matrix.shape[-1]
+    if matrix.shape[-2] != n:
+        raise ValueError("triangular solves need square matrices")
+    if b.shape[-1] != n:
+        raise ValueError(f"vector of size {b.shape[-1]} does not match operator of order {n}")
+    if b.dtype != matrix.dtype:
+        raise TypeError(f"vector dtype {b.dtype} differs from matrix dtype {matrix.dtype}")
+    op_lead = tuple(matrix.shape[:-2])
+    lead = tuple(torch.broadcast_shapes(op_lead, b.shape[:-1]))
+    x = b.expand(*lead, n).contiguous().clone()
+    if math.prod(lead) == 0 or n == 0:
+        return x
+    descs, count = _solve_descs(lead, op_lead, n, n, qr_block(matrix.dtype), n, matrix.device)
+    trsv_raw(descs, count, n, matrix.contiguous(), 1, x, lower=lower, trans=trans, unit_diag=unit_diag)
+    return x

@@ -1,0 +1,232 @@
+// C <- beta C + sign A B (beta in {0, 1}, sign = +-1) on the gfx950 matrix
+// cores, float32 storage, with the products taken in bfloat16, float16 or
+// float32 (FP32 accumulation in all three).  The Schur-complement updates of
+// the blocked LU and its triangular block products run here.  With beta = 0
+// the update may be done in place when one operand's tile is exactly the C
+// tile it produces (M <= 128 with B aliasing C, or N <= 128 with A aliasing
+// C): a workgroup reads all of its operand tiles before it writes C.
+//
+// Tiling: a 256-thread workgroup owns a 128 x 128 tile of C; each of its four
+// waves a 64 x 64 quarter as 2 x 2 blocks of 32 x 32 MFMA accumulators
+// (v_mfma_f32_32x32x16_{bf16,f16}, or v_mfma_f32_32x32x2_f32).  K advances 32
+// at a time through double-buffered LDS tiles: A as [row][k] and B
+// transposed as [col][k], so each lane's MFMA operand (8 consecutive k of one
+// row or column) is one 16-byte LDS read; rows are padded by 8 elements.
+// A is negated while it is converted when sign < 0; the MFMAs accumulate onto
+// the C tile loaded into the accumulators (or onto zero when beta = 0).
+//
+// 32 x 32 MFMA operand and result layout (lane l, result register r):
+//   A operand: row l % 32, k = kper * (l / 32) + [0, kper)
+//   B operand: col l % 32, same k
+//   D[r]:      row 8 (r / 4) + 4 (l / 32) + r % 4, col l % 32
+// with kper = 8 for the 16-bit forms and 1 for float32.
+
+#include "lxa_common.h"
+
+namespace lxa {
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+constexpr int kBM = 128, kBN = 128, kBK = 32;
+constexpr int kGemmThreads = 256;
+
+template <int MODE>
+struct GemmMode;
+template <>
+struct GemmMode<LXA_GEMM_BF16> {
+  using E = __bf16;
+  static constexpr int kPad = 8;
+};
+template <>
+struct GemmMode<LXA_GEMM_F16> {
+  using E = _Float16;
+  static constexpr int kPad = 8;
+};
+template <>
+struct GemmMode<LXA_GEMM_F32> {
+  using E = float;
+  static constexpr int kPad = 1;
+};
+
+template <int MODE>
+__device__ __forceinline__ void mfma_block(floatx16& acc, const typename GemmMode<MODE>::E* a_row,
+                                           const typename GemmMode<MODE>::E* b_col, int lane) {
+  if constexpr (MODE == LXA_GEMM_F32) {
+    // 16 MFMAs of K = 2 cover the 32-wide k tile.
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 2) {
+      const float av = a_row[kk + (lane >> 5)];
+      const float bv = b_col[kk + (lane >> 5)];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 16) {
+      const int ko = kk + 8 * (lane >> 5);
+      if constexpr (MODE == LXA_GEMM_BF16) {
+        const bf16x8 av = *reinterpret_cast<const bf16x8*>(a_row + ko);
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(b_col + ko);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+      } else {
+        const halfx8 av = *reinterpret_cast<const halfx8*>(a_row + ko);
+        const halfx8 bv = *reinterpret_cast<const halfx8*>(b_col + ko);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc, 0, 0, 0);
+      }
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kGemmThreads) void gemm_kernel(int M, int N, int K, const float* A, int64_t lda,
+                                                            const float* B, int64_t ldb, float* C, int64_t ldc,
+                                                            int beta, float sign) {
+  using E = typename GemmMode<MODE>::E;
+  constexpr int LDK = kBK + GemmMode<MODE>::kPad;
+  __shared__ __align__(16) E sa[2][kBM][LDK];
+  __shared__ __align__(16) E sb[2][kBN][LDK];
+
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+  const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+
+  // accumulators <- C tile
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int col = n0 + wn + 32 * bj + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = m0 + wm + 32 * bi + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        acc[bi][bj][r] = (beta && row < M && col < N) ? C[(int64_t)row * ldc + col] : 0.f;
+      }
+    }
+
+  // global -> register staging: A tile 128 x 32 and B tile 32 x 128, 4 float4 each per thread
+  float4 ra[4], rb[4];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + kGemmThreads * q;
+      {  // A: row idx / 8, k 4 (idx % 8)
+        const int row = m0 + (idx >> 3), k = k0 + 4 * (idx & 7);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row < M) {
+          const float* p = A + (int64_t)row * lda + k;
+          if (k + 3 < K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+            v = *reinterpret_cast<const float4*>(p);
+          } else {
+            v.x = k < K ? p[0] : 0.f;
+            v.y = k + 1 < K ? p[1] : 0.f;
+            v.z = k + 2 < K ? p[2] : 0.f;
+            v.w = k + 3 < K ? p[3] : 0.f;
+          }
+        }
+        ra[q] = v;
+      }
+      {  // B: k idx / 32, col 4 (idx % 32)
+        const int k = k0 + (idx >> 5), col = n0 + 4 * (idx & 31);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (k < K) {
+          const float* p = B + (int64_t)k * ldb + col;
+          if (col + 3 < N && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+            v = *reinterpret_cast<const float4*>(p);
+          } else {
+            v.x = col < N ? p[0] : 0.f;
+            v.y = col + 1 < N ? p[1] : 0.f;
+            v.z = col + 2 < N ? p[2] : 0.f;
+            v.w = col + 3 < N ? p[3] : 0.f;
+          }
+        }
+        rb[q] = v;
+      }
+    }
+  };
+  auto store_tiles = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + kGemmThreads * q;
+      const int ar = idx >> 3, ak = 4 * (idx & 7);
+      sa[buf][ar][ak + 0] = (E)(sign * ra[q].x);
+      sa[buf][ar][ak + 1] = (E)(sign * ra[q].y);
+      sa[buf][ar][ak + 2] = (E)(sign * ra[q].z);
+      sa[buf][ar][ak + 3] = (E)(sign * ra[q].w);
+      const int bk = idx >> 5, bc = 4 * (idx & 31);
+      sb[buf][bc + 0][bk] = (E)rb[q].x;
+      sb[buf][bc + 1][bk] = (E)rb[q].y;
+      sb[buf][bc + 2][bk] = (E)rb[q].z;
+      sb[buf][bc + 3][bk] = (E)rb[q].w;
+    }
+  };
+
+  const int ktiles = (K + kBK - 1) / kBK;
+  load_tiles(0);
+  store_tiles(0);
+  __syncthreads();
+  for (int kt = 0; kt < ktiles; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < ktiles) load_tiles((kt + 1) * kBK);
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+        mfma_block<MODE>(acc[bi][bj], &sa[buf][wm + 32 * bi + (lane & 31)][0], &sb[buf][wn + 32 * bj + (lane & 31)][0],
+                         lane);
+    if (kt + 1 < ktiles) store_tiles(buf ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int col = n0 + wn + 32 * bj + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = m0 + wm + 32 * bi + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (row < M && col < N) C[(int64_t)row * ldc + col] = acc[bi][bj][r];
+      }
+    }
+}
+
+}  // namespace
+
+int gemm_update(hipStream_t st, int mode, int M, int N, int K, const float* A, int64_t lda, const float* B,
+                int64_t ldb, float* C, int64_t ldc, int beta, float sign) {
+  if (M <= 0 || N <= 0) return LXA_OK;
+  if (K <= 0 && beta) return LXA_OK;
+  const dim3 grid((unsigned)((N + kBN - 1) / kBN), (unsigned)((M + kBM - 1) / kBM));
+  switch (mode) {
+    case LXA_GEMM_BF16:
+      gemm_kernel<LXA_GEMM_BF16><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign);
+      break;
+    case LXA_GEMM_F16:
+      gemm_kernel<LXA_GEMM_F16><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign);
+      break;
+    case LXA_GEMM_F32:
+      gemm_kernel<LXA_GEMM_F32><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign);
+      break;
+    default:
+      return set_error(LXA_EINVAL, "unknown gemm mode %d", mode);
+  }
+  LXA_LAUNCH_CHECK("gemm_kernel");
+  return LXA_OK;
+}
+
+}  // namespace lxa
+
+extern "C" int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_t k, const float* a, int64_t lda,
+                            const float* b, int64_t ldb, float* c, int64_t ldc) {
+  LXA_REQUIRE(m >= 0 && n >= 0 && k >= 0 && m < (1 << 30) && n < (1 << 30) && k < (1 << 30),
+              "invalid sizes m=%lld n=%lld k=%lld", (long long)m, (long long)n, (long long)k);
+  if (m == 0 || n == 0 || k == 0) return LXA_OK;
+  LXA_REQUIRE(a && b && c, "a, b and c must be non-NULL");
+  LXA_REQUIRE(lda >= k && ldb >= n && ldc >= n, "leading dimensions too small (lda=%lld ldb=%lld ldc=%lld)",
+              (long long)lda, (long long)ldb, (long long)ldc);
+  return lxa::gemm_update(static_cast<hipStream_t>(stream), mode, (int)m, (int)n, (int)k, a, lda, b, ldb, c, ldc, 1,
+                          -1.f);
+}

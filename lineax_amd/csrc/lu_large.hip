@@ -1,0 +1,518 @@
+// Large dense LU for one system of order n (thousands), gfx950.
+//
+// Pivoting.  getrf's partial pivoting puts one device-wide decision (the
+// arg-max of a column of up to n entries) on the critical path of every one
+// of the n columns; on a 256-CU device each costs a grid-wide exchange of a
+// few microseconds, more than the whole Schur-complement work at n = 16384.
+// This path instead randomises the system with a two-sided depth-2 random
+// butterfly transform (Parker, "Random butterfly transformations with
+// applications in computational linear algebra", 1995; Baboulin et al.,
+// "Accelerating linear system solutions using randomization techniques",
+// 2013), A' = U^T A V, factors A' = L U without pivoting, and recovers
+// accuracy by refinement against the original A (the host drives GMRES on
+// A preconditioned by these factors).  The solution is the same as with
+// partial pivoting to the refinement tolerance; the factors differ.
+//
+// Factorisation (blocked right-looking, two levels):
+//   outer panels of 512 columns: the panel is factored by 128-column inner
+//   steps, each  (1) LU of the 128 x 128 diagonal block in LDS, with the
+//   inverses of its unit-lower and upper triangles (kept for the block
+//   triangular solves), (2) L21 = A21 U11^-1 and the panel's U12 = L11^-1 A12
+//   as float32 matrix-core products, (3) the panel's Schur update in float32.
+//   Then the outer U12 row block (128 rows at a time) and the outer Schur
+//   update A22 -= L21 U12, whose products run in bfloat16 or float16 (or
+//   float32) with float32 accumulation.
+//
+// Solves: block triangular solves with 128-row blocks in one persistent
+// launch each.  Workgroups take blocks in ticket order (an atomic counter),
+// so a workgroup only waits for blocks already owned by running workgroups;
+// each block subtracts the products with the solved blocks as their flags
+// appear, multiplies by the stored inverse of its diagonal block, and
+// publishes its flag.
+
+#include "lxa_common.h"
+
+namespace lxa {
+namespace {
+
+constexpr int kB = 128;          // diagonal / triangular-solve block
+constexpr int kOuter = 512;      // outer panel width
+constexpr int kDiagThreads = 1024;
+constexpr int kTrsvThreads = 256;
+constexpr int kMaxRhs = 8;
+constexpr int kSpinLimit = 1 << 26;
+
+// ---------------------------------------------------------------------------
+// Random butterflies.  One level with half-width s mixes index pairs
+// (b 2s + i, b 2s + s + i), i < s, with diagonals d0 = d[b 2s + i] and
+// d1 = d[b 2s + s + i]:
+//   B   form: (x, y) -> ((d0 x + d1 y), (d0 x - d1 y)) / sqrt 2
+//   B^T form: (x, y) -> (d0 (x + y), d1 (x - y)) / sqrt 2
+// A' = U^T A V applies, per level, B^T with the row diagonals to row pairs
+// and B with the column diagonals to column pairs (the product of the two is
+// formed from the four entries of each 2 x 2 quad in one pass).
+// ---------------------------------------------------------------------------
+__global__ void rbt_matrix_kernel(int n, int s, float* __restrict__ a, int64_t lda, const float* __restrict__ du,
+                                  const float* __restrict__ dv) {
+  const int64_t half = n / 2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= half * half) return;
+  const int qi = (int)(q / half), qj = (int)(q % half);
+  const int i = (qi / s) * 2 * s + (qi % s);
+  const int j = (qj / s) * 2 * s + (qj % s);
+  const float r0 = du[i], r1 = du[i + s], c0 = dv[j], c1 = dv[j + s];
+  float* p = a + (int64_t)i * lda + j;
+  const float x = p[0], y = p[s], z = p[(int64_t)s * lda], w = p[(int64_t)s * lda + s];
+  // rows: (x y ; z w) -> (r0 (x+z), r0 (y+w) ; r1 (x-z), r1 (y-w)) / sqrt 2
+  const float u0 = x + z, u1 = y + w, v0 = x - z, v1 = y - w;
+  // columns: (p, q) -> (c0 (p + q) ... ) per B form on the right: (p c0 + q c0, p c1 - q c1)
+  p[0] = 0.5f * r0 * c0 * (u0 + u1);
+  p[s] = 0.5f * r0 * c1 * (u0 - u1);
+  p[(int64_t)s * lda] = 0.5f * r1 * c0 * (v0 + v1);
+  p[(int64_t)s * lda + s] = 0.5f * r1 * c1 * (v0 - v1);
+}
+
+// Vectors x[r][i] (r < nrhs, row stride ldx), one level, form 0 = B, 1 = B^T.
+template <typename T>
+__global__ void rbt_vec_kernel(int n, int s, int nrhs, T* __restrict__ x, int64_t ldx, const float* __restrict__ d,
+                               int form) {
+  const int half = n / 2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)half * nrhs) return;
+  const int r = (int)(q / half), qi = (int)(q % half);
+  const int i = (qi / s) * 2 * s + (qi % s);
+  T* p = x + (int64_t)r * ldx;
+  const T u = p[i], v = p[i + s];
+  const T d0 = (T)d[i], d1 = (T)d[i + s];
+  const T h = (T)0.70710678118654752440;
+  if (form == 0) {
+    p[i] = h * (d0 * u + d1 * v);
+    p[i + s] = h * (d0 * u - d1 * v);
+  } else {
+    p[i] = h * d0 * (u + v);
+    p[i + s] = h * d1 * (u - v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LU without pivoting of the nb x nb diagonal block at (k, k) (nb <= 128),
+// in place, and the inverses of its unit-lower and upper triangles written
+// as dense 128 x 128 blocks (zeros outside the triangle).  info[0] is set to
+// the 1-based global index of the first exactly-zero pivot.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, float* __restrict__ a, int64_t lda,
+                                                               float* __restrict__ linv, float* __restrict__ uinv,
+                                                               int32_t* __restrict__ info) {
+  extern __shared__ __align__(16) float dsm[];
+  float* S = dsm;                      // [128][129]
+  float* X = dsm + kB * (kB + 1);      // [128][129]
+  constexpr int LD = kB + 1;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
+    const int i = idx >> 7, j = idx & 127;
+    S[i * LD + j] = (i < nb && j < nb) ? a[(int64_t)(k + i) * lda + k + j] : (i == j ? 1.f : 0.f);
+  }
+  __syncthreads();
+  // right-looking LU, one column per step
+  for (int j = 0; j < kB; ++j) {
+    const float piv = S[j * LD + j];
+    if (tid == 0 && piv == 0.f && j < nb && info[0] == 0) info[0] = k + j + 1;
+    const int rows = kB - 1 - j;
+    // multipliers then rank-1 update; thread (r, c) over the trailing block
+    for (int idx = tid; idx < rows * (rows + 1); idx += kDiagThreads) {
+      const int r = j + 1 + idx / (rows + 1);
+      const int c = j + idx % (rows + 1);  // c == j: the multiplier itself
+      const float l = S[r * LD + j] / piv;
+      if (c == j)
+        X[r * LD + j] = l;  // staged: the column is overwritten after the barrier
+      else
+        S[r * LD + c] -= l * S[j * LD + c];
+    }
+    __syncthreads();
+    for (int r = j + 1 + tid; r < kB; r += kDiagThreads) S[r * LD + j] = X[r * LD + j];
+    __syncthreads();
+  }
+  // write the factors back
+  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
+    const int i = idx >> 7, j = idx & 127;
+    if (i < nb && j < nb) a[(int64_t)(k + i) * lda + k + j] = S[i * LD + j];
+  }
+  // inverse of the unit lower triangle: X = I; for j: X[i][:] -= L[i][j] X[j][:], i > j
+  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
+    const int i = idx >> 7, j = idx & 127;
+    X[i * LD + j] = i == j ? 1.f : 0.f;
+  }
+  __syncthreads();
+  for (int j = 0; j < kB - 1; ++j) {
+    const int rows = kB - 1 - j, cols = j + 1;
+    for (int idx = tid; idx < rows * cols; idx += kDiagThreads) {
+      const int i = j + 1 + idx / cols, c = idx % cols;
+      X[i * LD + c] -= S[i * LD + j] * X[j * LD + c];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
+    const int i = idx >> 7, j = idx & 127;
+    linv[idx] = X[i * LD + j];
+  }
+  __syncthreads();
+  // inverse of the upper triangle, as Y = (U^T)^-1 (lower): Y = I; for j:
+  // Y[j][:] /= U[j][j]; Y[i][:] -= U[j][i] Y[j][:], i > j.  uinv = Y^T.
+  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
+    const int i = idx >> 7, j = idx & 127;
+    X[i * LD + j] = i == j ? 1.f : 0.f;
+  }
+  __syncthreads();
+  for (int j = 0; j < kB; ++j) {
+    const float dj = S[j * LD + j];
+    for (int c = tid; c <= j; c += kDiagThreads) X[j * LD + c] /= dj;
+    __syncthreads();
+    const int rows = kB - 1 - j, cols = j + 1;
+    for (int idx = tid; idx < rows * cols; idx += kDiagThreads) {
+      const int i = j + 1 + idx / cols, c = idx % cols;
+      X[i * LD + c] -= S[j * LD + i] * X[j * LD + c];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
+    const int i = idx >> 7, j = idx & 127;
+    uinv[idx] = X[j * LD + i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block triangular solve, one persistent launch.
+//   kind 0: L y = x  (unit lower, forward)      kind 1: U y = x (upper, backward)
+//   kind 2: U^T y = x (lower, forward)          kind 3: L^T y = x (unit upper, backward)
+// inv holds the inverses of the diagonal blocks of L (kinds 0, 3) or U
+// (kinds 1, 2), 128 x 128 each; transposed kinds read blocks transposed.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float lu_block_elem(const float* lu, int64_t lda, int kind, int bi, int bj, int r, int c) {
+  const bool tr = kind >= 2;
+  const int64_t row = tr ? (int64_t)bj * kB + c : (int64_t)bi * kB + r;
+  const int64_t col = tr ? (int64_t)bi * kB + r : (int64_t)bj * kB + c;
+  return lu[row * lda + col];
+}
+
+__global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nrhs, const float* __restrict__ lu,
+                                                                   int64_t lda, const float* __restrict__ inv,
+                                                                   float* __restrict__ x, int64_t ldx, int kind,
+                                                                   int32_t* __restrict__ sync) {
+  __shared__ float xs[kMaxRhs][kB];   // the solved block being consumed
+  __shared__ float acc[kMaxRhs][kB];  // this block's right-hand side
+  __shared__ int s_ticket;
+  int32_t* ticket = sync;
+  int32_t* flags = sync + 1;
+  int32_t* failed = sync + 1 + ((n + kB - 1) / kB);
+  const int nblk = (n + kB - 1) / kB;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_ticket = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int t = s_ticket;
+  if (t >= nblk) return;
+  const bool forward = (kind == 0 || kind == 2);
+  const int bi = forward ? t : nblk - 1 - t;
+  const int r0 = bi * kB, rows = min(kB, n - r0);
+  for (int idx = tid; idx < kMaxRhs * kB; idx += kTrsvThreads) {
+    const int rr = idx / kB, i = idx % kB;
+    acc[rr][i] = (rr < nrhs && i < rows) ? x[(int64_t)rr * ldx + r0 + i] : 0.f;
+  }
+  __syncthreads();
+  // row i of the block handled by threads 2i, 2i + 1 (64 columns each)
+  const int row = tid >> 1, hcol = (tid & 1) * 64;
+  for (int step = 0; step < t; ++step) {
+    const int bj = forward ? step : nblk - 1 - step;
+    if (tid == 0) {
+      int spin = 0;
+      while (__hip_atomic_load(&flags[bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (++spin > kSpinLimit) {
+          __hip_atomic_store(failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    (void)__hip_atomic_load(&flags[bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const int c0 = bj * kB, cols = min(kB, n - c0);
+    for (int idx = tid; idx < kMaxRhs * kB; idx += kTrsvThreads) {
+      const int rr = idx / kB, i = idx % kB;
+      xs[rr][i] = (rr < nrhs && i < cols) ? x[(int64_t)rr * ldx + c0 + i] : 0.f;
+    }
+    __syncthreads();
+    float part[kMaxRhs];
+#pragma unroll
+    for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] = 0.f;
+    if (row < rows) {
+      for (int c = hcol; c < hcol + 64 && c < cols; ++c) {
+        const float e = lu_block_elem(lu, lda, kind, bi, bj, row, c);
+#pragma unroll
+        for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += e * xs[rr][c];
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += __shfl_xor(part[rr], 1, kWave);
+    if ((tid & 1) == 0 && row < rows) {
+#pragma unroll
+      for (int rr = 0; rr < kMaxRhs; ++rr)
+        if (rr < nrhs) acc[rr][row] -= part[rr];
+    }
+    __syncthreads();
+  }
+  // y = inv(D) acc
+  const float* D = inv + (int64_t)bi * kB * kB;
+  const bool tr = kind >= 2;
+  float part[kMaxRhs];
+#pragma unroll
+  for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] = 0.f;
+  for (int c = hcol; c < hcol + 64; ++c) {
+    const float e = tr ? D[(int64_t)c * kB + row] : D[(int64_t)row * kB + c];
+#pragma unroll
+    for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += e * acc[rr][c];
+  }
+#pragma unroll
+  for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += __shfl_xor(part[rr], 1, kWave);
+  if ((tid & 1) == 0 && row < rows) {
+#pragma unroll
+    for (int rr = 0; rr < kMaxRhs; ++rr)
+      if (rr < nrhs) x[(int64_t)rr * ldx + r0 + row] = part[rr];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    __hip_atomic_store(&flags[bi], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y = A x (b == null) or y = b - A x, A float32 [m, k] row-major; x, b, y of
+// either dtype, NR vectors each (row stride ldx / ldb / ldy); FP64
+// accumulation.  A wave owns 4 consecutive rows, so each x load serves four
+// rows; lanes stride the columns (coalesced rows of A).
+// ---------------------------------------------------------------------------
+constexpr int kGemvRows = 4;
+
+template <typename TX, typename TY, int NR>
+__global__ __launch_bounds__(256) void gemv_f32a_kernel(int m, int k, const float* __restrict__ a, int64_t lda,
+                                                        const TX* __restrict__ x, int64_t ldx,
+                                                        const TY* __restrict__ b, int64_t ldb, TY* __restrict__ y,
+                                                        int64_t ldy) {
+  const int lane = threadIdx.x & 63;
+  const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * kGemvRows;
+  if (row0 >= m) return;
+  double s[kGemvRows][NR];
+#pragma unroll
+  for (int i = 0; i < kGemvRows; ++i)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[i][r] = 0.0;
+  for (int c = lane; c < k; c += kWave) {
+    double xv[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) xv[r] = (double)x[(int64_t)r * ldx + c];
+#pragma unroll
+    for (int i = 0; i < kGemvRows; ++i) {
+      if (row0 + i < m) {
+        const double av = (double)a[(int64_t)(row0 + i) * lda + c];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) s[i][r] += av * xv[r];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kGemvRows; ++i)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[i][r] = wave_sum(s[i][r]);
+  if (lane < kGemvRows * NR) {
+    const int i = lane / NR, r = lane % NR;
+    double v = 0.0;
+#pragma unroll
+    for (int ii = 0; ii < kGemvRows; ++ii)
+#pragma unroll
+      for (int rr = 0; rr < NR; ++rr)
+        if (ii == i && rr == r) v = s[ii][rr];
+    const int row = row0 + i;
+    if (row < m) {
+      const double out = b ? (double)b[(int64_t)r * ldb + row] - v : v;
+      y[(int64_t)r * ldy + row] = (TY)out;
+    }
+  }
+}
+
+template <typename TX, typename TY>
+int gemv_f32a_launch(hipStream_t st, int m, int k, int nrhs, const float* a, int64_t lda, const TX* x, int64_t ldx,
+                     const TY* b, int64_t ldb, TY* y, int64_t ldy) {
+  const unsigned blocks = (unsigned)((m + 4 * kGemvRows - 1) / (4 * kGemvRows));
+  // split into launches of 8, 4, 2, 1 right-hand sides
+  int done = 0;
+  while (done < nrhs) {
+    const int left = nrhs - done;
+    const TX* xp = x + (int64_t)done * ldx;
+    const TY* bp = b ? b + (int64_t)done * ldb : nullptr;
+    TY* yp = y + (int64_t)done * ldy;
+    int take;
+    if (left >= 8) {
+      take = 8;
+      gemv_f32a_kernel<TX, TY, 8><<<blocks, 256, 0, st>>>(m, k, a, lda, xp, ldx, bp, ldb, yp, ldy);
+    } else if (left >= 4) {
+      take = 4;
+      gemv_f32a_kernel<TX, TY, 4><<<blocks, 256, 0, st>>>(m, k, a, lda, xp, ldx, bp, ldb, yp, ldy);
+    } else if (left >= 2) {
+      take = 2;
+      gemv_f32a_kernel<TX, TY, 2><<<blocks, 256, 0, st>>>(m, k, a, lda, xp, ldx, bp, ldb, yp, ldy);
+    } else {
+      take = 1;
+      gemv_f32a_kernel<TX, TY, 1><<<blocks, 256, 0, st>>>(m, k, a, lda, xp, ldx, bp, ldb, yp, ldy);
+    }
+    LXA_LAUNCH_CHECK("gemv_f32a_kernel");
+    done += take;
+  }
+  return LXA_OK;
+}
+
+int lu_large_factor_impl(hipStream_t st, int mode, int n, float* a, int64_t lda, float* linv, float* uinv,
+                         int32_t* info) {
+  const size_t dsmem = sizeof(float) * 2 * kB * (kB + 1);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&diag_lu_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)dsmem);
+  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(diag_lu_kernel)");
+  for (int K0 = 0; K0 < n; K0 += kOuter) {
+    const int K1 = min(n, K0 + kOuter);
+    for (int k = K0; k < K1; k += kB) {
+      const int k2 = min(K1, k + kB), nb = k2 - k;
+      float* Li = linv + (int64_t)(k / kB) * kB * kB;
+      float* Ui = uinv + (int64_t)(k / kB) * kB * kB;
+      diag_lu_kernel<<<1, kDiagThreads, dsmem, st>>>(k, nb, a, lda, Li, Ui, info);
+      LXA_LAUNCH_CHECK("diag_lu_kernel");
+      float* A11 = a + (int64_t)k * lda + k;
+      // L21 = A21 U11^-1 (in place, N = nb <= 128)
+      if (n - k2 > 0) {
+        if (int rc = gemm_update(st, LXA_GEMM_F32, n - k2, nb, nb, A11 + (int64_t)nb * lda, lda, Ui, kB,
+                                 A11 + (int64_t)nb * lda, lda, 0, 1.f))
+          return rc;
+      }
+      // panel U12 = L11^-1 A12 (in place, M = nb <= 128), then the panel's Schur update
+      if (K1 - k2 > 0) {
+        if (int rc = gemm_update(st, LXA_GEMM_F32, nb, K1 - k2, nb, Li, kB, A11 + nb, lda, A11 + nb, lda, 0, 1.f))
+          return rc;
+        if (n - k2 > 0) {
+          if (int rc = gemm_update(st, LXA_GEMM_F32, n - k2, K1 - k2, nb, A11 + (int64_t)nb * lda, lda, A11 + nb, lda,
+                                   A11 + (int64_t)nb * lda + nb, lda, 1, -1.f))
+            return rc;
+        }
+      }
+    }
+    if (n - K1 <= 0) continue;
+    // outer U12 = L11^-1 A12 over the panel's 128-row blocks
+    for (int k = K0; k < K1; k += kB) {
+      const int nb = min(K1, k + kB) - k;
+      float* rowblk = a + (int64_t)k * lda + K1;
+      if (k > K0) {
+        if (int rc = gemm_update(st, LXA_GEMM_F32, nb, n - K1, k - K0, a + (int64_t)k * lda + K0, lda,
+                                 a + (int64_t)K0 * lda + K1, lda, rowblk, lda, 1, -1.f))
+          return rc;
+      }
+      if (int rc = gemm_update(st, LXA_GEMM_F32, nb, n - K1, nb, linv + (int64_t)(k / kB) * kB * kB, kB, rowblk, lda,
+                               rowblk, lda, 0, 1.f))
+        return rc;
+    }
+    // outer Schur update
+    if (int rc = gemm_update(st, mode, n - K1, n - K1, K1 - K0, a + (int64_t)K1 * lda + K0, lda,
+                             a + (int64_t)K0 * lda + K1, lda, a + (int64_t)K1 * lda + K1, lda, 1, -1.f))
+      return rc;
+  }
+  return LXA_OK;
+}
+
+}  // namespace
+}  // namespace lxa
+
+using namespace lxa;
+
+extern "C" int lxa_lu_large_block(void) { return kB; }
+
+extern "C" int lxa_lu_large_factor(void* stream, int mode, int64_t n, float* a, int64_t lda, float* linv, float* uinv,
+                                   int32_t* info) {
+  LXA_REQUIRE(n >= 0 && n <= (1 << 20), "n out of range (%lld)", (long long)n);
+  LXA_REQUIRE(lda >= n, "lda (%lld) < n (%lld)", (long long)lda, (long long)n);
+  LXA_REQUIRE(mode == LXA_GEMM_F32 || mode == LXA_GEMM_BF16 || mode == LXA_GEMM_F16, "unknown mode %d", mode);
+  if (n == 0) return LXA_OK;
+  LXA_REQUIRE(a && linv && uinv && info, "a, linv, uinv and info must be non-NULL");
+  return lu_large_factor_impl(static_cast<hipStream_t>(stream), mode, (int)n, a, lda, linv, uinv, info);
+}
+
+extern "C" int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, const float* lu, int64_t lda,
+                                 const float* inv, float* x, int64_t ldx, int32_t* sync) {
+  LXA_REQUIRE(kind >= 0 && kind <= 3, "kind must be 0..3 (got %d)", kind);
+  LXA_REQUIRE(n >= 0 && n <= (1 << 20) && lda >= n, "invalid n / lda");
+  LXA_REQUIRE(nrhs >= 0 && nrhs <= kMaxRhs, "nrhs must be in [0, %d] (got %d)", kMaxRhs, nrhs);
+  if (n == 0 || nrhs == 0) return LXA_OK;
+  LXA_REQUIRE(lu && inv && x && sync && ldx >= n, "invalid pointers or ldx");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nblk = (int)((n + kB - 1) / kB);
+  hipError_t e = hipMemsetAsync(sync, 0, sizeof(int32_t) * (size_t)(nblk + 2), st);
+  if (e != hipSuccess) return hip_error(e, "hipMemsetAsync(trsv sync)");
+  trsv_blocks_kernel<<<nblk, kTrsvThreads, 0, st>>>((int)n, nrhs, lu, lda, inv, x, ldx, kind, sync);
+  LXA_LAUNCH_CHECK("trsv_blocks_kernel");
+  return LXA_OK;
+}
+
+extern "C" int lxa_rbt_matrix(void* stream, int64_t n, float* a, int64_t lda, const float* du, const float* dv) {
+  LXA_REQUIRE(n >= 0 && n % 4 == 0 && lda >= n, "butterflies need n divisible by 4 (got %lld)", (long long)n);
+  if (n == 0) return LXA_OK;
+  LXA_REQUIRE(a && du && dv, "a, du and dv must be non-NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t quads = (n / 2) * (n / 2);
+  const unsigned blocks = (unsigned)((quads + 255) / 256);
+  // inner level (two butterflies of order n/2), then the outer level
+  rbt_matrix_kernel<<<blocks, 256, 0, st>>>((int)n, (int)(n / 4), a, lda, du + n, dv + n);
+  LXA_LAUNCH_CHECK("rbt_matrix_kernel");
+  rbt_matrix_kernel<<<blocks, 256, 0, st>>>((int)n, (int)(n / 2), a, lda, du, dv);
+  LXA_LAUNCH_CHECK("rbt_matrix_kernel");
+  return LXA_OK;
+}
+
+extern "C" int lxa_rbt_vectors(void* stream, int dtype, int64_t n, int nrhs, void* x, int64_t ldx, const float* d,
+                               int form) {
+  LXA_REQUIRE(dtype == LXA_F32 || dtype == LXA_F64, "dtype must be LXA_F32 or LXA_F64");
+  LXA_REQUIRE(n >= 0 && n % 4 == 0 && ldx >= n && nrhs >= 0, "invalid n / ldx / nrhs");
+  LXA_REQUIRE(form == 0 || form == 1, "form must be 0 (B) or 1 (B^T)");
+  if (n == 0 || nrhs == 0) return LXA_OK;
+  LXA_REQUIRE(x && d, "x and d must be non-NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const unsigned blocks = (unsigned)(((n / 2) * nrhs + 255) / 256);
+  // B^T (applied for U^T b): inner level then outer; B (for V y): outer then inner
+  const int order[2][2] = {{(int)(n / 2), (int)(n / 4)}, {(int)(n / 4), (int)(n / 2)}};
+  for (int t = 0; t < 2; ++t) {
+    const int s = order[form][t];
+    const float* dd = (s == n / 2) ? d : d + n;
+    if (dtype == LXA_F32)
+      rbt_vec_kernel<float><<<blocks, 256, 0, st>>>((int)n, s, nrhs, static_cast<float*>(x), ldx, dd, form);
+    else
+      rbt_vec_kernel<double><<<blocks, 256, 0, st>>>((int)n, s, nrhs, static_cast<double*>(x), ldx, dd, form);
+    LXA_LAUNCH_CHECK("rbt_vec_kernel");
+  }
+  return LXA_OK;
+}
+
+extern "C" int lxa_gemv_f32a(void* stream, int64_t m, int64_t k, int nrhs, const float* a, int64_t lda,
+                             int x_dtype, const void* x, int64_t ldx, const void* b, int64_t ldb, int y_dtype, void* y,
+                             int64_t ldy) {
+  LXA_REQUIRE(m >= 0 && k >= 0 && lda >= k && m < (1 << 30) && k < (1 << 30), "invalid m / k / lda");
+  LXA_REQUIRE(nrhs >= 0 && nrhs <= 65535, "nrhs out of range (%d)", nrhs);
+  LXA_REQUIRE((x_dtype == LXA_F32 || x_dtype == LXA_F64) && (y_dtype == LXA_F32 || y_dtype == LXA_F64),
+              "dtypes must be LXA_F32 or LXA_F64");
+  if (m == 0 || nrhs == 0) return LXA_OK;
+  LXA_REQUIRE(a && x && y && ldx >= k && ldy >= m && (!b || ldb >= m), "invalid pointers or strides");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (x_dtype == LXA_F32 && y_dtype == LXA_F32)
+    return gemv_f32a_launch<float, float>(st, (int)m, (int)k, nrhs, a, lda, static_cast<const float*>(x), ldx,
+                                          static_cast<const float*>(b), ldb, static_cast<float*>(y), ldy);
+  if (x_dtype == LXA_F32)
+    return gemv_f32a_launch<float, double>(st, (int)m, (int)k, nrhs, a, lda, static_cast<const float*>(x), ldx,
+                                           static_cast<const double*>(b), ldb, static_cast<double*>(y), ldy);
+  if (y_dtype == LXA_F32)
+    return gemv_f32a_launch<double, float>(st, (int)m, (int)k, nrhs, a, lda, static_cast<const double*>(x), ldx,
+                                           static_cast<const float*>(b), ldb, static_cast<float*>(y), ldy);
+  return gemv_f32a_launch<double, double>(st, (int)m, (int)k, nrhs, a, lda, static_cast<const double*>(x), ldx,
+                                          static_cast<const double*>(b), ldb, static_cast<double*>(y), ldy);
+}

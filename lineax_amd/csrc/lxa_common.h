@@ -22,6 +22,11 @@ int num_cus();
 
 constexpr int kWave = 64;
 
+// C <- beta C + sign A B, float32 storage, products in the precision `mode`
+// (LXA_GEMM_*); see gemm.hip.
+int gemm_update(hipStream_t st, int mode, int M, int N, int K, const float* A, int64_t lda, const float* B,
+                int64_t ldb, float* C, int64_t ldc, int beta, float sign);
+
 template <typename T>
 __device__ __forceinline__ T quiet_nan();
 template <>

@@ -1,0 +1,682 @@
+// Householder QR over uneven batches (geqrf / ormqr) and triangular solves
+// for gfx950.
+//
+// Factorisation (lxa_qr_factor) is blocked, right-looking, compact WY:
+// each panel step factors nb columns of every system and then updates the
+// systems' trailing columns:
+//
+//   panel   one 1024-thread workgroup per system.  The panel (rows k..m-1,
+//           nb columns) lives in registers, up to eight rows x nb columns
+//           per thread of a 512-thread workgroup, i.e. 4096 rows (a
+//           global-memory variant covers taller panels).  Per column: one block reduction for the norm
+//           below the diagonal (larfg), one for v^T A over the remaining
+//           panel columns, which also carries v_a^T v_j for the T factor.
+//           The workgroup ends by forming T (larft) and writing tau, T and
+//           the panel back.
+//   trail   C <- (I - V T V^T)^T C = C - V (T^T (V^T C)) on the columns
+//           right of the panel: (1) partial V^T C per 256-row chunk and
+//           64-column block, (2) the chunk sum and T^T, (3) C -= V W.  Each
+//           pass is a coalesced sweep over C (read, read, read+write).
+//
+// Block sums are deterministic: a wave reduce-scatter (log2 K halving
+// exchanges, then a full reduction of one value per lane), then a fixed
+// order over waves.
+//
+// Reference: lineax/_solver/qr.py:55-92 calls jnp.linalg.qr(mode="raw")
+// (geqrf), jax.lax.linalg.ormqr and jax.scipy.linalg.solve_triangular; the
+// reflector convention here is LAPACK's (larfg: beta = -sign(alpha) ||x||,
+// tau = (beta - alpha) / beta, v = x / (alpha - beta), tau = 0 when x = 0).
+
+#include "lxa_common.h"
+
+namespace lxa {
+namespace {
+
+template <typename T>
+struct QrCfg;
+template <>
+struct QrCfg<float> {
+  static constexpr int NB = 16;
+};
+template <>
+struct QrCfg<double> {
+  static constexpr int NB = 8;
+};
+
+constexpr int kPanelThreads = 512;   // 2 waves per SIMD: 256 VGPRs each
+constexpr int kPanelWaves = kPanelThreads / kWave;
+constexpr int kPanelMaxRpt = 8;      // register panel: rows <= 4096
+constexpr int kBigThreads = 1024;    // apply / triangular solve workgroups
+constexpr int kBigWaves = kBigThreads / kWave;
+constexpr int kTrailCols = 64;
+constexpr int kTrailRows = 256;
+constexpr int kTrailThreads = 256;
+
+template <int K>
+struct Log2 {
+  static constexpr int value = 1 + Log2<K / 2>::value;
+};
+template <>
+struct Log2<1> {
+  static constexpr int value = 0;
+};
+
+// Sums K values (K a power of two <= 64) over the 64 lanes of a wave.  On
+// return, every lane holds the complete wave sum of value number
+// lane >> (6 - log2 K).  v is destroyed.
+template <int K, typename R>
+__device__ __forceinline__ R wave_reduce_scatter(R (&v)[K], int lane) {
+  int off = 32;
+#pragma unroll
+  for (int half = K / 2; half >= 1; half >>= 1) {
+    const bool upper = (lane & off) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      const R keep = upper ? v[i + half] : v[i];
+      const R send = upper ? v[i] : v[i + half];
+      v[i] = keep + __shfl_xor(send, off, kWave);
+    }
+    off >>= 1;
+  }
+  R x = v[0];
+#pragma unroll
+  for (int o = 32 / K; o >= 1; o >>= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+
+// Block sum of K values per thread over a workgroup of WAVES waves.
+// stage: LDS [WAVES][K]; out: LDS [K].  Every thread gets the sums.
+template <int K, int WAVES, typename R>
+__device__ __forceinline__ void block_sum(R (&v)[K], R* stage, R* out, int tid) {
+  const int lane = tid & (kWave - 1);
+  const int wv = tid >> 6;
+  const R x = wave_reduce_scatter<K>(v, lane);
+  constexpr int kShift = 6 - Log2<K>::value;
+  if ((lane & ((1 << kShift) - 1)) == 0) stage[wv * K + (lane >> kShift)] = x;
+  __syncthreads();
+  if (tid < K) {
+    R s = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) s += stage[w * K + tid];
+    out[tid] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < K; ++i) v[i] = out[i];
+}
+
+// Householder vector entry j of panel-relative row (row - k): 1 on the
+// diagonal, the stored value below it, 0 above.
+template <typename T>
+__device__ __forceinline__ T vpick(int row, int diag, T stored) {
+  return row > diag ? stored : (row == diag ? T(1) : T(0));
+}
+
+// ---------------------------------------------------------------------------
+// Panel factorisation.  RPT > 0: rows in registers (RPT per thread);
+// RPT == 0: rows in global memory (any height, slower).
+// ---------------------------------------------------------------------------
+template <typename T, int RPT>
+__global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_desc* __restrict__ descs, int k,
+                                                                 T* __restrict__ a, T* __restrict__ tau,
+                                                                 T* __restrict__ tblk) {
+  constexpr int NB = QrCfg<T>::NB;
+  constexpr int KW = NB;  // width of the per-column dot reduction (>= NB - 1)
+  __shared__ double s_stage2[kPanelWaves * 2];
+  __shared__ double s_out2[2];
+  __shared__ T s_stage[kPanelWaves * KW];
+  __shared__ T s_out[KW];
+  __shared__ T s_g[NB][NB];  // s_g[a][j] = v_a^T v_j, a < j
+
+  const lxa_qr_desc d = descs[blockIdx.x];
+  const int m = d.m, n = d.n, lda = d.lda;
+  const int kmax = min(m, n);
+  if (k >= kmax) return;
+  const int nb = min(NB, kmax - k);
+  const int tid = threadIdx.x;
+  T* A = a + d.a_offset;
+  const int nr = RPT ? RPT : (m - k + kPanelThreads - 1) / kPanelThreads;
+
+  T regs[RPT ? RPT : 1][NB];
+  auto P = [&](int r, int c) -> T& {
+    if constexpr (RPT > 0) {
+      return regs[r][c];
+    } else {
+      return A[(int64_t)(k + tid + r * kPanelThreads) * lda + k + c];
+    }
+  };
+
+  if constexpr (RPT > 0) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int row = k + tid + r * kPanelThreads;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) regs[r][c] = (row < m && c < nb) ? A[(int64_t)row * lda + k + c] : T(0);
+    }
+  }
+
+  T taus[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    taus[j] = T(0);
+    if (j >= nb) continue;  // uniform
+    const int diag = k + j;
+    // ---- larfg: norm of the column below the diagonal, and alpha
+    double nv[2] = {0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < nr; ++r) {
+      const int row = k + tid + r * kPanelThreads;
+      if (row < m) {
+        const double x = (double)P(r, j);
+        if (row > diag) nv[0] += x * x;
+        if (row == diag) nv[1] = x;
+      }
+    }
+    block_sum<2, kPanelWaves>(nv, s_stage2, s_out2, tid);
+    const double ss = nv[0], alpha = nv[1];
+    T tj = T(0), beta = (T)alpha, scale = T(1);
+    if (ss != 0.0) {
+      const double bd = -copysign(sqrt(alpha * alpha + ss), alpha);
+      beta = (T)bd;
+      tj = (T)((bd - alpha) / bd);
+      scale = (T)(1.0 / (alpha - bd));
+    }
+    taus[j] = tj;
+#pragma unroll
+    for (int r = 0; r < nr; ++r) {
+      const int row = k + tid + r * kPanelThreads;
+      if (row < m) {
+        if (row > diag && ss != 0.0) P(r, j) = P(r, j) * scale;
+        if (row == diag) P(r, j) = beta;
+      }
+    }
+    // ---- w_c = v^T A(:, c) for c > j, and g_a = v_a^T v_j for a < j
+    T acc[KW];
+#pragma unroll
+    for (int c = 0; c < KW; ++c) acc[c] = T(0);
+#pragma unroll
+    for (int r = 0; r < nr; ++r) {
+      const int row = k + tid + r * kPanelThreads;
+      if (row < m && row >= k) {
+        const T vj = vpick(row, diag, P(r, j));
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          if (c > j && c < nb) acc[c] += vj * P(r, c);
+          if (c < j) acc[c] += vj * vpick(row, k + c, P(r, c));
+        }
+      }
+    }
+    block_sum<KW, kPanelWaves>(acc, s_stage, s_out, tid);
+    if (tid == 0) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < j) s_g[c][j] = acc[c];
+    }
+#pragma unroll
+    for (int r = 0; r < nr; ++r) {
+      const int row = k + tid + r * kPanelThreads;
+      if (row < m && row >= diag && tj != T(0)) {
+        const T tv = tj * vpick(row, diag, P(r, j));
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c > j && c < nb) P(r, c) -= tv * acc[c];
+      }
+    }
+  }
+  __syncthreads();  // s_g complete
+
+  // ---- T (larft, forward, columnwise): lane a owns row a of T.
+  if (tid < NB) {
+    const int ra = tid;
+    T trow[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      T v = T(0);
+      if (i < nb) {
+        if (ra == i) {
+          v = taus[i];
+        } else if (ra < i) {
+          T s = T(0);
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+            if (b >= ra && b < i) s += trow[b] * s_g[b][i];
+          v = -taus[i] * s;
+        }
+      }
+      trow[i] = v;
+    }
+    T* Tp = tblk + d.t_offset + (int64_t)(k / NB) * NB * NB + ra * NB;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) Tp[i] = trow[i];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if (i == ra && i < nb) tau[d.tau_offset + k + i] = taus[i];
+  }
+
+  if constexpr (RPT > 0) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int row = k + tid + r * kPanelThreads;
+      if (row < m) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < nb) A[(int64_t)row * lda + k + c] = regs[r][c];
+      }
+    }
+  }
+}
+
+// Loads the Householder vectors of rows [r0, r0 + kTrailRows) of the panel at
+// column k into LDS (with the implied unit diagonal and zeros above).
+template <typename T, int NB>
+__device__ __forceinline__ void load_v_chunk(T (*sv)[NB], const T* A, int lda, int m, int k, int nb, int r0,
+                                             int tid) {
+  for (int i = tid; i < kTrailRows; i += kTrailThreads) {
+    const int row = r0 + i;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      T v = T(0);
+      if (row < m && j < nb) {
+        const int diag = k + j;
+        v = row > diag ? A[(int64_t)row * lda + k + j] : (row == diag ? T(1) : T(0));
+      }
+      sv[i][j] = v;
+    }
+  }
+}
+
+// (1) partial W = V^T C over one 256-row chunk and one 64-column block.
+template <typename T>
+__global__ __launch_bounds__(kTrailThreads) void qr_trail_partial_kernel(const lxa_qr_desc* __restrict__ descs, int k,
+                                                                        const T* __restrict__ a, T* __restrict__ work) {
+  constexpr int NB = QrCfg<T>::NB;
+  __shared__ T sv[kTrailRows][NB];
+  __shared__ T sp[kTrailThreads / kWave][NB][kWave];
+  const lxa_qr_desc d = descs[blockIdx.z];
+  const int m = d.m, n = d.n, lda = d.lda;
+  const int kmax = min(m, n);
+  if (k >= kmax) return;
+  const int nb = min(NB, kmax - k);
+  const int c0 = k + nb + kTrailCols * (int)blockIdx.x;
+  const int r0 = k + kTrailRows * (int)blockIdx.y;
+  if (c0 >= n || r0 >= m) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  const T* A = a + d.a_offset;
+  load_v_chunk<T, NB>(sv, A, lda, m, k, nb, r0, tid);
+  __syncthreads();
+  const int col = c0 + lane;
+  const bool cv = col < n;
+  T acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) acc[j] = T(0);
+  const int rows = min(kTrailRows, m - r0);
+  for (int i = wv; i < rows; i += kTrailThreads / kWave) {
+    const T c = cv ? A[(int64_t)(r0 + i) * lda + col] : T(0);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[j] += sv[i][j] * c;
+  }
+#pragma unroll
+  for (int j = 0; j < NB; ++j) sp[wv][j][lane] = acc[j];
+  __syncthreads();
+  T* W = work + d.w_offset + (int64_t)blockIdx.y * NB * n;
+  for (int idx = tid; idx < NB * kWave; idx += kTrailThreads) {
+    const int j = idx >> 6, l = idx & (kWave - 1);
+    T s = T(0);
+#pragma unroll
+    for (int w = 0; w < kTrailThreads / kWave; ++w) s += sp[w][j][l];
+    if (c0 + l < n) W[(int64_t)j * n + c0 + l] = s;
+  }
+}
+
+// (2) W = sum over chunks, then W <- T^T W; stored in chunk slot 0.
+template <typename T>
+__global__ __launch_bounds__(kTrailThreads) void qr_trail_reduce_kernel(const lxa_qr_desc* __restrict__ descs, int k,
+                                                                       const T* __restrict__ tblk,
+                                                                       T* __restrict__ work) {
+  constexpr int NB = QrCfg<T>::NB;
+  __shared__ T sw[NB][kWave];
+  __shared__ T st[NB][NB];
+  const lxa_qr_desc d = descs[blockIdx.y];
+  const int m = d.m, n = d.n;
+  const int kmax = min(m, n);
+  if (k >= kmax) return;
+  const int nb = min(NB, kmax - k);
+  const int c0 = k + nb + kTrailCols * (int)blockIdx.x;
+  if (c0 >= n) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), g = tid >> 6;
+  const int chunks = (m - k + kTrailRows - 1) / kTrailRows;
+  T* W = work + d.w_offset;
+  const T* Tp = tblk + d.t_offset + (int64_t)(k / NB) * NB * NB;
+  if (tid < NB * NB) st[tid / NB][tid % NB] = Tp[tid];
+  const int col = c0 + lane;
+  for (int j = g; j < NB; j += kTrailThreads / kWave) {
+    T s = T(0);
+    if (col < n)
+      for (int ch = 0; ch < chunks; ++ch) s += W[((int64_t)ch * NB + j) * n + col];
+    sw[j][lane] = s;
+  }
+  __syncthreads();
+  for (int j = g; j < NB; j += kTrailThreads / kWave) {
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if (i <= j) s += st[i][j] * sw[i][lane];
+    if (col < n) W[(int64_t)j * n + col] = s;
+  }
+}
+
+// (3) C -= V W over one chunk and column block.
+template <typename T>
+__global__ __launch_bounds__(kTrailThreads) void qr_trail_update_kernel(const lxa_qr_desc* __restrict__ descs, int k,
+                                                                       T* __restrict__ a,
+                                                                       const T* __restrict__ work) {
+  constexpr int NB = QrCfg<T>::NB;
+  __shared__ T sv[kTrailRows][NB];
+  const lxa_qr_desc d = descs[blockIdx.z];
+  const int m = d.m, n = d.n, lda = d.lda;
+  const int kmax = min(m, n);
+  if (k >= kmax) return;
+  const int nb = min(NB, kmax - k);
+  const int c0 = k + nb + kTrailCols * (int)blockIdx.x;
+  const int r0 = k + kTrailRows * (int)blockIdx.y;
+  if (c0 >= n || r0 >= m) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  T* A = a + d.a_offset;
+  load_v_chunk<T, NB>(sv, A, lda, m, k, nb, r0, tid);
+  const int col = c0 + lane;
+  const bool cv = col < n;
+  const T* W = work + d.w_offset;
+  T w[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) w[j] = cv ? W[(int64_t)j * n + col] : T(0);
+  __syncthreads();
+  if (!cv) return;
+  const int rows = min(kTrailRows, m - r0);
+  for (int i = wv; i < rows; i += kTrailThreads / kWave) {
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) s += sv[i][j] * w[j];
+    A[(int64_t)(r0 + i) * lda + col] -= s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// b <- Q^T b (trans) or Q b: one workgroup per (system, right-hand side),
+// panel by panel: w = V^T b; w = T^T w (or T w); b -= V w.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBigThreads) void qr_apply_kernel(const lxa_qr_desc* __restrict__ descs,
+                                                                 const T* __restrict__ a,
+                                                                 const T* __restrict__ tblk, T* __restrict__ b,
+                                                                 int trans) {
+  constexpr int NB = QrCfg<T>::NB;
+  __shared__ T s_stage[kBigWaves * NB];
+  __shared__ T s_out[NB];
+  __shared__ T s_w[NB];
+  const lxa_qr_desc d = descs[blockIdx.x];
+  const int m = d.m, n = d.n, lda = d.lda;
+  const int kmax = min(m, n);
+  if (kmax <= 0) return;
+  const int tid = threadIdx.x;
+  const T* A = a + d.a_offset;
+  T* x = b + d.b_offset + (int64_t)blockIdx.y * d.ldb;
+  const int panels = (kmax + NB - 1) / NB;
+  for (int q = 0; q < panels; ++q) {
+    const int p = trans ? q : panels - 1 - q;
+    const int k = p * NB;
+    const int nb = min(NB, kmax - k);
+    T acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[j] = T(0);
+    for (int row = k + tid; row < m; row += kBigThreads) {
+      const T xv = x[row];
+      const T* Ar = A + (int64_t)row * lda + k;
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        if (j < nb) acc[j] += vpick(row, k + j, Ar[j]) * xv;
+    }
+    block_sum<NB, kBigWaves>(acc, s_stage, s_out, tid);
+    if (tid < NB) {
+      const T* Tp = tblk + d.t_offset + (int64_t)p * NB * NB;
+      T s = T(0);
+      if (trans) {  // (T^T w)_tid = sum_{i <= tid} T[i][tid] w_i
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+          if (i <= tid) s += Tp[i * NB + tid] * acc[i];
+      } else {  // (T w)_tid = sum_{i >= tid} T[tid][i] w_i
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+          if (i >= tid) s += Tp[tid * NB + i] * acc[i];
+      }
+      s_w[tid] = s;
+    }
+    __syncthreads();
+    T w[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) w[j] = s_w[j];
+    for (int row = k + tid; row < m; row += kBigThreads) {
+      const T* Ar = A + (int64_t)row * lda + k;
+      T s = T(0);
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        if (j < nb) s += vpick(row, k + j, Ar[j]) * w[j];
+      x[row] -= s;
+    }
+    __syncthreads();  // x rows are re-read by other threads in the next panel
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solve with an upper triangular R, R[i][j] = a[i*rs + j*cs]:
+// back substitution for R x = y (tr = 0), forward for R^T x = y (tr = 1).
+// One workgroup per (system, right-hand side); x lives in LDS; 64-row
+// blocks: a parallel product with the solved part, then one wave solves the
+// diagonal block from an LDS copy of it.
+// ---------------------------------------------------------------------------
+constexpr int kTrsvBlock = 64;
+constexpr int kTrsvMaxK = 8192;
+
+template <typename T>
+__global__ __launch_bounds__(kBigThreads) void trsv_kernel(const lxa_qr_desc* __restrict__ descs,
+                                                             const T* __restrict__ a, T* __restrict__ b, int lower,
+                                                             int trans, int unit_diag) {
+  extern __shared__ __align__(16) unsigned char trsv_smem[];
+  T* xs = reinterpret_cast<T*>(trsv_smem);                      // [kTrsvMaxK] (only k used)
+  T* tile = xs + ((kTrsvMaxK + 1) & ~1);                        // [64][65]
+  T* red = tile + kTrsvBlock * (kTrsvBlock + 1);                // [16][64]
+  const lxa_qr_desc d = descs[blockIdx.x];
+  const int kk = min(d.m, d.n);
+  if (kk <= 0) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  const T* A = a + d.a_offset;
+  const int64_t rs = lower ? 1 : d.lda, cs = lower ? d.lda : 1;
+  const int tr = trans ^ lower;
+  T* y = b + d.b_offset + (int64_t)blockIdx.y * d.ldb;
+  for (int i = tid; i < kk; i += kBigThreads) xs[i] = y[i];
+  __syncthreads();
+  const int blocks = (kk + kTrsvBlock - 1) / kTrsvBlock;
+  for (int q = 0; q < blocks; ++q) {
+    const int bi = tr ? q : blocks - 1 - q;
+    const int r0 = bi * kTrsvBlock;
+    const int r1 = min(kk, r0 + kTrsvBlock);
+    const int bs = r1 - r0;
+    // diagonal block copy: tile[i][j] = R[r0 + i][r0 + j]
+    for (int idx = tid; idx < kTrsvBlock * kTrsvBlock; idx += kBigThreads) {
+      const int i = idx >> 6, j = idx & 63;
+      tile[i * (kTrsvBlock + 1) + j] =
+          (i < bs && j < bs && j >= i) ? A[(int64_t)(r0 + i) * rs + (int64_t)(r0 + j) * cs] : T(0);
+    }
+    if (!tr) {
+      // z_i = y_i - sum_{c >= r1} R[i][c] x_c: wave wv takes rows r0 + wv + 16 t
+      for (int i = wv; i < bs; i += kBigWaves) {
+        T s = T(0);
+        const T* Ri = A + (int64_t)(r0 + i) * rs;
+        for (int c = r1 + lane; c < kk; c += kWave) s += Ri[(int64_t)c * cs] * xs[c];
+        s = wave_sum(s);
+        if (lane == 0) red[i] = s;
+      }
+    } else {
+      // z_i = y_i - sum_{c < r0} R[c][i] x_c: thread (column i, group g)
+      const int i = lane, g = wv;
+      T s = T(0);
+      if (i < bs)
+        for (int c = g; c < r0; c += kBigWaves) s += A[(int64_t)c * rs + (int64_t)(r0 + i) * cs] * xs[c];
+      red[g * kWave + i] = s;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      T z = T(0);
+      if (lane < bs) {
+        if (!tr) {
+          z = xs[r0 + lane] - red[lane];
+        } else {
+          T s = T(0);
+#pragma unroll
+          for (int g = 0; g < kBigWaves; ++g) s += red[g * kWave + lane];
+          z = xs[r0 + lane] - s;
+        }
+      }
+      if (!tr) {
+        for (int j = bs - 1; j >= 0; --j) {
+          if (lane == j && !unit_diag) z = z / tile[j * (kTrsvBlock + 1) + j];
+          const T xj = __shfl(z, j, kWave);
+          if (lane < j) z -= tile[lane * (kTrsvBlock + 1) + j] * xj;
+        }
+      } else {
+        for (int j = 0; j < bs; ++j) {
+          if (lane == j && !unit_diag) z = z / tile[j * (kTrsvBlock + 1) + j];
+          const T xj = __shfl(z, j, kWave);
+          if (lane > j && lane < bs) z -= tile[j * (kTrsvBlock + 1) + lane] * xj;
+        }
+      }
+      if (lane < bs) xs[r0 + lane] = z;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < kk; i += kBigThreads) y[i] = xs[i];
+}
+
+size_t trsv_smem_bytes(int dtype) {
+  const size_t e = dtype == LXA_F64 ? 8 : 4;
+  return e * (((kTrsvMaxK + 1) & ~1) + kTrsvBlock * (kTrsvBlock + 1) + kBigWaves * kWave);
+}
+
+template <typename T>
+int qr_factor_impl(hipStream_t st, int64_t count, const lxa_qr_desc* desc, int max_m, int max_n, int max_k, T* a,
+                   T* tau, T* tblk, T* work) {
+  constexpr int NB = QrCfg<T>::NB;
+  for (int k = 0; k < max_k; k += NB) {
+    const int rows = max_m - k;
+    const int rpt = (rows + kPanelThreads - 1) / kPanelThreads;
+    const dim3 pg((unsigned)count);
+    if (rpt <= 1)
+      qr_panel_kernel<T, 1><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
+    else if (rpt <= 2)
+      qr_panel_kernel<T, 2><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
+    else if (rpt <= 4)
+      qr_panel_kernel<T, 4><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
+    else if (rpt <= kPanelMaxRpt)
+      qr_panel_kernel<T, kPanelMaxRpt><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
+    else
+      qr_panel_kernel<T, 0><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
+    LXA_LAUNCH_CHECK("qr_panel_kernel");
+    const int trail_cols = max_n - k - 1;  // >= trailing width of every system that has one
+    if (trail_cols <= 0) continue;
+    const unsigned cb = (unsigned)((trail_cols + kTrailCols - 1) / kTrailCols);
+    const unsigned rc = (unsigned)((rows + kTrailRows - 1) / kTrailRows);
+    qr_trail_partial_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k, a, work);
+    LXA_LAUNCH_CHECK("qr_trail_partial_kernel");
+    qr_trail_reduce_kernel<T><<<dim3(cb, (unsigned)count), kTrailThreads, 0, st>>>(desc, k, tblk, work);
+    LXA_LAUNCH_CHECK("qr_trail_reduce_kernel");
+    qr_trail_update_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k, a, work);
+    LXA_LAUNCH_CHECK("qr_trail_update_kernel");
+  }
+  return LXA_OK;
+}
+
+int check_batch(int dtype, int64_t count, const void* desc) {
+  LXA_REQUIRE(dtype == LXA_F32 || dtype == LXA_F64, "dtype must be LXA_F32 or LXA_F64 (got %d)", dtype);
+  LXA_REQUIRE(count >= 0 && count <= 0x7fffffff, "count out of range (%lld)", (long long)count);
+  LXA_REQUIRE(count == 0 || desc != nullptr, "desc is NULL");
+  return LXA_OK;
+}
+
+}  // namespace
+}  // namespace lxa
+
+using namespace lxa;
+
+extern "C" int lxa_qr_block(int dtype) { return dtype == LXA_F64 ? QrCfg<double>::NB : QrCfg<float>::NB; }
+
+extern "C" int lxa_qr_work_elems(int dtype, int64_t m, int64_t n, int64_t* elems) {
+  LXA_REQUIRE(dtype == LXA_F32 || dtype == LXA_F64, "dtype must be LXA_F32 or LXA_F64 (got %d)", dtype);
+  LXA_REQUIRE(m >= 0 && n >= 0 && elems != nullptr, "invalid arguments");
+  const int64_t nb = lxa_qr_block(dtype);
+  *elems = ((m + kTrailRows - 1) / kTrailRows) * nb * n;
+  return LXA_OK;
+}
+
+extern "C" int lxa_qr_factor(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, int max_m, int max_n,
+                             int max_k, void* a, void* tau, void* tblk, void* work) {
+  if (int rc = check_batch(dtype, count, desc)) return rc;
+  LXA_REQUIRE(max_m >= 0 && max_n >= 0 && max_k >= 0 && max_k <= max_m && max_k <= max_n,
+              "inconsistent bounds max_m=%d max_n=%d max_k=%d", max_m, max_n, max_k);
+  if (count == 0 || max_k == 0) return LXA_OK;
+  LXA_REQUIRE(a && tau && tblk && work, "a, tau, tblk and work must be non-NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == LXA_F32)
+    return qr_factor_impl<float>(st, count, desc, max_m, max_n, max_k, static_cast<float*>(a),
+                                 static_cast<float*>(tau), static_cast<float*>(tblk), static_cast<float*>(work));
+  return qr_factor_impl<double>(st, count, desc, max_m, max_n, max_k, static_cast<double*>(a),
+                                static_cast<double*>(tau), static_cast<double*>(tblk), static_cast<double*>(work));
+}
+
+extern "C" int lxa_qr_apply(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, int max_m, int max_k,
+                            const void* a, const void* tblk, int nrhs, void* b, int trans) {
+  if (int rc = check_batch(dtype, count, desc)) return rc;
+  LXA_REQUIRE(nrhs >= 0 && nrhs <= 65535, "nrhs out of range (%d)", nrhs);
+  LXA_REQUIRE(max_k >= 0 && max_m >= max_k, "inconsistent bounds max_m=%d max_k=%d", max_m, max_k);
+  if (count == 0 || nrhs == 0 || max_k == 0) return LXA_OK;
+  LXA_REQUIRE(a && tblk && b, "a, tblk and b must be non-NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid((unsigned)count, (unsigned)nrhs);
+  if (dtype == LXA_F32)
+    qr_apply_kernel<float><<<grid, kBigThreads, 0, st>>>(desc, static_cast<const float*>(a),
+                                                          static_cast<const float*>(tblk), static_cast<float*>(b),
+                                                          trans ? 1 : 0);
+  else
+    qr_apply_kernel<double><<<grid, kBigThreads, 0, st>>>(desc, static_cast<const double*>(a),
+                                                           static_cast<const double*>(tblk), static_cast<double*>(b),
+                                                           trans ? 1 : 0);
+  LXA_LAUNCH_CHECK("qr_apply_kernel");
+  return LXA_OK;
+}
+
+extern "C" int lxa_trsv(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, int max_k, const void* a,
+                        int nrhs, void* b, int lower, int trans, int unit_diag) {
+  if (int rc = check_batch(dtype, count, desc)) return rc;
+  LXA_REQUIRE(nrhs >= 0 && nrhs <= 65535, "nrhs out of range (%d)", nrhs);
+  if (max_k > kTrsvMaxK)
+    return set_error(LXA_EUNSUPPORTED, "triangular solves support order <= %d (got %d)", kTrsvMaxK, max_k);
+  if (count == 0 || nrhs == 0 || max_k <= 0) return LXA_OK;
+  LXA_REQUIRE(a && b, "a and b must be non-NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid((unsigned)count, (unsigned)nrhs);
+  const size_t smem = trsv_smem_bytes(dtype);
+  if (dtype == LXA_F32) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&trsv_kernel<float>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(trsv)");
+    trsv_kernel<float><<<grid, kBigThreads, smem, st>>>(desc, static_cast<const float*>(a), static_cast<float*>(b),
+                                                          lower ? 1 : 0, trans ? 1 : 0, unit_diag ? 1 : 0);
+  } else {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&trsv_kernel<double>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(trsv)");
+    trsv_kernel<double><<<grid, kBigThreads, smem, st>>>(desc, static_cast<const double*>(a),
+                                                           static_cast<double*>(b), lower ? 1 : 0, trans ? 1 : 0,
+                                                           unit_diag ? 1 : 0);
+  }
+  LXA_LAUNCH_CHECK("trsv_kernel");
+  return LXA_OK;
+}
