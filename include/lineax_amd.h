@@ -285,13 +285,16 @@ int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_t k, const 
  * solve behind lineax/_solver/lu.py:53 (lu_factor) and :64 (lu_solve) for
  * large operators.  The system is randomised by a two-sided depth-2 random
  * butterfly transform A' = U^T A V (lxa_rbt_matrix; n divisible by 4: pad
- * with an identity block), A' = L U is factored without pivoting, and
+ * with an identity block), A' is factored with pivoting only inside each
+ * 128 x 128 diagonal block, and
  * solutions are refined against A by the caller (GMRES preconditioned with
  * these factors, FP64 residuals from lxa_gemv_f32a).
  *
- * lxa_lu_large_factor: blocked LU in place (L unit lower below the diagonal,
- *   U on and above); linv / uinv receive the inverses of the 128 x 128
- *   diagonal blocks of L and U (ceil(n/128) blocks each, row-major, dense);
+ * lxa_lu_large_factor: blocked LU in place, P A = L U with P's row swaps
+ *   confined to each 128-row diagonal block (L unit lower below the
+ *   diagonal, U on and above); piv[i] (n entries, 0-based) is the row swapped
+ *   with row i, as getrf records it; linv / uinv receive the inverses of the
+ *   128 x 128 diagonal blocks of L and U (ceil(n/128) each, row-major, dense);
  *   mode (LXA_GEMM_*) is the precision of the outer Schur-complement
  *   products (panels are float32).  info[0] (zeroed by the caller) receives
  *   1 + the index of the first exactly-zero pivot.
@@ -309,7 +312,11 @@ int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_t k, const 
  * ------------------------------------------------------------------------- */
 int lxa_lu_large_block(void);
 int lxa_lu_large_factor(void* stream, int mode, int64_t n, float* a, int64_t lda, float* linv, float* uinv,
-                        int32_t* info);
+                        int32_t* piv, int32_t* info);
+/* The factorisation's row swaps on nrhs vectors x[r * ldx + i]: reverse = 0
+ * applies P (before the L solve), reverse = 1 applies P^T (after the L^T
+ * solve of a transposed system). */
+int lxa_lu_large_swaps(void* stream, int64_t n, int nrhs, float* x, int64_t ldx, const int32_t* piv, int reverse);
 int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, const float* lu, int64_t lda, const float* inv,
                       float* x, int64_t ldx, int32_t* sync);
 int lxa_rbt_matrix(void* stream, int64_t n, float* a, int64_t lda, const float* du, const float* dv);
@@ -353,6 +360,24 @@ int lxa_qr_work_elems(int dtype, int64_t m, int64_t n, int64_t* elems);
  * min(m, n) over the batch (they size the launch grids). */
 int lxa_qr_factor(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, int max_m, int max_n,
                   int max_k, void* a, void* tau, void* tblk, void* work);
+/* Blocked Cholesky over an uneven batch of float32 SPD systems of order
+ * n <= 1024 (the lxa_qr_desc fields used: a_offset, b_offset, n, lda), one
+ * workgroup per system: the cho_factor / cho_solve pair of
+ * lineax/_solver/cholesky.py:60, 74 for batches whose orders differ.
+ * factor: upper triangle in place (the strict lower triangle is left as is);
+ * info[s] = k + 1 at a non-positive pivot (factor then NaN), -1 if n > 1024.
+ * solve: x = (U^T U)^-1 b in place at b + b_offset (one vector per system). */
+int lxa_chol_blocked_factor(void* stream, int64_t count, const lxa_qr_desc* desc, float* a, int32_t* info);
+int lxa_chol_blocked_solve(void* stream, int64_t count, const lxa_qr_desc* desc, const float* u, float* b);
+
+/* One panel step of lxa_qr_factor for each system at its own column
+ * kcol[s] (a multiple of lxa_qr_block(dtype); systems with kcol[s] >=
+ * min(m, n) are skipped): the continuous-batching form, where systems that
+ * joined the batch at different times advance together.  max_rows >= m - kcol
+ * and max_trail_cols >= n - kcol - 1 over the systems with columns left.
+ * The caller advances kcol by the block after the call. */
+int lxa_qr_factor_step(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, const int32_t* kcol,
+                       int max_rows, int max_trail_cols, void* a, void* tau, void* tblk, void* work);
 /* b <- Q^T b (trans = 1, ormqr with transpose=True, qr.py:89) or b <- Q b
  * (trans = 0, the min-norm branch's ormqr, qr.py:86) for vectors of length m. */
 int lxa_qr_apply(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, int max_m, int max_k,

@@ -62,10 +62,21 @@ _SIGNATURES = {
     "lxa_gmres_residual": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _int, _vp, _vp, _vp],
     "lxa_gmres_cycle_end": [_vp, _i64, _i64, _int, _vp, _vp, _vp, _vp],
     "lxa_gmres_result": [_vp, _i64, _vp, _vp, _vp],
+    "lxa_gemm_sub": [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64],
+    "lxa_lu_large_block": [],
+    "lxa_lu_large_factor": [_vp, _int, _i64, _vp, _i64, _vp, _vp, _vp, _vp],
+    "lxa_lu_large_swaps": [_vp, _i64, _int, _vp, _i64, _vp, _int],
+    "lxa_lu_large_trsv": [_vp, _int, _i64, _int, _vp, _i64, _vp, _vp, _i64, _vp],
+    "lxa_rbt_matrix": [_vp, _i64, _vp, _i64, _vp, _vp],
+    "lxa_rbt_vectors": [_vp, _int, _i64, _int, _vp, _i64, _vp, _int],
+    "lxa_gemv_f32a": [_vp, _i64, _i64, _int, _vp, _i64, _int, _vp, _i64, _vp, _i64, _int, _vp, _i64],
     "lxa_qr_block": [_int],
     "lxa_qr_work_elems": [_int, _i64, _i64, _i64p],
     "lxa_qr_factor": [_vp, _int, _i64, _vp, _int, _int, _int, _vp, _vp, _vp, _vp],
     "lxa_qr_apply": [_vp, _int, _i64, _vp, _int, _int, _vp, _vp, _int, _vp, _int],
+    "lxa_qr_factor_step": [_vp, _int, _i64, _vp, _vp, _int, _int, _vp, _vp, _vp, _vp],
+    "lxa_chol_blocked_factor": [_vp, _i64, _vp, _vp, _vp],
+    "lxa_chol_blocked_solve": [_vp, _i64, _vp, _vp, _vp],
     "lxa_trsv": [_vp, _int, _i64, _vp, _int, _vp, _int, _vp, _int, _int, _int],
     "lxa_version": [],
     "lxa_last_error": [],
@@ -520,6 +531,20 @@ def qr_factor_raw(descs: torch.Tensor, count: int, max_m: int, max_n: int, max_k
     )
 
 
+def qr_factor_step_raw(descs: torch.Tensor, count: int, kcol: torch.Tensor, max_rows: int, max_trail_cols: int,
+                       a: torch.Tensor, tau: torch.Tensor, tblk: torch.Tensor, work: torch.Tensor) -> None:
+    """One panel step of every system at its own column kcol[s] (int32, device)."""
+    for t, what in ((descs, "descs"), (kcol, "kcol"), (a, "a"), (tau, "tau"), (tblk, "tblk"), (work, "work")):
+        _gpu(t, what)
+    if kcol.dtype != torch.int32:
+        raise TypeError("kcol must be int32")
+    _check(
+        load().lxa_qr_factor_step(_stream(a), _dtype_code(a.dtype), int(count), _ptr(descs), _ptr(kcol),
+                                  int(max_rows), int(max_trail_cols), _ptr(a), _ptr(tau), _ptr(tblk), _ptr(work)),
+        "lxa_qr_factor_step",
+    )
+
+
 def qr_apply_raw(descs: torch.Tensor, count: int, max_m: int, max_k: int, a: torch.Tensor, tblk: torch.Tensor,
                  nrhs: int, b: torch.Tensor, trans: bool) -> None:
     for t, what in ((descs, "descs"), (a, "a"), (tblk, "tblk"), (b, "b")):
@@ -652,3 +677,113 @@ def tri_solve(matrix: torch.Tensor, b: torch.Tensor, *, lower: bool, trans: bool
     descs, count = _solve_descs(lead, op_lead, n, n, qr_block(matrix.dtype), n, matrix.device)
     trsv_raw(descs, count, n, matrix.contiguous(), 1, x, lower=lower, trans=trans, unit_diag=unit_diag)
     return x
+
+
+# ---------------------------------------------------------------------------
+# Matrix-core update and the large-LU building blocks
+# ---------------------------------------------------------------------------
+
+GEMM_F32 = 0
+GEMM_BF16 = 1
+GEMM_F16 = 2
+GEMM_MODES = {"fp32": GEMM_F32, "bf16": GEMM_BF16, "fp16": GEMM_F16}
+
+
+def _f32_2d(t: torch.Tensor, what: str) -> None:
+    _gpu(t, what)
+    if t.dtype != torch.float32 or t.dim() != 2 or t.stride(1) != 1:
+        raise TypeError(f"{what} must be a float32 matrix with unit column stride")
+
+
+def gemm_sub(c: torch.Tensor, a: torch.Tensor, b: torch.Tensor, mode: str = "fp32") -> torch.Tensor:
+    """C <- C - A B in place on the matrix cores (float32 storage)."""
+    for t, what in ((a, "a"), (b, "b"), (c, "c")):
+        _f32_2d(t, what)
+    m, k = a.shape
+    k2, n = b.shape
+    if k2 != k or tuple(c.shape) != (m, n):
+        raise ValueError(f"shape mismatch: {tuple(a.shape)} x {tuple(b.shape)} -> {tuple(c.shape)}")
+    _check(load().lxa_gemm_sub(_stream(c), GEMM_MODES[mode], m, n, k, _ptr(a), a.stride(0), _ptr(b), b.stride(0),
+                               _ptr(c), c.stride(0)), "lxa_gemm_sub")
+    return c
+
+
+def lu_large_block() -> int:
+    return int(load().lxa_lu_large_block())
+
+
+def lu_large_factor(a: torch.Tensor, mode: str, linv: torch.Tensor, uinv: torch.Tensor, piv: torch.Tensor,
+                    info: torch.Tensor) -> None:
+    _f32_2d(a, "a")
+    n = a.shape[0]
+    if a.shape[1] != n:
+        raise ValueError("lu_large_factor needs a square matrix")
+    if piv.dtype != torch.int32 or piv.numel() < n:
+        raise TypeError("piv must be int32 with n entries")
+    _check(load().lxa_lu_large_factor(_stream(a), GEMM_MODES[mode], n, _ptr(a), a.stride(0), _ptr(linv), _ptr(uinv),
+                                      _ptr(piv), _ptr(info)), "lxa_lu_large_factor")
+
+
+def lu_large_swaps(x: torch.Tensor, piv: torch.Tensor, reverse: bool) -> None:
+    _f32_2d(x, "x")
+    _check(load().lxa_lu_large_swaps(_stream(x), x.shape[1], x.shape[0], _ptr(x), x.stride(0), _ptr(piv),
+                                     int(bool(reverse))), "lxa_lu_large_swaps")
+
+
+def lu_large_trsv(kind: int, lu: torch.Tensor, inv: torch.Tensor, x: torch.Tensor, sync: torch.Tensor) -> None:
+    """Block triangular solve in place on x [nrhs <= 8, ldx] (float32)."""
+    _f32_2d(lu, "lu")
+    _f32_2d(x, "x")
+    n = lu.shape[0]
+    _check(load().lxa_lu_large_trsv(_stream(x), int(kind), n, x.shape[0], _ptr(lu), lu.stride(0), _ptr(inv), _ptr(x),
+                                    x.stride(0), _ptr(sync)), "lxa_lu_large_trsv")
+
+
+def rbt_matrix(a: torch.Tensor, du: torch.Tensor, dv: torch.Tensor) -> None:
+    _f32_2d(a, "a")
+    _check(load().lxa_rbt_matrix(_stream(a), a.shape[0], _ptr(a), a.stride(0), _ptr(du), _ptr(dv)), "lxa_rbt_matrix")
+
+
+def rbt_vectors(x: torch.Tensor, d: torch.Tensor, form: int) -> None:
+    _gpu(x, "x")
+    if x.dim() != 2 or x.stride(1) != 1:
+        raise TypeError("x must be [nrhs, n] with unit column stride")
+    _check(load().lxa_rbt_vectors(_stream(x), _dtype_code(x.dtype), x.shape[1], x.shape[0], _ptr(x), x.stride(0),
+                                  _ptr(d), int(form)), "lxa_rbt_vectors")
+
+
+def gemv_f32a(a: torch.Tensor, x: torch.Tensor, out_dtype: torch.dtype, b: torch.Tensor | None = None) -> torch.Tensor:
+    """A x (or b - A x) for float32 A [m, k] and x [nrhs, k]; FP64 accumulation."""
+    _f32_2d(a, "a")
+    _gpu(x, "x")
+    m, k = a.shape
+    x = x if (x.dim() == 2 and x.stride(1) == 1) else x.reshape(-1, k).contiguous()
+    if x.shape[1] != k:
+        raise ValueError(f"x has {x.shape[1]} columns, expected {k}")
+    nrhs = x.shape[0]
+    y = torch.empty((nrhs, m), dtype=out_dtype, device=a.device)
+    if b is not None:
+        b = b.to(out_dtype).reshape(nrhs, m).contiguous()
+    _check(load().lxa_gemv_f32a(_stream(y), m, k, nrhs, _ptr(a), a.stride(0), _dtype_code(x.dtype), _ptr(x),
+                                x.stride(0), _ptr(b), (b.stride(0) if b is not None else m), _dtype_code(out_dtype),
+                                _ptr(y), y.stride(0)), "lxa_gemv_f32a")
+    return y
+
+
+def chol_blocked_factor_raw(descs: torch.Tensor, count: int, a: torch.Tensor, info: torch.Tensor) -> None:
+    """Blocked Cholesky of an uneven batch (orders <= 1024), float32, in place."""
+    for t, what in ((descs, "descs"), (a, "a"), (info, "info")):
+        _gpu(t, what)
+    if a.dtype != torch.float32 or info.dtype != torch.int32:
+        raise TypeError("a must be float32 and info int32")
+    _check(load().lxa_chol_blocked_factor(_stream(a), int(count), _ptr(descs), _ptr(a), _ptr(info)),
+           "lxa_chol_blocked_factor")
+
+
+def chol_blocked_solve_raw(descs: torch.Tensor, count: int, u: torch.Tensor, b: torch.Tensor) -> None:
+    for t, what in ((descs, "descs"), (u, "u"), (b, "b")):
+        _gpu(t, what)
+    if u.dtype != torch.float32 or b.dtype != torch.float32:
+        raise TypeError("u and b must be float32")
+    _check(load().lxa_chol_blocked_solve(_stream(b), int(count), _ptr(descs), _ptr(u), _ptr(b)),
+           "lxa_chol_blocked_solve")

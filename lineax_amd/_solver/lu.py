@@ -18,13 +18,33 @@ class LU(AbstractLinearSolver):
     getrf kernel); ``compute`` solves with ``trans`` taken from the state
     (lu.py:56-66), so ``transpose`` only flips a flag (lu.py:68-81).
     State: ``((lu, piv), packed_structures, transposed)``.
+
+    One unbatched float32 operator of order >= ``large_threshold`` takes the
+    large path instead (``lineax_amd.large_lu``): butterfly randomisation,
+    unpivoted blocked LU with ``large_precision`` Schur products, and
+    refinement against A to a normwise backward error <= ``large_tol``
+    (default: float32 epsilon), i.e. at least getrf's accuracy.  Its state is
+    ``(("large", factorisation), packed_structures, transposed)``.
+    (The constructor arguments are not in lineax; their defaults keep
+    lineax's behaviour.)
     """
+
+    def __init__(self, large_threshold: int = 4096, large_precision: str = "bf16", large_tol: float | None = None):
+        self.large_threshold = int(large_threshold)
+        self.large_precision = large_precision
+        self.large_tol = large_tol
 
     def init(self, operator, options):
         if operator.in_size() != operator.out_size():
             raise ValueError("`LU` may only be used for linear solves with square matrices")
         packed = pack_structures(operator)
         matrix = operator.as_matrix()
+        n = operator.in_size()
+        if (matrix.dim() == 2 and n >= self.large_threshold and matrix.dtype == torch.float32
+                and not is_diagonal(operator)):
+            from ..large_lu import LargeLU
+
+            return ("large", LargeLU(matrix, precision=self.large_precision)), packed, False
         if is_diagonal(operator):
             # lineax/_solver/lu.py:50-51: the diagonal is its own LU factor.
             n = operator.in_size()
@@ -37,6 +57,13 @@ class LU(AbstractLinearSolver):
     def compute(self, state, vector, options):
         (lu, piv), packed, transpose = state
         flat, batch = ravel_vector(vector, packed)
+        if isinstance(lu, str):
+            fac = piv
+            tol = self.large_tol if self.large_tol is not None else float(torch.finfo(torch.float32).eps)
+            rhs = flat.reshape(-1, fac.n)
+            x, _eta, _stats = fac.solve(rhs, tol=tol, trans=transpose)
+            x = x.to(flat.dtype).reshape(*flat.shape[:-1], fac.n)
+            return unravel_solution(x, packed, tuple(x.shape[:-1])), RESULTS.successful, {}
         x = K.lu_solve(lu, piv, flat.to(lu.dtype), trans=1 if transpose else 0)
         batch = tuple(x.shape[:-1])
         return unravel_solution(x, packed, batch), RESULTS.successful, {}
@@ -48,6 +75,8 @@ class LU(AbstractLinearSolver):
     def conj(self, state, options):
         # As lineax/_solver/lu.py:83-95 (which also flips the transpose flag).
         (lu, piv), packed, transpose = state
+        if isinstance(lu, str):
+            return ((lu, piv), packed, not transpose), {}
         return ((lu.conj(), piv), packed, not transpose), {}
 
     def assume_full_rank(self):

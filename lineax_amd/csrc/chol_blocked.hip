@@ -1,0 +1,281 @@
+// Blocked Cholesky (upper, A = U^T U) for uneven batches of order <= 1024,
+// one 256-thread workgroup per system, float32; and the matching solve.
+//
+// Per 32-column panel at k0 (kb = min(32, n - k0) columns):
+//   (1) the kb x kb diagonal block is factored in LDS by one wave (lane j
+//       owns column j; readlane-free: the pivot row is read from LDS);
+//   (2) the row block U12 = U11^-T A12 is staged in LDS (32 x (n - k0 - kb)
+//       floats), solved column-parallel in place, written back to A and kept
+//       in LDS for the update;
+//   (3) the trailing upper triangle takes the rank-kb update
+//       A22 -= U12^T U12 in 4 x 4 register micro-tiles reading U12 from LDS.
+// The trailing matrix is read and written once per panel (from L2 / HBM).
+// A non-positive pivot fills the factor with NaN and sets info = k + 1, as
+// lxa_chol_factor does.  Only the upper triangle is read; the strictly lower
+// triangle is not touched (the solve reads only the upper triangle).
+//
+// Reference semantics: lineax/_solver/cholesky.py:43-78 (cho_factor upper,
+// cho_solve); this is the uneven-batch form the request loop batches.
+
+#include "lxa_common.h"
+
+namespace lxa {
+namespace {
+
+constexpr int kP = 32;           // panel width
+constexpr int kThreads = 256;
+constexpr int kMaxN = 1024;
+constexpr int kPanelLd = kMaxN;  // LDS row stride of the U12 panel
+constexpr int kDLd = kP + 4;     // LDS row stride of the diagonal block (16-byte rows)
+
+struct Geometry {
+  int n;
+  int64_t a_off, b_off;
+  int lda;
+};
+
+__device__ __forceinline__ Geometry geometry(const lxa_qr_desc* descs, int n_all, int64_t a_stride, int64_t b_stride) {
+  Geometry g;
+  if (descs) {
+    const lxa_qr_desc d = descs[blockIdx.x];
+    g.n = d.n;
+    g.a_off = d.a_offset;
+    g.b_off = d.b_offset;
+    g.lda = d.lda;
+  } else {
+    g.n = n_all;
+    g.a_off = (int64_t)blockIdx.x * a_stride;
+    g.b_off = (int64_t)blockIdx.x * b_stride;
+    g.lda = n_all;
+  }
+  return g;
+}
+
+__global__ __launch_bounds__(kThreads) void chol_blocked_factor_kernel(const lxa_qr_desc* __restrict__ descs, int n_all,
+                                                                       int64_t a_stride, float* __restrict__ a,
+                                                                       int32_t* __restrict__ info) {
+  extern __shared__ __align__(16) float cbs[];
+  float* D = cbs;                      // [32][36]
+  float* P = cbs + kP * kDLd;          // [32][kPanelLd]
+  __shared__ int s_fail;
+  const Geometry g = geometry(descs, n_all, a_stride, 0);
+  const int n = g.n, lda = g.lda;
+  float* A = a + g.a_off;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) s_fail = 0;
+  __syncthreads();
+  if (n <= 0 || n > kMaxN) {
+    if (tid == 0) info[blockIdx.x] = n > kMaxN ? -1 : 0;
+    return;
+  }
+  for (int k0 = 0; k0 < n; k0 += kP) {
+    const int kb = min(kP, n - k0);
+    const int rest = n - k0 - kb;
+    for (int idx = tid; idx < kP * kP; idx += kThreads) {
+      const int i = idx / kP, j = idx % kP;
+      D[i * kDLd + j] =
+          (i < kb && j < kb) ? (j >= i ? A[(int64_t)(k0 + i) * lda + k0 + j] : 0.f) : (i == j ? 1.f : 0.f);
+    }
+    __syncthreads();
+    if (wv == 0) {
+      // right-looking on the upper triangle; lane j owns column j
+      int fail = 0;
+      for (int t = 0; t < kb; ++t) {
+        const float dtt = D[t * kDLd + t];
+        if (!(dtt > 0.f)) {
+          fail = k0 + t + 1;
+          break;
+        }
+        const float utt = sqrtf(dtt);
+        wave_lds_sync();
+        if (lane > t && lane < kP) D[t * kDLd + lane] /= utt;
+        if (lane == t) D[t * kDLd + t] = utt;
+        wave_lds_sync();
+        if (lane > t && lane < kP) {
+          const float utj = D[t * kDLd + lane];
+          for (int i = t + 1; i <= lane; ++i) D[i * kDLd + lane] -= D[t * kDLd + i] * utj;
+        }
+        wave_lds_sync();
+      }
+      if (lane == 0 && fail) s_fail = fail;
+    }
+    __syncthreads();
+    if (s_fail) break;
+    for (int idx = tid; idx < kb * kb; idx += kThreads) {
+      const int i = idx / kb, j = idx % kb;
+      if (j >= i) A[(int64_t)(k0 + i) * lda + k0 + j] = D[i * kDLd + j];
+    }
+    if (rest <= 0) break;
+    // (2) U12 = U11^-T A12: the row block is staged in the LDS panel
+    // (coalesced), swept in place one column per thread, and written back.
+    for (int idx = tid; idx < kP * rest; idx += kThreads) {
+      const int i = idx / rest, c = idx % rest;
+      P[i * kPanelLd + c] = i < kb ? A[(int64_t)(k0 + i) * lda + k0 + kb + c] : 0.f;
+    }
+    __syncthreads();
+    for (int c = tid; c < rest; c += kThreads) {
+      for (int t = 0; t < kb; ++t) {
+        const float xt = P[t * kPanelLd + c] / D[t * kDLd + t];
+        P[t * kPanelLd + c] = xt;
+        for (int i = t + 1; i < kb; ++i) P[i * kPanelLd + c] -= D[t * kDLd + i] * xt;
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kb * rest; idx += kThreads) {
+      const int i = idx / rest, c = idx % rest;
+      A[(int64_t)(k0 + i) * lda + k0 + kb + c] = P[i * kPanelLd + c];
+    }
+    // (3) A22 -= U12^T U12 on the upper triangle, 4 x 4 micro-tiles
+    const int T4 = (rest + 3) / 4;
+    for (int idx = tid; idx < T4 * T4; idx += kThreads) {
+      const int bi = idx / T4, bj = idx % T4;
+      if (bj < bi) continue;
+      float acc[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = 0.f;
+      for (int t = 0; t < kb; ++t) {
+        const float4 u = *reinterpret_cast<const float4*>(&P[t * kPanelLd + 4 * bi]);
+        const float4 v = *reinterpret_cast<const float4*>(&P[t * kPanelLd + 4 * bj]);
+        const float ur[4] = {u.x, u.y, u.z, u.w};
+        const float vc[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[r][c] += ur[r] * vc[c];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 4 * bi + r;
+        if (i >= rest) continue;
+        float* Ar = A + (int64_t)(k0 + kb + i) * lda + k0 + kb;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int j = 4 * bj + c;
+          if (j >= i && j < rest) Ar[j] -= acc[r][c];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  const int fail = s_fail;
+  if (fail) {
+    const float nan = quiet_nan<float>();
+    for (int64_t t = tid; t < (int64_t)n * n; t += kThreads) A[(t / n) * lda + (t % n)] = nan;
+  }
+  if (tid == 0) info[blockIdx.x] = fail;
+}
+
+// Solves U^T U x = b for one right-hand side per system (in place in b).
+__global__ __launch_bounds__(kThreads) void chol_blocked_solve_kernel(const lxa_qr_desc* __restrict__ descs, int n_all,
+                                                                      int64_t a_stride, int64_t b_stride,
+                                                                      const float* __restrict__ a,
+                                                                      float* __restrict__ b) {
+  __shared__ float y[kMaxN];
+  __shared__ float red[kThreads / kWave][kP];
+  __shared__ float part[8][kP];
+  const Geometry g = geometry(descs, n_all, a_stride, b_stride);
+  const int n = g.n, lda = g.lda;
+  if (n <= 0 || n > kMaxN) return;
+  const float* U = a + g.a_off;
+  float* B = b + g.b_off;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = tid; i < n; i += kThreads) y[i] = B[i];
+  __syncthreads();
+  const int nblk = (n + kP - 1) / kP;
+  // forward: U^T y = b
+  for (int q = 0; q < nblk; ++q) {
+    const int k0 = q * kP, kb = min(kP, n - k0);
+    {  // thread (column i = tid % 32, group gp = tid / 32) over rows t < k0
+      const int i = tid & (kP - 1), gp = tid >> 5;
+      float s = 0.f;
+      if (i < kb)
+        for (int t = gp; t < k0; t += 8) s += U[(int64_t)t * lda + k0 + i] * y[t];
+      part[gp][i] = s;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      float z = 0.f;
+      if (lane < kb) {
+        float s = 0.f;
+#pragma unroll
+        for (int gp = 0; gp < 8; ++gp) s += part[gp][lane];
+        z = y[k0 + lane] - s;
+      }
+      for (int t = 0; t < kb; ++t) {
+        if (lane == t) z = z / U[(int64_t)(k0 + t) * lda + k0 + t];
+        const float yt = __shfl(z, t, kWave);
+        if (lane > t && lane < kb) z -= U[(int64_t)(k0 + t) * lda + k0 + lane] * yt;
+      }
+      if (lane < kb) y[k0 + lane] = z;
+    }
+    __syncthreads();
+  }
+  // backward: U x = y
+  for (int q = nblk - 1; q >= 0; --q) {
+    const int k0 = q * kP, kb = min(kP, n - k0), k1 = k0 + kb;
+    for (int i = wv; i < kb; i += kThreads / kWave) {
+      float s = 0.f;
+      const float* Ur = U + (int64_t)(k0 + i) * lda;
+      for (int t = k1 + lane; t < n; t += kWave) s += Ur[t] * y[t];
+      s = wave_sum(s);
+      if (lane == 0) red[0][i] = s;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      float z = lane < kb ? y[k0 + lane] - red[0][lane] : 0.f;
+      for (int t = kb - 1; t >= 0; --t) {
+        if (lane == t) z = z / U[(int64_t)(k0 + t) * lda + k0 + t];
+        const float xt = __shfl(z, t, kWave);
+        if (lane < t) z -= U[(int64_t)(k0 + lane) * lda + k0 + t] * xt;
+      }
+      if (lane < kb) y[k0 + lane] = z;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += kThreads) B[i] = y[i];
+}
+
+size_t factor_smem() { return sizeof(float) * (kP * kDLd + kP * kPanelLd); }
+
+}  // namespace
+
+int chol_blocked_factor(hipStream_t st, int64_t count, const lxa_qr_desc* descs, int n_all, int64_t a_stride, float* a,
+                        int32_t* info) {
+  if (count <= 0) return LXA_OK;
+  const size_t smem = factor_smem();
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&chol_blocked_factor_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(chol_blocked_factor_kernel)");
+  chol_blocked_factor_kernel<<<(unsigned)count, kThreads, smem, st>>>(descs, n_all, a_stride, a, info);
+  LXA_LAUNCH_CHECK("chol_blocked_factor_kernel");
+  return LXA_OK;
+}
+
+int chol_blocked_solve(hipStream_t st, int64_t count, const lxa_qr_desc* descs, int n_all, int64_t a_stride,
+                       int64_t b_stride, const float* a, float* b) {
+  if (count <= 0) return LXA_OK;
+  chol_blocked_solve_kernel<<<(unsigned)count, kThreads, 0, st>>>(descs, n_all, a_stride, b_stride, a, b);
+  LXA_LAUNCH_CHECK("chol_blocked_solve_kernel");
+  return LXA_OK;
+}
+
+}  // namespace lxa
+
+extern "C" int lxa_chol_blocked_factor(void* stream, int64_t count, const lxa_qr_desc* desc, float* a,
+                                       int32_t* info) {
+  LXA_REQUIRE(count >= 0 && count <= 0x7fffffff, "count out of range");
+  if (count == 0) return LXA_OK;
+  LXA_REQUIRE(desc && a && info, "desc, a and info must be non-NULL");
+  return lxa::chol_blocked_factor(static_cast<hipStream_t>(stream), count, desc, 0, 0, a, info);
+}
+
+extern "C" int lxa_chol_blocked_solve(void* stream, int64_t count, const lxa_qr_desc* desc, const float* u,
+                                      float* b) {
+  LXA_REQUIRE(count >= 0 && count <= 0x7fffffff, "count out of range");
+  if (count == 0) return LXA_OK;
+  LXA_REQUIRE(desc && u && b, "desc, u and b must be non-NULL");
+  return lxa::chol_blocked_solve(static_cast<hipStream_t>(stream), count, desc, 0, 0, 0, u, b);
+}

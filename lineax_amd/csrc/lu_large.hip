@@ -8,16 +8,18 @@
 // butterfly transform (Parker, "Random butterfly transformations with
 // applications in computational linear algebra", 1995; Baboulin et al.,
 // "Accelerating linear system solutions using randomization techniques",
-// 2013), A' = U^T A V, factors A' = L U without pivoting, and recovers
+// 2013), A' = U^T A V, factors P A' = L U with pivoting confined to each
+// 128 x 128 diagonal block (no device-wide exchange), and recovers
 // accuracy by refinement against the original A (the host drives GMRES on
 // A preconditioned by these factors).  The solution is the same as with
 // partial pivoting to the refinement tolerance; the factors differ.
 //
 // Factorisation (blocked right-looking, two levels):
 //   outer panels of 512 columns: the panel is factored by 128-column inner
-//   steps, each  (1) LU of the 128 x 128 diagonal block in LDS, with the
-//   inverses of its unit-lower and upper triangles (kept for the block
-//   triangular solves), (2) L21 = A21 U11^-1 and the panel's U12 = L11^-1 A12
+//   steps, each  (1) LU with in-block partial pivoting of the 128 x 128
+//   diagonal block in LDS, with the inverses of its unit-lower and upper
+//   triangles (kept for the block triangular solves), and the block's row
+//   swaps applied across the matrix, (2) L21 = A21 U11^-1 and U12 = L11^-1 A12
 //   as float32 matrix-core products, (3) the panel's Schur update in float32.
 //   Then the outer U12 row block (128 rows at a time) and the outer Schur
 //   update A22 -= L21 U12, whose products run in bfloat16 or float16 (or
@@ -95,15 +97,22 @@ __global__ void rbt_vec_kernel(int n, int s, int nrhs, T* __restrict__ x, int64_
 }
 
 // ---------------------------------------------------------------------------
-// LU without pivoting of the nb x nb diagonal block at (k, k) (nb <= 128),
-// in place, and the inverses of its unit-lower and upper triangles written
-// as dense 128 x 128 blocks (zeros outside the triangle).  info[0] is set to
-// the 1-based global index of the first exactly-zero pivot.
+// LU with partial pivoting restricted to the nb x nb diagonal block at (k, k)
+// (nb <= 128), in place, and the inverses of its unit-lower and upper
+// triangles written as dense 128 x 128 blocks (zeros outside the triangle).
+// Pivoting inside the block keeps its multipliers <= 1 in magnitude, so the
+// block's L is well conditioned and its explicit inverse accurate; the
+// butterflies keep the growth across blocks in check.  piv[k + j] receives
+// the 0-based global row swapped with row k + j (always inside the block);
+// the caller applies the swaps to the rest of the block's rows.  info[0] is
+// set to 1 + the global index of the first exactly-zero pivot.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, float* __restrict__ a, int64_t lda,
                                                                float* __restrict__ linv, float* __restrict__ uinv,
-                                                               int32_t* __restrict__ info) {
+                                                               int32_t* __restrict__ piv, int32_t* __restrict__ info) {
   extern __shared__ __align__(16) float dsm[];
+  __shared__ float s_pv[2];
+  __shared__ int s_pi[2];
   float* S = dsm;                      // [128][129]
   float* X = dsm + kB * (kB + 1);      // [128][129]
   constexpr int LD = kB + 1;
@@ -115,14 +124,45 @@ __global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, fl
   __syncthreads();
   // right-looking LU, one column per step
   for (int j = 0; j < kB; ++j) {
-    const float piv = S[j * LD + j];
-    if (tid == 0 && piv == 0.f && j < nb && info[0] == 0) info[0] = k + j + 1;
+    // arg-max of |S[r][j]| over r >= j (NaN counts as largest; ties: lowest row)
+    if (tid < kB) {
+      float v = -1.f;
+      if (tid >= j) {
+        v = fabsf(S[tid * LD + j]);
+        if (v != v) v = INFINITY;
+      }
+      int idx = tid;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_xor(v, off, kWave);
+        const int oi = __shfl_xor(idx, off, kWave);
+        if (ov > v || (ov == v && oi < idx)) {
+          v = ov;
+          idx = oi;
+        }
+      }
+      if ((tid & 63) == 0) {
+        s_pv[tid >> 6] = v;
+        s_pi[tid >> 6] = idx;
+      }
+    }
+    __syncthreads();
+    const int p = (s_pv[1] > s_pv[0] || (s_pv[1] == s_pv[0] && s_pi[1] < s_pi[0])) ? s_pi[1] : s_pi[0];
+    if (p != j && tid < kB) {
+      const float t = S[j * LD + tid];
+      S[j * LD + tid] = S[p * LD + tid];
+      S[p * LD + tid] = t;
+    }
+    if (tid == 0 && j < nb) piv[k + j] = k + p;
+    __syncthreads();
+    const float pv = S[j * LD + j];
+    if (tid == 0 && pv == 0.f && j < nb && info[0] == 0) info[0] = k + j + 1;
     const int rows = kB - 1 - j;
     // multipliers then rank-1 update; thread (r, c) over the trailing block
     for (int idx = tid; idx < rows * (rows + 1); idx += kDiagThreads) {
       const int r = j + 1 + idx / (rows + 1);
       const int c = j + idx % (rows + 1);  // c == j: the multiplier itself
-      const float l = S[r * LD + j] / piv;
+      const float l = S[r * LD + j] / pv;
       if (c == j)
         X[r * LD + j] = l;  // staged: the column is overwritten after the barrier
       else
@@ -177,6 +217,61 @@ __global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, fl
   for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
     const int i = idx >> 7, j = idx & 127;
     uinv[idx] = X[j * LD + i];
+  }
+}
+
+// The row swaps of diagonal block k (rows k .. k + nb - 1, in order) applied
+// to every column outside the block: one thread per column, the block's rows
+// staged in LDS.
+constexpr int kSwapCols = 256;
+
+__global__ __launch_bounds__(kSwapCols) void block_row_swaps_kernel(int n, int k, int nb, float* __restrict__ a,
+                                                                    int64_t lda, const int32_t* __restrict__ piv) {
+  extern __shared__ __align__(16) float col[];  // [nb][kSwapCols]
+  const int t = threadIdx.x;
+  const int c = blockIdx.x * kSwapCols + t;
+  const bool live = c < n && (c < k || c >= k + nb);
+  if (live)
+    for (int r = 0; r < nb; ++r) col[r * kSwapCols + t] = a[(int64_t)(k + r) * lda + c];
+  if (!live) return;
+  for (int j = 0; j < nb; ++j) {
+    const int p = piv[k + j] - k;
+    if (p != j) {
+      const float v = col[j * kSwapCols + t];
+      col[j * kSwapCols + t] = col[p * kSwapCols + t];
+      col[p * kSwapCols + t] = v;
+    }
+  }
+  for (int r = 0; r < nb; ++r) a[(int64_t)(k + r) * lda + c] = col[r * kSwapCols + t];
+}
+
+// The same swaps on right-hand sides x[r * ldx + i]: forward order (P x) or
+// reverse order (P^T x).  One thread per (vector, 128-row block).
+__global__ void rhs_swaps_kernel(int n, int nrhs, float* __restrict__ x, int64_t ldx,
+                                 const int32_t* __restrict__ piv, int reverse) {
+  const int nblk = (n + kB - 1) / kB;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nblk * nrhs) return;
+  float* v = x + (int64_t)(q / nblk) * ldx;
+  const int j0 = (q % nblk) * kB, j1 = min(n, j0 + kB);
+  if (!reverse) {
+    for (int j = j0; j < j1; ++j) {
+      const int p = piv[j];
+      if (p != j) {
+        const float t = v[j];
+        v[j] = v[p];
+        v[p] = t;
+      }
+    }
+  } else {
+    for (int j = j1 - 1; j >= j0; --j) {
+      const int p = piv[j];
+      if (p != j) {
+        const float t = v[j];
+        v[j] = v[p];
+        v[p] = t;
+      }
+    }
   }
 }
 
@@ -277,11 +372,11 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
     for (int rr = 0; rr < kMaxRhs; ++rr)
       if (rr < nrhs) x[(int64_t)rr * ldx + r0 + row] = part[rr];
   }
+  // Every wave makes its own stores visible device-wide before the flag is
+  // raised (a fence orders only the issuing wave's memory operations).
+  __threadfence();
   __syncthreads();
-  if (tid == 0) {
-    __threadfence();
-    __hip_atomic_store(&flags[bi], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (tid == 0) __hip_atomic_store(&flags[bi], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -370,19 +465,25 @@ int gemv_f32a_launch(hipStream_t st, int m, int k, int nrhs, const float* a, int
 }
 
 int lu_large_factor_impl(hipStream_t st, int mode, int n, float* a, int64_t lda, float* linv, float* uinv,
-                         int32_t* info) {
+                         int32_t* piv, int32_t* info) {
   const size_t dsmem = sizeof(float) * 2 * kB * (kB + 1);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&diag_lu_kernel),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dsmem);
   if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(diag_lu_kernel)");
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_row_swaps_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float) * kB * kSwapCols));
+  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(block_row_swaps_kernel)");
   for (int K0 = 0; K0 < n; K0 += kOuter) {
     const int K1 = min(n, K0 + kOuter);
     for (int k = K0; k < K1; k += kB) {
       const int k2 = min(K1, k + kB), nb = k2 - k;
       float* Li = linv + (int64_t)(k / kB) * kB * kB;
       float* Ui = uinv + (int64_t)(k / kB) * kB * kB;
-      diag_lu_kernel<<<1, kDiagThreads, dsmem, st>>>(k, nb, a, lda, Li, Ui, info);
+      diag_lu_kernel<<<1, kDiagThreads, dsmem, st>>>(k, nb, a, lda, Li, Ui, piv, info);
       LXA_LAUNCH_CHECK("diag_lu_kernel");
+      block_row_swaps_kernel<<<(n + kSwapCols - 1) / kSwapCols, kSwapCols, sizeof(float) * nb * kSwapCols, st>>>(
+          n, k, nb, a, lda, piv);
+      LXA_LAUNCH_CHECK("block_row_swaps_kernel");
       float* A11 = a + (int64_t)k * lda + k;
       // L21 = A21 U11^-1 (in place, N = nb <= 128)
       if (n - k2 > 0) {
@@ -431,13 +532,25 @@ using namespace lxa;
 extern "C" int lxa_lu_large_block(void) { return kB; }
 
 extern "C" int lxa_lu_large_factor(void* stream, int mode, int64_t n, float* a, int64_t lda, float* linv, float* uinv,
-                                   int32_t* info) {
+                                   int32_t* piv, int32_t* info) {
   LXA_REQUIRE(n >= 0 && n <= (1 << 20), "n out of range (%lld)", (long long)n);
   LXA_REQUIRE(lda >= n, "lda (%lld) < n (%lld)", (long long)lda, (long long)n);
   LXA_REQUIRE(mode == LXA_GEMM_F32 || mode == LXA_GEMM_BF16 || mode == LXA_GEMM_F16, "unknown mode %d", mode);
   if (n == 0) return LXA_OK;
-  LXA_REQUIRE(a && linv && uinv && info, "a, linv, uinv and info must be non-NULL");
-  return lu_large_factor_impl(static_cast<hipStream_t>(stream), mode, (int)n, a, lda, linv, uinv, info);
+  LXA_REQUIRE(a && linv && uinv && piv && info, "a, linv, uinv, piv and info must be non-NULL");
+  return lu_large_factor_impl(static_cast<hipStream_t>(stream), mode, (int)n, a, lda, linv, uinv, piv, info);
+}
+
+extern "C" int lxa_lu_large_swaps(void* stream, int64_t n, int nrhs, float* x, int64_t ldx, const int32_t* piv,
+                                  int reverse) {
+  LXA_REQUIRE(n >= 0 && n <= (1 << 20) && nrhs >= 0 && ldx >= n, "invalid n / nrhs / ldx");
+  if (n == 0 || nrhs == 0) return LXA_OK;
+  LXA_REQUIRE(x && piv, "x and piv must be non-NULL");
+  const int64_t threads = ((n + kB - 1) / kB) * nrhs;
+  rhs_swaps_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      (int)n, nrhs, x, ldx, piv, reverse ? 1 : 0);
+  LXA_LAUNCH_CHECK("rhs_swaps_kernel");
+  return LXA_OK;
 }
 
 extern "C" int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, const float* lu, int64_t lda,

@@ -117,7 +117,8 @@ __device__ __forceinline__ T vpick(int row, int diag, T stored) {
 // RPT == 0: rows in global memory (any height, slower).
 // ---------------------------------------------------------------------------
 template <typename T, int RPT>
-__global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_desc* __restrict__ descs, int k,
+__global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_desc* __restrict__ descs, int k_all,
+                                                                 const int32_t* __restrict__ kcol,
                                                                  T* __restrict__ a, T* __restrict__ tau,
                                                                  T* __restrict__ tblk) {
   constexpr int NB = QrCfg<T>::NB;
@@ -131,6 +132,7 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_de
   const lxa_qr_desc d = descs[blockIdx.x];
   const int m = d.m, n = d.n, lda = d.lda;
   const int kmax = min(m, n);
+  const int k = kcol ? kcol[blockIdx.x] : k_all;
   if (k >= kmax) return;
   const int nb = min(NB, kmax - k);
   const int tid = threadIdx.x;
@@ -287,7 +289,8 @@ __device__ __forceinline__ void load_v_chunk(T (*sv)[NB], const T* A, int lda, i
 
 // (1) partial W = V^T C over one 256-row chunk and one 64-column block.
 template <typename T>
-__global__ __launch_bounds__(kTrailThreads) void qr_trail_partial_kernel(const lxa_qr_desc* __restrict__ descs, int k,
+__global__ __launch_bounds__(kTrailThreads) void qr_trail_partial_kernel(const lxa_qr_desc* __restrict__ descs,
+                                                                        int k_all, const int32_t* __restrict__ kcol,
                                                                         const T* __restrict__ a, T* __restrict__ work) {
   constexpr int NB = QrCfg<T>::NB;
   __shared__ T sv[kTrailRows][NB];
@@ -295,6 +298,7 @@ __global__ __launch_bounds__(kTrailThreads) void qr_trail_partial_kernel(const l
   const lxa_qr_desc d = descs[blockIdx.z];
   const int m = d.m, n = d.n, lda = d.lda;
   const int kmax = min(m, n);
+  const int k = kcol ? kcol[blockIdx.z] : k_all;
   if (k >= kmax) return;
   const int nb = min(NB, kmax - k);
   const int c0 = k + nb + kTrailCols * (int)blockIdx.x;
@@ -330,7 +334,8 @@ __global__ __launch_bounds__(kTrailThreads) void qr_trail_partial_kernel(const l
 
 // (2) W = sum over chunks, then W <- T^T W; stored in chunk slot 0.
 template <typename T>
-__global__ __launch_bounds__(kTrailThreads) void qr_trail_reduce_kernel(const lxa_qr_desc* __restrict__ descs, int k,
+__global__ __launch_bounds__(kTrailThreads) void qr_trail_reduce_kernel(const lxa_qr_desc* __restrict__ descs,
+                                                                       int k_all, const int32_t* __restrict__ kcol,
                                                                        const T* __restrict__ tblk,
                                                                        T* __restrict__ work) {
   constexpr int NB = QrCfg<T>::NB;
@@ -339,6 +344,7 @@ __global__ __launch_bounds__(kTrailThreads) void qr_trail_reduce_kernel(const lx
   const lxa_qr_desc d = descs[blockIdx.y];
   const int m = d.m, n = d.n;
   const int kmax = min(m, n);
+  const int k = kcol ? kcol[blockIdx.y] : k_all;
   if (k >= kmax) return;
   const int nb = min(NB, kmax - k);
   const int c0 = k + nb + kTrailCols * (int)blockIdx.x;
@@ -367,7 +373,8 @@ __global__ __launch_bounds__(kTrailThreads) void qr_trail_reduce_kernel(const lx
 
 // (3) C -= V W over one chunk and column block.
 template <typename T>
-__global__ __launch_bounds__(kTrailThreads) void qr_trail_update_kernel(const lxa_qr_desc* __restrict__ descs, int k,
+__global__ __launch_bounds__(kTrailThreads) void qr_trail_update_kernel(const lxa_qr_desc* __restrict__ descs,
+                                                                       int k_all, const int32_t* __restrict__ kcol,
                                                                        T* __restrict__ a,
                                                                        const T* __restrict__ work) {
   constexpr int NB = QrCfg<T>::NB;
@@ -375,6 +382,7 @@ __global__ __launch_bounds__(kTrailThreads) void qr_trail_update_kernel(const lx
   const lxa_qr_desc d = descs[blockIdx.z];
   const int m = d.m, n = d.n, lda = d.lda;
   const int kmax = min(m, n);
+  const int k = kcol ? kcol[blockIdx.z] : k_all;
   if (k >= kmax) return;
   const int nb = min(NB, kmax - k);
   const int c0 = k + nb + kTrailCols * (int)blockIdx.x;
@@ -561,35 +569,43 @@ size_t trsv_smem_bytes(int dtype) {
   return e * (((kTrsvMaxK + 1) & ~1) + kTrsvBlock * (kTrsvBlock + 1) + kBigWaves * kWave);
 }
 
+// One panel step for every system: column k_all for all (kcol == nullptr)
+// or kcol[s] for system s.  max_rows bounds m - k and max_trail_cols bounds
+// n - k - 1 over the systems that still have columns.
+template <typename T>
+int qr_step_impl(hipStream_t st, int64_t count, const lxa_qr_desc* desc, int k_all, const int32_t* kcol,
+                 int max_rows, int max_trail_cols, T* a, T* tau, T* tblk, T* work) {
+  const int rpt = (max_rows + kPanelThreads - 1) / kPanelThreads;
+  const dim3 pg((unsigned)count);
+  if (rpt <= 1)
+    qr_panel_kernel<T, 1><<<pg, kPanelThreads, 0, st>>>(desc, k_all, kcol, a, tau, tblk);
+  else if (rpt <= 2)
+    qr_panel_kernel<T, 2><<<pg, kPanelThreads, 0, st>>>(desc, k_all, kcol, a, tau, tblk);
+  else if (rpt <= 4)
+    qr_panel_kernel<T, 4><<<pg, kPanelThreads, 0, st>>>(desc, k_all, kcol, a, tau, tblk);
+  else if (rpt <= kPanelMaxRpt)
+    qr_panel_kernel<T, kPanelMaxRpt><<<pg, kPanelThreads, 0, st>>>(desc, k_all, kcol, a, tau, tblk);
+  else
+    qr_panel_kernel<T, 0><<<pg, kPanelThreads, 0, st>>>(desc, k_all, kcol, a, tau, tblk);
+  LXA_LAUNCH_CHECK("qr_panel_kernel");
+  if (max_trail_cols <= 0) return LXA_OK;
+  const unsigned cb = (unsigned)((max_trail_cols + kTrailCols - 1) / kTrailCols);
+  const unsigned rc = (unsigned)((max_rows + kTrailRows - 1) / kTrailRows);
+  qr_trail_partial_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, a, work);
+  LXA_LAUNCH_CHECK("qr_trail_partial_kernel");
+  qr_trail_reduce_kernel<T><<<dim3(cb, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, tblk, work);
+  LXA_LAUNCH_CHECK("qr_trail_reduce_kernel");
+  qr_trail_update_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, a, work);
+  LXA_LAUNCH_CHECK("qr_trail_update_kernel");
+  return LXA_OK;
+}
+
 template <typename T>
 int qr_factor_impl(hipStream_t st, int64_t count, const lxa_qr_desc* desc, int max_m, int max_n, int max_k, T* a,
                    T* tau, T* tblk, T* work) {
   constexpr int NB = QrCfg<T>::NB;
   for (int k = 0; k < max_k; k += NB) {
-    const int rows = max_m - k;
-    const int rpt = (rows + kPanelThreads - 1) / kPanelThreads;
-    const dim3 pg((unsigned)count);
-    if (rpt <= 1)
-      qr_panel_kernel<T, 1><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
-    else if (rpt <= 2)
-      qr_panel_kernel<T, 2><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
-    else if (rpt <= 4)
-      qr_panel_kernel<T, 4><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
-    else if (rpt <= kPanelMaxRpt)
-      qr_panel_kernel<T, kPanelMaxRpt><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
-    else
-      qr_panel_kernel<T, 0><<<pg, kPanelThreads, 0, st>>>(desc, k, a, tau, tblk);
-    LXA_LAUNCH_CHECK("qr_panel_kernel");
-    const int trail_cols = max_n - k - 1;  // >= trailing width of every system that has one
-    if (trail_cols <= 0) continue;
-    const unsigned cb = (unsigned)((trail_cols + kTrailCols - 1) / kTrailCols);
-    const unsigned rc = (unsigned)((rows + kTrailRows - 1) / kTrailRows);
-    qr_trail_partial_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k, a, work);
-    LXA_LAUNCH_CHECK("qr_trail_partial_kernel");
-    qr_trail_reduce_kernel<T><<<dim3(cb, (unsigned)count), kTrailThreads, 0, st>>>(desc, k, tblk, work);
-    LXA_LAUNCH_CHECK("qr_trail_reduce_kernel");
-    qr_trail_update_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k, a, work);
-    LXA_LAUNCH_CHECK("qr_trail_update_kernel");
+    if (int rc = qr_step_impl<T>(st, count, desc, k, nullptr, max_m - k, max_n - k - 1, a, tau, tblk, work)) return rc;
   }
   return LXA_OK;
 }
@@ -629,6 +645,22 @@ extern "C" int lxa_qr_factor(void* stream, int dtype, int64_t count, const lxa_q
                                  static_cast<float*>(tau), static_cast<float*>(tblk), static_cast<float*>(work));
   return qr_factor_impl<double>(st, count, desc, max_m, max_n, max_k, static_cast<double*>(a),
                                 static_cast<double*>(tau), static_cast<double*>(tblk), static_cast<double*>(work));
+}
+
+extern "C" int lxa_qr_factor_step(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc,
+                                  const int32_t* kcol, int max_rows, int max_trail_cols, void* a, void* tau,
+                                  void* tblk, void* work) {
+  if (int rc = check_batch(dtype, count, desc)) return rc;
+  LXA_REQUIRE(kcol != nullptr, "kcol must be non-NULL");
+  LXA_REQUIRE(max_rows >= 0, "max_rows must be >= 0");
+  if (count == 0 || max_rows == 0) return LXA_OK;
+  LXA_REQUIRE(a && tau && tblk && work, "a, tau, tblk and work must be non-NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == LXA_F32)
+    return qr_step_impl<float>(st, count, desc, 0, kcol, max_rows, max_trail_cols, static_cast<float*>(a),
+                               static_cast<float*>(tau), static_cast<float*>(tblk), static_cast<float*>(work));
+  return qr_step_impl<double>(st, count, desc, 0, kcol, max_rows, max_trail_cols, static_cast<double*>(a),
+                              static_cast<double*>(tau), static_cast<double*>(tblk), static_cast<double*>(work));
 }
 
 extern "C" int lxa_qr_apply(void* stream, int dtype, int64_t count, const lxa_qr_desc* desc, int max_m, int max_k,

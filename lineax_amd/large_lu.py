@@ -1,0 +1,182 @@
+"""Large dense LU: butterfly-randomised, unpivoted, refined (float32 data).
+
+``LargeLU(a)`` factors one float32 matrix of order n (thousands):
+
+1. A' = U^T A V with two-sided depth-2 random butterflies (seeded), padded
+   to a multiple of 4 with an identity block;
+2. P A' = L U with pivoting confined to each 128 x 128 diagonal block,
+   blocked: float32 panels and triangular block products, Schur-complement
+   products in bfloat16 / float16 / float32 on the matrix cores with float32
+   accumulation (``precision``);
+3. ``solve`` refines against the original A: FP64 residuals r = b - A x
+   (float32 A, FP64 accumulation), each correction from GMRES on A
+   preconditioned by x -> V (L U)^-1 P U^T x (float32 basis, FP64 inner
+   products), until the normwise backward error
+   eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol or ``max_refine``
+   residuals.
+
+The answer is that of lineax's LU (lineax/_solver/lu.py: getrf with partial
+pivoting, then getrs) to the refinement tolerance; the factors are not
+getrf's.  Pivoting inside diagonal blocks of the randomised matrix replaces
+one device-wide arg-max per column with a workgroup-local one.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _kernels as K
+from ._operator import FunctionLinearOperator
+from ._structure import TensorStruct
+
+MAX_RHS_PER_SOLVE = 8
+
+
+class _F32Matrix(FunctionLinearOperator):
+    """x -> A x for a float32 matrix, in the dtype asked for (FP64 accumulation)."""
+
+    def __init__(self, a: torch.Tensor, dtype: torch.dtype):
+        self.a = a
+        n = a.shape[0]
+        struct = TensorStruct((n,), dtype)
+        super().__init__(self._apply, struct, (), output_structure=struct, device=a.device)
+
+    def _apply(self, x):
+        return self.mv_as(x.reshape(-1, self.a.shape[1]), x.dtype).reshape(x.shape)
+
+    def mv_as(self, x: torch.Tensor, out_dtype: torch.dtype) -> torch.Tensor:
+        return K.gemv_f32a(self.a, x.reshape(-1, self.a.shape[1]), out_dtype).reshape(*x.shape[:-1], self.a.shape[0])
+
+
+class _Preconditioner(FunctionLinearOperator):
+    """x -> V (L U)^-1 U^T x (or its transpose), float32 arithmetic."""
+
+    def __init__(self, fac: "LargeLU", dtype: torch.dtype, trans: bool):
+        self.fac = fac
+        self.trans = trans
+        struct = TensorStruct((fac.n,), dtype)
+        super().__init__(self._apply, struct, (), output_structure=struct, device=fac.lu.device)
+
+    def _apply(self, x):
+        return self.mv_as(x.reshape(-1, self.fac.n), x.dtype).reshape(x.shape)
+
+    def mv_as(self, x: torch.Tensor, out_dtype: torch.dtype) -> torch.Tensor:
+        y = self.fac.apply_inverse(x.reshape(-1, self.fac.n).to(torch.float32), trans=self.trans)
+        return y.to(out_dtype).reshape(*x.shape[:-1], self.fac.n)
+
+
+class LargeLU:
+    """Factorisation of one large float32 matrix (see the module docstring)."""
+
+    def __init__(self, a: torch.Tensor, precision: str = "bf16", seed: int = 0):
+        K._gpu(a, "matrix")
+        if a.dim() != 2 or a.shape[0] != a.shape[1]:
+            raise ValueError(f"LargeLU needs one square matrix, got {tuple(a.shape)}")
+        if a.dtype != torch.float32:
+            raise TypeError("LargeLU factors float32 matrices")
+        if precision not in K.GEMM_MODES:
+            raise ValueError(f"precision must be one of {sorted(K.GEMM_MODES)}")
+        self.precision = precision
+        self.a = a.detach().contiguous()
+        self.n = n = a.shape[0]
+        self.np = np_ = -(-n // 4) * 4
+        dev = a.device
+        lu = torch.zeros((np_, np_), dtype=torch.float32, device=dev)
+        lu[:n, :n] = self.a
+        if np_ > n:
+            idx = torch.arange(n, np_, device=dev)
+            lu[idx, idx] = 1.0
+        gen = torch.Generator(device="cpu").manual_seed(int(seed))
+        # butterfly diagonals exp(r / 10), r ~ U(-1/2, 1/2) (Parker 1995)
+        self.du = torch.exp((torch.rand(2 * np_, generator=gen, dtype=torch.float64) - 0.5) / 10).float().to(dev)
+        self.dv = torch.exp((torch.rand(2 * np_, generator=gen, dtype=torch.float64) - 0.5) / 10).float().to(dev)
+        K.rbt_matrix(lu, self.du, self.dv)
+        blk = K.lu_large_block()
+        self.nblk = -(-np_ // blk)
+        self.linv = torch.empty(self.nblk * blk * blk, dtype=torch.float32, device=dev)
+        self.uinv = torch.empty_like(self.linv)
+        self.info = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.piv = torch.empty(np_, dtype=torch.int32, device=dev)
+        K.lu_large_factor(lu, precision, self.linv, self.uinv, self.piv, self.info)
+        self.lu = lu
+        self.sync = torch.empty(self.nblk + 2, dtype=torch.int32, device=dev)
+        self.norm_inf = self.a.abs().sum(dim=1, dtype=torch.float64).max()
+
+    def apply_inverse(self, v: torch.Tensor, trans: bool = False) -> torch.Tensor:
+        """A^-1 v through the factors: V (LU)^-1 P U^T v, or U P^T (LU)^-T V^T v."""
+        rows = v.shape[0]
+        w = torch.zeros((rows, self.np), dtype=torch.float32, device=v.device)
+        w[:, : self.n] = v
+        first, last = (self.dv, self.du) if trans else (self.du, self.dv)
+        K.rbt_vectors(w, first, form=1)
+        if not trans:
+            K.lu_large_swaps(w, self.piv, reverse=False)
+        for r0 in range(0, rows, MAX_RHS_PER_SOLVE):
+            chunk = w[r0: r0 + MAX_RHS_PER_SOLVE]
+            if trans:
+                K.lu_large_trsv(2, self.lu, self.uinv, chunk, self.sync)
+                K.lu_large_trsv(3, self.lu, self.linv, chunk, self.sync)
+            else:
+                K.lu_large_trsv(0, self.lu, self.linv, chunk, self.sync)
+                K.lu_large_trsv(1, self.lu, self.uinv, chunk, self.sync)
+        if trans:
+            K.lu_large_swaps(w, self.piv, reverse=True)
+        K.rbt_vectors(w, last, form=0)
+        return w[:, : self.n]
+
+    def residual(self, x: torch.Tensor, b: torch.Tensor, trans: bool = False) -> torch.Tensor:
+        """b - A x (or b - A^T x) in FP64."""
+        a = self.a.T.contiguous() if trans else self.a
+        return K.gemv_f32a(a, x, torch.float64, b=b)
+
+    def backward_error(self, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor, trans: bool = False):
+        norm_a = self.a.abs().sum(dim=0, dtype=torch.float64).max() if trans else self.norm_inf
+        num = r.abs().amax(dim=1)
+        den = norm_a * x.abs().amax(dim=1) + b.abs().amax(dim=1)
+        return torch.where(num == 0, torch.zeros_like(num), num / den)
+
+    def solve(self, b: torch.Tensor, *, tol: float = 1e-12, max_refine: int = 10, trans: bool = False,
+              inner_rtol: float = 1e-4, restart: int = 30, inner_max_steps: int = 300):
+        """Refined solution x (float64, [nrhs, n]) of A x = b (A^T x = b if trans).
+
+        Returns (x, eta, stats) with eta the last backward error per
+        right-hand side; stats counts residuals and GMRES steps.
+        """
+        from . import _solve, _solver
+
+        b64 = b.reshape(-1, self.n).to(torch.float64).contiguous()
+        x = self.apply_inverse(b64.to(torch.float32), trans=trans).to(torch.float64)
+        op = _F32Matrix(self.a.T.contiguous() if trans else self.a, torch.float64)
+        prec = _Preconditioner(self, torch.float64, trans)
+        residuals = 0
+        gmres_steps = 0
+        eta = None
+        for _ in range(max(1, max_refine)):
+            r = self.residual(x, b64, trans)
+            residuals += 1
+            eta = self.backward_error(x, b64, r, trans)
+            # stop when converged, out of residuals, or non-finite (a zero
+            # pivot: the output stays non-finite, reported as singular)
+            if bool((eta <= tol).all()) or residuals >= max_refine or not bool(torch.isfinite(eta).all()):
+                break
+            d0 = prec.mv_as(r, torch.float64)
+            atol = inner_rtol * float(d0.abs().max())
+            sol = _solve.linear_solve(
+                op, r, _solver.GMRES(rtol=inner_rtol, atol=atol, restart=restart, max_steps=inner_max_steps,
+                                     working_dtype=torch.float32),
+                options={"preconditioner": prec, "y0": d0}, throw=False,
+            )
+            gmres_steps += int(sol.stats["num_steps"].max())
+            x = x + sol.value
+        return x, eta, {"residuals": residuals, "gmres_steps": gmres_steps, "zero_pivot": int(self.info.item())}
+
+
+def factor_flops(n: int) -> float:
+    """Algorithmic flops of an LU of order n with its solves of 4 right-hand sides (2n^3/3 + 8n^2)."""
+    return 2.0 * n ** 3 / 3.0 + 8.0 * n ** 2
+
+
+def padded_order(n: int) -> int:
+    return int(math.ceil(n / 4) * 4)

@@ -1,0 +1,447 @@
+"""A request loop that solves a stream of dense least-squares (QR) and SPD
+(Cholesky) systems on one GPU with continuous batching.
+
+Requests arrive over time (``Request.arrival``, seconds after the start).
+The chosen form (``SolveServer(form="chosen")``) runs in ticks:
+
+1. admission: arrived requests are admitted first-come first-served while
+   their working memory fits in the HBM arena (``hbm_budget`` bytes; the
+   operator, its factors, scratch and right-hand side are allocated there);
+2. one launch of ``lxa_qr_factor_step`` advances every admitted QR system by
+   one 16-column panel, each at its own column, so systems admitted in
+   different ticks share the launches (continuous batching);
+3. systems whose factorisation completed, and requests that reuse a resident
+   factorisation with a new right-hand side, are solved together (one
+   ``lxa_qr_apply`` + ``lxa_trsv`` launch pair);
+4. newly admitted Cholesky systems are factored and solved in one uneven
+   launch pair (``lxa_chol_blocked_factor`` / ``_solve``);
+5. the tick ends with one synchronisation; requests completed in it get that
+   completion time.  Finished factorisations stay resident (up to
+   ``resident_budget`` bytes, least recently used evicted) for reuse
+   requests.
+
+The plain form runs the same kernels one request at a time in arrival order
+(no batching, no reuse: a reuse request factors its operator again).
+
+Semantically each request is ``lx.linear_solve(lx.MatrixLinearOperator(A),
+b, lx.QR())`` (lineax/_solver/qr.py:55-95; least squares for m >= n) or
+``lx.linear_solve(lx.MatrixLinearOperator(A, positive_semidefinite_tag), b,
+lx.Cholesky())`` (lineax/_solver/cholesky.py:43-78), in float32.
+"""
+
+from __future__ import annotations
+
+import bisect
+import dataclasses
+import time
+
+import numpy as np
+import torch
+
+from . import _kernels as K
+
+QR_ELEM = 4  # float32 bytes
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    kind: str  # "qr" or "chol"
+    arrival: float
+    m: int
+    n: int
+    op_id: int  # operator identity; reuse requests share an earlier request's op_id
+    reuse: bool
+    a: torch.Tensor | None  # the operator (device, float32), None for reuse requests
+    b: torch.Tensor  # right-hand side (device, float32, length m)
+    # filled in by the server
+    x: torch.Tensor | None = None
+    admitted: float | None = None
+    done: float | None = None
+
+
+class Arena:
+    """First-fit allocator over one device buffer (offsets in float32 elements)."""
+
+    ALIGN = 64  # elements (256 bytes)
+
+    def __init__(self, elems: int, device):
+        self.size = int(elems)
+        self.buf = torch.empty(self.size, dtype=torch.float32, device=device)
+        self.free = [(0, self.size)]  # sorted (offset, length)
+        self.used = 0
+        self.high_water = 0
+
+    def alloc(self, elems: int) -> int | None:
+        need = max(self.ALIGN, -(-int(elems) // self.ALIGN) * self.ALIGN)
+        for i, (off, ln) in enumerate(self.free):
+            if ln >= need:
+                if ln == need:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (off + need, ln - need)
+                self.used += need
+                self.high_water = max(self.high_water, self.used)
+                return off
+        return None
+
+    def release(self, off: int, elems: int) -> None:
+        need = max(self.ALIGN, -(-int(elems) // self.ALIGN) * self.ALIGN)
+        self.used -= need
+        i = bisect.bisect_left(self.free, (off, 0))
+        self.free.insert(i, (off, need))
+        # coalesce with neighbours
+        if i + 1 < len(self.free) and self.free[i][0] + self.free[i][1] == self.free[i + 1][0]:
+            self.free[i] = (self.free[i][0], self.free[i][1] + self.free[i + 1][1])
+            self.free.pop(i + 1)
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] == self.free[i][0]:
+            self.free[i - 1] = (self.free[i - 1][0], self.free[i - 1][1] + self.free[i][1])
+            self.free.pop(i)
+
+    def view(self, off: int, elems: int) -> torch.Tensor:
+        return self.buf[off: off + elems]
+
+
+def qr_footprint(m: int, n: int, nb: int = 16) -> dict:
+    """Arena elements of one QR system: matrix, tau, T blocks, scratch, rhs."""
+    k = min(m, n)
+    return {"a": m * n, "tau": k, "t": -(-k // nb) * nb * nb, "w": -(-m // 256) * nb * n, "b": m}
+
+
+def chol_footprint(n: int) -> dict:
+    return {"a": n * n, "b": n}
+
+
+class _Resident:
+    """A finished factorisation kept in the arena for reuse requests."""
+
+    def __init__(self, kind, m, n, blocks):
+        self.kind, self.m, self.n, self.blocks = kind, m, n, blocks
+        self.last_use = 0
+
+
+class SolveServer:
+    def __init__(self, device, *, form: str = "chosen", hbm_budget: int = 8 << 30,
+                 resident_budget: int = 2 << 30):
+        if form not in ("chosen", "plain"):
+            raise ValueError("form must be 'chosen' or 'plain'")
+        self.device = torch.device(device)
+        self.form = form
+        self.arena = Arena(hbm_budget // 4, self.device)
+        self.resident_budget = resident_budget // 4
+        self.resident: dict[int, _Resident] = {}
+        self.resident_elems = 0
+        self.nb = K.qr_block(torch.float32)
+        self.ticks = 0
+        self.tick_log: list[dict] = []
+
+    # ------------------------------------------------------------------ memory
+    def _alloc_blocks(self, sizes: dict) -> dict | None:
+        got = {}
+        for key, elems in sizes.items():
+            off = self.arena.alloc(elems)
+            if off is None:
+                for k2, (o2, e2) in got.items():
+                    self.arena.release(o2, e2)
+                return None
+            got[key] = (off, elems)
+        return got
+
+    def _free_blocks(self, blocks: dict) -> None:
+        for off, elems in blocks.values():
+            self.arena.release(off, elems)
+
+    def _evict_one(self) -> bool:
+        if not self.resident:
+            return False
+        victim = min(self.resident, key=lambda k: self.resident[k].last_use)
+        r = self.resident.pop(victim)
+        self.resident_elems -= sum(e for _, e in r.blocks.values())
+        self._free_blocks(r.blocks)
+        return True
+
+    def _evict_until(self, need_elems: int) -> None:
+        while self.resident and self.resident_elems + need_elems > self.resident_budget:
+            victim = min(self.resident, key=lambda k: self.resident[k].last_use)
+            r = self.resident.pop(victim)
+            self.resident_elems -= sum(e for _, e in r.blocks.values())
+            self._free_blocks(r.blocks)
+
+    # ------------------------------------------------------------------ kernels
+    def _descs(self, items, with_b=True):
+        d = np.zeros(len(items), dtype=K.QR_DESC)
+        for i, (m, n, blocks) in enumerate(items):
+            d[i]["a_offset"] = blocks["a"][0]
+            if "tau" in blocks:
+                d[i]["tau_offset"] = blocks["tau"][0]
+                d[i]["t_offset"] = blocks["t"][0]
+                d[i]["w_offset"] = blocks["w"][0]
+            d[i]["b_offset"] = blocks["b"][0]
+            d[i]["m"], d[i]["n"], d[i]["lda"], d[i]["ldb"] = m, n, n, m
+        return K.descs_to_device(d, self.device)
+
+    def _qr_solve_batch(self, items):
+        """items: (m, n, blocks) with the rhs already in blocks['b']; x = first n entries."""
+        if not items:
+            return
+        buf = self.arena.buf
+        dd = self._descs(items)
+        max_m = max(m for m, _, _ in items)
+        max_k = max(min(m, n) for m, n, _ in items)
+        K.qr_apply_raw(dd, len(items), max_m, max_k, buf, buf, 1, buf, trans=True)
+        K.trsv_raw(dd, len(items), max_k, buf, 1, buf, lower=False, trans=False, unit_diag=False)
+
+    def _chol_batch(self, items, factor: bool):
+        if not items:
+            return
+        buf = self.arena.buf
+        dd = self._descs(items)
+        if factor:
+            info = torch.empty(len(items), dtype=torch.int32, device=self.device)
+            K.chol_blocked_factor_raw(dd, len(items), buf, info)
+        K.chol_blocked_solve_raw(dd, len(items), buf, buf)
+
+    # ------------------------------------------------------------------ run
+    def run(self, requests: list[Request], operators: dict, *, time_limit: float = 120.0) -> dict:
+        """Serves the stream in real time; returns latency and throughput figures.
+
+        ``operators`` maps op_id to the operator's device matrix (reuse
+        requests carry no matrix of their own)."""
+        self._operator_data = operators
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        for r in reqs:
+            if r.kind == "qr" and r.m < r.n:
+                raise ValueError("the request loop serves least-squares QR requests (m >= n)")
+            fp = qr_footprint(r.m, r.n, self.nb) if r.kind == "qr" else chol_footprint(r.n)
+            if sum(fp.values()) + len(fp) * Arena.ALIGN > self.arena.size:
+                raise MemoryError(f"request {r.rid} ({r.kind} {r.m}x{r.n}) exceeds the HBM arena")
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        if self.form == "plain":
+            self._run_plain(reqs, t0, time_limit)
+        else:
+            self._run_chosen(reqs, t0, time_limit)
+        makespan = time.perf_counter() - t0
+        lat = np.array([r.done - r.arrival for r in reqs if r.done is not None])
+        done = int(len(lat))
+        return {
+            "requests": len(reqs),
+            "completed": done,
+            "makespan_s": makespan,
+            "solves_per_s": done / makespan if makespan > 0 else 0.0,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3) if done else None,
+            "p99_ms": float(np.percentile(lat, 99) * 1e3) if done else None,
+            "ticks": self.ticks,
+            "arena_high_water_bytes": self.arena.high_water * 4,
+        }
+
+    def _now(self, t0):
+        return time.perf_counter() - t0
+
+    def _run_plain(self, reqs, t0, time_limit):
+        for r in reqs:
+            while self._now(t0) < r.arrival:
+                pass
+            if self._now(t0) > time_limit:
+                break
+            r.admitted = self._now(t0)
+            if r.kind == "qr":
+                fp = qr_footprint(r.m, r.n, self.nb)
+                blocks = self._alloc_blocks(fp)
+                if blocks is None:
+                    raise MemoryError("request does not fit in the arena")
+                a = r.a if r.a is not None else self._operator_data[r.op_id]
+                self.arena.view(*blocks["a"]).copy_(a.reshape(-1))
+                self.arena.view(*blocks["b"]).copy_(r.b)
+                dd = self._descs([(r.m, r.n, blocks)])
+                buf = self.arena.buf
+                K.qr_factor_raw(dd, 1, r.m, r.n, min(r.m, r.n), buf, buf, buf, buf)
+                self._qr_solve_batch([(r.m, r.n, blocks)])
+                r.x = self.arena.view(blocks["b"][0], r.n).clone()
+            else:
+                fp = chol_footprint(r.n)
+                blocks = self._alloc_blocks(fp)
+                if blocks is None:
+                    raise MemoryError("request does not fit in the arena")
+                a = r.a if r.a is not None else self._operator_data[r.op_id]
+                self.arena.view(*blocks["a"]).copy_(a.reshape(-1))
+                self.arena.view(*blocks["b"]).copy_(r.b)
+                self._chol_batch([(r.n, r.n, blocks)], factor=True)
+                r.x = self.arena.view(*blocks["b"]).clone()
+            torch.cuda.synchronize(self.device)
+            r.done = self._now(t0)
+            self._free_blocks(blocks)
+            self.ticks += 1
+
+    def _run_chosen(self, reqs, t0, time_limit):
+        pending = list(reqs)  # arrival order
+        waiting: list[Request] = []
+        factoring: list[tuple[Request, dict, int]] = []  # (request, blocks, next column)
+        owner_pending: dict[int, Request] = {}
+        i_next = 0
+        while True:
+            now = self._now(t0)
+            if now > time_limit:
+                break
+            while i_next < len(pending) and pending[i_next].arrival <= now:
+                waiting.append(pending[i_next])
+                i_next += 1
+            if not waiting and not factoring:
+                if i_next >= len(pending):
+                    break
+                continue  # idle until the next arrival
+            qr_solve, chol_new, chol_reuse, finished = [], [], [], []
+            # admission, first come first served
+            still = []
+            blocked = False
+            for r in waiting:
+                if blocked:
+                    still.append(r)
+                    continue
+                if r.reuse and r.op_id in self.resident:
+                    res = self.resident[r.op_id]
+                    blocks = self._alloc_blocks({"b": r.m})
+                    if blocks is None:
+                        blocked = True
+                        still.append(r)
+                        continue
+                    self.arena.view(*blocks["b"]).copy_(r.b)
+                    res.last_use = self.ticks
+                    merged = dict(res.blocks)
+                    merged["b"] = blocks["b"]
+                    item = (r.m, r.n, merged)
+                    (qr_solve if r.kind == "qr" else chol_reuse).append((r, item, blocks))
+                    continue
+                if r.reuse and r.op_id in owner_pending:
+                    still.append(r)  # its operator is still being factored
+                    continue
+                a = r.a if r.a is not None else self._operator_data[r.op_id]
+                fp = qr_footprint(r.m, r.n, self.nb) if r.kind == "qr" else chol_footprint(r.n)
+                blocks = self._alloc_blocks(fp)
+                if blocks is None:
+                    blocked = True
+                    still.append(r)
+                    continue
+                r.admitted = now
+                self.arena.view(*blocks["a"]).copy_(a.reshape(-1))
+                self.arena.view(*blocks["b"]).copy_(r.b)
+                if r.kind == "qr":
+                    factoring.append((r, blocks, 0))
+                    owner_pending[r.op_id] = r
+                else:
+                    chol_new.append((r, (r.n, r.n, blocks), blocks))
+                    owner_pending[r.op_id] = r
+            waiting = still
+            if blocked and not (factoring or qr_solve or chol_new or chol_reuse):
+                # nothing in flight frees memory: make room from the resident factors
+                if not self._evict_one():
+                    raise MemoryError("admission blocked with nothing in flight")
+                continue
+            # one panel step for every QR system being factored
+            if factoring:
+                buf = self.arena.buf
+                dd = self._descs([(r.m, r.n, blk) for r, blk, _ in factoring])
+                kcol = torch.tensor([k for _, _, k in factoring], dtype=torch.int32).to(self.device)
+                max_rows = max(r.m - k for r, _, k in factoring)
+                max_tc = max(r.n - k - 1 for r, _, k in factoring)
+                K.qr_factor_step_raw(dd, len(factoring), kcol, max_rows, max_tc, buf, buf, buf, buf)
+                nxt = []
+                for r, blk, k in factoring:
+                    k2 = k + self.nb
+                    if k2 >= min(r.m, r.n):
+                        qr_solve.append((r, (r.m, r.n, blk), None))
+                    else:
+                        nxt.append((r, blk, k2))
+                factoring = nxt
+            self._qr_solve_batch([item for _, item, _ in qr_solve])
+            self._chol_batch([item for _, item, _ in chol_new], factor=True)
+            self._chol_batch([item for _, item, _ in chol_reuse], factor=False)
+            outs = []
+            for r, item, rhs_blocks in qr_solve + chol_new + chol_reuse:
+                m, n, blk = item
+                outs.append((r, self.arena.view(blk["b"][0], r.n).clone()))
+            torch.cuda.synchronize(self.device)
+            t_done = self._now(t0)
+            self.ticks += 1
+            for r, x in outs:
+                r.x = x
+                r.done = t_done
+            # release or keep resident
+            for r, item, rhs_blocks in qr_solve + chol_new + chol_reuse:
+                m, n, blk = item
+                if rhs_blocks is not None and r.reuse and r.op_id in self.resident:
+                    self._free_blocks(rhs_blocks)  # the reused factor stays
+                    continue
+                owner_pending.pop(r.op_id, None)
+                keep = {k: v for k, v in blk.items() if k != "b"}
+                self.arena.release(*blk["b"])
+                elems = sum(e for _, e in keep.values())
+                if elems <= self.resident_budget:
+                    self._evict_until(elems)
+                    if r.op_id in self.resident:  # an older copy
+                        old = self.resident.pop(r.op_id)
+                        self.resident_elems -= sum(e for _, e in old.blocks.values())
+                        self._free_blocks(old.blocks)
+                    self.resident[r.op_id] = _Resident(r.kind, m, n, keep)
+                    self.resident[r.op_id].last_use = self.ticks
+                    self.resident_elems += elems
+                else:
+                    self._free_blocks(keep)
+
+
+
+def make_request_stream(rate: float, count: int, *, seed: int = 5, device="cuda:0", qr_m=(64, 4096),
+                        chol_n=(16, 1024), reuse_fraction: float = 0.1,
+                        qr_fraction: float = 0.5) -> tuple[list[Request], dict]:
+    """Synthetic Poisson stream: exponential gaps at `rate`/s; half QR
+    (m ~ U{qr_m}, n ~ U{ceil(m/4)..m}, N(0,1) entries), half SPD Cholesky
+    (n ~ U{chol_n}, A = M M^T / n + 0.1 I); a `reuse_fraction` of requests
+    reuse an earlier operator with a new right-hand side.  Data is generated
+    on the device (seeded per request) before serving starts.  Returns the
+    requests and the operators by op_id."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs: list[Request] = []
+    ops: dict[int, tuple[str, int, int]] = {}
+    data: dict[int, torch.Tensor] = {}
+    gen = torch.Generator(device=device)
+    for i in range(count):
+        reuse = bool(ops) and rng.random() < reuse_fraction
+        if reuse:
+            op_id = int(rng.choice(list(ops)))
+            kind, m, n = ops[op_id]
+            a = None
+        else:
+            op_id = i
+            if rng.random() < qr_fraction:
+                kind = "qr"
+                m = int(rng.integers(qr_m[0], qr_m[1] + 1))
+                n = int(rng.integers(-(-m // 4), m + 1))
+            else:
+                kind = "chol"
+                n = int(rng.integers(chol_n[0], chol_n[1] + 1))
+                m = n
+            ops[op_id] = (kind, m, n)
+            gen.manual_seed(seed * 1_000_003 + i)
+            if kind == "qr":
+                a = torch.randn((m, n), generator=gen, device=device, dtype=torch.float32)
+            else:
+                mm = torch.randn((n, n), generator=gen, device=device, dtype=torch.float64)
+                a = (mm @ mm.T / n + 0.1 * torch.eye(n, device=device, dtype=torch.float64)).to(torch.float32)
+            data[op_id] = a
+        gen.manual_seed(seed * 1_000_003 + 500_000 + i)
+        b = torch.randn((m,), generator=gen, device=device, dtype=torch.float32)
+        reqs.append(Request(rid=i, kind=kind, arrival=float(arrivals[i]), m=m, n=n, op_id=op_id, reuse=reuse,
+                            a=a, b=b))
+    return reqs, data
+
+
+def request_flops(r: Request) -> float:
+    """Algorithmic flops of one request (a reuse request: the solve only)."""
+    if r.kind == "qr":
+        m, n = r.m, r.n
+        solve = 4.0 * m * n
+        return solve if r.reuse else 2.0 * m * n * n - 2.0 * n ** 3 / 3.0 + solve
+    n = r.n
+    return 2.0 * n * n if r.reuse else n ** 3 / 3.0 + 2.0 * n * n

@@ -1,0 +1,202 @@
+"""GPU parity of the matrix-core update, the butterfly transform and the large LU.
+
+* gemm_sub against float64 products of the same operands rounded to the
+  product precision (bf16 / fp16 / fp32): only the FP32 accumulation order
+  differs, so the tolerance is tight and a wrong MFMA lane layout fails.
+* rbt_matrix / rbt_vectors against the butterfly matrices formed explicitly.
+* LargeLU: L U reproduces the row-swapped randomised matrix (float32
+  products; swaps stay inside 128-row diagonal blocks); the
+  refined solve reaches normwise backward error <= 1e-12 against the float64
+  oracle matrix and matches numpy's float64 solve; the transposed solve and
+  the LU solver's large path agree with float64 solves; an all-zero matrix
+  gives non-finite output (RESULTS.singular).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import lineax_amd as lx
+from lineax_amd import _kernels as K
+from lineax_amd.large_lu import LargeLU
+
+pytestmark = pytest.mark.gpu
+
+
+def _round(x, mode):
+    t = torch.as_tensor(x, dtype=torch.float32)
+    if mode == "bf16":
+        t = t.to(torch.bfloat16)
+    elif mode == "fp16":
+        t = t.to(torch.float16)
+    return t.to(torch.float64).numpy()
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (130, 70, 33), (257, 300, 512), (128, 128, 128), (64, 1000, 17)])
+def test_gemm_sub_matches_rounded_products(device, rng, mode, shape):
+    m, n, k = shape
+    a = rng.standard_normal((m, k)).astype(np.float32)
+    b = rng.standard_normal((k, n)).astype(np.float32)
+    c = rng.standard_normal((m, n)).astype(np.float32)
+    ref = c.astype(np.float64) - _round(a, mode) @ _round(b, mode)
+    cd = torch.as_tensor(c, device=device)
+    K.gemm_sub(cd, torch.as_tensor(a, device=device), torch.as_tensor(b, device=device), mode)
+    got = cd.cpu().numpy().astype(np.float64)
+    scale = np.abs(_round(a, mode)) @ np.abs(_round(b, mode)) + np.abs(c)
+    assert np.max(np.abs(got - ref) / scale) <= 4e-6
+
+
+def test_gemm_sub_strided_views(device, rng):
+    big = torch.as_tensor(rng.standard_normal((300, 400)).astype(np.float32), device=device)
+    a = big[10:110, 5:69]
+    b = big[150:214, 100:300]
+    c = big[200:300, 150:350].clone()
+    ref = c.double() - a.double() @ b.double()
+    K.gemm_sub(c, a, b, "fp32")
+    assert torch.max(torch.abs(c.double() - ref)).item() <= 1e-4
+
+
+def _butterfly(d, s_half, n):
+    """Explicit level matrix B (B form) of half-width s over n."""
+    B = np.zeros((n, n))
+    h = 1 / np.sqrt(2)
+    for blk in range(0, n, 2 * s_half):
+        for i in range(s_half):
+            a, b_ = blk + i, blk + s_half + i
+            B[a, a] = h * d[a]
+            B[a, b_] = h * d[b_]
+            B[b_, a] = h * d[a]
+            B[b_, b_] = -h * d[b_]
+    return B
+
+
+def test_rbt_matches_explicit_butterflies(device, rng):
+    n = 16
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    du = np.exp((rng.random(2 * n) - 0.5) / 10).astype(np.float32)
+    dv = np.exp((rng.random(2 * n) - 0.5) / 10).astype(np.float32)
+    U = _butterfly(du[n:].astype(np.float64), n // 4, n) @ _butterfly(du[:n].astype(np.float64), n // 2, n)
+    V = _butterfly(dv[n:].astype(np.float64), n // 4, n) @ _butterfly(dv[:n].astype(np.float64), n // 2, n)
+    ad = torch.as_tensor(a, device=device)
+    K.rbt_matrix(ad, torch.as_tensor(du, device=device), torch.as_tensor(dv, device=device))
+    np.testing.assert_allclose(ad.cpu().numpy(), U.T @ a @ V, rtol=1e-5, atol=1e-5)
+    x = rng.standard_normal((3, n))
+    xd = torch.as_tensor(x, device=device)
+    K.rbt_vectors(xd, torch.as_tensor(du, device=device), form=1)
+    np.testing.assert_allclose(xd.cpu().numpy(), (U.T @ x.T).T, rtol=1e-6, atol=1e-6)
+    xd = torch.as_tensor(x, device=device)
+    K.rbt_vectors(xd, torch.as_tensor(dv, device=device), form=0)
+    np.testing.assert_allclose(xd.cpu().numpy(), (V @ x.T).T, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("n", [128, 512, 1000, 1300])
+def test_large_lu_factors_reproduce_randomised_matrix(device, rng, n):
+    a = (rng.standard_normal((n, n)) / np.sqrt(n) + 2 * np.eye(n)).astype(np.float32)
+    fac = LargeLU(torch.as_tensor(a, device=device), precision="fp32", seed=3)
+    lu = fac.lu.cpu().numpy().astype(np.float64)
+    L = np.tril(lu, -1) + np.eye(fac.np)
+    U = np.triu(lu)
+    # the randomised matrix, formed on the host from the same diagonals
+    ap = np.zeros((fac.np, fac.np))
+    ap[:n, :n] = a
+    ap[np.arange(n, fac.np), np.arange(n, fac.np)] = 1.0
+    du = fac.du.cpu().numpy().astype(np.float64)
+    dv = fac.dv.cpu().numpy().astype(np.float64)
+    m = fac.np
+    Ub = _butterfly(du[m:], m // 4, m) @ _butterfly(du[:m], m // 2, m)
+    Vb = _butterfly(dv[m:], m // 4, m) @ _butterfly(dv[:m], m // 2, m)
+    target = Ub.T @ ap @ Vb
+    piv = fac.piv.cpu().numpy()
+    for i in range(fac.np):  # the factorisation's row swaps, in order
+        if piv[i] != i:
+            assert i // 128 == piv[i] // 128  # swaps stay inside a diagonal block
+            target[[i, piv[i]]] = target[[piv[i], i]]
+    err = np.max(np.abs(L @ U - target)) / np.max(np.abs(target))
+    assert err <= 1e-4
+    assert int(fac.info.item()) == 0
+    assert np.max(np.abs(np.tril(lu, -1))) <= 1.0 + 1e-6 or n > 128  # |l| <= 1 inside one block
+
+
+def test_large_lu_block_inverses_and_solves_on_gaussian(device, rng):
+    n = 640
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    fac = LargeLU(torch.as_tensor(a, device=device), precision="fp32", seed=4)
+    lu = fac.lu.cpu().numpy().astype(np.float64)
+    m = fac.np
+    L = np.tril(lu, -1) + np.eye(m)
+    U = np.triu(lu)
+    li = fac.linv.cpu().numpy().reshape(-1, 128, 128)
+    ui = fac.uinv.cpu().numpy().reshape(-1, 128, 128)
+    for j in range(m // 128):
+        blk = slice(128 * j, 128 * j + 128)
+        assert np.max(np.abs(li[j] @ L[blk, blk] - np.eye(128))) <= 1e-3
+        assert np.max(np.abs(ui[j] @ U[blk, blk] - np.eye(128))) <= 1e-2
+    b = rng.standard_normal((3, m)).astype(np.float32)
+    for kind, mat, inv in ((0, L, fac.linv), (1, U, fac.uinv), (2, U.T, fac.uinv), (3, L.T, fac.linv)):
+        xd = torch.as_tensor(b, device=device).contiguous()
+        K.lu_large_trsv(kind, fac.lu, inv, xd, fac.sync)
+        ref = np.linalg.solve(mat, b.T.astype(np.float64)).T
+        assert np.max(np.abs(xd.cpu().numpy() - ref)) <= 1e-3 * np.max(np.abs(ref)), kind
+    bb = rng.standard_normal((2, n)).astype(np.float32)
+    for trans in (False, True):
+        y = fac.apply_inverse(torch.as_tensor(bb, device=device), trans=trans).cpu().numpy()
+        mat = a.astype(np.float64).T if trans else a.astype(np.float64)
+        ref = np.linalg.solve(mat, bb.T.astype(np.float64)).T
+        assert np.max(np.abs(y - ref)) <= 1e-2 * np.max(np.abs(ref)), trans
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "fp32"])
+@pytest.mark.parametrize("n", [1024, 2000])
+def test_large_lu_refined_solve(device, rng, precision, n):
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    b = rng.standard_normal((4, n)).astype(np.float32)
+    fac = LargeLU(torch.as_tensor(a, device=device), precision=precision)
+    x, eta, stats = fac.solve(torch.as_tensor(b, device=device), tol=1e-12, max_refine=10)
+    x = x.cpu().numpy()
+    a64 = a.astype(np.float64)
+    for i in range(4):
+        r = b[i] - a64 @ x[i]
+        e = np.max(np.abs(r)) / (np.max(np.abs(a64).sum(1)) * np.max(np.abs(x[i])) + np.max(np.abs(b[i])))
+        assert e <= 1e-12, (i, e, stats)
+        ref = np.linalg.solve(a64, b[i].astype(np.float64))
+        assert np.max(np.abs(x[i] - ref)) <= 1e-8 * np.max(np.abs(ref))
+    assert float(eta.max()) <= 1e-12
+
+
+def test_large_lu_transposed_solve(device, rng):
+    n = 777
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    b = rng.standard_normal((2, n)).astype(np.float32)
+    fac = LargeLU(torch.as_tensor(a, device=device), precision="bf16")
+    x, eta, _ = fac.solve(torch.as_tensor(b, device=device), tol=1e-12, trans=True)
+    for i in range(2):
+        ref = np.linalg.solve(a.astype(np.float64).T, b[i].astype(np.float64))
+        assert np.max(np.abs(x[i].cpu().numpy() - ref)) <= 1e-8 * np.max(np.abs(ref))
+
+
+def test_lu_solver_large_path(device, rng):
+    n = 1536
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    op = lx.MatrixLinearOperator(torch.as_tensor(a, device=device))
+    solver = lx.LU(large_threshold=1024)
+    sol = lx.linear_solve(op, torch.as_tensor(b, device=device), solver)
+    assert sol.value.dtype == torch.float32 and int(sol.result) == 0
+    ref = np.linalg.solve(a.astype(np.float64), b.astype(np.float64))
+    x = sol.value.cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(x - ref)) <= 1e-4 * np.max(np.abs(ref))
+    # transposed state (used by reverse-mode derivatives) solves with A^T
+    state = solver.init(op, {})
+    st_t, _ = solver.transpose(state, {})
+    xt, _, _ = solver.compute(st_t, torch.as_tensor(b, device=device), {})
+    ref_t = np.linalg.solve(a.astype(np.float64).T, b.astype(np.float64))
+    assert np.max(np.abs(xt.cpu().numpy() - ref_t)) <= 1e-4 * np.max(np.abs(ref_t))
+
+
+def test_large_lu_zero_matrix_is_singular(device):
+    n = 256
+    op = lx.MatrixLinearOperator(torch.zeros((n, n), dtype=torch.float32, device=device))
+    sol = lx.linear_solve(op, torch.ones(n, dtype=torch.float32, device=device), lx.LU(large_threshold=128),
+                          throw=False)
+    assert int(sol.result) == lx.RESULTS.singular

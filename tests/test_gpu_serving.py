@@ -1,0 +1,167 @@
+"""GPU parity of the request loop and its kernels.
+
+* lxa_chol_blocked_factor / _solve (uneven orders up to 1024) against the
+  oracle's float64 Cholesky of the same float32 data: normwise backward
+  error <= 1e-5 (float32 factorisation), and RESULTS-style failure (info,
+  NaN factor) on a non-positive-definite system.
+* lxa_qr_factor_step with systems at different columns (one joins later)
+  gives the factors of lxa_qr_factor to float32 rounding (1e-4 relative).
+* The request loop: every request's answer matches float64 numpy (lstsq /
+  solve) to float32 accuracy; each request solved alone through the loop
+  gives the same answer as in the mix to 1e-5 relative (the batch can change
+  the panel kernel's row split and so the float32 summation order); the
+  plain form agrees with the chosen form; reuse requests are answered from
+  the resident factorisation.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from lineax_amd import _kernels as K
+from lineax_amd import serving as S
+from oracle import lineax_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _spd(rng, n):
+    m = rng.standard_normal((n, n))
+    return (m @ m.T / n + 0.1 * np.eye(n)).astype(np.float32)
+
+
+def test_chol_blocked_uneven_vs_oracle(device, rng):
+    ns = [1, 5, 31, 32, 33, 100, 257, 513, 1024]
+    mats = [_spd(rng, n) for n in ns]
+    rhs = [rng.standard_normal(n).astype(np.float32) for n in ns]
+    d = np.zeros(len(ns), dtype=K.QR_DESC)
+    a_off = b_off = 0
+    for i, n in enumerate(ns):
+        d[i]["a_offset"], d[i]["b_offset"], d[i]["m"], d[i]["n"], d[i]["lda"], d[i]["ldb"] = a_off, b_off, n, n, n, n
+        a_off += n * n
+        b_off += n
+    a = torch.as_tensor(np.concatenate([x.ravel() for x in mats]), device=device)
+    b = torch.as_tensor(np.concatenate(rhs), device=device)
+    info = torch.full((len(ns),), 7, dtype=torch.int32, device=device)
+    dd = K.descs_to_device(d, device)
+    K.chol_blocked_factor_raw(dd, len(ns), a, info)
+    K.chol_blocked_solve_raw(dd, len(ns), a, b)
+    assert info.cpu().tolist() == [0] * len(ns)
+    x = b.cpu().numpy().astype(np.float64)
+    for i, n in enumerate(ns):
+        xi = x[int(d[i]["b_offset"]): int(d[i]["b_offset"]) + n]
+        assert O.normwise_backward_error(mats[i].astype(np.float64), xi, rhs[i].astype(np.float64)) <= 1e-5
+        ref = O.solve_dense("cholesky", mats[i].astype(np.float64), rhs[i].astype(np.float64))[0]
+        assert np.max(np.abs(xi - ref)) <= 1e-3 * np.max(np.abs(ref))
+
+
+def test_chol_blocked_reports_non_pd(device, rng):
+    n = 70
+    a = _spd(rng, n)
+    a[40, 40] = -5.0
+    d = np.zeros(1, dtype=K.QR_DESC)
+    d[0]["m"], d[0]["n"], d[0]["lda"], d[0]["ldb"] = n, n, n, n
+    at = torch.as_tensor(a.ravel(), device=device)
+    info = torch.zeros(1, dtype=torch.int32, device=device)
+    K.chol_blocked_factor_raw(K.descs_to_device(d, device), 1, at, info)
+    assert int(info.item()) > 0
+    assert torch.isnan(at).all()
+
+
+def test_qr_steps_with_staggered_columns(device, rng):
+    shapes = [(700, 300), (300, 200)]
+    mats = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+    nb = K.qr_block(torch.float32)
+    d = np.zeros(2, dtype=K.QR_DESC)
+    offs = {"a": 0, "tau": 0, "t": 0, "w": 0}
+    for i, (m, n) in enumerate(shapes):
+        fp = S.qr_footprint(m, n, nb)
+        d[i]["a_offset"], d[i]["tau_offset"], d[i]["t_offset"], d[i]["w_offset"] = (offs["a"], offs["tau"],
+                                                                                     offs["t"], offs["w"])
+        d[i]["m"], d[i]["n"], d[i]["lda"] = m, n, n
+        for key in offs:
+            offs[key] += fp[key]
+    a = torch.as_tensor(np.concatenate([x.ravel() for x in mats]), device=device)
+    tau = torch.zeros(offs["tau"], device=device)
+    tblk = torch.zeros(offs["t"], device=device)
+    work = torch.zeros(offs["w"], device=device)
+    dd = K.descs_to_device(d, device)
+    # system 1 joins after system 0 has done 5 panels
+    k0, k1 = 0, None
+    steps = 0
+    while True:
+        active_k = [k0 if k0 < 300 else 10**9, (k1 if k1 is not None and k1 < 200 else 10**9)]
+        if active_k[0] >= 10**9 and active_k[1] >= 10**9:
+            break
+        kcol = torch.tensor(active_k, dtype=torch.int32).to(device)
+        rows = max(shapes[i][0] - k for i, k in enumerate(active_k) if k < 10**9)
+        tc = max(shapes[i][1] - k - 1 for i, k in enumerate(active_k) if k < 10**9)
+        K.qr_factor_step_raw(dd, 2, kcol, rows, tc, a, tau, tblk, work)
+        steps += 1
+        if k0 < 300:
+            k0 += nb
+        if k1 is not None and k1 < 200:
+            k1 += nb
+        if steps == 5:
+            k1 = 0
+    got = a.cpu().numpy()
+    for i, (m, n) in enumerate(shapes):
+        ref, _, _ = K.qr_factor(torch.as_tensor(mats[i], device=device))
+        gi = got[int(d[i]["a_offset"]): int(d[i]["a_offset"]) + m * n].reshape(m, n)
+        r = ref.cpu().numpy()
+        assert np.max(np.abs(gi - r)) <= 1e-4 * np.max(np.abs(r))
+
+
+def _check_answer(r, ops):
+    a = (r.a if r.a is not None else ops[r.op_id]).double().cpu().numpy()
+    b = r.b.double().cpu().numpy()
+    x = r.x.double().cpu().numpy()
+    if r.kind == "qr":
+        # least-squares optimality A^T (A x - b) ~ 0, to float32 accuracy
+        res = a @ x - b
+        fa = np.linalg.norm(a)
+        assert np.linalg.norm(a.T @ res) <= 1e-4 * fa * (fa * np.linalg.norm(x) + np.linalg.norm(b)), (r.rid, r.m, r.n)
+    else:
+        assert O.normwise_backward_error(a, x, b) <= 1e-5, (r.rid, r.n)
+
+
+def test_request_loop_chosen_matches_float64_and_alone(device):
+    reqs, ops = S.make_request_stream(2000.0, 40, seed=11, device=device, qr_m=(64, 400), chol_n=(16, 300),
+                                      reuse_fraction=0.2)
+    server = S.SolveServer(device, form="chosen", hbm_budget=1 << 30, resident_budget=256 << 20)
+    stats = server.run(reqs, ops)
+    assert stats["completed"] == len(reqs)
+    assert any(r.reuse for r in reqs)
+    for r in reqs:
+        _check_answer(r, ops)
+    # each request alone gives the same answer as in the mix
+    for r in reqs[:12]:
+        alone = S.Request(rid=r.rid, kind=r.kind, arrival=0.0, m=r.m, n=r.n, op_id=r.op_id, reuse=False,
+                          a=(r.a if r.a is not None else ops[r.op_id]), b=r.b)
+        S.SolveServer(device, form="chosen", hbm_budget=1 << 28).run([alone], ops)
+        scale = float(torch.max(torch.abs(alone.x)))
+        assert float(torch.max(torch.abs(alone.x - r.x))) <= 1e-5 * max(1.0, scale), (r.rid, r.kind)
+
+
+def test_request_loop_plain_matches_chosen(device):
+    reqs, ops = S.make_request_stream(5000.0, 16, seed=12, device=device, qr_m=(64, 300), chol_n=(16, 200))
+    S.SolveServer(device, form="chosen", hbm_budget=1 << 29).run(reqs, ops)
+    chosen = [r.x.clone() for r in reqs]
+    reqs2, ops2 = S.make_request_stream(5000.0, 16, seed=12, device=device, qr_m=(64, 300), chol_n=(16, 200))
+    stats = S.SolveServer(device, form="plain", hbm_budget=1 << 29).run(reqs2, ops2)
+    assert stats["completed"] == 16
+    for x1, r in zip(chosen, reqs2):
+        scale = float(torch.max(torch.abs(x1)))
+        assert float(torch.max(torch.abs(x1 - r.x))) <= 1e-5 * max(1.0, scale)
+
+
+def test_request_loop_admission_respects_budget(device):
+    reqs, ops = S.make_request_stream(50000.0, 30, seed=13, device=device, qr_m=(512, 1024), chol_n=(16, 64),
+                                      reuse_fraction=0.0)
+    budget = 24 << 20
+    server = S.SolveServer(device, form="chosen", hbm_budget=budget, resident_budget=0)
+    stats = server.run(reqs, ops)
+    assert stats["completed"] == 30
+    assert stats["arena_high_water_bytes"] <= budget
+    for r in reqs:
+        _check_answer(r, ops)
