@@ -6,8 +6,10 @@ solved as one step per pass over the whole batch, inputs resident in HBM.
 
   chosen form: one persistent launch, FP32 factorisation in LDS, FP64
                refinement to normwise backward error <= 1e-12 (float64 x).
-  plain form:  one FP32 launch per order bucket (249 launches), no
-               refinement (float32 x).
+  plain form:  the kernels lx.Cholesky runs (factor + solve launch pair) on
+               one order bucket at a time (249 buckets), FP32, no
+               refinement (float32 x).  The per-bucket schedule on the
+               chosen form's kernel is reported beside it.
 
 ``value`` is the chosen form's solves/s.  Per-step latency percentiles, the
 plain form's figures, accuracy against the float64 oracle (on a sample of
@@ -52,12 +54,16 @@ def _time_steps(fn, steps, warmup):
     stream = torch.cuda.current_stream()
     events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
+    host = []
     for start, end in events:
         start.record(stream)
+        h0 = time.perf_counter()
         fn()
+        host.append((time.perf_counter() - h0) * 1e3)
         end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    _time_steps.last_host_ms = host
     return [s.elapsed_time(e) for s, e in events], wall
 
 
@@ -129,7 +135,8 @@ def main():
                     choices=["uneven_cholesky", "small_lu", "large_lu", "krylov_cg", "krylov_gmres", "serving"],
                     help="default: the uneven-batch Cholesky configuration (the headline metric)")
     ap.add_argument("--n", type=int, default=16384, help="large_lu: order (8192 or 16384)")
-    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16", "fp32"], help="large_lu: Schur products")
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16", "fp16", "bf16x3", "fp32"],
+                    help="large_lu: Schur products")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: arrivals per second")
     ap.add_argument("--requests", type=int, default=400, help="serving: stream length")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -187,6 +194,7 @@ def bench_uneven_cholesky(args):
         chosen["sol"] = batch.cholesky_solve(form="chosen", tol=args.tol, out=chosen.get("sol"))
 
     ms, wall = _time_steps(run_chosen, args.steps, args.warmup)
+    host_ms = float(np.mean(_time_steps.last_host_ms))
     sol = chosen["sol"]
     ok = int((sol.result == 0).sum())
     eta_dev = sol.eta.cpu().numpy()
@@ -196,34 +204,72 @@ def bench_uneven_cholesky(args):
           file=sys.stderr, flush=True)
 
     plain_info = None
+    fused_info = None
     if not args.no_plain:
-        plain = {}
-
-        def run_plain():
-            plain["sol"] = batch.cholesky_solve(form="plain", out=plain.get("sol"))
-
-        pms, _ = _time_steps(run_plain, max(3, args.steps // 2), 1)
-        psol = plain["sol"]
+        from lineax_amd import _kernels as K
         from oracle import lineax_oracle as O
 
         rng = np.random.default_rng(5)
         pidx = rng.choice(host.count, size=200, replace=False)
-        px = psol.x.cpu().numpy().astype(np.float64)
-        peta = max(
-            O.normwise_backward_error(host.matrix(i), px[int(host.b_offset[i]): int(host.b_offset[i]) + int(host.n[i])],
-                                      host.rhs(i))
-            for i in pidx
-        )
+
+        def eta_sample(x_of):
+            return max(O.normwise_backward_error(host.matrix(i), x_of(i), host.rhs(i)) for i in pidx)
+
+        # Plain form: what lx.Cholesky runs under a vmap over one order bucket
+        # (lxa_chol_factor + lxa_chol_solve on a [k, n, n] batch), bucket by
+        # bucket; the buckets' matrices and right-hand sides are resident.
+        order_asc = batch.order_asc.cpu().numpy()
+        buckets = []
+        for order_n, s0, e0 in batch.buckets:
+            idx = order_asc[s0:e0]
+            mats = np.stack([host.matrix(int(i)) for i in idx])
+            rhs = np.stack([host.rhs(int(i)) for i in idx]).astype(np.float32)
+            buckets.append((idx, torch.as_tensor(mats, device="cuda:0"), torch.as_tensor(rhs, device="cuda:0")))
+        torch.cuda.synchronize()
+        plain = {}
+
+        def run_plain():
+            plain["x"] = [K.chol_solve(K.chol_factor(a_k)[0], b_k) for _, a_k, b_k in buckets]
+
+        pms, _ = _time_steps(run_plain, max(3, args.steps // 2), 1)
+        where = {}
+        for (idx, _, _), xk in zip(buckets, plain["x"]):
+            xh = xk.cpu().numpy().astype(np.float64)
+            for j, i in enumerate(idx):
+                where[int(i)] = xh[j]
         plain_info = {
             "value": host.count / (float(np.mean(pms)) / 1e3),
             "ms_per_step": float(np.mean(pms)),
             "p50_ms": _percentile(pms, 50),
             "p99_ms": _percentile(pms, 99),
-            "launches_per_step": len(batch.buckets),
-            "max_backward_error_sample": peta,
+            "kernels": "lxa_chol_factor + lxa_chol_solve per order bucket (the lx.Cholesky path)",
+            "launches_per_step": 2 * len(batch.buckets),
+            "max_backward_error_sample": eta_sample(lambda i: where[int(i)]),
             "x_dtype": "float32",
+            "achieved_gbs": batch.algorithmic_bytes / (float(np.mean(pms)) / 1e3) / 1e9,
         }
+        del buckets, plain
         print(f"[bench] plain form: {plain_info['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+
+        # The same per-bucket schedule on the register-blocked kernel of the
+        # chosen form (FP32, no refinement): separates launch structure from
+        # kernel quality.
+        fused = {}
+
+        def run_fused():
+            fused["sol"] = batch.cholesky_solve(form="plain", out=fused.get("sol"))
+
+        fms, _ = _time_steps(run_fused, max(3, args.steps // 2), 1)
+        fx = fused["sol"].x.cpu().numpy().astype(np.float64)
+        fused_info = {
+            "value": host.count / (float(np.mean(fms)) / 1e3),
+            "ms_per_step": float(np.mean(fms)),
+            "kernels": "chol_uneven_kernel<false,float> per order bucket",
+            "launches_per_step": len(batch.buckets),
+            "max_backward_error_sample": eta_sample(
+                lambda i: fx[int(host.b_offset[i]): int(host.b_offset[i]) + int(host.n[i])]),
+        }
+        print(f"[bench] fused per-bucket form: {fused_info['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
 
     acc = accuracy_sample(host, sol)
     value = host.count / (mean_ms / 1e3)
@@ -272,12 +318,14 @@ def bench_uneven_cholesky(args):
             "tol_backward_error": args.tol,
             "digest": host.digest(),
         },
+        "host_ms_per_step": host_ms,
         "systems_ok": ok,
         "device_eta_max": float(eta_dev.max()),
         "refinement_residuals_mean": float(iters.mean()),
         "accuracy": acc,
         "plain_form": plain_info,
         "speedup_vs_plain": (value / plain_info["value"]) if plain_info else None,
+        "per_bucket_fused_kernel": fused_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "wall_s_timed": wall,
@@ -289,7 +337,7 @@ def bench_uneven_cholesky(args):
 # Other configurations (one JSON line each; not the headline line)
 # ---------------------------------------------------------------------------
 
-MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "bf16x3": 2500.0 / 3, "fp32": 157.3}
 
 
 def _cpu_threads():
@@ -333,7 +381,7 @@ def bench_small_lu(args):
     achieved = 1024 * per_bytes / (t / 1e3) / 1e9
     t0 = time.perf_counter()
     for i in range(1024):
-        lu, piv = O.lu_factor(a_h[i])
+        lu, piv, _ = O.lu_factor(a_h[i])
         O.lu_solve(lu, piv, b_h[i])
     cpu_s = time.perf_counter() - t0
     return _line("solves/s", 1024 / (t / 1e3), "solves/s", t, args,
@@ -392,7 +440,7 @@ def bench_large_lu(args):
     rng = np.random.default_rng(2)
     a_s = rng.standard_normal((ns, ns))
     t0 = time.perf_counter()
-    lu, piv = O.lu_factor(a_s)
+    lu, piv, _ = O.lu_factor(a_s)
     for _ in range(4):
         O.lu_solve(lu, piv, rng.standard_normal(ns))
     cpu_s = (time.perf_counter() - t0) * (n / ns) ** 3
@@ -422,13 +470,13 @@ def _krylov_leg(args, kind):
     if kind == "cg":
         nx = 2048
         op = poisson2d(nx, dtype=torch.float32, device="cuda:0")
-        solver = lx.CG(rtol=1e-6, atol=1e-6)
+        solver = lx.CG(rtol=1e-6, atol=1e-6, graph=True)
         # 2-sweep fused schedule: 9 float32 passes per point per iteration
         bytes_per_iter = 36 * nx * nx * rows
     else:
         nx = 1024
         op = convection_diffusion2d(nx, dtype=torch.float32, device="cuda:0")
-        solver = lx.GMRES(rtol=1e-6, atol=1e-6, restart=30, max_steps=int(args.steps) * 30 + 2)
+        solver = lx.GMRES(rtol=1e-6, atol=1e-6, restart=30, max_steps=int(args.steps) + 1, graph=True)
         m = 30
         bytes_per_iter = 4 * nx * nx * (m * m + 5 * m + 5) * rows  # per restart cycle
     gen = torch.Generator(device="cpu").manual_seed(3 if kind == "cg" else 4)
@@ -440,7 +488,7 @@ def _krylov_leg(args, kind):
     wall = time.perf_counter() - t0
     steps = sol.stats["num_steps"].cpu().numpy()
     results = sol.result.cpu().numpy()
-    iters = float(steps.max()) if kind == "cg" else float(steps.max()) / 30.0
+    iters = float(steps.max())  # CG: iterations; GMRES: restart cycles (lineax counts a cycle as a step)
     per_iter_ms = wall * 1e3 / max(iters, 1.0)
     achieved = bytes_per_iter / (per_iter_ms / 1e3) / 1e9
     return _line("solves/s", rows / wall, "right-hand sides solved per second", wall * 1e3, args,

@@ -272,11 +272,13 @@ int lxa_gmres_result(void* stream, int64_t rows, const lxa_krylov_params* prm, l
  * Schur-complement update of the blocked LU (the trailing-matrix GEMM inside
  * LAPACK getrf that jax.scipy.linalg.lu_factor reaches from
  * lineax/_solver/lu.py:53).  mode selects the product precision: float32,
- * or bfloat16 / float16 inputs rounded to nearest, FP32 accumulation.
+ * bfloat16 / float16 inputs rounded to nearest, or split bfloat16 (bf16x3);
+ * FP32 accumulation.
  * ------------------------------------------------------------------------- */
 #define LXA_GEMM_F32 0
 #define LXA_GEMM_BF16 1
 #define LXA_GEMM_F16 2
+#define LXA_GEMM_BF16X3 3 /* split bf16: hi*hi + hi*lo + lo*hi, ~16-bit operands */
 int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_t k, const float* a, int64_t lda,
                  const float* b, int64_t ldb, float* c, int64_t ldc);
 

@@ -686,7 +686,8 @@ def tri_solve(matrix: torch.Tensor, b: torch.Tensor, *, lower: bool, trans: bool
 GEMM_F32 = 0
 GEMM_BF16 = 1
 GEMM_F16 = 2
-GEMM_MODES = {"fp32": GEMM_F32, "bf16": GEMM_BF16, "fp16": GEMM_F16}
+GEMM_BF16X3 = 3
+GEMM_MODES = {"fp32": GEMM_F32, "bf16": GEMM_BF16, "fp16": GEMM_F16, "bf16x3": GEMM_BF16X3}
 
 
 def _f32_2d(t: torch.Tensor, what: str) -> None:

@@ -23,6 +23,8 @@ NORM_KINDS = {max_norm: 0, two_norm: 1, rms_norm: 2}
 # Iterations between host checks of the per-row `active` flags: grows from
 # 1 to this, so short solves stop promptly and long ones sync rarely.
 MAX_CHECK_BLOCK = 32
+# Iterations per captured graph (rounded up to a multiple of stabilise_every).
+GRAPH_BLOCK = 30
 
 
 def norm_kind(norm) -> int:
@@ -58,11 +60,14 @@ class CG(AbstractLinearSolver):
     default and the RESULTS mapping.  ``working_dtype`` (not in lineax)
     stores r, p and A p in a narrower dtype than the vector's: with a float64
     right-hand side and ``working_dtype=torch.float32`` the iterate, the
-    stabilising residuals and all inner products stay float64.
+    stabilising residuals and all inner products stay float64.  ``graph``
+    (not in lineax) captures blocks of iterations in a HIP graph and replays
+    them, checking convergence on the host between replays; the operator
+    must then be capturable (device work only, no host synchronisation).
     """
 
     def __init__(self, rtol, atol, norm=max_norm, stabilise_every: int | None = 10, max_steps: int | None = None,
-                 working_dtype: torch.dtype | None = None):
+                 working_dtype: torch.dtype | None = None, graph: bool = False):
         if isinstance(rtol, (int, float)) and rtol < 0:
             raise ValueError("Tolerances must be non-negative.")
         if isinstance(atol, (int, float)) and atol < 0:
@@ -75,6 +80,7 @@ class CG(AbstractLinearSolver):
         self.stabilise_every = stabilise_every
         self.max_steps = max_steps
         self.working_dtype = working_dtype
+        self.graph = graph
 
     def init(self, operator, options):
         is_nsd = is_negative_semidefinite(operator)
@@ -123,32 +129,54 @@ class CG(AbstractLinearSolver):
         p = z.clone()
         K.call("lxa_cg_begin", stream, vd, rows, size, z, r, prm, st, part)
 
+        se = self.stabilise_every
+
+        def iteration(i):
+            stream = K.current_stream(b)  # the capture stream while a graph is recorded
+            q = A(p, vdt)
+            K.call("lxa_cg_alpha", stream, vd, rows, size, q, p, prm, st, part)
+            stable = se == 1 or (se is not None and (i + 1) % se == 0)
+            fuse = (not precond) and (not stable)
+            K.call("lxa_cg_update", stream, yd, vd, rows, size, p, q, y, r, b, int(stable), int(fuse), prm, st,
+                   part)
+            if stable:
+                t = A(y, ydt)
+                K.call("lxa_cg_residual", stream, yd, vd, rows, size, b, t, r, int(not precond), prm, st, part)
+            if precond:
+                z = M(r, vdt)
+                K.call("lxa_cg_precond_gamma", stream, yd, vd, rows, size, b, z, r, prm, st, part)
+            else:
+                z = r
+            K.call("lxa_cg_finish", stream, vd, rows, size, z, p, prm, st, part)
+
         it = 0
         block = 1
+        # Graph mode: a block of `period` iterations (a multiple of the
+        # stabilisation interval, so every replay has the same schedule) is
+        # captured once and replayed between host checks of the active flags.
+        period = 0
+        if self.graph and dev.type == "cuda":
+            period = (se or 1) * max(1, -(-GRAPH_BLOCK // (se or 1)))
+        graph = None
         while True:
-            for _ in range(block):
-                q = A(p, vdt)
-                K.call("lxa_cg_alpha", stream, vd, rows, size, q, p, prm, st, part)
-                se = self.stabilise_every
-                stable = se == 1 or (se is not None and (it + 1) % se == 0)
-                fuse = (not precond) and (not stable)
-                K.call("lxa_cg_update", stream, yd, vd, rows, size, p, q, y, r, b, int(stable), int(fuse), prm, st,
-                       part)
-                if stable:
-                    t = A(y, ydt)
-                    K.call("lxa_cg_residual", stream, yd, vd, rows, size, b, t, r, int(not precond), prm, st, part)
-                if precond:
-                    z = M(r, vdt)
-                    K.call("lxa_cg_precond_gamma", stream, yd, vd, rows, size, b, z, r, prm, st, part)
-                else:
-                    z = r
-                K.call("lxa_cg_finish", stream, vd, rows, size, z, p, prm, st, part)
-                it += 1
-                if it >= max_steps:
-                    break
+            if graph is None and period and it % period == 0 and it + period <= max_steps and it >= period:
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    for j in range(period):
+                        iteration(it + j)
+            if graph is not None and it + period <= max_steps:
+                graph.replay()
+                it += period
+            else:
+                for _ in range(block):
+                    iteration(it)
+                    it += 1
+                    if it >= max_steps or (period and it % period == 0):
+                        break
             if it >= max_steps or not bool(st.active().any()):
                 break
             block = min(block * 2, MAX_CHECK_BLOCK)
+        stream = K.current_stream(b)
         K.call("lxa_cg_result", stream, rows, prm, st)
 
         if is_nsd:

@@ -26,10 +26,12 @@ class GMRES(AbstractLinearSolver):
     residual norm stop the solve.  ``working_dtype`` (not in lineax) stores
     the basis and residual in a narrower dtype than the vector's; inner
     products, the Hessenberg matrix and the least-squares solve are FP64.
+    ``graph`` (not in lineax) captures a restart cycle in a HIP graph and
+    replays it, checking the active flags on the host between cycles.
     """
 
     def __init__(self, rtol, atol, norm=max_norm, max_steps: int | None = None, restart: int = 20,
-                 stagnation_iters: int = 20, working_dtype: torch.dtype | None = None):
+                 stagnation_iters: int = 20, working_dtype: torch.dtype | None = None, graph: bool = False):
         if isinstance(rtol, (int, float)) and rtol < 0:
             raise ValueError("Tolerances must be non-negative.")
         if isinstance(atol, (int, float)) and atol < 0:
@@ -43,6 +45,7 @@ class GMRES(AbstractLinearSolver):
         self.restart = restart
         self.stagnation_iters = stagnation_iters
         self.working_dtype = working_dtype
+        self.graph = graph
 
     def init(self, operator, options):
         if not structure_equal(operator.in_structure(), operator.out_structure()):
@@ -88,17 +91,15 @@ class GMRES(AbstractLinearSolver):
         K.call("lxa_gmres_init", stream, rows, st, gs)
 
         def residual():
+            stream = K.current_stream(b)
             t = A(y, ydt)
             K.call("lxa_gmres_residual", stream, yd, vd, rows, size, b, t, r, 1, prm, st, part)
             if precond:
                 r.copy_(M(r, vdt))
                 K.call("lxa_gmres_residual", stream, yd, vd, rows, size, b, t, r, 0, prm, st, part)
 
-        # The dummy first pass only forms r0 (gmres.py:313-317).
-        residual()
-        K.call("lxa_gmres_cycle_end", stream, rows, size, 1, prm, st, gs, part)
-        cycles = 1
-        while cycles < max_steps and bool(st.active().any()):
+        def cycle():
+            stream = K.current_stream(b)  # the capture stream while a graph is recorded
             K.call("lxa_gmres_cycle_begin", stream, vd, rows, size, r, basis, prm, st, gs, hess, part)
             for j in range(m):
                 w = M(A(basis[:, j, :], vdt), vdt).contiguous()
@@ -106,7 +107,26 @@ class GMRES(AbstractLinearSolver):
             K.call("lxa_gmres_update", stream, yd, vd, rows, size, basis, y, prm, st, gs, hess, z, part)
             residual()
             K.call("lxa_gmres_cycle_end", stream, rows, size, 0, prm, st, gs, part)
+
+        # The dummy first pass only forms r0 (gmres.py:313-317).
+        residual()
+        K.call("lxa_gmres_cycle_end", stream, rows, size, 1, prm, st, gs, part)
+        cycles = 1
+        graph = None
+        use_graph = self.graph and dev.type == "cuda"
+        while cycles < max_steps and bool(st.active().any()):
+            if graph is not None:
+                graph.replay()
+            elif use_graph and cycles >= 2:
+                # one restart cycle captured once (after an eager one), then replayed
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    cycle()
+                graph.replay()
+            else:
+                cycle()
             cycles += 1
+        stream = K.current_stream(b)
         K.call("lxa_gmres_result", stream, rows, prm, st, gs)
 
         solution = unravel(y.reshape(*batch, size), struct, batch)

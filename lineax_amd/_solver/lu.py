@@ -29,7 +29,7 @@ class LU(AbstractLinearSolver):
     lineax's behaviour.)
     """
 
-    def __init__(self, large_threshold: int = 4096, large_precision: str = "bf16", large_tol: float | None = None):
+    def __init__(self, large_threshold: int = 4096, large_precision: str = "bf16x3", large_tol: float | None = None):
         self.large_threshold = int(large_threshold)
         self.large_precision = large_precision
         self.large_tol = large_tol

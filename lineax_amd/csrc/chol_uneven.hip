@@ -55,7 +55,7 @@ struct Lds {
   double* rowpart;   // [256]                    }
   double* rowabs;    // [256]                    }
   double* acc;       // [4][256]                 }
-  double* red;       // [8]                      }
+  double* red;       // [16]                     }
   float* rinv;       // [256] 1 / U_kk
   int* item;         // work-queue slot
   float* T;          // packed U, order N16
@@ -144,11 +144,14 @@ __device__ int factor_registers(float (&R)[16][16], int n, const Lds& s) {
       float u[16];
 #pragma unroll
       for (int p = 0; p < 16; ++p) u[p] = T[pku(N16, k0 + p, jc)];
+      // column sweep: once u[st] is final its multiples leave every later
+      // entry at once (15 independent FMAs), and the pivot division is a
+      // multiplication by the stored reciprocal
 #pragma unroll
-      for (int p = 0; p < 16; ++p) {
+      for (int st = 0; st < 16; ++st) {
+        u[st] *= s.rinv[k0 + st];
 #pragma unroll
-        for (int st = 0; st < p; ++st) u[p] -= T[pku(N16, k0 + st, k0 + p)] * u[st];
-        u[p] = u[p] / T[pku(N16, k0 + p, k0 + p)];
+        for (int p = st + 1; p < 16; ++p) u[p] -= T[pku(N16, k0 + st, k0 + p)] * u[st];
       }
       if (jvalid) {
 #pragma unroll
@@ -191,51 +194,116 @@ __device__ int factor_registers(float (&R)[16][16], int n, const Lds& s) {
 }
 
 // U^T U x = v for v held by one wave (lane l: v[l + 64c]); uses k < n only.
+// The solves are a chain of n dependent steps (broadcast the finished entry,
+// scale by the reciprocal pivot, update the rest).  The LDS operands of the
+// next group of steps (a row of U forward, a column backward, and the
+// reciprocal pivots) are loaded while the current group runs, so the LDS
+// latency stays off the chain.  (Groups of kG steps.)
+__device__ __forceinline__ void solve_load_row(const Lds& s, int N16, int n, int l, int k, bool live, float (&t)[4],
+                                               float& r) {
+  const float* T = s.T;
+  r = live ? s.rinv[k] : 0.0f;
+  const int rk = pku(N16, k, k);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int jj = l + 64 * c;
+    t[c] = (live && jj > k && jj < n) ? T[rk + (jj - k)] : 0.0f;
+  }
+}
+
+__device__ __forceinline__ void solve_load_col(const Lds& s, int N16, int l, int k, bool live, float (&t)[4],
+                                               float& r) {
+  const float* T = s.T;
+  r = live ? s.rinv[k] : 0.0f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int i = l + 64 * c;
+    t[c] = (live && i < k) ? T[pku(N16, i, k)] : 0.0f;
+  }
+}
+
+constexpr int kG = 1;  // steps per prefetch group (1 keeps the refined kernel free of scratch)
+
 __device__ void wave_solve(int n, const Lds& s, float (&v)[4]) {
   const int l = threadIdx.x & 63;
   const int N16 = round16(n);
-  const float* T = s.T;
 #pragma unroll
   for (int c0 = 0; c0 < 4; ++c0) {
     const int kend = min(64, n - 64 * c0);
-    for (int kk = 0; kk < kend; ++kk) {
-      const int k = 64 * c0 + kk;
-      const int rk = pku(N16, k, k);
-      const float yk = bcast(v[c0], kk) * s.rinv[k];
-      if (l == kk) v[c0] = yk;
+    if (kend <= 0) continue;
+    float ta[kG][4], ra[kG];
 #pragma unroll
-      for (int c = c0; c < 4; ++c) {
-        const int jj = l + 64 * c;
-        if (jj > k && jj < n) v[c] -= T[rk + (jj - k)] * yk;
+    for (int u = 0; u < kG; ++u) solve_load_row(s, N16, n, l, 64 * c0 + u, u < kend, ta[u], ra[u]);
+    for (int g = 0; g < kend; g += kG) {
+      float tb[kG][4], rb[kG];
+#pragma unroll
+      for (int u = 0; u < kG; ++u) solve_load_row(s, N16, n, l, 64 * c0 + g + kG + u, g + kG + u < kend, tb[u], rb[u]);
+#pragma unroll
+      for (int u = 0; u < kG; ++u) {
+        const int kk = g + u;
+        if (kk < kend) {
+          const int k = 64 * c0 + kk;
+          const float yk = bcast(v[c0], kk) * ra[u];
+          if (l == kk) v[c0] = yk;
+#pragma unroll
+          for (int c = c0; c < 4; ++c) v[c] -= ta[u][c] * yk;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kG; ++u) {
+        ra[u] = rb[u];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ta[u][c] = tb[u][c];
       }
     }
   }
 #pragma unroll
   for (int c0 = 3; c0 >= 0; --c0) {
     const int kend = min(64, n - 64 * c0);
-    for (int kk = kend - 1; kk >= 0; --kk) {
-      const int k = 64 * c0 + kk;
-      const float xk = bcast(v[c0], kk) * s.rinv[k];
-      if (l == kk) v[c0] = xk;
+    if (kend <= 0) continue;
+    float ta[kG][4], ra[kG];
 #pragma unroll
-      for (int c = 0; c <= c0; ++c) {
-        const int i = l + 64 * c;
-        if (i < k) v[c] -= T[pku(N16, i, k)] * xk;
+    for (int u = 0; u < kG; ++u) solve_load_col(s, N16, l, 64 * c0 + kend - 1 - u, kend - 1 - u >= 0, ta[u], ra[u]);
+    for (int g = kend - 1; g >= 0; g -= kG) {
+      float tb[kG][4], rb[kG];
+#pragma unroll
+      for (int u = 0; u < kG; ++u) solve_load_col(s, N16, l, 64 * c0 + g - kG - u, g - kG - u >= 0, tb[u], rb[u]);
+#pragma unroll
+      for (int u = 0; u < kG; ++u) {
+        const int kk = g - u;
+        if (kk >= 0) {
+          const float xk = bcast(v[c0], kk) * ra[u];
+          if (l == kk) v[c0] = xk;
+#pragma unroll
+          for (int c = 0; c <= c0; ++c) v[c] -= ta[u][c] * xk;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kG; ++u) {
+        ra[u] = rb[u];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ta[u][c] = tb[u][c];
       }
     }
   }
 }
 
-__device__ double block_max(double v, double* red) {
-  v = wave_max(v);
+// Four block-wide maxima in one exchange (red holds 4 doubles per wave).
+__device__ void block_max4(double (&v)[4], double* red) {
   const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = wave_max(v[i]);
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[w] = v;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[4 * w + i] = v[i];
+  }
   __syncthreads();
-  const double out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = fmax(fmax(red[i], red[4 + i]), fmax(red[8 + i], red[12 + i]));
   __syncthreads();
-  return out;
 }
+
 
 // r = b - S x, S the symmetric matrix of A's upper triangle; optionally the
 // row sums of |S|.  Every upper-triangle element is read once from global
@@ -379,50 +447,53 @@ __global__ __launch_bounds__(kT) void chol_uneven_kernel(int64_t count, int max_
       __syncthreads();
       continue;
     }
-    if (w == 0) {
-      float v[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) v[c] = (l + 64 * c < n) ? B[l + 64 * c] : 0.0f;
-      wave_solve(n, s, v);
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (l + 64 * c < n) s.x[l + 64 * c] = static_cast<double>(v[c]);
-    }
-    __syncthreads();
-
+    // Pass -1 solves with b; later passes form the FP64 residual, test the
+    // backward error and solve for the correction (one solve call site).
     int steps = 0;
     double last_eta = -1.0;
-    if constexpr (kRefine) {
-      double bnorm = 0.0;
-      for (int i = tid; i < n; i += kT) bnorm = fmax(bnorm, fabs(static_cast<double>(B[i])));
-      bnorm = block_max(bnorm, s.red);
-      double anorm = 0.0;
-      for (int it = 0; it < max_refine; ++it) {
-        residual_sym_upper(n, A, B, s, it == 0);
-        double rmax = 0.0, xmax = 0.0, amax = 0.0;
-        for (int i = tid; i < n; i += kT) {
-          rmax = fmax(rmax, fabs(s.r[i]));
-          xmax = fmax(xmax, fabs(s.x[i]));
-          if (it == 0) amax = fmax(amax, s.rowabs[i]);
+    double bnorm = 0.0, anorm = 0.0;
+    const int passes = kRefine ? max_refine : 0;
+    for (int it = -1; it < passes; ++it) {
+      if constexpr (kRefine) {
+        if (it >= 0) {
+          residual_sym_upper(n, A, B, s, it == 0);
+          // |r|_inf, |x|_inf and (first pass) |A|_inf, |b|_inf in one exchange
+          double m4[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int i = tid; i < n; i += kT) {
+            m4[0] = fmax(m4[0], fabs(s.r[i]));
+            m4[1] = fmax(m4[1], fabs(s.x[i]));
+            if (it == 0) {
+              m4[2] = fmax(m4[2], s.rowabs[i]);
+              m4[3] = fmax(m4[3], fabs(static_cast<double>(B[i])));
+            }
+          }
+          block_max4(m4, s.red);
+          const double rmax = m4[0], xmax = m4[1];
+          if (it == 0) {
+            anorm = m4[2];
+            bnorm = m4[3];
+          }
+          const double denom = anorm * xmax + bnorm;
+          last_eta = denom > 0.0 ? rmax / denom : (rmax > 0.0 ? INFINITY : 0.0);
+          ++steps;
+          if (!(last_eta > tol)) break;  // also stops on NaN
         }
-        rmax = block_max(rmax, s.red);
-        xmax = block_max(xmax, s.red);
-        if (it == 0) anorm = block_max(amax, s.red);
-        const double denom = anorm * xmax + bnorm;
-        last_eta = denom > 0.0 ? rmax / denom : (rmax > 0.0 ? INFINITY : 0.0);
-        ++steps;
-        if (!(last_eta > tol)) break;  // also stops on NaN
-        if (w == 0) {
-          float v[4];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) v[c] = (l + 64 * c < n) ? static_cast<float>(s.r[l + 64 * c]) : 0.0f;
-          wave_solve(n, s, v);
-#pragma unroll
-          for (int c = 0; c < 4; ++c)
-            if (l + 64 * c < n) s.x[l + 64 * c] += static_cast<double>(v[c]);
-        }
-        __syncthreads();
       }
+      if (w == 0) {
+        float v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int i = l + 64 * c;
+          v[c] = i < n ? (it < 0 ? B[i] : static_cast<float>(s.r[i])) : 0.0f;
+        }
+        wave_solve(n, s, v);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int i = l + 64 * c;
+          if (i < n) s.x[i] = (it < 0 ? 0.0 : s.x[i]) + static_cast<double>(v[c]);
+        }
+      }
+      __syncthreads();
     }
     for (int i = tid; i < n; i += kT) X[i] = static_cast<TX>(s.x[i]);
     if (tid == 0) {

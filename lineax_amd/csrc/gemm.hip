@@ -1,6 +1,9 @@
 // C <- beta C + sign A B (beta in {0, 1}, sign = +-1) on the gfx950 matrix
 // cores, float32 storage, with the products taken in bfloat16, float16 or
-// float32 (FP32 accumulation in all three).  The Schur-complement updates of
+// float32 (FP32 accumulation in all), or in split bfloat16 ("bf16x3": each
+// operand x = hi + lo with hi = bf16(x), lo = bf16(x - hi), and A B taken as
+// A_hi B_hi + A_hi B_lo + A_lo B_hi, about 16 significant bits per operand
+// for three bf16 MFMAs).  The Schur-complement updates of
 // the blocked LU and its triangular block products run here.  With beta = 0
 // the update may be done in place when one operand's tile is exactly the C
 // tile it produces (M <= 128 with B aliasing C, or N <= 128 with A aliasing
@@ -50,6 +53,11 @@ struct GemmMode<LXA_GEMM_F32> {
   using E = float;
   static constexpr int kPad = 1;
 };
+template <>
+struct GemmMode<LXA_GEMM_BF16X3> {
+  using E = __bf16;
+  static constexpr int kPad = 8;
+};
 
 template <int MODE>
 __device__ __forceinline__ void mfma_block(floatx16& acc, const typename GemmMode<MODE>::E* a_row,
@@ -66,7 +74,7 @@ __device__ __forceinline__ void mfma_block(floatx16& acc, const typename GemmMod
 #pragma unroll
     for (int kk = 0; kk < kBK; kk += 16) {
       const int ko = kk + 8 * (lane >> 5);
-      if constexpr (MODE == LXA_GEMM_BF16) {
+      if constexpr (MODE == LXA_GEMM_BF16 || MODE == LXA_GEMM_BF16X3) {
         const bf16x8 av = *reinterpret_cast<const bf16x8*>(a_row + ko);
         const bf16x8 bv = *reinterpret_cast<const bf16x8*>(b_col + ko);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
@@ -85,8 +93,10 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(int M, int N, int K,
                                                             int beta, float sign) {
   using E = typename GemmMode<MODE>::E;
   constexpr int LDK = kBK + GemmMode<MODE>::kPad;
-  __shared__ __align__(16) E sa[2][kBM][LDK];
-  __shared__ __align__(16) E sb[2][kBN][LDK];
+  constexpr bool kSplit = MODE == LXA_GEMM_BF16X3;
+  constexpr int kLo = kSplit ? 2 : 1;  // [0] hi (or the only) part, [1] lo part
+  __shared__ __align__(16) E sa[kLo][2][kBM][LDK];
+  __shared__ __align__(16) E sb[kLo][2][kBN][LDK];
 
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
@@ -146,20 +156,25 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(int M, int N, int K,
       }
     }
   };
+  auto put = [&](E (*dst)[2][kBM][LDK], int buf, int r, int k, float v) {
+    const E hi = (E)v;
+    dst[0][buf][r][k] = hi;
+    if constexpr (kSplit) dst[1][buf][r][k] = (E)(v - (float)hi);
+  };
   auto store_tiles = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int idx = tid + kGemmThreads * q;
       const int ar = idx >> 3, ak = 4 * (idx & 7);
-      sa[buf][ar][ak + 0] = (E)(sign * ra[q].x);
-      sa[buf][ar][ak + 1] = (E)(sign * ra[q].y);
-      sa[buf][ar][ak + 2] = (E)(sign * ra[q].z);
-      sa[buf][ar][ak + 3] = (E)(sign * ra[q].w);
+      put(sa, buf, ar, ak + 0, sign * ra[q].x);
+      put(sa, buf, ar, ak + 1, sign * ra[q].y);
+      put(sa, buf, ar, ak + 2, sign * ra[q].z);
+      put(sa, buf, ar, ak + 3, sign * ra[q].w);
       const int bk = idx >> 5, bc = 4 * (idx & 31);
-      sb[buf][bc + 0][bk] = (E)rb[q].x;
-      sb[buf][bc + 1][bk] = (E)rb[q].y;
-      sb[buf][bc + 2][bk] = (E)rb[q].z;
-      sb[buf][bc + 3][bk] = (E)rb[q].w;
+      put(sb, buf, bc + 0, bk, rb[q].x);
+      put(sb, buf, bc + 1, bk, rb[q].y);
+      put(sb, buf, bc + 2, bk, rb[q].z);
+      put(sb, buf, bc + 3, bk, rb[q].w);
     }
   };
 
@@ -173,9 +188,16 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(int M, int N, int K,
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-        mfma_block<MODE>(acc[bi][bj], &sa[buf][wm + 32 * bi + (lane & 31)][0], &sb[buf][wn + 32 * bj + (lane & 31)][0],
-                         lane);
+      for (int bj = 0; bj < 2; ++bj) {
+        const E* ah = &sa[0][buf][wm + 32 * bi + (lane & 31)][0];
+        const E* bh = &sb[0][buf][wn + 32 * bj + (lane & 31)][0];
+        if constexpr (kSplit) {
+          // small terms first: lo parts, then hi x hi
+          mfma_block<MODE>(acc[bi][bj], ah, &sb[1][buf][wn + 32 * bj + (lane & 31)][0], lane);
+          mfma_block<MODE>(acc[bi][bj], &sa[1][buf][wm + 32 * bi + (lane & 31)][0], bh, lane);
+        }
+        mfma_block<MODE>(acc[bi][bj], ah, bh, lane);
+      }
     if (kt + 1 < ktiles) store_tiles(buf ^ 1);
     __syncthreads();
   }
@@ -209,6 +231,9 @@ int gemm_update(hipStream_t st, int mode, int M, int N, int K, const float* A, i
       break;
     case LXA_GEMM_F32:
       gemm_kernel<LXA_GEMM_F32><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign);
+      break;
+    case LXA_GEMM_BF16X3:
+      gemm_kernel<LXA_GEMM_BF16X3><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign);
       break;
     default:
       return set_error(LXA_EINVAL, "unknown gemm mode %d", mode);

@@ -22,8 +22,8 @@
 //   swaps applied across the matrix, (2) L21 = A21 U11^-1 and U12 = L11^-1 A12
 //   as float32 matrix-core products, (3) the panel's Schur update in float32.
 //   Then the outer U12 row block (128 rows at a time) and the outer Schur
-//   update A22 -= L21 U12, whose products run in bfloat16 or float16 (or
-//   float32) with float32 accumulation.
+//   update A22 -= L21 U12, whose products run in bfloat16, float16, split
+//   bfloat16 (bf16x3) or float32, with float32 accumulation.
 //
 // Solves: block triangular solves with 128-row blocks in one persistent
 // launch each.  Workgroups take blocks in ticket order (an atomic counter),
@@ -220,6 +220,11 @@ __global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, fl
   }
 }
 
+__global__ void zero_i32_kernel(int32_t* __restrict__ p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
 // The row swaps of diagonal block k (rows k .. k + nb - 1, in order) applied
 // to every column outside the block: one thread per column, the block's rows
 // staged in LDS.
@@ -282,19 +287,72 @@ __global__ void rhs_swaps_kernel(int n, int nrhs, float* __restrict__ x, int64_t
 // inv holds the inverses of the diagonal blocks of L (kinds 0, 3) or U
 // (kinds 1, 2), 128 x 128 each; transposed kinds read blocks transposed.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float lu_block_elem(const float* lu, int64_t lda, int kind, int bi, int bj, int r, int c) {
+constexpr int kTileLd = kB + 4;  // LDS row stride of a staged 128 x 128 block (16-byte rows)
+
+// Stages block (bi, bj) of the operator of `kind` into tile[r][c] (r, c <
+// 128; zeros outside rows < rows, cols < cols).  Loads are float4 along the
+// stored rows: for the transposed kinds the stored rows are the tile's
+// columns, so they are written transposed.
+__device__ void stage_block(float* tile, const float* lu, int64_t lda, int kind, int bi, int bj, int rows, int cols,
+                            int tid) {
   const bool tr = kind >= 2;
-  const int64_t row = tr ? (int64_t)bj * kB + c : (int64_t)bi * kB + r;
-  const int64_t col = tr ? (int64_t)bi * kB + r : (int64_t)bj * kB + c;
-  return lu[row * lda + col];
+  // stored block: row index sr (0..127), 4-column group sc (0..31)
+  for (int idx = tid; idx < kB * (kB / 4); idx += kTrsvThreads) {
+    const int sr = idx >> 5, sc = 4 * (idx & 31);
+    // stored element (sr, sc..sc+3) is tile (sr, sc..) or, transposed, tile (sc.., sr)
+    const int srow_lim = tr ? cols : rows, scol_lim = tr ? rows : cols;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (sr < srow_lim) {
+      const int64_t grow = tr ? (int64_t)bj * kB + sr : (int64_t)bi * kB + sr;
+      const int64_t gcol = tr ? (int64_t)bi * kB + sc : (int64_t)bj * kB + sc;
+      const float* p = lu + grow * lda + gcol;
+      if (sc + 3 < scol_lim) {
+        v = *reinterpret_cast<const float4*>(p);
+      } else {
+        v.x = sc < scol_lim ? p[0] : 0.f;
+        v.y = sc + 1 < scol_lim ? p[1] : 0.f;
+        v.z = sc + 2 < scol_lim ? p[2] : 0.f;
+        v.w = sc + 3 < scol_lim ? p[3] : 0.f;
+      }
+    }
+    if (!tr) {
+      *reinterpret_cast<float4*>(tile + sr * kTileLd + sc) = v;
+    } else {
+      tile[(sc + 0) * kTileLd + sr] = v.x;
+      tile[(sc + 1) * kTileLd + sr] = v.y;
+      tile[(sc + 2) * kTileLd + sr] = v.z;
+      tile[(sc + 3) * kTileLd + sr] = v.w;
+    }
+  }
+}
+
+// part[rr] = sum_c tile[row][c] * vec[rr][c] over this thread's 64 columns,
+// combined with the partner thread holding the other 64.
+__device__ __forceinline__ void tile_product(const float* tile, const float (*vec)[kB], int row, int hcol,
+                                             float (&part)[kMaxRhs]) {
+#pragma unroll
+  for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] = 0.f;
+  const float* tr = tile + row * kTileLd + hcol;
+#pragma unroll 4
+  for (int c = 0; c < 64; c += 4) {
+    const float4 e = *reinterpret_cast<const float4*>(tr + c);
+#pragma unroll
+    for (int rr = 0; rr < kMaxRhs; ++rr)
+      part[rr] += e.x * vec[rr][hcol + c] + e.y * vec[rr][hcol + c + 1] + e.z * vec[rr][hcol + c + 2] +
+                  e.w * vec[rr][hcol + c + 3];
+  }
+#pragma unroll
+  for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += __shfl_xor(part[rr], 1, kWave);
 }
 
 __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nrhs, const float* __restrict__ lu,
                                                                    int64_t lda, const float* __restrict__ inv,
                                                                    float* __restrict__ x, int64_t ldx, int kind,
                                                                    int32_t* __restrict__ sync) {
-  __shared__ float xs[kMaxRhs][kB];   // the solved block being consumed
-  __shared__ float acc[kMaxRhs][kB];  // this block's right-hand side
+  extern __shared__ __align__(16) float tsm[];
+  float* tile = tsm;                                                     // [128][132]
+  float(*xs)[kB] = reinterpret_cast<float(*)[kB]>(tsm + kB * kTileLd);   // [8][128] solved block
+  float(*acc)[kB] = reinterpret_cast<float(*)[kB]>(tsm + kB * kTileLd + kMaxRhs * kB);  // [8][128]
   __shared__ int s_ticket;
   int32_t* ticket = sync;
   int32_t* flags = sync + 1;
@@ -312,11 +370,13 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
     const int rr = idx / kB, i = idx % kB;
     acc[rr][i] = (rr < nrhs && i < rows) ? x[(int64_t)rr * ldx + r0 + i] : 0.f;
   }
-  __syncthreads();
   // row i of the block handled by threads 2i, 2i + 1 (64 columns each)
   const int row = tid >> 1, hcol = (tid & 1) * 64;
   for (int step = 0; step < t; ++step) {
     const int bj = forward ? step : nblk - 1 - step;
+    const int c0 = bj * kB, cols = min(kB, n - c0);
+    // the operator block does not depend on the solution: stage it first
+    stage_block(tile, lu, lda, kind, bi, bj, rows, cols, tid);
     if (tid == 0) {
       int spin = 0;
       while (__hip_atomic_load(&flags[bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
@@ -329,24 +389,13 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
     }
     __syncthreads();
     (void)__hip_atomic_load(&flags[bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const int c0 = bj * kB, cols = min(kB, n - c0);
     for (int idx = tid; idx < kMaxRhs * kB; idx += kTrsvThreads) {
       const int rr = idx / kB, i = idx % kB;
       xs[rr][i] = (rr < nrhs && i < cols) ? x[(int64_t)rr * ldx + c0 + i] : 0.f;
     }
     __syncthreads();
     float part[kMaxRhs];
-#pragma unroll
-    for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] = 0.f;
-    if (row < rows) {
-      for (int c = hcol; c < hcol + 64 && c < cols; ++c) {
-        const float e = lu_block_elem(lu, lda, kind, bi, bj, row, c);
-#pragma unroll
-        for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += e * xs[rr][c];
-      }
-    }
-#pragma unroll
-    for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += __shfl_xor(part[rr], 1, kWave);
+    tile_product(tile, xs, row, hcol, part);
     if ((tid & 1) == 0 && row < rows) {
 #pragma unroll
       for (int rr = 0; rr < kMaxRhs; ++rr)
@@ -354,19 +403,27 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
     }
     __syncthreads();
   }
-  // y = inv(D) acc
-  const float* D = inv + (int64_t)bi * kB * kB;
-  const bool tr = kind >= 2;
-  float part[kMaxRhs];
-#pragma unroll
-  for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] = 0.f;
-  for (int c = hcol; c < hcol + 64; ++c) {
-    const float e = tr ? D[(int64_t)c * kB + row] : D[(int64_t)row * kB + c];
-#pragma unroll
-    for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += e * acc[rr][c];
+  // y = inv(D) acc, with D staged like an operator block (transposed kinds
+  // use inv(D)^T)
+  {
+    const float* D = inv + (int64_t)bi * kB * kB;
+    const bool tr = kind >= 2;
+    for (int idx = tid; idx < kB * (kB / 4); idx += kTrsvThreads) {
+      const int sr = idx >> 5, sc = 4 * (idx & 31);
+      const float4 v = *reinterpret_cast<const float4*>(D + sr * kB + sc);
+      if (!tr) {
+        *reinterpret_cast<float4*>(tile + sr * kTileLd + sc) = v;
+      } else {
+        tile[(sc + 0) * kTileLd + sr] = v.x;
+        tile[(sc + 1) * kTileLd + sr] = v.y;
+        tile[(sc + 2) * kTileLd + sr] = v.z;
+        tile[(sc + 3) * kTileLd + sr] = v.w;
+      }
+    }
   }
-#pragma unroll
-  for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += __shfl_xor(part[rr], 1, kWave);
+  __syncthreads();
+  float part[kMaxRhs];
+  tile_product(tile, acc, row, hcol, part);
   if ((tid & 1) == 0 && row < rows) {
 #pragma unroll
     for (int rr = 0; rr < kMaxRhs; ++rr)
@@ -535,7 +592,8 @@ extern "C" int lxa_lu_large_factor(void* stream, int mode, int64_t n, float* a, 
                                    int32_t* piv, int32_t* info) {
   LXA_REQUIRE(n >= 0 && n <= (1 << 20), "n out of range (%lld)", (long long)n);
   LXA_REQUIRE(lda >= n, "lda (%lld) < n (%lld)", (long long)lda, (long long)n);
-  LXA_REQUIRE(mode == LXA_GEMM_F32 || mode == LXA_GEMM_BF16 || mode == LXA_GEMM_F16, "unknown mode %d", mode);
+  LXA_REQUIRE(mode == LXA_GEMM_F32 || mode == LXA_GEMM_BF16 || mode == LXA_GEMM_F16 || mode == LXA_GEMM_BF16X3,
+              "unknown mode %d", mode);
   if (n == 0) return LXA_OK;
   LXA_REQUIRE(a && linv && uinv && piv && info, "a, linv, uinv, piv and info must be non-NULL");
   return lu_large_factor_impl(static_cast<hipStream_t>(stream), mode, (int)n, a, lda, linv, uinv, piv, info);
@@ -562,9 +620,14 @@ extern "C" int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, co
   LXA_REQUIRE(lu && inv && x && sync && ldx >= n, "invalid pointers or ldx");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int nblk = (int)((n + kB - 1) / kB);
-  hipError_t e = hipMemsetAsync(sync, 0, sizeof(int32_t) * (size_t)(nblk + 2), st);
-  if (e != hipSuccess) return hip_error(e, "hipMemsetAsync(trsv sync)");
-  trsv_blocks_kernel<<<nblk, kTrsvThreads, 0, st>>>((int)n, nrhs, lu, lda, inv, x, ldx, kind, sync);
+  // a fill kernel rather than a memset node, so the call can sit in a captured graph
+  zero_i32_kernel<<<(nblk + 2 + 255) / 256, 256, 0, st>>>(sync, nblk + 2);
+  LXA_LAUNCH_CHECK("zero_i32_kernel");
+  const size_t smem = sizeof(float) * (kB * kTileLd + 2 * kMaxRhs * kB);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&trsv_blocks_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(trsv_blocks_kernel)");
+  trsv_blocks_kernel<<<nblk, kTrsvThreads, smem, st>>>((int)n, nrhs, lu, lda, inv, x, ldx, kind, sync);
   LXA_LAUNCH_CHECK("trsv_blocks_kernel");
   return LXA_OK;
 }

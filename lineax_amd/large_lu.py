@@ -6,8 +6,11 @@
    to a multiple of 4 with an identity block;
 2. P A' = L U with pivoting confined to each 128 x 128 diagonal block,
    blocked: float32 panels and triangular block products, Schur-complement
-   products in bfloat16 / float16 / float32 on the matrix cores with float32
-   accumulation (``precision``);
+   products on the matrix cores with float32 accumulation in bfloat16,
+   float16, split bfloat16 ("bf16x3": hi*hi + hi*lo + lo*hi, the default) or
+   float32 (``precision``).  Plain bf16 / fp16 factors are too inexact for
+   ill-conditioned operators (GMRES then needs hundreds of steps, and fp16
+   can overflow); bf16x3 keeps float32-level factors at bf16 MFMA rates;
 3. ``solve`` refines against the original A: FP64 residuals r = b - A x
    (float32 A, FP64 accumulation), each correction from GMRES on A
    preconditioned by x -> V (L U)^-1 P U^T x (float32 basis, FP64 inner
@@ -70,7 +73,7 @@ class _Preconditioner(FunctionLinearOperator):
 class LargeLU:
     """Factorisation of one large float32 matrix (see the module docstring)."""
 
-    def __init__(self, a: torch.Tensor, precision: str = "bf16", seed: int = 0):
+    def __init__(self, a: torch.Tensor, precision: str = "bf16x3", seed: int = 0):
         K._gpu(a, "matrix")
         if a.dim() != 2 or a.shape[0] != a.shape[1]:
             raise ValueError(f"LargeLU needs one square matrix, got {tuple(a.shape)}")
