@@ -317,8 +317,10 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
   double xi[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) xi[c] = (l + 64 * c < n) ? s.x[l + 64 * c] : 0.0;
-  for (int j0 = w; j0 < n; j0 += 32) {
-    float av[8][4];
+  // Rows j0 + 4u (u < 8) per batch; the next batch's loads are issued before
+  // this batch is reduced (the eight row reductions are independent, so their
+  // shuffles interleave).
+  auto load_rows = [&](int j0, float (&av)[8][4]) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int j = j0 + 4 * u;
@@ -329,11 +331,17 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
         av[u][c] = (j < n && i >= j && i < n) ? Aj[i] : 0.0f;
       }
     }
+  };
+  float av[8][4];
+  load_rows(w, av);
+  for (int j0 = w; j0 < n; j0 += 32) {
+    float nv[8][4];
+    load_rows(j0 + 32, nv);  // rows past n load nothing (zero)
+    double red[16];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int j = j0 + 4 * u;
-      if (j >= n) break;
-      const double xj = s.x[j];
+      const double xj = j < n ? s.x[j] : 0.0;
       double dot = 0.0, rab = 0.0;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -346,13 +354,23 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
           aab[c] += fabs(aji);
         }
       }
-      dot = wave_sum(dot);
-      if (want_abs) rab = wave_sum(rab);
-      if (l == 0) {
+      red[u] = dot;
+      red[8 + u] = rab;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + 4 * u;
+      const double dot = wave_sum(red[u]);
+      const double rab = want_abs ? wave_sum(red[8 + u]) : 0.0;
+      if (l == 0 && j < n) {
         s.rowpart[j] = dot;
         if (want_abs) s.rowabs[j] = rab;
       }
     }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) av[u][c] = nv[u][c];
   }
 #pragma unroll
   for (int c = 0; c < 4; ++c) {

@@ -61,6 +61,38 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
+template <int K>
+struct Log2 {
+  static constexpr int value = 1 + Log2<K / 2>::value;
+};
+template <>
+struct Log2<1> {
+  static constexpr int value = 0;
+};
+
+// Sums K values (K a power of two <= 64) over the 64 lanes of a wave.  On
+// return, every lane holds the complete wave sum of value number
+// lane >> (6 - log2 K).  v is destroyed.
+template <int K, typename R>
+__device__ __forceinline__ R wave_reduce_scatter(R (&v)[K], int lane) {
+  int off = 32;
+#pragma unroll
+  for (int half = K / 2; half >= 1; half >>= 1) {
+    const bool upper = (lane & off) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      const R keep = upper ? v[i + half] : v[i];
+      const R send = upper ? v[i] : v[i + half];
+      v[i] = keep + __shfl_xor(send, off, kWave);
+    }
+    off >>= 1;
+  }
+  R x = v[0];
+#pragma unroll
+  for (int o = 32 / K; o >= 1; o >>= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+
 // Index of element (i, j), j >= i, in a row-packed upper triangle of order n.
 __device__ __forceinline__ int packed_upper(int n, int i, int j) {
   return i * n - ((i * (i - 1)) >> 1) + (j - i);

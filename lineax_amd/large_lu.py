@@ -141,11 +141,15 @@ class LargeLU:
         return torch.where(num == 0, torch.zeros_like(num), num / den)
 
     def solve(self, b: torch.Tensor, *, tol: float = 1e-12, max_refine: int = 10, trans: bool = False,
-              inner_rtol: float = 1e-4, restart: int = 30, inner_max_steps: int = 300):
+              inner_rtol: float = 1e-4, restart: int = 4, inner_max_steps: int = 50):
         """Refined solution x (float64, [nrhs, n]) of A x = b (A^T x = b if trans).
 
         Returns (x, eta, stats) with eta the last backward error per
-        right-hand side; stats counts residuals and GMRES steps.
+        right-hand side; stats counts residuals and GMRES steps (restart
+        cycles of ``restart`` Arnoldi iterations: with factors this close to
+        A, a few iterations per correction suffice, so short cycles avoid
+        preconditioner applications that lineax's cycle-end convergence test
+        would otherwise spend).
         """
         from . import _solve, _solver
 
