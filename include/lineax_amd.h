@@ -239,6 +239,35 @@ int lxa_cg_finish(void* stream, int vdtype, int64_t rows, int64_t n, const void*
                   const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
 int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state);
 
+/* Fused CG for a constant-coefficient five-point stencil operator (the
+ * matrix-free FunctionLinearOperator of lineax/_operator.py:664-732 whose mv
+ * is lxa_stencil5_apply) with the identity preconditioner.  One iteration of
+ * lineax/_solver/cg.py:169-206 is two launches:
+ *   lxa_cg_stencil_direction: p_new = r + beta p_old (p_new = r when first),
+ *     <A p_new, p_new> and alpha with the rcond guard (cg.py:172-178);
+ *   lxa_cg_stencil_update: y += alpha p_new; r -= alpha A p_new, or, when
+ *     stable, r = b - A y (cg.py:187-198); gamma, beta, step, both stopping
+ *     norms and the loop condition (cg.py:150-167, 202-206).
+ * The per-row scalars are finished on the device by a one-block-per-row
+ * kernel after each sweep.  Rows are grid functions [rows][nx][ny] (row
+ * stride nx*ny); coeffs = (center, west, east, south, north) as in
+ * lxa_stencil5_apply.  p_new must not alias r or p_old (ping-pong two
+ * direction buffers; p_old is read even when first, so pass a valid buffer).
+ * Stabilising steps read y around each tile, so they write y_new to
+ * y_out != y_in; plain steps pass y_out == y_in (in place); a finished row's
+ * iterate is copied across.  b may be stored in float32 (bdtype) when that is
+ * exact.  Dtypes (y, vectors, b): (f64, f32, f32|f64), (f64, f64, f64), (f32,
+ * f32, f32).  Workspace: FP64 partials of the size lxa_cg_stencil_workspace
+ * returns. */
+int lxa_cg_stencil_workspace(int64_t rows, int64_t nx, int64_t ny, size_t* partial_bytes);
+int lxa_cg_stencil_direction(void* stream, int vdtype, int64_t rows, int64_t nx, int64_t ny, const double* coeffs,
+                             const void* r, const void* p_old, void* p_new, int first,
+                             const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
+int lxa_cg_stencil_update(void* stream, int ydtype, int vdtype, int bdtype, int64_t rows, int64_t nx, int64_t ny,
+                          const double* coeffs, const void* p, const void* y_in, void* y_out, void* r,
+                          const void* b, int stable,
+                          const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
+
 /* GMRES: init, then per restart cycle: cycle_begin (normalise r into basis
  * column 0), for j < restart: [caller: w = M A v_j] arnoldi(j), then update
  * (Hessenberg least squares by Householder QR, y += V z), [caller:

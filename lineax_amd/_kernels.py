@@ -55,6 +55,10 @@ _SIGNATURES = {
     "lxa_cg_precond_gamma": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "lxa_cg_finish": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "lxa_cg_result": [_vp, _i64, _vp, _vp],
+    "lxa_cg_stencil_workspace": [_i64, _i64, _i64, _szp],
+    "lxa_cg_stencil_direction": [_vp, _int, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _int, _vp, _vp, _vp],
+    "lxa_cg_stencil_update": [_vp, _int, _int, _int, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _vp,
+                              _vp],
     "lxa_gmres_init": [_vp, _i64, _vp, _vp],
     "lxa_gmres_cycle_begin": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "lxa_gmres_arnoldi": [_vp, _int, _i64, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -398,15 +402,18 @@ def stencil5(x: torch.Tensor, nx: int, ny: int, coeffs, out_dtype=None) -> torch
         raise ValueError(f"vector of shape {tuple(x.shape)} is not a batch of {nx}x{ny} grids")
     rows = x.numel() // size
     xv = x.reshape(rows, size)
-    if not xv.is_contiguous():
+    # rows may be strided (e.g. one column of a Krylov basis); each row's grid
+    # must be contiguous
+    if xv.stride(1) != 1 or (rows > 1 and xv.stride(0) < size):
         xv = xv.contiguous()
+    xstride = xv.stride(0) if rows > 1 else size
     y = torch.empty((rows, size), dtype=out_dtype, device=x.device)
     if rows == 0:
         return y.reshape(x.shape)
     c = (ctypes.c_double * 5)(*[float(v) for v in coeffs])
     _check(
         load().lxa_stencil5_apply(_stream(x), _dtype_code(x.dtype), _dtype_code(out_dtype), rows, nx, ny,
-                                  ctypes.cast(c, ctypes.c_void_p), _ptr(xv), size, _ptr(y), size),
+                                  ctypes.cast(c, ctypes.c_void_p), _ptr(xv), xstride, _ptr(y), size),
         "lxa_stencil5_apply",
     )
     return y.reshape(x.shape)
@@ -461,6 +468,17 @@ def partials_buffer(rows: int, n: int, restart: int, device) -> torch.Tensor:
     need = ctypes.c_size_t(0)
     _check(load().lxa_krylov_partials_size(rows, n, restart, ctypes.byref(need)), "lxa_krylov_partials_size")
     return torch.empty(max(8, int(need.value) // 8), dtype=torch.float64, device=device)
+
+
+class StencilCGWorkspace:
+    """Partials and coefficients of the fused stencil CG sweeps."""
+
+    def __init__(self, rows: int, nx: int, ny: int, coeffs, device):
+        pb = ctypes.c_size_t(0)
+        _check(load().lxa_cg_stencil_workspace(rows, nx, ny, ctypes.byref(pb)), "lxa_cg_stencil_workspace")
+        self.partials = torch.empty(max(1, int(pb.value) // 8), dtype=torch.float64, device=device)
+        self._coeffs = (ctypes.c_double * 5)(*[float(c) for c in coeffs])
+        self.coeffs = ctypes.cast(self._coeffs, ctypes.c_void_p).value
 
 
 def call(name: str, *args) -> None:

@@ -9,6 +9,8 @@ from .._misc import resolve_rcond
 from .._norm import max_norm, rms_norm, two_norm
 from .._operator import (
     IdentityLinearOperator,
+    NegLinearOperator,
+    TaggedLinearOperator,
     conj,
     mv_as,
     is_negative_semidefinite,
@@ -25,6 +27,28 @@ NORM_KINDS = {max_norm: 0, two_norm: 1, rms_norm: 2}
 MAX_CHECK_BLOCK = 32
 # Iterations per captured graph (rounded up to a multiple of stabilise_every).
 GRAPH_BLOCK = 30
+
+
+def stencil_of(operator):
+    """(nx, ny, coeffs) when ``operator`` is a five-point stencil operator,
+    possibly tagged or negated; else None."""
+    from ..stencil import StencilOperator
+
+    sign = 1.0
+    while True:
+        if isinstance(operator, TaggedLinearOperator):
+            operator = operator.operator
+        elif isinstance(operator, NegLinearOperator):
+            operator = operator.operator
+            sign = -sign
+        else:
+            break
+    if isinstance(operator, StencilOperator):
+        return operator.nx, operator.ny, tuple(sign * c for c in operator.coeffs)
+    return None
+
+
+_FUSED_DTYPES = {(torch.float64, torch.float32), (torch.float64, torch.float64), (torch.float32, torch.float32)}
 
 
 def norm_kind(norm) -> int:
@@ -64,10 +88,13 @@ class CG(AbstractLinearSolver):
     (not in lineax) captures blocks of iterations in a HIP graph and replays
     them, checking convergence on the host between replays; the operator
     must then be capturable (device work only, no host synchronisation).
+    ``fused`` (not in lineax): for a five-point stencil operator with no
+    preconditioner, run each iteration as the two fused sweeps of
+    lxa_cg_stencil_direction / _update instead of six separate kernels.
     """
 
     def __init__(self, rtol, atol, norm=max_norm, stabilise_every: int | None = 10, max_steps: int | None = None,
-                 working_dtype: torch.dtype | None = None, graph: bool = False):
+                 working_dtype: torch.dtype | None = None, graph: bool = False, fused: bool = True):
         if isinstance(rtol, (int, float)) and rtol < 0:
             raise ValueError("Tolerances must be non-negative.")
         if isinstance(atol, (int, float)) and atol < 0:
@@ -81,6 +108,7 @@ class CG(AbstractLinearSolver):
         self.max_steps = max_steps
         self.working_dtype = working_dtype
         self.graph = graph
+        self.fused = fused
 
     def init(self, operator, options):
         is_nsd = is_negative_semidefinite(operator)
@@ -130,24 +158,55 @@ class CG(AbstractLinearSolver):
         K.call("lxa_cg_begin", stream, vd, rows, size, z, r, prm, st, part)
 
         se = self.stabilise_every
+        stencil = stencil_of(operator) if (self.fused and not precond and dev.type == "cuda") else None
+        if stencil is not None and (stencil[0] * stencil[1] != size or (ydt, vdt) not in _FUSED_DTYPES):
+            stencil = None
 
-        def iteration(i):
-            stream = K.current_stream(b)  # the capture stream while a graph is recorded
-            q = A(p, vdt)
-            K.call("lxa_cg_alpha", stream, vd, rows, size, q, p, prm, st, part)
-            stable = se == 1 or (se is not None and (i + 1) % se == 0)
-            fuse = (not precond) and (not stable)
-            K.call("lxa_cg_update", stream, yd, vd, rows, size, p, q, y, r, b, int(stable), int(fuse), prm, st,
-                   part)
-            if stable:
-                t = A(y, ydt)
-                K.call("lxa_cg_residual", stream, yd, vd, rows, size, b, t, r, int(not precond), prm, st, part)
-            if precond:
-                z = M(r, vdt)
-                K.call("lxa_cg_precond_gamma", stream, yd, vd, rows, size, b, z, r, prm, st, part)
-            else:
-                z = r
-            K.call("lxa_cg_finish", stream, vd, rows, size, z, p, prm, st, part)
+        if stencil is None:
+            def iteration(i):
+                stream = K.current_stream(b)  # the capture stream while a graph is recorded
+                q = A(p, vdt)
+                K.call("lxa_cg_alpha", stream, vd, rows, size, q, p, prm, st, part)
+                stable = se == 1 or (se is not None and (i + 1) % se == 0)
+                fuse = (not precond) and (not stable)
+                K.call("lxa_cg_update", stream, yd, vd, rows, size, p, q, y, r, b, int(stable), int(fuse), prm, st,
+                       part)
+                if stable:
+                    t = A(y, ydt)
+                    K.call("lxa_cg_residual", stream, yd, vd, rows, size, b, t, r, int(not precond), prm, st, part)
+                if precond:
+                    z = M(r, vdt)
+                    K.call("lxa_cg_precond_gamma", stream, yd, vd, rows, size, b, z, r, prm, st, part)
+                else:
+                    z = r
+                K.call("lxa_cg_finish", stream, vd, rows, size, z, p, prm, st, part)
+        else:
+            nx, ny, coeffs = stencil
+            ws = K.StencilCGWorkspace(rows, nx, ny, coeffs, dev)
+            # b is read in float32 when that is exact (the usual case of float32
+            # data promoted to a float64 solve): same values, half the traffic
+            b_read, bd = b, yd
+            if ydt == torch.float64 and vdt == torch.float32:
+                b32 = b.to(torch.float32)
+                if torch.equal(b32.to(torch.float64), b):
+                    b_read, bd = b32, K.dtype_code(torch.float32)
+            pbuf = (p, torch.empty_like(p))
+            # stabilising steps write the iterate to the other buffer
+            ybuf = [y, torch.empty_like(y)]
+            ycur = [0]
+
+            def iteration(i):
+                stream = K.current_stream(b)
+                src, dst = pbuf[i % 2], pbuf[(i + 1) % 2]
+                K.call("lxa_cg_stencil_direction", stream, vd, rows, nx, ny, ws.coeffs, r, src, dst, int(i == 0), prm,
+                       st, ws.partials)
+                stable = se == 1 or (se is not None and (i + 1) % se == 0)
+                y_in = ybuf[ycur[0]]
+                if stable:
+                    ycur[0] ^= 1
+                y_out = ybuf[ycur[0]]
+                K.call("lxa_cg_stencil_update", stream, yd, vd, bd, rows, nx, ny, ws.coeffs, dst, y_in, y_out, r,
+                       b_read, int(stable), prm, st, ws.partials)
 
         it = 0
         block = 1
@@ -156,7 +215,13 @@ class CG(AbstractLinearSolver):
         # captured once and replayed between host checks of the active flags.
         period = 0
         if self.graph and dev.type == "cuda":
-            period = (se or 1) * max(1, -(-GRAPH_BLOCK // (se or 1)))
+            unit = se or 1
+            if stencil is not None:
+                # the fused sweeps alternate two direction buffers each step and
+                # two iterate buffers each stabilising step: a replayed block
+                # holds an even number of both
+                unit = 2 * (se or 1)
+            period = unit * max(1, -(-GRAPH_BLOCK // unit))
         graph = None
         while True:
             if graph is None and period and it % period == 0 and it + period <= max_steps and it >= period:
@@ -179,6 +244,8 @@ class CG(AbstractLinearSolver):
         stream = K.current_stream(b)
         K.call("lxa_cg_result", stream, rows, prm, st)
 
+        if stencil is not None:
+            y = ybuf[ycur[0]]
         if is_nsd:
             y = -y
         solution = unravel(y.reshape(*batch, size), struct, batch)

@@ -405,26 +405,59 @@ __global__ __launch_bounds__(kB) void gm_project_kernel(int64_t n, const TV* __r
   }
   ww = block_sum(ww, red);
   if (threadIdx.x == 0) part[pbase + static_cast<int64_t>(m + 1) * gridDim.x] = ww;
-  for (int k = 0; k <= j; ++k) {
-    const TV* vk = Vr + static_cast<int64_t>(k) * n;
-    double acc = 0.0;
-    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB)
-      acc = fma(static_cast<double>(vk[i]), static_cast<double>(wr[i]), acc);
-    acc = block_sum(acc, red);
-    if (threadIdx.x == 0) part[pbase + static_cast<int64_t>(k) * gridDim.x] = acc;
+  // eight basis vectors per pass over the chunk: eight independent loads per
+  // element and one block reduction per eight coefficients
+  __shared__ double red8[kB / 64][8];
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  for (int k0 = 0; k0 <= j; k0 += 8) {
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+      const double wi = static_cast<double>(wr[i]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u <= j) acc[u] = fma(static_cast<double>(Vr[static_cast<int64_t>(k0 + u) * n + i]), wi, acc[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = wave_sum(acc[u]);
+    if (ln == 0) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) red8[wv][u] = acc[u];
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && k0 + static_cast<int>(threadIdx.x) <= j) {
+      const int u = threadIdx.x;
+      const double v = (red8[0][u] + red8[1][u]) + (red8[2][u] + red8[3][u]);
+      part[pbase + static_cast<int64_t>(k0 + u) * gridDim.x] = v;
+    }
+    __syncthreads();
   }
 }
 
-// Finalize the projections: proj[row][k] = h_k, and step_norm = |w|_2.
-__global__ void gm_project_finish_kernel(int64_t rows, int nblk, int j, int m, const double* __restrict__ part,
-                                         double* __restrict__ proj, const lxa_krylov_row* __restrict__ st,
-                                         lxa_gmres_row* __restrict__ gs) {
-  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// Finalize the projections: proj[row][k] = h_k, and step_norm = |w|_2.  One
+// block per row; each wave sums whole coefficients (lanes strided over the
+// block partials, then a fixed butterfly), so the order is fixed.
+__global__ __launch_bounds__(kB) void gm_project_finish_kernel(int64_t rows, int nblk, int j, int m,
+                                                               const double* __restrict__ part,
+                                                               double* __restrict__ proj,
+                                                               const lxa_krylov_row* __restrict__ st,
+                                                               lxa_gmres_row* __restrict__ gs) {
+  const int64_t row = blockIdx.x;
   if (row >= rows || !st[row].active || gs[row].inner_done) return;
   const double* pr = part + row * static_cast<int64_t>(m + 2) * nblk;
   double* h = proj + row * (m + 1);
-  for (int k = 0; k <= m; ++k) h[k] = k <= j ? sum_partials(pr + static_cast<int64_t>(k) * nblk, nblk) : 0.0;
-  gs[row].step_norm = sqrt(sum_partials(pr + static_cast<int64_t>(m + 1) * nblk, nblk));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (int k = w; k <= m + 1; k += kB / 64) {
+    double v = 0.0;
+    if (k <= j || k == m + 1)
+      for (int q = l; q < nblk; q += 64) v += pr[static_cast<int64_t>(k) * nblk + q];
+    v = wave_sum(v);
+    if (l == 0) {
+      if (k == m + 1) gs[row].step_norm = sqrt(v);
+      else h[k] = k <= j ? v : 0.0;
+    }
+  }
 }
 
 // w' = w - sum_{k<=j} h_k v_k, written to basis column j+1 (unnormalised);
@@ -959,8 +992,8 @@ extern "C" int lxa_gmres_arnoldi(void* stream, int vdtype, int64_t rows, int64_t
     using TV = double;
     hipLaunchKernelGGL((lxa::gm_project_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
                        static_cast<const TV*>(basis), static_cast<const TV*>(w), j, m, state, gstate, partials);
-    hipLaunchKernelGGL(lxa::gm_project_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, j, m, partials,
-                       proj, state, gstate);
+    hipLaunchKernelGGL(lxa::gm_project_finish_kernel, dim3(static_cast<unsigned>(rows)), dim3(lxa::kB), 0, s, rows,
+                       nb, j, m, partials, proj, state, gstate);
     hipLaunchKernelGGL((lxa::gm_orthogonalise_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
                        static_cast<TV*>(basis), static_cast<const TV*>(w), proj, j, m, state, gstate, partials);
     hipLaunchKernelGGL(lxa::gm_arnoldi_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, j, m, partials,
@@ -972,8 +1005,8 @@ extern "C" int lxa_gmres_arnoldi(void* stream, int vdtype, int64_t rows, int64_t
     using TV = float;
     hipLaunchKernelGGL((lxa::gm_project_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
                        static_cast<const TV*>(basis), static_cast<const TV*>(w), j, m, state, gstate, partials);
-    hipLaunchKernelGGL(lxa::gm_project_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, j, m, partials,
-                       proj, state, gstate);
+    hipLaunchKernelGGL(lxa::gm_project_finish_kernel, dim3(static_cast<unsigned>(rows)), dim3(lxa::kB), 0, s, rows,
+                       nb, j, m, partials, proj, state, gstate);
     hipLaunchKernelGGL((lxa::gm_orthogonalise_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
                        static_cast<TV*>(basis), static_cast<const TV*>(w), proj, j, m, state, gstate, partials);
     hipLaunchKernelGGL(lxa::gm_arnoldi_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, j, m, partials,

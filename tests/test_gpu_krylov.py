@@ -193,3 +193,115 @@ def test_gmres_graph_replays_match_eager(device, rng):
         assert torch.equal(g.value, eager.value)
         assert torch.equal(g.stats["num_steps"], eager.stats["num_steps"])
         assert torch.equal(g.result, eager.result)
+
+
+# --------------------------------------------------------------------------
+# Fused two-sweep CG for stencil operators (lxa_cg_stencil_*)
+# --------------------------------------------------------------------------
+
+def _stencil_mv(nx, ny, coeffs):
+    return lambda v: O.stencil_apply(v.reshape(nx, ny), coeffs).ravel()
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (5, 3), (70, 300), (129, 257)])
+def test_cg_fused_stencil_vs_oracle(device, rng, nx, ny):
+    # tile edges in both directions (64-row, 256-column tiles), float64
+    coeffs = (4.0, -1.0, -1.0, -1.0, -1.0)
+    op = poisson2d(nx, ny, device=device)
+    b = rng.standard_normal((2, nx, ny))
+    sol = lx.linear_solve(op, _t(b, device), lx.CG(rtol=1e-9, atol=1e-9))
+    for i in range(2):
+        x_ref, res_ref, steps_ref = O.cg(_stencil_mv(nx, ny, coeffs), b[i].ravel(), 1e-9, 1e-9)
+        assert int(sol.result[i]) == res_ref == 0
+        assert abs(int(sol.stats["num_steps"][i]) - steps_ref) <= 1
+        np.testing.assert_allclose(sol.value[i].cpu().numpy().ravel(), x_ref, rtol=1e-6, atol=1e-7)
+
+
+def test_cg_fused_matches_unfused(device, rng):
+    # the fused sweeps and the six-kernel iteration run the same recurrence:
+    # equal step counts, iterates equal to rounding of the reduction order
+    nx, ny = 40, 260
+    op = poisson2d(nx, ny, device=device)
+    b = _t(rng.standard_normal((3, nx, ny)), device)
+    f = lx.linear_solve(op, b, lx.CG(rtol=1e-10, atol=1e-10), throw=False)
+    u = lx.linear_solve(op, b, lx.CG(rtol=1e-10, atol=1e-10, fused=False), throw=False)
+    assert torch.equal(f.result, u.result)
+    assert torch.max(torch.abs(f.stats["num_steps"] - u.stats["num_steps"])) <= 1
+    scale = float(torch.max(torch.abs(u.value)))
+    assert float(torch.max(torch.abs(f.value - u.value))) <= 1e-8 * scale
+
+
+def test_cg_fused_nsd_stencil(device, rng):
+    # a negative definite stencil: CG runs on -A and negates (cg.py:100-101, 224-225)
+    from lineax_amd.stencil import StencilOperator
+
+    nx, ny = 33, 47
+    coeffs = (-4.0, 1.0, 1.0, 1.0, 1.0)
+    op = StencilOperator(nx, ny, coeffs, device=device, tags=(lx.negative_semidefinite_tag, lx.symmetric_tag))
+    b = rng.standard_normal((nx, ny))
+    sol = lx.linear_solve(op, _t(b, device), lx.CG(rtol=1e-10, atol=1e-10))
+    x_ref, res_ref, steps_ref = O.cg(_stencil_mv(nx, ny, coeffs), b.ravel(), 1e-10, 1e-10, negate=True)
+    assert int(sol.result) == res_ref == 0
+    assert abs(int(sol.stats["num_steps"]) - steps_ref) <= 1
+    np.testing.assert_allclose(sol.value.cpu().numpy().ravel(), x_ref, rtol=1e-7, atol=1e-8)
+
+
+@pytest.mark.parametrize("exact_f32_b", [True, False])
+def test_cg_fused_mixed_precision(device, rng, exact_f32_b):
+    # float64 iterate, float32 working vectors; b read in float32 only when exact
+    nx = ny = 64
+    coeffs = (4.0, -1.0, -1.0, -1.0, -1.0)
+    b = rng.standard_normal((2, nx, ny))
+    if exact_f32_b:
+        b = b.astype(np.float32).astype(np.float64)
+    sol = lx.linear_solve(poisson2d(nx, ny, device=device), _t(b, device),
+                          lx.CG(rtol=1e-6, atol=1e-6, working_dtype=torch.float32))
+    assert torch.all(sol.result == 0)
+    A = O.stencil_matrix(nx, ny, coeffs)
+    for i in range(2):
+        x = sol.value[i].cpu().numpy().ravel()
+        r = b[i].ravel() - A @ x
+        assert np.max(np.abs(r) / (1e-6 + 1e-6 * np.abs(b[i].ravel()))) <= 1.0
+
+
+def test_cg_fused_float32_vs_unfused(device, rng):
+    nx = ny = 48
+    b = _t(rng.standard_normal((2, nx, ny)), device, torch.float32)
+    op = poisson2d(nx, ny, dtype=torch.float32, device=device)
+    f = lx.linear_solve(op, b, lx.CG(rtol=1e-4, atol=1e-4), throw=False)
+    u = lx.linear_solve(op, b, lx.CG(rtol=1e-4, atol=1e-4, fused=False), throw=False)
+    assert torch.equal(f.result, u.result)
+    assert torch.max(torch.abs(f.stats["num_steps"] - u.stats["num_steps"])) <= 2
+    scale = float(torch.max(torch.abs(u.value)))
+    assert float(torch.max(torch.abs(f.value - u.value))) <= 1e-3 * scale
+
+
+def test_cg_fused_max_steps_rules(device, rng):
+    nx, ny = 20, 30
+    op = poisson2d(nx, ny, device=device)
+    b = _t(rng.standard_normal((nx, ny)), device)
+    s = lx.linear_solve(op, b, lx.CG(0.0, 0.0, max_steps=2), throw=False)
+    assert int(s.stats["num_steps"]) == 2 and int(s.result) == 0
+    s = lx.linear_solve(op, b, lx.CG(1e-14, 1e-14, max_steps=3), throw=False)
+    assert int(s.stats["num_steps"]) == 3 and int(s.result) == lx.RESULTS.max_steps_reached
+    # a row that starts converged (b = 0) never iterates beside one that does
+    b2 = torch.stack([b, torch.zeros_like(b)])
+    s = lx.linear_solve(op, b2, lx.CG(1e-8, 1e-8), throw=False)
+    assert int(s.stats["num_steps"][1]) == 0 and int(s.stats["num_steps"][0]) > 0
+    assert torch.all(s.value[1] == 0)
+
+
+@pytest.mark.parametrize("stabilise_every", [10, 7])
+def test_cg_fused_graph_replays_match_eager(device, rng, stabilise_every):
+    # odd stabilisation intervals double the captured block (two direction buffers)
+    nx, ny = 50, 270
+    op = poisson2d(nx, ny, device=device)
+    b = _t(rng.standard_normal((2, nx, ny)), device)
+    kw = dict(rtol=1e-9, atol=1e-9, stabilise_every=stabilise_every)
+    eager = lx.linear_solve(op, b, lx.CG(**kw), throw=False)
+    for _ in range(3):
+        g = lx.linear_solve(op, b, lx.CG(graph=True, **kw), throw=False)
+        _ = (b * 2.0).sum().item()
+        assert torch.equal(g.value, eager.value)
+        assert torch.equal(g.stats["num_steps"], eager.stats["num_steps"])
+        assert torch.equal(g.result, eager.result)
