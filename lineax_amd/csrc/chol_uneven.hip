@@ -29,6 +29,9 @@ namespace lxa {
 namespace {
 
 constexpr int kT = 256;
+#ifndef LXA_SMALL_WAVES
+#define LXA_SMALL_WAVES 2
+#endif
 constexpr int kMaxN = 256;
 constexpr int kPTStride = 18;  // 16 panel values + 2 pad: b64 reads hit distinct banks
 constexpr size_t kUnionBytes = 256 * kPTStride * sizeof(float);  // >= (8 * 256 + 8) doubles
@@ -79,7 +82,8 @@ __device__ Lds carve(unsigned char* base) {
 
 // Factorises the system held in R (see the file comment).  Returns 0 or the
 // 1-based index of the first non-positive pivot; uniform over the block.
-__device__ int factor_registers(float (&R)[16][16], int n, const Lds& s) {
+template <int KR>
+__device__ int factor_registers(float (&R)[KR][KR], int n, const Lds& s) {
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4, w = tid >> 6, l = tid & 63;
   const int N16 = round16(n);
@@ -90,10 +94,10 @@ __device__ int factor_registers(float (&R)[16][16], int n, const Lds& s) {
     const int k0 = 16 * K;
     // (a) panel rows to T
 #pragma unroll
-    for (int a = 0; a < 16; ++a) {
+    for (int a = 0; a < KR; ++a) {
       if (a == K) {
 #pragma unroll
-        for (int b = a; b < 16; ++b) {
+        for (int b = a; b < KR; ++b) {
           if (b < nk) {
             const int i = ty + 16 * a, j = tx + 16 * b;
             if (j >= i) T[pku(N16, i, j)] = R[a][b];
@@ -168,18 +172,18 @@ __device__ int factor_registers(float (&R)[16][16], int n, const Lds& s) {
     // instruction cache.
 #pragma unroll 1
     for (int pc = 0; pc < 16; pc += 2) {
-      float2 uj[16];
+      float2 uj[KR];
 #pragma unroll
-      for (int b = 0; b < 16; ++b) {
+      for (int b = 0; b < KR; ++b) {
         uj[b] = make_float2(0.0f, 0.0f);
         if (b > K && b < nk) uj[b] = *reinterpret_cast<const float2*>(PT + (tx + 16 * b) * kPTStride + pc);
       }
 #pragma unroll
-      for (int a = 0; a < 16; ++a) {
+      for (int a = 0; a < KR; ++a) {
         if (a > K && a < nk) {
           const float2 ui = *reinterpret_cast<const float2*>(PT + (ty + 16 * a) * kPTStride + pc);
 #pragma unroll
-          for (int b = a; b < 16; ++b) {
+          for (int b = a; b < KR; ++b) {
             if (b < nk) {
               R[a][b] -= ui.x * uj[b].x;
               R[a][b] -= ui.y * uj[b].y;
@@ -199,24 +203,26 @@ __device__ int factor_registers(float (&R)[16][16], int n, const Lds& s) {
 // next group of steps (a row of U forward, a column backward, and the
 // reciprocal pivots) are loaded while the current group runs, so the LDS
 // latency stays off the chain.  (Groups of kG steps.)
-__device__ __forceinline__ void solve_load_row(const Lds& s, int N16, int n, int l, int k, bool live, float (&t)[4],
+template <int NC>
+__device__ __forceinline__ void solve_load_row(const Lds& s, int N16, int n, int l, int k, bool live, float (&t)[NC],
                                                float& r) {
   const float* T = s.T;
   r = live ? s.rinv[k] : 0.0f;
   const int rk = pku(N16, k, k);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < NC; ++c) {
     const int jj = l + 64 * c;
     t[c] = (live && jj > k && jj < n) ? T[rk + (jj - k)] : 0.0f;
   }
 }
 
-__device__ __forceinline__ void solve_load_col(const Lds& s, int N16, int l, int k, bool live, float (&t)[4],
+template <int NC>
+__device__ __forceinline__ void solve_load_col(const Lds& s, int N16, int l, int k, bool live, float (&t)[NC],
                                                float& r) {
   const float* T = s.T;
   r = live ? s.rinv[k] : 0.0f;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < NC; ++c) {
     const int i = l + 64 * c;
     t[c] = (live && i < k) ? T[pku(N16, i, k)] : 0.0f;
   }
@@ -224,18 +230,19 @@ __device__ __forceinline__ void solve_load_col(const Lds& s, int N16, int l, int
 
 constexpr int kG = 1;  // steps per prefetch group (1 keeps the refined kernel free of scratch)
 
-__device__ void wave_solve(int n, const Lds& s, float (&v)[4]) {
+template <int NC>
+__device__ void wave_solve(int n, const Lds& s, float (&v)[NC]) {
   const int l = threadIdx.x & 63;
   const int N16 = round16(n);
 #pragma unroll
-  for (int c0 = 0; c0 < 4; ++c0) {
+  for (int c0 = 0; c0 < NC; ++c0) {
     const int kend = min(64, n - 64 * c0);
     if (kend <= 0) continue;
-    float ta[kG][4], ra[kG];
+    float ta[kG][NC], ra[kG];
 #pragma unroll
     for (int u = 0; u < kG; ++u) solve_load_row(s, N16, n, l, 64 * c0 + u, u < kend, ta[u], ra[u]);
     for (int g = 0; g < kend; g += kG) {
-      float tb[kG][4], rb[kG];
+      float tb[kG][NC], rb[kG];
 #pragma unroll
       for (int u = 0; u < kG; ++u) solve_load_row(s, N16, n, l, 64 * c0 + g + kG + u, g + kG + u < kend, tb[u], rb[u]);
 #pragma unroll
@@ -246,26 +253,26 @@ __device__ void wave_solve(int n, const Lds& s, float (&v)[4]) {
           const float yk = bcast(v[c0], kk) * ra[u];
           if (l == kk) v[c0] = yk;
 #pragma unroll
-          for (int c = c0; c < 4; ++c) v[c] -= ta[u][c] * yk;
+          for (int c = c0; c < NC; ++c) v[c] -= ta[u][c] * yk;
         }
       }
 #pragma unroll
       for (int u = 0; u < kG; ++u) {
         ra[u] = rb[u];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) ta[u][c] = tb[u][c];
+        for (int c = 0; c < NC; ++c) ta[u][c] = tb[u][c];
       }
     }
   }
 #pragma unroll
-  for (int c0 = 3; c0 >= 0; --c0) {
+  for (int c0 = NC - 1; c0 >= 0; --c0) {
     const int kend = min(64, n - 64 * c0);
     if (kend <= 0) continue;
-    float ta[kG][4], ra[kG];
+    float ta[kG][NC], ra[kG];
 #pragma unroll
     for (int u = 0; u < kG; ++u) solve_load_col(s, N16, l, 64 * c0 + kend - 1 - u, kend - 1 - u >= 0, ta[u], ra[u]);
     for (int g = kend - 1; g >= 0; g -= kG) {
-      float tb[kG][4], rb[kG];
+      float tb[kG][NC], rb[kG];
 #pragma unroll
       for (int u = 0; u < kG; ++u) solve_load_col(s, N16, l, 64 * c0 + g - kG - u, g - kG - u >= 0, tb[u], rb[u]);
 #pragma unroll
@@ -282,7 +289,7 @@ __device__ void wave_solve(int n, const Lds& s, float (&v)[4]) {
       for (int u = 0; u < kG; ++u) {
         ra[u] = rb[u];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) ta[u][c] = tb[u][c];
+        for (int c = 0; c < NC; ++c) ta[u][c] = tb[u][c];
       }
     }
   }
@@ -309,33 +316,35 @@ __device__ void block_max4(double (&v)[4], double* red) {
 // row sums of |S|.  Every upper-triangle element is read once from global
 // memory: row j gives sum_{i>=j} A_ji x_i to r_j (wave reduction) and
 // A_ji x_j to r_i for i > j (lane accumulators).
+template <int NC>
 __device__ void residual_sym_upper(int n, const float* __restrict__ A, const float* __restrict__ b, const Lds& s,
                                    bool want_abs) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  double aab[4] = {0.0, 0.0, 0.0, 0.0};
-  double xi[4];
+  double acc[NC], aab[NC], xi[NC];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) xi[c] = (l + 64 * c < n) ? s.x[l + 64 * c] : 0.0;
+  for (int c = 0; c < NC; ++c) {
+    acc[c] = aab[c] = 0.0;
+    xi[c] = (l + 64 * c < n) ? s.x[l + 64 * c] : 0.0;
+  }
   // Rows j0 + 4u (u < 8) per batch; the next batch's loads are issued before
   // this batch is reduced (the eight row reductions are independent, so their
   // shuffles interleave).
-  auto load_rows = [&](int j0, float (&av)[8][4]) {
+  auto load_rows = [&](int j0, float (&av)[8][NC]) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int j = j0 + 4 * u;
       const float* Aj = A + static_cast<int64_t>(j) * n;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < NC; ++c) {
         const int i = l + 64 * c;
         av[u][c] = (j < n && i >= j && i < n) ? Aj[i] : 0.0f;
       }
     }
   };
-  float av[8][4];
+  float av[8][NC];
   load_rows(w, av);
   for (int j0 = w; j0 < n; j0 += 32) {
-    float nv[8][4];
+    float nv[8][NC];
     load_rows(j0 + 32, nv);  // rows past n load nothing (zero)
     double red[16];
 #pragma unroll
@@ -344,7 +353,7 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
       const double xj = j < n ? s.x[j] : 0.0;
       double dot = 0.0, rab = 0.0;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < NC; ++c) {
         const int i = l + 64 * c;
         const double aji = static_cast<double>(av[u][c]);
         dot = fma(aji, xi[c], dot);
@@ -370,10 +379,10 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) av[u][c] = nv[u][c];
+      for (int c = 0; c < NC; ++c) av[u][c] = nv[u][c];
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < NC; ++c) {
     const int i = l + 64 * c;
     if (i < n) s.acc[w * 256 + i] = acc[c];
   }
@@ -385,7 +394,7 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
   if (want_abs) {
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < NC; ++c) {
       const int i = l + 64 * c;
       if (i < n) s.acc[w * 256 + i] = aab[c];
     }
@@ -396,8 +405,14 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
   __syncthreads();
 }
 
-template <bool kRefine, typename TX>
-__global__ __launch_bounds__(kT) void chol_uneven_kernel(int64_t count, int max_n, const int32_t* __restrict__ ns,
+// KR: register tile extent; a workgroup holds orders up to 16 KR.  Each
+// launch takes the systems with n_lo < n <= n_hi from the shared order and
+// skips the rest, so two launches with different tiles split one batch.
+// report_invalid: this launch marks orders outside [1, max_n].
+template <int KR, bool kRefine, typename TX>
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(KR == 8 ? LXA_SMALL_WAVES : 1)))
+void chol_uneven_kernel(int64_t count, int max_n, int n_lo, int n_hi,
+                                                         int report_invalid, const int32_t* __restrict__ ns,
                                                          const int64_t* __restrict__ a_off,
                                                          const float* __restrict__ a,
                                                          const int64_t* __restrict__ b_off,
@@ -423,7 +438,7 @@ __global__ __launch_bounds__(kT) void chol_uneven_kernel(int64_t count, int max_
     if (!persistent) next = count;
     const int n = ns[sys];
     if (n < 1 || n > max_n) {
-      if (tid == 0) {
+      if (tid == 0 && report_invalid) {
         info[sys] = -1;
         if (iters) iters[sys] = 0;
         if (eta) eta[sys] = -1.0;
@@ -431,16 +446,18 @@ __global__ __launch_bounds__(kT) void chol_uneven_kernel(int64_t count, int max_
       __syncthreads();
       continue;
     }
+    if (n <= n_lo || n > n_hi) continue;  // the other tile class's system (uniform)
     const float* A = a + a_off[sys];
     const float* B = b + b_off[sys];
     TX* X = x + b_off[sys];
     const int nk = round16(n) >> 4;
 
-    float R[16][16];
+    constexpr int NC = KR / 4;  // 64-entry chunks of a vector (orders <= 64 NC)
+    float R[KR][KR];
 #pragma unroll
-    for (int aa = 0; aa < 16; ++aa) {
+    for (int aa = 0; aa < KR; ++aa) {
 #pragma unroll
-      for (int bb = aa; bb < 16; ++bb) {
+      for (int bb = aa; bb < KR; ++bb) {
         R[aa][bb] = 0.0f;
         if (aa < nk && bb < nk) {
           const int i = ty + 16 * aa, j = tx + 16 * bb;
@@ -453,7 +470,7 @@ __global__ __launch_bounds__(kT) void chol_uneven_kernel(int64_t count, int max_
       }
     }
 
-    const int fail = factor_registers(R, n, s);
+    const int fail = factor_registers<KR>(R, n, s);
     if (fail) {
       __syncthreads();
       for (int i = tid; i < n; i += kT) X[i] = static_cast<TX>(quiet_nan<double>());
@@ -474,7 +491,7 @@ __global__ __launch_bounds__(kT) void chol_uneven_kernel(int64_t count, int max_
     for (int it = -1; it < passes; ++it) {
       if constexpr (kRefine) {
         if (it >= 0) {
-          residual_sym_upper(n, A, B, s, it == 0);
+          residual_sym_upper<NC>(n, A, B, s, it == 0);
           // |r|_inf, |x|_inf and (first pass) |A|_inf, |b|_inf in one exchange
           double m4[4] = {0.0, 0.0, 0.0, 0.0};
           for (int i = tid; i < n; i += kT) {
@@ -498,15 +515,15 @@ __global__ __launch_bounds__(kT) void chol_uneven_kernel(int64_t count, int max_
         }
       }
       if (w == 0) {
-        float v[4];
+        float v[NC];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < NC; ++c) {
           const int i = l + 64 * c;
           v[c] = i < n ? (it < 0 ? B[i] : static_cast<float>(s.r[i])) : 0.0f;
         }
-        wave_solve(n, s, v);
+        wave_solve<NC>(n, s, v);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < NC; ++c) {
           const int i = l + 64 * c;
           if (i < n) s.x[i] = (it < 0 ? 0.0 : s.x[i]) + static_cast<double>(v[c]);
         }
@@ -535,6 +552,36 @@ int allow_lds(K kernel, size_t bytes) {
 
 constexpr int64_t kMaxGrid = 2147483647;
 
+// One launch of the KR tile class over `order`; persistent launches are sized
+// by the occupancy the kernel's registers and LDS allow.
+template <int KR, bool kRefine, typename TX>
+int launch_class(hipStream_t s, int64_t count, int max_n, int n_lo, int n_hi, int report, const int32_t* n,
+                 const int64_t* a_offset, const float* a, const int64_t* b_offset, const float* b, void* x,
+                 const int32_t* order, bool persistent, unsigned int* queue, int max_refine, double tol, int32_t* info,
+                 int32_t* iters, double* eta) {
+  auto kernel = chol_uneven_kernel<KR, kRefine, TX>;
+  const int lds_n = n_hi < 16 * KR ? n_hi : 16 * KR;
+  const size_t shmem = uneven_lds_bytes(lds_n);
+  if (shmem > 160 * 1024) return set_error(LXA_EINVAL, "lxa_chol_uneven_solve: LDS need %zu exceeds 160 KiB", shmem);
+  if (int rc = allow_lds(kernel, shmem)) return rc;
+  int64_t grid = count;
+  if (persistent) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, queue);
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), kT,
+                                                                shmem);
+    if (e != hipSuccess) return hip_error(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (per_cu < 1) per_cu = 1;
+    grid = static_cast<int64_t>(num_cus()) * per_cu;
+    if (grid > count) grid = count;
+  }
+  hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(grid)), dim3(kT), shmem, s, count, max_n, n_lo, n_hi, report,
+                     n, a_offset, a, b_offset, b, static_cast<TX*>(x), order, persistent ? 1 : 0, queue, max_refine,
+                     tol, info, iters, eta);
+  LXA_LAUNCH_CHECK("lxa_chol_uneven_solve launch");
+  return LXA_OK;
+}
+
 }  // namespace
 }  // namespace lxa
 
@@ -556,41 +603,42 @@ extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, con
   LXA_REQUIRE(max_refine >= 0 && max_refine <= 64, "lxa_chol_uneven_solve: max_refine in [0, 64]");
   LXA_REQUIRE(max_refine == 0 || x_dtype == LXA_F64, "lxa_chol_uneven_solve: refinement needs x_dtype F64");
   const bool persistent = (flags & LXA_UNEVEN_PERSISTENT) != 0;
-  LXA_REQUIRE(!persistent || (workspace && workspace_bytes >= 4), "lxa_chol_uneven_solve: workspace required");
+  LXA_REQUIRE(!persistent || (workspace && workspace_bytes >= 8), "lxa_chol_uneven_solve: workspace required");
   if (count == 0) return LXA_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool refine = max_refine > 0;
-  const size_t shmem = lxa::uneven_lds_bytes(max_n);
-  LXA_REQUIRE(shmem <= 160 * 1024, "lxa_chol_uneven_solve: LDS need %zu exceeds 160 KiB", shmem);
   unsigned int* queue = static_cast<unsigned int*>(workspace);
-  int64_t grid = count;
-  if (persistent) {
-    hipLaunchKernelGGL(lxa::zero_u32_kernel, dim3(1), dim3(1), 0, s, queue);
-    LXA_LAUNCH_CHECK("lxa_chol_uneven_solve queue reset");
-    int per_cu = static_cast<int>((160 * 1024) / shmem);
-    per_cu = per_cu < 1 ? 1 : (per_cu > 2 ? 2 : per_cu);  // register file: at most two 256-thread groups
-    grid = static_cast<int64_t>(lxa::num_cus()) * per_cu;
-    if (grid > count) grid = count;
-  }
-  const int pers = persistent ? 1 : 0;
+  // Two register-tile classes: orders <= 128 run with 8 x 8 tiles (a quarter
+  // of the registers, so several systems share a compute unit), larger ones
+  // with 16 x 16.  A persistent solve is one launch per class over the same
+  // order (the large class first, each with its own queue counter); one
+  // workgroup per system uses the class that covers max_n.
+  const int split = 128;
   int rc = LXA_OK;
-  if (refine) rc = lxa::allow_lds(lxa::chol_uneven_kernel<true, double>, shmem);
-  else if (x_dtype == LXA_F64) rc = lxa::allow_lds(lxa::chol_uneven_kernel<false, double>, shmem);
-  else rc = lxa::allow_lds(lxa::chol_uneven_kernel<false, float>, shmem);
-  if (rc) return rc;
-  if (refine) {
-    hipLaunchKernelGGL((lxa::chol_uneven_kernel<true, double>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kT),
-                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<double*>(x), order, pers,
-                       queue, max_refine, tol, info, iters, eta);
-  } else if (x_dtype == LXA_F64) {
-    hipLaunchKernelGGL((lxa::chol_uneven_kernel<false, double>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kT),
-                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<double*>(x), order, pers,
-                       queue, 0, tol, info, iters, eta);
+#define LXA_UNEVEN_CLASS(KR, LO, HI, REPORT, Q)                                                                    \
+  do {                                                                                                           \
+    if (refine)                                                                                                  \
+      rc = lxa::launch_class<KR, true, double>(s, count, max_n, LO, HI, REPORT, n, a_offset, a, b_offset, b, x,    \
+                                               order, persistent, Q, max_refine, tol, info, iters, eta);        \
+    else if (x_dtype == LXA_F64)                                                                                 \
+      rc = lxa::launch_class<KR, false, double>(s, count, max_n, LO, HI, REPORT, n, a_offset, a, b_offset, b, x,   \
+                                                order, persistent, Q, 0, tol, info, iters, eta);                \
+    else                                                                                                         \
+      rc = lxa::launch_class<KR, false, float>(s, count, max_n, LO, HI, REPORT, n, a_offset, a, b_offset, b, x,    \
+                                               order, persistent, Q, 0, tol, info, iters, eta);                 \
+  } while (0)
+  if (!persistent) {
+    if (max_n > split) LXA_UNEVEN_CLASS(16, 0, max_n, 1, queue);
+    else LXA_UNEVEN_CLASS(8, 0, max_n, 1, queue);
+  } else if (max_n > split) {
+    LXA_UNEVEN_CLASS(16, split, max_n, 1, queue);
+    if (rc) return rc;
+    LXA_UNEVEN_CLASS(8, 0, split, 0, queue + 1);
   } else {
-    hipLaunchKernelGGL((lxa::chol_uneven_kernel<false, float>), dim3(static_cast<unsigned>(grid)), dim3(lxa::kT),
-                       shmem, s, count, max_n, n, a_offset, a, b_offset, b, static_cast<float*>(x), order, pers,
-                       queue, 0, tol, info, iters, eta);
+    LXA_UNEVEN_CLASS(8, 0, max_n, 1, queue);
   }
+#undef LXA_UNEVEN_CLASS
+  if (rc) return rc;
   LXA_LAUNCH_CHECK("lxa_chol_uneven_solve launch");
   return LXA_OK;
 }
