@@ -107,117 +107,168 @@ __global__ void rbt_vec_kernel(int n, int s, int nrhs, T* __restrict__ x, int64_
 // the caller applies the swaps to the rest of the block's rows.  info[0] is
 // set to 1 + the global index of the first exactly-zero pivot.
 // ---------------------------------------------------------------------------
+// 1024 threads as a 32 x 32 grid; thread (ty, tx) keeps the 4 x 4 elements
+// (ty + 32 a, tx + 32 b) in registers.  Per column: the owners of column j
+// publish it, every wave finds the pivot from it redundantly, the owners of
+// rows j and p publish those rows, and each thread swaps and updates its own
+// elements -- two barriers per column, LDS buffers alternating by column.
+// The inverses run the same way on register tiles with the factors staged
+// in LDS, one barrier per column.
 __global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, float* __restrict__ a, int64_t lda,
                                                                float* __restrict__ linv, float* __restrict__ uinv,
                                                                int32_t* __restrict__ piv, int32_t* __restrict__ info) {
   extern __shared__ __align__(16) float dsm[];
-  __shared__ float s_pv[2];
-  __shared__ int s_pi[2];
-  float* S = dsm;                      // [128][129]
-  float* X = dsm + kB * (kB + 1);      // [128][129]
+  float* S = dsm;  // [128][129]: the factors, for the inverse sweeps
   constexpr int LD = kB + 1;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
-    const int i = idx >> 7, j = idx & 127;
-    S[i * LD + j] = (i < nb && j < nb) ? a[(int64_t)(k + i) * lda + k + j] : (i == j ? 1.f : 0.f);
-  }
-  __syncthreads();
-  // right-looking LU, one column per step
-  for (int j = 0; j < kB; ++j) {
-    // arg-max of |S[r][j]| over r >= j (NaN counts as largest; ties: lowest row)
-    if (tid < kB) {
-      float v = -1.f;
-      if (tid >= j) {
-        v = fabsf(S[tid * LD + j]);
-        if (v != v) v = INFINITY;
-      }
-      int idx = tid;
+  __shared__ float colv[2][kB];
+  __shared__ float rowj[2][kB];
+  __shared__ float rowp[2][kB];
+  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63;
+  float R[4][4];
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const float ov = __shfl_xor(v, off, kWave);
-        const int oi = __shfl_xor(idx, off, kWave);
-        if (ov > v || (ov == v && oi < idx)) {
-          v = ov;
-          idx = oi;
+  for (int ai = 0; ai < 4; ++ai)
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+      const int r = ty + 32 * ai, c = tx + 32 * bi;
+      R[ai][bi] = (r < nb && c < nb) ? a[(int64_t)(k + r) * lda + k + c] : (r == c ? 1.f : 0.f);
+    }
+  for (int j = 0; j < kB; ++j) {
+    const int buf = j & 1, jb = j >> 5, jt = j & 31;
+    if (tx == jt) {
+#pragma unroll
+      for (int ai = 0; ai < 4; ++ai)
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+          if (bi == jb) colv[buf][ty + 32 * ai] = R[ai][bi];
+    }
+    __syncthreads();
+    // arg-max of |column j| over rows >= j (NaN counts as largest; ties:
+    // lowest row), computed by every wave
+    float v0 = -1.f, v1 = -1.f;
+    if (lane >= j) {
+      v0 = fabsf(colv[buf][lane]);
+      if (v0 != v0) v0 = INFINITY;
+    }
+    if (lane + 64 >= j) {
+      v1 = fabsf(colv[buf][lane + 64]);
+      if (v1 != v1) v1 = INFINITY;
+    }
+    float v = v0;
+    int idx = lane;
+    if (v1 > v0) {
+      v = v1;
+      idx = lane + 64;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const float ov = __shfl_xor(v, off, kWave);
+      const int oi = __shfl_xor(idx, off, kWave);
+      if (ov > v || (ov == v && oi < idx)) {
+        v = ov;
+        idx = oi;
+      }
+    }
+    const int p = idx;
+    const float pv = colv[buf][p];
+    // rows j and p to LDS (before the swap)
+    if (ty == jt) {
+#pragma unroll
+      for (int ai = 0; ai < 4; ++ai)
+        if (ai == jb) {
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi) rowj[buf][tx + 32 * bi] = R[ai][bi];
+        }
+    }
+    if (ty == (p & 31)) {
+#pragma unroll
+      for (int ai = 0; ai < 4; ++ai)
+        if (ai == (p >> 5)) {
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi) rowp[buf][tx + 32 * bi] = R[ai][bi];
+        }
+    }
+    if (tid == 0 && j < nb) {
+      piv[k + j] = k + p;
+      if (pv == 0.f && info[0] == 0) info[0] = k + j + 1;
+    }
+    __syncthreads();
+    // swap rows j and p, multipliers into column j, rank-1 update
+#pragma unroll
+    for (int ai = 0; ai < 4; ++ai) {
+      const int r = ty + 32 * ai;
+      if (r == j) {
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) R[ai][bi] = rowp[buf][tx + 32 * bi];
+      } else if (r > j) {
+        const bool isp = r == p;
+        const float l = (isp ? colv[buf][j] : colv[buf][r]) / pv;
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) {
+          const int c = tx + 32 * bi;
+          const float old = isp ? rowj[buf][c] : R[ai][bi];
+          R[ai][bi] = c == j ? l : (c > j ? old - l * rowp[buf][c] : old);
         }
       }
-      if ((tid & 63) == 0) {
-        s_pv[tid >> 6] = v;
-        s_pi[tid >> 6] = idx;
+    }
+  }
+  // the factors to global memory and to LDS
+#pragma unroll
+  for (int ai = 0; ai < 4; ++ai)
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+      const int r = ty + 32 * ai, c = tx + 32 * bi;
+      S[r * LD + c] = R[ai][bi];
+      if (r < nb && c < nb) a[(int64_t)(k + r) * lda + k + c] = R[ai][bi];
+    }
+  __syncthreads();
+  // X = L^-1 (unit lower): for j: X[i][:] -= L[i][j] X[j][:], i > j.
+  // Y = (U^T)^-1 (lower): for j: Y[j][:] /= U[j][j]; Y[i][:] -= U[j][i] Y[j][:].
+  // uinv = Y^T.  Row j of each is final when column j is processed.
+  float X[4][4], Y[4][4];
+#pragma unroll
+  for (int ai = 0; ai < 4; ++ai)
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) X[ai][bi] = Y[ai][bi] = (ty + 32 * ai == tx + 32 * bi) ? 1.f : 0.f;
+  for (int j = 0; j < kB; ++j) {
+    const int buf = j & 1, jb = j >> 5, jt = j & 31;
+    if (ty == jt) {
+      const float d = S[j * LD + j];
+#pragma unroll
+      for (int ai = 0; ai < 4; ++ai)
+        if (ai == jb) {
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi) {
+            Y[ai][bi] /= d;
+            rowj[buf][tx + 32 * bi] = X[ai][bi];
+            rowp[buf][tx + 32 * bi] = Y[ai][bi];
+          }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ai = 0; ai < 4; ++ai) {
+      const int i = ty + 32 * ai;
+      if (i > j) {
+        const float lij = S[i * LD + j], uji = S[j * LD + i];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) {
+          const int c = tx + 32 * bi;
+          if (c <= j) {
+            X[ai][bi] -= lij * rowj[buf][c];
+            Y[ai][bi] -= uji * rowp[buf][c];
+          }
+        }
       }
     }
-    __syncthreads();
-    const int p = (s_pv[1] > s_pv[0] || (s_pv[1] == s_pv[0] && s_pi[1] < s_pi[0])) ? s_pi[1] : s_pi[0];
-    if (p != j && tid < kB) {
-      const float t = S[j * LD + tid];
-      S[j * LD + tid] = S[p * LD + tid];
-      S[p * LD + tid] = t;
+  }
+#pragma unroll
+  for (int ai = 0; ai < 4; ++ai)
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+      const int r = ty + 32 * ai, c = tx + 32 * bi;
+      linv[r * kB + c] = X[ai][bi];
+      uinv[c * kB + r] = Y[ai][bi];
     }
-    if (tid == 0 && j < nb) piv[k + j] = k + p;
-    __syncthreads();
-    const float pv = S[j * LD + j];
-    if (tid == 0 && pv == 0.f && j < nb && info[0] == 0) info[0] = k + j + 1;
-    const int rows = kB - 1 - j;
-    // multipliers then rank-1 update; thread (r, c) over the trailing block
-    for (int idx = tid; idx < rows * (rows + 1); idx += kDiagThreads) {
-      const int r = j + 1 + idx / (rows + 1);
-      const int c = j + idx % (rows + 1);  // c == j: the multiplier itself
-      const float l = S[r * LD + j] / pv;
-      if (c == j)
-        X[r * LD + j] = l;  // staged: the column is overwritten after the barrier
-      else
-        S[r * LD + c] -= l * S[j * LD + c];
-    }
-    __syncthreads();
-    for (int r = j + 1 + tid; r < kB; r += kDiagThreads) S[r * LD + j] = X[r * LD + j];
-    __syncthreads();
-  }
-  // write the factors back
-  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
-    const int i = idx >> 7, j = idx & 127;
-    if (i < nb && j < nb) a[(int64_t)(k + i) * lda + k + j] = S[i * LD + j];
-  }
-  // inverse of the unit lower triangle: X = I; for j: X[i][:] -= L[i][j] X[j][:], i > j
-  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
-    const int i = idx >> 7, j = idx & 127;
-    X[i * LD + j] = i == j ? 1.f : 0.f;
-  }
-  __syncthreads();
-  for (int j = 0; j < kB - 1; ++j) {
-    const int rows = kB - 1 - j, cols = j + 1;
-    for (int idx = tid; idx < rows * cols; idx += kDiagThreads) {
-      const int i = j + 1 + idx / cols, c = idx % cols;
-      X[i * LD + c] -= S[i * LD + j] * X[j * LD + c];
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
-    const int i = idx >> 7, j = idx & 127;
-    linv[idx] = X[i * LD + j];
-  }
-  __syncthreads();
-  // inverse of the upper triangle, as Y = (U^T)^-1 (lower): Y = I; for j:
-  // Y[j][:] /= U[j][j]; Y[i][:] -= U[j][i] Y[j][:], i > j.  uinv = Y^T.
-  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
-    const int i = idx >> 7, j = idx & 127;
-    X[i * LD + j] = i == j ? 1.f : 0.f;
-  }
-  __syncthreads();
-  for (int j = 0; j < kB; ++j) {
-    const float dj = S[j * LD + j];
-    for (int c = tid; c <= j; c += kDiagThreads) X[j * LD + c] /= dj;
-    __syncthreads();
-    const int rows = kB - 1 - j, cols = j + 1;
-    for (int idx = tid; idx < rows * cols; idx += kDiagThreads) {
-      const int i = j + 1 + idx / cols, c = idx % cols;
-      X[i * LD + c] -= S[j * LD + i] * X[j * LD + c];
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < kB * kB; idx += kDiagThreads) {
-    const int i = idx >> 7, j = idx & 127;
-    uinv[idx] = X[j * LD + i];
-  }
 }
 
 __global__ void zero_i32_kernel(int32_t* __restrict__ p, int count) {
@@ -293,58 +344,77 @@ constexpr int kTileLd = kB + 4;  // LDS row stride of a staged 128 x 128 block (
 // 128; zeros outside rows < rows, cols < cols).  Loads are float4 along the
 // stored rows: for the transposed kinds the stored rows are the tile's
 // columns, so they are written transposed.
-__device__ void stage_block(float* tile, const float* lu, int64_t lda, int kind, int bi, int bj, int rows, int cols,
-                            int tid) {
+// A 128 x 128 operator block staged through registers: tile_load issues all
+// of a thread's loads (16 float4, rows of the stored block) and tile_store
+// writes them to LDS, transposed for kinds 2/3, so the loads of one block are
+// in flight together and can overlap the product of the previous one.
+constexpr int kStageVec = kB * (kB / 4) / kTrsvThreads;  // float4 per thread
+
+__device__ void tile_load(float4 (&v)[kStageVec], const float* lu, int64_t lda, int kind, int bi, int bj, int rows,
+                          int cols, int tid) {
   const bool tr = kind >= 2;
-  // stored block: row index sr (0..127), 4-column group sc (0..31)
-  for (int idx = tid; idx < kB * (kB / 4); idx += kTrsvThreads) {
+  const int srow_lim = tr ? cols : rows, scol_lim = tr ? rows : cols;
+  const bool full = srow_lim == kB && scol_lim == kB;
+#pragma unroll
+  for (int u = 0; u < kStageVec; ++u) {
+    const int idx = tid + u * kTrsvThreads;
     const int sr = idx >> 5, sc = 4 * (idx & 31);
-    // stored element (sr, sc..sc+3) is tile (sr, sc..) or, transposed, tile (sc.., sr)
-    const int srow_lim = tr ? cols : rows, scol_lim = tr ? rows : cols;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (sr < srow_lim) {
-      const int64_t grow = tr ? (int64_t)bj * kB + sr : (int64_t)bi * kB + sr;
-      const int64_t gcol = tr ? (int64_t)bi * kB + sc : (int64_t)bj * kB + sc;
-      const float* p = lu + grow * lda + gcol;
-      if (sc + 3 < scol_lim) {
-        v = *reinterpret_cast<const float4*>(p);
-      } else {
-        v.x = sc < scol_lim ? p[0] : 0.f;
-        v.y = sc + 1 < scol_lim ? p[1] : 0.f;
-        v.z = sc + 2 < scol_lim ? p[2] : 0.f;
-        v.w = sc + 3 < scol_lim ? p[3] : 0.f;
-      }
-    }
-    if (!tr) {
-      *reinterpret_cast<float4*>(tile + sr * kTileLd + sc) = v;
+    const int64_t grow = tr ? (int64_t)bj * kB + sr : (int64_t)bi * kB + sr;
+    const int64_t gcol = tr ? (int64_t)bi * kB + sc : (int64_t)bj * kB + sc;
+    const float* p = lu + grow * lda + gcol;
+    if (full) {
+      v[u] = *reinterpret_cast<const float4*>(p);
     } else {
-      tile[(sc + 0) * kTileLd + sr] = v.x;
-      tile[(sc + 1) * kTileLd + sr] = v.y;
-      tile[(sc + 2) * kTileLd + sr] = v.z;
-      tile[(sc + 3) * kTileLd + sr] = v.w;
+      float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (sr < srow_lim) {
+        w.x = sc < scol_lim ? p[0] : 0.f;
+        w.y = sc + 1 < scol_lim ? p[1] : 0.f;
+        w.z = sc + 2 < scol_lim ? p[2] : 0.f;
+        w.w = sc + 3 < scol_lim ? p[3] : 0.f;
+      }
+      v[u] = w;
+    }
+  }
+}
+
+__device__ void tile_store(float* tile, const float4 (&v)[kStageVec], bool tr, int tid) {
+#pragma unroll
+  for (int u = 0; u < kStageVec; ++u) {
+    const int idx = tid + u * kTrsvThreads;
+    const int sr = idx >> 5, sc = 4 * (idx & 31);
+    if (!tr) {
+      *reinterpret_cast<float4*>(tile + sr * kTileLd + sc) = v[u];
+    } else {
+      tile[(sc + 0) * kTileLd + sr] = v[u].x;
+      tile[(sc + 1) * kTileLd + sr] = v[u].y;
+      tile[(sc + 2) * kTileLd + sr] = v[u].z;
+      tile[(sc + 3) * kTileLd + sr] = v[u].w;
     }
   }
 }
 
 // part[rr] = sum_c tile[row][c] * vec[rr][c] over this thread's 64 columns,
 // combined with the partner thread holding the other 64.
+template <int R>
 __device__ __forceinline__ void tile_product(const float* tile, const float (*vec)[kB], int row, int hcol,
-                                             float (&part)[kMaxRhs]) {
+                                             float (&part)[R]) {
 #pragma unroll
-  for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] = 0.f;
+  for (int rr = 0; rr < R; ++rr) part[rr] = 0.f;
   const float* tr = tile + row * kTileLd + hcol;
 #pragma unroll 4
   for (int c = 0; c < 64; c += 4) {
     const float4 e = *reinterpret_cast<const float4*>(tr + c);
 #pragma unroll
-    for (int rr = 0; rr < kMaxRhs; ++rr)
+    for (int rr = 0; rr < R; ++rr)
       part[rr] += e.x * vec[rr][hcol + c] + e.y * vec[rr][hcol + c + 1] + e.z * vec[rr][hcol + c + 2] +
                   e.w * vec[rr][hcol + c + 3];
   }
 #pragma unroll
-  for (int rr = 0; rr < kMaxRhs; ++rr) part[rr] += __shfl_xor(part[rr], 1, kWave);
+  for (int rr = 0; rr < R; ++rr) part[rr] += __shfl_xor(part[rr], 1, kWave);
 }
 
+// R: right-hand sides computed per product (4 or kMaxRhs; nrhs <= R).
+template <int R>
 __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nrhs, const float* __restrict__ lu,
                                                                    int64_t lda, const float* __restrict__ inv,
                                                                    float* __restrict__ x, int64_t ldx, int kind,
@@ -372,11 +442,23 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
   }
   // row i of the block handled by threads 2i, 2i + 1 (64 columns each)
   const int row = tid >> 1, hcol = (tid & 1) * 64;
+  const bool tr = kind >= 2;
+  // operator blocks do not depend on the solution: block `step + 1` is
+  // loaded while block `step` waits for its x and is applied
+  float4 pre[kStageVec];
+  auto block_col = [&](int step) { return forward ? step : nblk - 1 - step; };
+  if (t > 0) {
+    const int bj = block_col(0);
+    tile_load(pre, lu, lda, kind, bi, bj, rows, min(kB, n - bj * kB), tid);
+  }
   for (int step = 0; step < t; ++step) {
-    const int bj = forward ? step : nblk - 1 - step;
+    const int bj = block_col(step);
     const int c0 = bj * kB, cols = min(kB, n - c0);
-    // the operator block does not depend on the solution: stage it first
-    stage_block(tile, lu, lda, kind, bi, bj, rows, cols, tid);
+    tile_store(tile, pre, tr, tid);
+    if (step + 1 < t) {
+      const int bn = block_col(step + 1);
+      tile_load(pre, lu, lda, kind, bi, bn, rows, min(kB, n - bn * kB), tid);
+    }
     if (tid == 0) {
       int spin = 0;
       while (__hip_atomic_load(&flags[bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
@@ -394,11 +476,11 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
       xs[rr][i] = (rr < nrhs && i < cols) ? x[(int64_t)rr * ldx + c0 + i] : 0.f;
     }
     __syncthreads();
-    float part[kMaxRhs];
-    tile_product(tile, xs, row, hcol, part);
+    float part[R];
+    tile_product<R>(tile, xs, row, hcol, part);
     if ((tid & 1) == 0 && row < rows) {
 #pragma unroll
-      for (int rr = 0; rr < kMaxRhs; ++rr)
+      for (int rr = 0; rr < R; ++rr)
         if (rr < nrhs) acc[rr][row] -= part[rr];
     }
     __syncthreads();
@@ -407,26 +489,19 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
   // use inv(D)^T)
   {
     const float* D = inv + (int64_t)bi * kB * kB;
-    const bool tr = kind >= 2;
-    for (int idx = tid; idx < kB * (kB / 4); idx += kTrsvThreads) {
-      const int sr = idx >> 5, sc = 4 * (idx & 31);
-      const float4 v = *reinterpret_cast<const float4*>(D + sr * kB + sc);
-      if (!tr) {
-        *reinterpret_cast<float4*>(tile + sr * kTileLd + sc) = v;
-      } else {
-        tile[(sc + 0) * kTileLd + sr] = v.x;
-        tile[(sc + 1) * kTileLd + sr] = v.y;
-        tile[(sc + 2) * kTileLd + sr] = v.z;
-        tile[(sc + 3) * kTileLd + sr] = v.w;
-      }
+#pragma unroll
+    for (int u = 0; u < kStageVec; ++u) {
+      const int idx = tid + u * kTrsvThreads;
+      pre[u] = *reinterpret_cast<const float4*>(D + (idx >> 5) * kB + 4 * (idx & 31));
     }
+    tile_store(tile, pre, tr, tid);
   }
   __syncthreads();
-  float part[kMaxRhs];
-  tile_product(tile, acc, row, hcol, part);
+  float part[R];
+  tile_product<R>(tile, acc, row, hcol, part);
   if ((tid & 1) == 0 && row < rows) {
 #pragma unroll
-    for (int rr = 0; rr < kMaxRhs; ++rr)
+    for (int rr = 0; rr < R; ++rr)
       if (rr < nrhs) x[(int64_t)rr * ldx + r0 + row] = part[rr];
   }
   // Every wave makes its own stores visible device-wide before the flag is
@@ -523,7 +598,7 @@ int gemv_f32a_launch(hipStream_t st, int m, int k, int nrhs, const float* a, int
 
 int lu_large_factor_impl(hipStream_t st, int mode, int n, float* a, int64_t lda, float* linv, float* uinv,
                          int32_t* piv, int32_t* info) {
-  const size_t dsmem = sizeof(float) * 2 * kB * (kB + 1);
+  const size_t dsmem = sizeof(float) * kB * (kB + 1);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&diag_lu_kernel),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dsmem);
   if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(diag_lu_kernel)");
@@ -624,10 +699,16 @@ extern "C" int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, co
   zero_i32_kernel<<<(nblk + 2 + 255) / 256, 256, 0, st>>>(sync, nblk + 2);
   LXA_LAUNCH_CHECK("zero_i32_kernel");
   const size_t smem = sizeof(float) * (kB * kTileLd + 2 * kMaxRhs * kB);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&trsv_blocks_kernel),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&trsv_blocks_kernel<4>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&trsv_blocks_kernel<kMaxRhs>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(trsv_blocks_kernel)");
-  trsv_blocks_kernel<<<nblk, kTrsvThreads, smem, st>>>((int)n, nrhs, lu, lda, inv, x, ldx, kind, sync);
+  if (nrhs <= 4)
+    trsv_blocks_kernel<4><<<nblk, kTrsvThreads, smem, st>>>((int)n, nrhs, lu, lda, inv, x, ldx, kind, sync);
+  else
+    trsv_blocks_kernel<kMaxRhs><<<nblk, kTrsvThreads, smem, st>>>((int)n, nrhs, lu, lda, inv, x, ldx, kind, sync);
   LXA_LAUNCH_CHECK("trsv_blocks_kernel");
   return LXA_OK;
 }
