@@ -117,7 +117,9 @@ class _Resident:
 
     def __init__(self, kind, m, n, blocks):
         self.kind, self.m, self.n, self.blocks = kind, m, n, blocks
+        self.lda = n
         self.last_use = 0
+        self.busy = 0  # reuse solves in flight
 
 
 class SolveServer:
@@ -152,43 +154,58 @@ class SolveServer:
             self.arena.release(off, elems)
 
     def _evict_one(self) -> bool:
-        if not self.resident:
+        idle = [k for k, r in self.resident.items() if r.busy == 0]
+        if not idle:
             return False
-        victim = min(self.resident, key=lambda k: self.resident[k].last_use)
+        victim = min(idle, key=lambda k: self.resident[k].last_use)
         r = self.resident.pop(victim)
         self.resident_elems -= sum(e for _, e in r.blocks.values())
         self._free_blocks(r.blocks)
         return True
 
     def _evict_until(self, need_elems: int) -> None:
-        while self.resident and self.resident_elems + need_elems > self.resident_budget:
-            victim = min(self.resident, key=lambda k: self.resident[k].last_use)
+        while self.resident_elems + need_elems > self.resident_budget:
+            idle = [k for k, r in self.resident.items() if r.busy == 0]
+            if not idle:
+                break
+            victim = min(idle, key=lambda k: self.resident[k].last_use)
             r = self.resident.pop(victim)
             self.resident_elems -= sum(e for _, e in r.blocks.values())
             self._free_blocks(r.blocks)
 
     # ------------------------------------------------------------------ kernels
-    def _descs(self, items, with_b=True):
+    def _descs(self, items):
+        """items: (m, n, lda, blocks)."""
         d = np.zeros(len(items), dtype=K.QR_DESC)
-        for i, (m, n, blocks) in enumerate(items):
+        for i, (m, n, lda, blocks) in enumerate(items):
             d[i]["a_offset"] = blocks["a"][0]
             if "tau" in blocks:
                 d[i]["tau_offset"] = blocks["tau"][0]
                 d[i]["t_offset"] = blocks["t"][0]
-                d[i]["w_offset"] = blocks["w"][0]
+                if "w" in blocks:
+                    d[i]["w_offset"] = blocks["w"][0]
             d[i]["b_offset"] = blocks["b"][0]
-            d[i]["m"], d[i]["n"], d[i]["lda"], d[i]["ldb"] = m, n, n, m
+            d[i]["m"], d[i]["n"], d[i]["lda"], d[i]["ldb"] = m, n, lda, m
         return K.descs_to_device(d, self.device)
 
     def _qr_solve_batch(self, items):
-        """items: (m, n, blocks) with the rhs already in blocks['b']; x = first n entries."""
+        """items: (m, n, lda, blocks) with the rhs in blocks['b']; x = first n entries."""
         if not items:
             return
         buf = self.arena.buf
         dd = self._descs(items)
-        max_m = max(m for m, _, _ in items)
-        max_k = max(min(m, n) for m, n, _ in items)
+        max_m = max(it[0] for it in items)
+        max_k = max(min(it[0], it[1]) for it in items)
         K.qr_apply_raw(dd, len(items), max_m, max_k, buf, buf, 1, buf, trans=True)
+        K.trsv_raw(dd, len(items), max_k, buf, 1, buf, lower=False, trans=False, unit_diag=False)
+
+    def _trsv_batch(self, items):
+        """R x = c for factored systems whose rhs blocks hold c = (Q^T b)[:n]."""
+        if not items:
+            return
+        buf = self.arena.buf
+        dd = self._descs(items)
+        max_k = max(min(it[0], it[1]) for it in items)
         K.trsv_raw(dd, len(items), max_k, buf, 1, buf, lower=False, trans=False, unit_diag=False)
 
     def _chol_batch(self, items, factor: bool):
@@ -253,10 +270,10 @@ class SolveServer:
                 a = r.a if r.a is not None else self._operator_data[r.op_id]
                 self.arena.view(*blocks["a"]).copy_(a.reshape(-1))
                 self.arena.view(*blocks["b"]).copy_(r.b)
-                dd = self._descs([(r.m, r.n, blocks)])
+                dd = self._descs([(r.m, r.n, r.n, blocks)])
                 buf = self.arena.buf
                 K.qr_factor_raw(dd, 1, r.m, r.n, min(r.m, r.n), buf, buf, buf, buf)
-                self._qr_solve_batch([(r.m, r.n, blocks)])
+                self._qr_solve_batch([(r.m, r.n, r.n, blocks)])
                 r.x = self.arena.view(blocks["b"][0], r.n).clone()
             else:
                 fp = chol_footprint(r.n)
@@ -266,7 +283,7 @@ class SolveServer:
                 a = r.a if r.a is not None else self._operator_data[r.op_id]
                 self.arena.view(*blocks["a"]).copy_(a.reshape(-1))
                 self.arena.view(*blocks["b"]).copy_(r.b)
-                self._chol_batch([(r.n, r.n, blocks)], factor=True)
+                self._chol_batch([(r.n, r.n, r.n, blocks)], factor=True)
                 r.x = self.arena.view(*blocks["b"]).clone()
             torch.cuda.synchronize(self.device)
             r.done = self._now(t0)
@@ -274,120 +291,184 @@ class SolveServer:
             self.ticks += 1
 
     def _run_chosen(self, reqs, t0, time_limit):
-        pending = list(reqs)  # arrival order
+        """Continuous batching on two streams.
+
+        Factor stream: one batched panel step per tick for every QR system being
+        factored.  A new QR request carries its right-hand side as an extra
+        column of A (leading dimension n + 1), so the panel updates leave Q^T b
+        in that column and only the triangular solve remains.  Solve stream:
+        triangular solves of finished QR systems (after an event on the factor
+        stream), Cholesky factor + solve of new requests, and reuse solves.
+        Completions are found by polling events; arena blocks are returned only
+        after the work that uses them has finished.  At most `depth` ticks are
+        in flight on the factor stream."""
+        dev = self.device
+        fstream = torch.cuda.Stream(device=dev)
+        sstream = torch.cuda.Stream(device=dev)
+        depth = 2
+        pending = list(reqs)
         waiting: list[Request] = []
-        factoring: list[tuple[Request, dict, int]] = []  # (request, blocks, next column)
+        factoring: list[list] = []  # [request, blocks, next column]
         owner_pending: dict[int, Request] = {}
+        inflight: list[tuple] = []  # (event, [(request, x, item, rhs_blocks)])
+        tick_events: list = []
         i_next = 0
+        buf = self.arena.buf
+
+        def finish_batches(block: bool) -> None:
+            while inflight and (block or inflight[0][0].query()):
+                ev, batch = inflight.pop(0)
+                ev.synchronize()
+                t_done = self._now(t0)
+                for r, x, item, rhs_blocks in batch:
+                    r.x, r.done = x, t_done
+                    self._retire(r, item, rhs_blocks, owner_pending)
+                block = False
+
         while True:
             now = self._now(t0)
             if now > time_limit:
                 break
+            finish_batches(False)
             while i_next < len(pending) and pending[i_next].arrival <= now:
                 waiting.append(pending[i_next])
                 i_next += 1
             if not waiting and not factoring:
                 if i_next >= len(pending):
                     break
-                continue  # idle until the next arrival
-            qr_solve, chol_new, chol_reuse, finished = [], [], [], []
-            # admission, first come first served
-            still = []
-            blocked = False
-            for r in waiting:
-                if blocked:
-                    still.append(r)
-                    continue
-                if r.reuse and r.op_id in self.resident:
-                    res = self.resident[r.op_id]
-                    blocks = self._alloc_blocks({"b": r.m})
+                if inflight:
+                    finish_batches(False)
+                continue
+            qr_done, chol_new, solve_reuse = [], [], []
+            still, blocked = [], False
+            with torch.cuda.stream(fstream):
+                for r in waiting:
+                    if blocked:
+                        still.append(r)
+                        continue
+                    if r.reuse and r.op_id in self.resident:
+                        res = self.resident[r.op_id]
+                        blocks = self._alloc_blocks({"b": r.m})
+                        if blocks is None:
+                            blocked = True
+                            still.append(r)
+                            continue
+                        with torch.cuda.stream(sstream):
+                            self.arena.view(*blocks["b"]).copy_(r.b)
+                        res.last_use = self.ticks
+                        res.busy += 1
+                        merged = dict(res.blocks)
+                        merged["b"] = blocks["b"]
+                        solve_reuse.append((r, (r.m, r.n, res.lda, merged), blocks))
+                        continue
+                    if r.reuse and r.op_id in owner_pending:
+                        still.append(r)
+                        continue
+                    a = r.a if r.a is not None else self._operator_data[r.op_id]
+                    if r.kind == "qr":
+                        fp = qr_footprint(r.m, r.n + 1, self.nb)
+                        fp["b"] = r.n
+                    else:
+                        fp = chol_footprint(r.n)
+                    blocks = self._alloc_blocks(fp)
                     if blocks is None:
                         blocked = True
                         still.append(r)
                         continue
-                    self.arena.view(*blocks["b"]).copy_(r.b)
-                    res.last_use = self.ticks
-                    merged = dict(res.blocks)
-                    merged["b"] = blocks["b"]
-                    item = (r.m, r.n, merged)
-                    (qr_solve if r.kind == "qr" else chol_reuse).append((r, item, blocks))
-                    continue
-                if r.reuse and r.op_id in owner_pending:
-                    still.append(r)  # its operator is still being factored
-                    continue
-                a = r.a if r.a is not None else self._operator_data[r.op_id]
-                fp = qr_footprint(r.m, r.n, self.nb) if r.kind == "qr" else chol_footprint(r.n)
-                blocks = self._alloc_blocks(fp)
-                if blocks is None:
-                    blocked = True
-                    still.append(r)
-                    continue
-                r.admitted = now
-                self.arena.view(*blocks["a"]).copy_(a.reshape(-1))
-                self.arena.view(*blocks["b"]).copy_(r.b)
-                if r.kind == "qr":
-                    factoring.append((r, blocks, 0))
+                    r.admitted = now
                     owner_pending[r.op_id] = r
-                else:
-                    chol_new.append((r, (r.n, r.n, blocks), blocks))
-                    owner_pending[r.op_id] = r
-            waiting = still
-            if blocked and not (factoring or qr_solve or chol_new or chol_reuse):
-                # nothing in flight frees memory: make room from the resident factors
-                if not self._evict_one():
-                    raise MemoryError("admission blocked with nothing in flight")
-                continue
-            # one panel step for every QR system being factored
-            if factoring:
-                buf = self.arena.buf
-                dd = self._descs([(r.m, r.n, blk) for r, blk, _ in factoring])
-                kcol = torch.tensor([k for _, _, k in factoring], dtype=torch.int32).to(self.device)
-                max_rows = max(r.m - k for r, _, k in factoring)
-                max_tc = max(r.n - k - 1 for r, _, k in factoring)
-                K.qr_factor_step_raw(dd, len(factoring), kcol, max_rows, max_tc, buf, buf, buf, buf)
-                nxt = []
-                for r, blk, k in factoring:
-                    k2 = k + self.nb
-                    if k2 >= min(r.m, r.n):
-                        qr_solve.append((r, (r.m, r.n, blk), None))
+                    if r.kind == "qr":
+                        aug = self.arena.view(*blocks["a"]).view(r.m, r.n + 1)
+                        aug[:, : r.n].copy_(a)
+                        aug[:, r.n].copy_(r.b)
+                        factoring.append([r, blocks, 0])
                     else:
-                        nxt.append((r, blk, k2))
-                factoring = nxt
-            self._qr_solve_batch([item for _, item, _ in qr_solve])
-            self._chol_batch([item for _, item, _ in chol_new], factor=True)
-            self._chol_batch([item for _, item, _ in chol_reuse], factor=False)
-            outs = []
-            for r, item, rhs_blocks in qr_solve + chol_new + chol_reuse:
-                m, n, blk = item
-                outs.append((r, self.arena.view(blk["b"][0], r.n).clone()))
-            torch.cuda.synchronize(self.device)
-            t_done = self._now(t0)
-            self.ticks += 1
-            for r, x in outs:
-                r.x = x
-                r.done = t_done
-            # release or keep resident
-            for r, item, rhs_blocks in qr_solve + chol_new + chol_reuse:
-                m, n, blk = item
-                if rhs_blocks is not None and r.reuse and r.op_id in self.resident:
-                    self._free_blocks(rhs_blocks)  # the reused factor stays
+                        with torch.cuda.stream(sstream):
+                            self.arena.view(*blocks["a"]).copy_(a.reshape(-1))
+                            self.arena.view(*blocks["b"]).copy_(r.b)
+                        chol_new.append((r, (r.n, r.n, r.n, blocks), None))
+                waiting = still
+                if blocked and not (factoring or qr_done or chol_new or solve_reuse or inflight):
+                    if not self._evict_one():
+                        raise MemoryError("admission blocked with nothing in flight")
                     continue
-                owner_pending.pop(r.op_id, None)
-                keep = {k: v for k, v in blk.items() if k != "b"}
-                self.arena.release(*blk["b"])
-                elems = sum(e for _, e in keep.values())
-                if elems <= self.resident_budget:
-                    self._evict_until(elems)
-                    if r.op_id in self.resident:  # an older copy
-                        old = self.resident.pop(r.op_id)
-                        self.resident_elems -= sum(e for _, e in old.blocks.values())
-                        self._free_blocks(old.blocks)
-                    self.resident[r.op_id] = _Resident(r.kind, m, n, keep)
-                    self.resident[r.op_id].last_use = self.ticks
-                    self.resident_elems += elems
-                else:
-                    self._free_blocks(keep)
+                # one panel step for every QR system being factored (factor stream)
+                if factoring:
+                    items = [(r.m, r.n + 1, r.n + 1, blk) for r, blk, _ in factoring]
+                    dd = self._descs(items)
+                    kcol = torch.tensor([k for _, _, k in factoring], dtype=torch.int32).to(dev)
+                    max_rows = max(r.m - k for r, _, k in factoring)
+                    max_tc = max(r.n - k for r, _, k in factoring)
+                    K.qr_factor_step_raw(dd, len(factoring), kcol, max_rows, max_tc, buf, buf, buf, buf)
+                    nxt = []
+                    for f in factoring:
+                        f[2] += self.nb
+                        (qr_done if f[2] >= f[0].n else nxt).append(f)
+                    factoring = nxt
+                ev_f = torch.cuda.Event()
+                ev_f.record(fstream)
+            # solves (solve stream)
+            batch = []
+            with torch.cuda.stream(sstream):
+                if qr_done:
+                    sstream.wait_event(ev_f)
+                    items = []
+                    for r, blk, _ in qr_done:
+                        # Q^T b sits in column n of the augmented matrix
+                        col = self.arena.buf[blk["a"][0] + r.n: blk["a"][0] + r.n + r.n * (r.n + 1): r.n + 1]
+                        self.arena.view(*blk["b"]).copy_(col)
+                        items.append((r.m, r.n, r.n + 1, blk))
+                    self._trsv_batch(items)
+                    for (r, blk, _), item in zip(qr_done, items):
+                        batch.append((r, self.arena.view(blk["b"][0], r.n).clone(), item, None))
+                qr_reuse = [(r, it, rb) for r, it, rb in solve_reuse if r.kind == "qr"]
+                ch_reuse = [(r, it, rb) for r, it, rb in solve_reuse if r.kind != "qr"]
+                if qr_reuse:
+                    self._qr_solve_batch([it for _, it, _ in qr_reuse])
+                self._chol_batch([it for _, it, _ in chol_new], factor=True)
+                self._chol_batch([it for _, it, _ in ch_reuse], factor=False)
+                for r, item, rb in qr_reuse + chol_new + ch_reuse:
+                    batch.append((r, self.arena.view(item[3]["b"][0], r.n).clone(), item, rb))
+                if batch:
+                    ev = torch.cuda.Event()
+                    ev.record(sstream)
+                    inflight.append((ev, batch))
+            self.ticks += 1
+            tick_events.append(ev_f)
+            if len(tick_events) > depth:
+                tick_events.pop(0).synchronize()
+        finish_batches(True)
+        while inflight:
+            finish_batches(True)
 
+    def _retire(self, r, item, rhs_blocks, owner_pending):
+        """Returns a finished request's blocks to the arena or keeps its
+        factorisation resident for reuse requests."""
+        m, n, lda, blk = item
+        if rhs_blocks is not None:
+            self._free_blocks(rhs_blocks)  # a reuse solve: the resident factor stays
+            self.resident[r.op_id].busy -= 1
+            return
+        owner_pending.pop(r.op_id, None)
+        # keep what a reuse solve needs: the factor (and tau, T blocks for QR)
+        keep = {k: v for k, v in blk.items() if k not in ("b", "w")}
+        self.arena.release(*blk["b"])
+        if "w" in blk:
+            self.arena.release(*blk["w"])
+        elems = sum(e for _, e in keep.values())
+        if elems <= self.resident_budget:
+            self._evict_until(elems)
+            if r.op_id in self.resident:  # an older copy
+                old = self.resident.pop(r.op_id)
+                self.resident_elems -= sum(e for _, e in old.blocks.values())
+                self._free_blocks(old.blocks)
+            res = _Resident(r.kind, m, n, keep)
+            res.lda = lda
+            res.last_use = self.ticks
+            self.resident[r.op_id] = res
+            self.resident_elems += elems
+        else:
+            self._free_blocks(keep)
 
 
 def make_request_stream(rate: float, count: int, *, seed: int = 5, device="cuda:0", qr_m=(64, 4096),
