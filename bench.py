@@ -72,7 +72,9 @@ def _traffic_from_profiles(kernel_prefix):
     path = ROOT / "profiles" / "uneven_cholesky_pmc.txt"
     if not path.exists():
         return None
-    m = re.search(r"bytes_per_launch:\s*([0-9.eE+]+)", path.read_text())
+    m = re.search(r"^bytes_per_step:\s*([0-9.eE+]+)", path.read_text(), re.MULTILINE)
+    if m is None:
+        m = re.search(r"^bytes_per_launch:\s*([0-9.eE+]+)", path.read_text(), re.MULTILINE)
     return float(m.group(1)) if m else None
 
 
@@ -283,7 +285,8 @@ def bench_uneven_cholesky(args):
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic,
-        "kernel": "chol_uneven_kernel<true,double> (persistent)",
+        "kernel": "chol_uneven_kernel<16|8, true, double>: one persistent launch per register-tile class "
+                  "(orders > 128, orders <= 128) per step; achieved over the step's two launches",
         "algorithmic_bytes_per_launch": batch.algorithmic_bytes,
         "fp32_flop_view": {
             "achieved_tflops": batch.algorithmic_flops / (mean_ms / 1e3) / 1e12,

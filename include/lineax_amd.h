@@ -288,6 +288,34 @@ int lxa_gmres_arnoldi(void* stream, int vdtype, int64_t rows, int64_t n, int j, 
 int lxa_gmres_update(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* basis, void* y,
                      const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
                      const double* hessenberg, double* z, double* partials);
+
+/* As lxa_gmres_update, for a basis stored as columns v_k / colscale[k]
+ * ([rows][restart + 1] FP64, lxa_gmres_stencil_step's layout). */
+int lxa_gmres_update_scaled(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* basis,
+                            void* y, const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                            const double* hessenberg, double* z, const double* colscale, double* partials);
+
+/* Fused Arnoldi steps for a constant-coefficient five-point stencil operator
+ * (coeffs as in lxa_stencil5_apply) with no preconditioner, grids of width
+ * ny <= 1024, restart <= 32.  Replaces lxa_gmres_arnoldi's classical
+ * Gram-Schmidt step (lineax/_solver/gmres.py:355-398) with one sweep per
+ * step that reads the basis once:
+ *   mode 0, j = 0: t = A v_0, its projections;
+ *   mode 1, 1 <= j < restart: u_j = w'_{j-1} (the orthogonalised previous
+ *     step), stored unnormalised as basis column j; t = A u_j; the
+ *     projections of step j.  Closes step j-1 (|w'|, breakdown test,
+ *     Hessenberg row j-1);
+ *   mode 2, j = restart: closes the last step.
+ * w_in / w_out: the raw previous / next w ([rows][nx*ny], ping-pong).
+ * coefs ([rows][33] FP64), colscale ([rows][restart + 1] FP64), proj
+ * ([rows][restart + 1] FP64) carry the step to step state; basis, the
+ * Hessenberg buffer and the row states are lxa_gmres_*'s.  Begin a cycle
+ * with lxa_gmres_cycle_begin, end it with lxa_gmres_update_scaled. */
+int lxa_gmres_stencil_workspace(int64_t rows, int64_t nx, int64_t ny, size_t* partial_bytes, size_t* coef_bytes);
+int lxa_gmres_stencil_step(void* stream, int vdtype, int64_t rows, int64_t nx, int64_t ny, const double* coeffs,
+                           int mode, int j, const lxa_krylov_params* prm, void* basis, const void* w_in, void* w_out,
+                           double* coefs, double* colscale, double* proj, double* hessenberg, lxa_krylov_row* state,
+                           lxa_gmres_row* gstate, double* partials);
 int lxa_gmres_residual(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* b,
                        const void* t, void* r, int write, const lxa_krylov_params* prm, lxa_krylov_row* state,
                        double* partials);

@@ -56,6 +56,10 @@ _SIGNATURES = {
     "lxa_cg_finish": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "lxa_cg_result": [_vp, _i64, _vp, _vp],
     "lxa_cg_stencil_workspace": [_i64, _i64, _i64, _szp],
+    "lxa_gmres_stencil_workspace": [_i64, _i64, _i64, _szp, _szp],
+    "lxa_gmres_stencil_step": [_vp, _int, _i64, _i64, _i64, _vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                               _vp, _vp, _vp],
+    "lxa_gmres_update_scaled": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "lxa_cg_stencil_direction": [_vp, _int, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _int, _vp, _vp, _vp],
     "lxa_cg_stencil_update": [_vp, _int, _int, _int, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _vp,
                               _vp],
@@ -477,6 +481,20 @@ class StencilCGWorkspace:
         pb = ctypes.c_size_t(0)
         _check(load().lxa_cg_stencil_workspace(rows, nx, ny, ctypes.byref(pb)), "lxa_cg_stencil_workspace")
         self.partials = torch.empty(max(1, int(pb.value) // 8), dtype=torch.float64, device=device)
+        self._coeffs = (ctypes.c_double * 5)(*[float(c) for c in coeffs])
+        self.coeffs = ctypes.cast(self._coeffs, ctypes.c_void_p).value
+
+
+class StencilGMRESWorkspace:
+    """Partials, step coefficients and coefficients of the fused stencil GMRES steps."""
+
+    def __init__(self, rows: int, nx: int, ny: int, restart: int, coeffs, device):
+        pb, cb = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        _check(load().lxa_gmres_stencil_workspace(rows, nx, ny, ctypes.byref(pb), ctypes.byref(cb)),
+               "lxa_gmres_stencil_workspace")
+        self.partials = torch.empty(max(1, int(pb.value) // 8), dtype=torch.float64, device=device)
+        self.coefs = torch.zeros(max(1, int(cb.value) // 8), dtype=torch.float64, device=device)
+        self.colscale = torch.zeros((rows, restart + 1), dtype=torch.float64, device=device)
         self._coeffs = (ctypes.c_double * 5)(*[float(c) for c in coeffs])
         self.coeffs = ctypes.cast(self._coeffs, ctypes.c_void_p).value
 

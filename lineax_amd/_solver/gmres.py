@@ -9,10 +9,13 @@ from .._norm import max_norm
 from .._operator import conj, linearise
 from .._solve import AbstractLinearSolver
 from .._structure import batch_shape_of, ravel, structure_equal, unravel
-from .cg import FlatOperator, norm_kind
+from .cg import FlatOperator, norm_kind, stencil_of
 from .misc import preconditioner_and_y0
 
 MAX_RESTART = 128
+# fused stencil Arnoldi steps: restart and grid-width limits
+FUSED_MAX_RESTART = 32
+FUSED_MAX_WIDTH = 1024
 
 
 class GMRES(AbstractLinearSolver):
@@ -28,10 +31,14 @@ class GMRES(AbstractLinearSolver):
     products, the Hessenberg matrix and the least-squares solve are FP64.
     ``graph`` (not in lineax) captures a restart cycle in a HIP graph and
     replays it, checking the active flags on the host between cycles.
+    ``fused`` (not in lineax): for a five-point stencil operator with no
+    preconditioner (grid width <= 1024, restart <= 32), each Arnoldi step is
+    one sweep of lxa_gmres_stencil_step that reads the basis once.
     """
 
     def __init__(self, rtol, atol, norm=max_norm, max_steps: int | None = None, restart: int = 20,
-                 stagnation_iters: int = 20, working_dtype: torch.dtype | None = None, graph: bool = False):
+                 stagnation_iters: int = 20, working_dtype: torch.dtype | None = None, graph: bool = False,
+                 fused: bool = True):
         if isinstance(rtol, (int, float)) and rtol < 0:
             raise ValueError("Tolerances must be non-negative.")
         if isinstance(atol, (int, float)) and atol < 0:
@@ -46,6 +53,7 @@ class GMRES(AbstractLinearSolver):
         self.stagnation_iters = stagnation_iters
         self.working_dtype = working_dtype
         self.graph = graph
+        self.fused = fused
 
     def init(self, operator, options):
         if not structure_equal(operator.in_structure(), operator.out_structure()):
@@ -98,15 +106,40 @@ class GMRES(AbstractLinearSolver):
                 r.copy_(M(r, vdt))
                 K.call("lxa_gmres_residual", stream, yd, vd, rows, size, b, t, r, 0, prm, st, part)
 
-        def cycle():
-            stream = K.current_stream(b)  # the capture stream while a graph is recorded
-            K.call("lxa_gmres_cycle_begin", stream, vd, rows, size, r, basis, prm, st, gs, hess, part)
-            for j in range(m):
-                w = M(A(basis[:, j, :], vdt), vdt).contiguous()
-                K.call("lxa_gmres_arnoldi", stream, vd, rows, size, j, basis, w, prm, st, gs, hess, proj, part)
-            K.call("lxa_gmres_update", stream, yd, vd, rows, size, basis, y, prm, st, gs, hess, z, part)
-            residual()
-            K.call("lxa_gmres_cycle_end", stream, rows, size, 0, prm, st, gs, part)
+        stencil = None
+        if self.fused and not precond and dev.type == "cuda" and m <= FUSED_MAX_RESTART:
+            stencil = stencil_of(operator)
+            if stencil is not None and (stencil[0] * stencil[1] != size or stencil[1] > FUSED_MAX_WIDTH):
+                stencil = None
+
+        if stencil is None:
+            def cycle():
+                stream = K.current_stream(b)  # the capture stream while a graph is recorded
+                K.call("lxa_gmres_cycle_begin", stream, vd, rows, size, r, basis, prm, st, gs, hess, part)
+                for j in range(m):
+                    w = M(A(basis[:, j, :], vdt), vdt).contiguous()
+                    K.call("lxa_gmres_arnoldi", stream, vd, rows, size, j, basis, w, prm, st, gs, hess, proj, part)
+                K.call("lxa_gmres_update", stream, yd, vd, rows, size, basis, y, prm, st, gs, hess, z, part)
+                residual()
+                K.call("lxa_gmres_cycle_end", stream, rows, size, 0, prm, st, gs, part)
+        else:
+            nx, ny, coeffs = stencil
+            ws = K.StencilGMRESWorkspace(rows, nx, ny, m, coeffs, dev)
+            wbuf = (torch.empty((rows, size), dtype=vdt, device=dev), torch.empty((rows, size), dtype=vdt, device=dev))
+
+            def cycle():
+                stream = K.current_stream(b)
+                K.call("lxa_gmres_cycle_begin", stream, vd, rows, size, r, basis, prm, st, gs, hess, part)
+                for j in range(m + 1):
+                    mode = 0 if j == 0 else (2 if j == m else 1)
+                    w_in = wbuf[(j - 1) % 2] if j > 0 else None
+                    w_out = wbuf[j % 2] if j < m else None
+                    K.call("lxa_gmres_stencil_step", stream, vd, rows, nx, ny, ws.coeffs, mode, j, prm, basis, w_in,
+                           w_out, ws.coefs, ws.colscale, proj, hess, st, gs, ws.partials)
+                K.call("lxa_gmres_update_scaled", stream, yd, vd, rows, size, basis, y, prm, st, gs, hess, z,
+                       ws.colscale, part)
+                residual()
+                K.call("lxa_gmres_cycle_end", stream, rows, size, 0, prm, st, gs, part)
 
         # The dummy first pass only forms r0 (gmres.py:313-317).
         residual()

@@ -584,11 +584,14 @@ __global__ __launch_bounds__(64) void gm_lstsq_kernel(int64_t rows, int m, const
 
 // diff = sum_k z_k v_k over the written columns; y += diff; partial
 // |diff / y_scale| (gmres.py:304-310, 130-141).
+// colscale (may be null): basis column k is stored as v_k / colscale[k]
+// (the fused stencil Arnoldi steps), so its coefficient is z_k colscale[k].
 template <typename TY, typename TV>
 __global__ __launch_bounds__(kB) void gm_update_kernel(int64_t n, const TV* __restrict__ V, const double* __restrict__ z,
                                                        int m, TY* __restrict__ y, GmParams prm,
                                                        const lxa_krylov_row* __restrict__ st,
                                                        const lxa_gmres_row* __restrict__ gs,
+                                                       const double* __restrict__ colscale,
                                                        double* __restrict__ part) {
   __shared__ double red[4];
   __shared__ double zs[kMaxRestart];
@@ -597,16 +600,45 @@ __global__ __launch_bounds__(kB) void gm_update_kernel(int64_t n, const TV* __re
   const int cols = gs[row].columns < m ? gs[row].columns : m;
   const Chunk c = chunk_of(n);
   const TV* Vr = V + row * static_cast<int64_t>(m + 1) * n;
-  for (int k = threadIdx.x; k < m; k += kB) zs[k] = z[row * m + k];
+  for (int k = threadIdx.x; k < m; k += kB)
+    zs[k] = colscale ? z[row * m + k] * colscale[row * (m + 1) + k] : z[row * m + k];
   __syncthreads();
   const int kind = prm.norm_kind;
   double n2 = 0.0;
-  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
-    double d = 0.0;
-    for (int k = 0; k < cols; ++k) d = fma(zs[k], static_cast<double>(Vr[static_cast<int64_t>(k) * n + i]), d);
+  auto point = [&](int64_t i, double d) {
     const double yn = static_cast<double>(y[row * n + i]) + d;
     y[row * n + i] = static_cast<TY>(yn);
     if (prm.has_scale) n2 = norm_accum(kind, n2, norm_elem(kind, d / (prm.atol + prm.rtol * fabs(yn))));
+  };
+  bool done = false;
+  if constexpr (sizeof(TV) == 4) {
+    if (n % 4 == 0) {
+      done = true;
+    // four points per thread with 16-byte basis loads (one contiguous 1 KiB
+    // request per wave and column instead of 256 bytes)
+    const int64_t q0 = c.begin & ~int64_t(3), q1 = blockIdx.x + 1 == gridDim.x ? n : (c.end & ~int64_t(3));
+    for (int64_t i = q0 + 4 * static_cast<int64_t>(threadIdx.x); i < q1; i += 4 * kB) {
+      double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+      for (int k = 0; k < cols; ++k) {
+        const float4 v = *reinterpret_cast<const float4*>(Vr + static_cast<int64_t>(k) * n + i);
+        d0 = fma(zs[k], static_cast<double>(v.x), d0);
+        d1 = fma(zs[k], static_cast<double>(v.y), d1);
+        d2 = fma(zs[k], static_cast<double>(v.z), d2);
+        d3 = fma(zs[k], static_cast<double>(v.w), d3);
+      }
+      point(i, d0);
+      point(i + 1, d1);
+      point(i + 2, d2);
+      point(i + 3, d3);
+    }
+    }
+  }
+  if (!done) {
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+      double d = 0.0;
+      for (int k = 0; k < cols; ++k) d = fma(zs[k], static_cast<double>(Vr[static_cast<int64_t>(k) * n + i]), d);
+      point(i, d);
+    }
   }
   n2 = norm_block(kind, n2, red);
   if (threadIdx.x == 0) part[(row * 3) * gridDim.x + blockIdx.x] = n2;
@@ -1021,9 +1053,29 @@ extern "C" int lxa_gmres_arnoldi(void* stream, int vdtype, int64_t rows, int64_t
   return LXA_OK;
 }
 
+static int gmres_update_impl(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* basis,
+                             void* y, const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                             const double* hessenberg, double* z, const double* colscale, double* partials);
+
 extern "C" int lxa_gmres_update(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* basis,
                                 void* y, const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
                                 const double* hessenberg, double* z, double* partials) {
+  return gmres_update_impl(stream, ydtype, vdtype, rows, n, basis, y, prm, state, gstate, hessenberg, z, nullptr,
+                           partials);
+}
+
+extern "C" int lxa_gmres_update_scaled(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n,
+                                       const void* basis, void* y, const lxa_krylov_params* prm,
+                                       lxa_krylov_row* state, lxa_gmres_row* gstate, const double* hessenberg,
+                                       double* z, const double* colscale, double* partials) {
+  LXA_REQUIRE(colscale, "lxa_gmres_update_scaled: null colscale");
+  return gmres_update_impl(stream, ydtype, vdtype, rows, n, basis, y, prm, state, gstate, hessenberg, z, colscale,
+                           partials);
+}
+
+static int gmres_update_impl(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* basis,
+                             void* y, const lxa_krylov_params* prm, lxa_krylov_row* state, lxa_gmres_row* gstate,
+                             const double* hessenberg, double* z, const double* colscale, double* partials) {
   if (int rc = check_gmres(rows, n, prm, state, gstate, partials)) return rc;
   LXA_REQUIRE(basis && y && hessenberg && z, "lxa_gmres_update: null pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1041,7 +1093,7 @@ extern "C" int lxa_gmres_update(void* stream, int ydtype, int vdtype, int64_t ro
   LXA_DISPATCH_YV(ydtype, vdtype,
                   hipLaunchKernelGGL((lxa::gm_update_kernel<TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
                                      static_cast<const TV*>(basis), z, m, static_cast<TY*>(y), g, state, gstate,
-                                     partials));
+                                     colscale, partials));
   LXA_LAUNCH_CHECK("lxa_gmres_update");
   return LXA_OK;
 }

@@ -305,3 +305,81 @@ def test_cg_fused_graph_replays_match_eager(device, rng, stabilise_every):
         assert torch.equal(g.value, eager.value)
         assert torch.equal(g.stats["num_steps"], eager.stats["num_steps"])
         assert torch.equal(g.result, eager.result)
+
+
+# --------------------------------------------------------------------------
+# Fused stencil Arnoldi steps (lxa_gmres_stencil_step)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("nx,ny,restart", [(20, 20, 10), (37, 50, 7), (40, 1024, 5), (17, 3, 30)])
+def test_gmres_fused_matches_unfused(device, rng, nx, ny, restart):
+    # the one-sweep Arnoldi step runs lineax's classical Gram-Schmidt with the
+    # basis stored unnormalised: equal RESULTS and step counts, iterates equal
+    # to rounding
+    from lineax_amd.stencil import StencilOperator, convection_diffusion_coeffs
+
+    op = StencilOperator(nx, ny, convection_diffusion_coeffs(max(nx, ny)), device=device)
+    b = _t(rng.standard_normal((2, nx, ny)), device)
+    kw = dict(rtol=1e-9, atol=1e-9, restart=restart, max_steps=60)
+    f = lx.linear_solve(op, b, lx.GMRES(**kw), throw=False)
+    u = lx.linear_solve(op, b, lx.GMRES(fused=False, **kw), throw=False)
+    assert torch.equal(f.result, u.result)
+    assert torch.max(torch.abs(f.stats["num_steps"] - u.stats["num_steps"])) <= 1
+    scale = float(torch.max(torch.abs(u.value)))
+    assert float(torch.max(torch.abs(f.value - u.value))) <= 1e-7 * scale
+
+
+def test_gmres_fused_vs_oracle(device, rng):
+    from lineax_amd.stencil import convection_diffusion_coeffs
+
+    n = 24
+    coeffs = convection_diffusion_coeffs(n)
+    op = convection_diffusion2d(n, device=device)
+    b = rng.standard_normal((n, n))
+    sol = lx.linear_solve(op, _t(b, device), lx.GMRES(rtol=1e-10, atol=1e-10, restart=20))
+    x_ref, res_ref, steps_ref = O.gmres(lambda v: O.stencil_apply(v.reshape(n, n), coeffs).ravel(), b.ravel(), 1e-10,
+                                       1e-10, restart=20)
+    assert int(sol.result) == res_ref == 0
+    assert abs(int(sol.stats["num_steps"]) - steps_ref) <= 1
+    np.testing.assert_allclose(sol.value.cpu().numpy().ravel(), x_ref, rtol=1e-7, atol=1e-8)
+
+
+def test_gmres_fused_breakdown_identity_stencil(device, rng):
+    # A = I as a stencil: the first Arnoldi step breaks down (w' = 0); the
+    # fused and unfused paths give lineax's RESULTS and step counts
+    from lineax_amd.stencil import StencilOperator
+
+    op = StencilOperator(12, 9, (1.0, 0.0, 0.0, 0.0, 0.0), device=device)
+    b = _t(rng.standard_normal((3, 12, 9)), device)
+    f = lx.linear_solve(op, b, lx.GMRES(rtol=1e-8, atol=1e-8, restart=5), throw=False)
+    u = lx.linear_solve(op, b, lx.GMRES(rtol=1e-8, atol=1e-8, restart=5, fused=False), throw=False)
+    assert torch.equal(f.result, u.result)
+    assert torch.equal(f.stats["num_steps"], u.stats["num_steps"])
+    assert float(torch.max(torch.abs(f.value - b))) <= 1e-12
+
+
+def test_gmres_fused_float32_working(device, rng):
+    n = 64
+    op = convection_diffusion2d(n, device=device)
+    b = _t(rng.standard_normal((2, n, n)), device)
+    kw = dict(rtol=1e-5, atol=1e-5, restart=30, working_dtype=torch.float32, max_steps=200)
+    f = lx.linear_solve(op, b, lx.GMRES(**kw), throw=False)
+    u = lx.linear_solve(op, b, lx.GMRES(fused=False, **kw), throw=False)
+    assert torch.equal(f.result, u.result)
+    assert torch.max(torch.abs(f.stats["num_steps"] - u.stats["num_steps"])) <= 2
+    r = b - op.mv(f.value)
+    assert float(torch.max(torch.abs(r) / (1e-5 + 1e-5 * torch.abs(b)))) <= 1.0
+
+
+def test_gmres_fused_graph_replays_match_eager(device, rng):
+    n = 30
+    op = convection_diffusion2d(n, device=device)
+    b = _t(rng.standard_normal((2, n, n)), device)
+    kw = dict(rtol=1e-9, atol=1e-9, restart=10)
+    eager = lx.linear_solve(op, b, lx.GMRES(**kw), throw=False)
+    for _ in range(3):
+        g = lx.linear_solve(op, b, lx.GMRES(graph=True, **kw), throw=False)
+        _ = (b * 2.0).sum().item()
+        assert torch.equal(g.value, eager.value)
+        assert torch.equal(g.stats["num_steps"], eager.stats["num_steps"])
+        assert torch.equal(g.result, eager.result)
