@@ -60,6 +60,34 @@ __device__ __forceinline__ double norm_final(int kind, double v, int64_t n) {
   return n > 0 ? sqrt(v / static_cast<double>(n)) : 0.0;
 }
 
+// Reduces three values over a block of nt threads (a multiple of 64, at
+// most 1024; sum, sum, or norm).  Every thread gets the results; red holds
+// 3 * (nt / 64) doubles.
+__device__ void block_reduce3_n(int kind, double& a, double& b, double& c, double* red, int nt) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a = norm_acc(kind, a, __shfl_xor(a, off, 64));
+    b += __shfl_xor(b, off, 64);
+    c = norm_acc(kind, c, __shfl_xor(c, off, 64));
+  }
+  const int w = threadIdx.x >> 6, nw = nt >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = a;
+    red[16 + w] = b;
+    red[32 + w] = c;
+  }
+  __syncthreads();
+  a = red[0];
+  b = red[16];
+  c = red[32];
+  for (int q = 1; q < nw; ++q) {
+    a = norm_acc(kind, a, red[q]);
+    b += red[16 + q];
+    c = norm_acc(kind, c, red[32 + q]);
+  }
+  __syncthreads();
+}
+
 // Reduces three values over the 256-thread block (sum, sum, or norm).  Every
 // thread gets the results.
 __device__ void block_reduce3(int kind, double& a, double& b, double& c, double* red) {
@@ -421,6 +449,309 @@ __global__ __launch_bounds__(kTW) void cgs_finish_kernel(int nblk, int64_t n, Cg
   }
 }
 
+// ---------------------------------------------------------------------------
+// Full-row variants for grid widths that are a multiple of 4 (at most 2048):
+// thread t owns the four adjacent columns 4t .. 4t + 3 (16-byte loads of
+// float vectors), a workgroup covers kTH4 whole grid rows, so there is no
+// column halo; the columns either side of a thread's group come from its
+// neighbours through LDS.  Same arithmetic, order and results as the tiled
+// kernels above.
+// ---------------------------------------------------------------------------
+constexpr int kTH4 = 64;     // grid rows per tile, update sweep
+constexpr int kTH4Dir = 16;  // grid rows per tile, direction sweep
+constexpr int kQ4 = 4;       // rows of loads in flight, update sweep
+constexpr int kQ4Dir = 8;    // rows of loads in flight, direction sweep
+constexpr int kMaxThreads4 = 512;  // grid widths up to 2048
+
+template <typename T>
+struct Quad;
+template <>
+struct Quad<float> {
+  using type = float4;
+};
+template <>
+struct Quad<double> {
+  using type = double4;
+};
+
+template <typename T>
+__device__ __forceinline__ typename Quad<T>::type ld4(const T* base, int ny, int i, int c) {
+  return *reinterpret_cast<const typename Quad<T>::type*>(base + static_cast<int64_t>(i) * ny + c);
+}
+
+template <typename Q>
+__device__ __forceinline__ double q_at(const Q& q, int e) {
+  return e == 0 ? static_cast<double>(q.x) : e == 1 ? static_cast<double>(q.y) : e == 2 ? static_cast<double>(q.z)
+                                                                                     : static_cast<double>(q.w);
+}
+
+// The four stencil values of a thread's columns from the window rows and the
+// neighbouring columns (west of column 0, east of column 3).
+__device__ __forceinline__ void stencil4(const St5& k, const double (&pr)[4], const double (&cu)[4],
+                                         const double (&nx4)[4], double west, double east, double (&out)[4]) {
+  out[0] = k.c * cu[0] + k.w * west + k.e * cu[1] + k.s * pr[0] + k.n * nx4[0];
+  out[1] = k.c * cu[1] + k.w * cu[0] + k.e * cu[2] + k.s * pr[1] + k.n * nx4[1];
+  out[2] = k.c * cu[2] + k.w * cu[1] + k.e * cu[3] + k.s * pr[2] + k.n * nx4[2];
+  out[3] = k.c * cu[3] + k.w * cu[2] + k.e * east + k.s * pr[3] + k.n * nx4[3];
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kMaxThreads4) void cgs_direction4_kernel(int nx, int ny, St5 k,
+                                                                      const TV* __restrict__ r,
+                                                                      const TV* __restrict__ pold,
+                                                                      TV* __restrict__ pnew, int first,
+                                                                      int64_t vstride,
+                                                                      const lxa_krylov_row* __restrict__ st,
+                                                                      double* __restrict__ part) {
+  using Q = typename Quad<TV>::type;
+  __shared__ double eL[2][kMaxThreads4 + 1], eR[2][kMaxThreads4 + 1];
+  __shared__ double red[48];
+  const int row = blockIdx.y;
+  if (!st[row].active) return;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int i0 = blockIdx.x * kTH4Dir, i1 = i0 + kTH4Dir < nx ? i0 + kTH4Dir : nx;
+  const int imax = i1 < nx - 1 ? i1 : nx - 1;
+  const int c0 = 4 * t;
+  const bool cin = c0 < ny;
+  const int cc = cin ? c0 : ny - 4;
+  const double beta = first ? 0.0 : st[row].beta;
+  const TV* R = r + row * vstride;
+  const TV* P = pold + row * vstride;
+  TV* PN = pnew + row * vstride;
+  auto pn = [&](double rv, double pv) -> double {
+    const double v = first ? rv : fma(beta, pv, rv);
+    return static_cast<double>(static_cast<TV>(v));
+  };
+  auto row4 = [&](const Q& qr, const Q& qp, bool in, double (&out)[4]) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[e] = in ? pn(q_at(qr, e), q_at(qp, e)) : 0.0;
+  };
+  Q qr[kQ4Dir], qp[kQ4Dir];
+#pragma unroll
+  for (int u = 0; u < kQ4Dir; ++u) {
+    const int i = clampi(i0 + 1 + u, 0, imax);
+    qr[u] = ld4(R, ny, i, cc);
+    qp[u] = ld4(P, ny, i, cc);
+  }
+  double prev[4], cur[4], next[4];
+  {
+    const int ip = clampi(i0 - 1, 0, nx - 1);
+    row4(ld4(R, ny, ip, cc), ld4(P, ny, ip, cc), cin && i0 >= 1, prev);
+    row4(ld4(R, ny, i0, cc), ld4(P, ny, i0, cc), cin, cur);
+  }
+  double acc = 0.0;
+  for (int ib = i0; ib < i1; ib += kQ4Dir) {
+#pragma unroll
+    for (int u = 0; u < kQ4Dir; ++u) {
+      const int i = ib + u;
+      const bool live = i < i1;
+      row4(qr[u], qp[u], cin && i + 1 <= imax, next);
+      const int il = clampi(i + 1 + kQ4Dir, 0, imax);
+      qr[u] = ld4(R, ny, il, cc);
+      qp[u] = ld4(P, ny, il, cc);
+      const int buf = u & 1;
+      eL[buf][t] = cur[0];
+      eR[buf][t] = cur[3];
+      lds_barrier();
+      if (live && cin) {
+        const double west = t > 0 ? eR[buf][t - 1] : 0.0;
+        const double east = t + 1 < nt ? eL[buf][t + 1] : 0.0;
+        double ap[4];
+        stencil4(k, prev, cur, next, west, east, ap);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc = fma(ap[e], cur[e], acc);
+        Q o;
+        o.x = static_cast<TV>(cur[0]);
+        o.y = static_cast<TV>(cur[1]);
+        o.z = static_cast<TV>(cur[2]);
+        o.w = static_cast<TV>(cur[3]);
+        *reinterpret_cast<Q*>(PN + static_cast<int64_t>(i) * ny + c0) = o;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        prev[e] = cur[e];
+        cur[e] = next[e];
+      }
+    }
+  }
+  double z0 = 0.0, z1 = 0.0;
+  block_reduce3_n(1, z0, acc, z1, red, nt);
+  if (t == 0) part[static_cast<int64_t>(row) * 3 * gridDim.x + blockIdx.x] = acc;
+}
+
+template <bool kStable, typename TY, typename TV, typename TB>
+__global__ __launch_bounds__(kMaxThreads4) void cgs_update4_kernel(int nx, int ny, St5 k, const TV* __restrict__ p,
+                                                                   const TY* y_in, TY* y_out, TV* __restrict__ r,
+                                                                   const TB* __restrict__ b, int64_t vstride,
+                                                                   CgP prm, const lxa_krylov_row* __restrict__ st,
+                                                                   double* __restrict__ part) {
+  using QV = typename Quad<TV>::type;
+  using QY = typename Quad<TY>::type;
+  using QB = typename Quad<TB>::type;
+  __shared__ double eL[2][kMaxThreads4 + 1], eR[2][kMaxThreads4 + 1];
+  __shared__ double red[48];
+  const int row = blockIdx.y;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int i0 = blockIdx.x * kTH4, i1 = i0 + kTH4 < nx ? i0 + kTH4 : nx;
+  const int c0 = 4 * t;
+  const bool cin = c0 < ny;
+  if (!st[row].active) {
+    // a finished row keeps its iterate across the buffer swap
+    if (kStable && cin)
+      for (int i = i0; i < i1; ++i) {
+        const int64_t o = row * vstride + static_cast<int64_t>(i) * ny + c0;
+        *reinterpret_cast<QY*>(y_out + o) = *reinterpret_cast<const QY*>(y_in + o);
+      }
+    return;
+  }
+  const int imax = i1 < nx - 1 ? i1 : nx - 1;
+  const int own = i1 - 1;
+  const int cc = cin ? c0 : ny - 4;
+  const double alpha = st[row].alpha;
+  const int kind = prm.norm_kind;
+  const TV* P = p + row * vstride;
+  const TY* Y = y_in + row * vstride;
+  TY* YO = y_out + row * vstride;
+  TV* Rv = r + row * vstride;
+  const TB* B = b + row * vstride;
+  // stencil operand: p (plain step) or y_new = y + alpha p rounded to storage
+  auto opnd = [&](const QV& qp, const QY& qy, bool in, double (&out)[4]) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double pv = q_at(qp, e);
+      out[e] = !in ? 0.0 : (kStable ? static_cast<double>(static_cast<TY>(q_at(qy, e) + alpha * pv)) : pv);
+    }
+  };
+  QV qp[kQ4], cr[kQ4];
+  QY qy[kQ4], cy[kQ4];
+  QB cb[kQ4];
+#pragma unroll
+  for (int u = 0; u < kQ4; ++u) {
+    const int i = clampi(i0 + 1 + u, 0, imax);
+    qp[u] = ld4(P, ny, i, cc);
+    if (kStable) qy[u] = ld4(Y, ny, i, cc);
+    const int ic = clampi(i0 + u, 0, own);
+    if (!kStable) {
+      cy[u] = ld4(Y, ny, ic, cc);
+      cr[u] = ld4(Rv, ny, ic, cc);
+    }
+    cb[u] = ld4(B, ny, ic, cc);
+  }
+  double prev[4], cur[4], next[4], pcur[4], ycur[4];
+  {
+    const int ip = clampi(i0 - 1, 0, nx - 1);
+    QY yz{};
+    opnd(ld4(P, ny, ip, cc), kStable ? ld4(Y, ny, ip, cc) : yz, cin && i0 >= 1, prev);
+    const QV p0 = ld4(P, ny, i0, cc);
+    const QY y0 = kStable ? ld4(Y, ny, i0, cc) : yz;
+    opnd(p0, y0, cin, cur);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      pcur[e] = q_at(p0, e);
+      ycur[e] = q_at(y0, e);
+    }
+  }
+  double n2 = 0.0, rr = 0.0, n1 = 0.0;
+  // lineax's loop condition is (norm1 > 1) | (norm2 > 1) (cg.py:150-158).
+  // Once a wave has seen norm2's contribution exceed 1 (max norm, or the
+  // two-norm's partial sum), the row cannot stop this step whatever norm1
+  // is, so the wave stops reading b for norm1 (plain steps; stabilising steps
+  // need b for the residual).  Should norm2 later turn NaN (NaN compares
+  // false), norm1 is unknown and is set to +inf: the row runs one more step.
+  const bool can_skip = !kStable && prm.has_scale && (kind == 0 || kind == 1);
+  bool skipped = false;
+  for (int ib = i0; ib < i1; ib += kQ4) {
+#pragma unroll
+    for (int u = 0; u < kQ4; ++u) {
+      const int i = ib + u;
+      const bool live = i < i1;
+      const bool nin = cin && i + 1 <= imax;
+      double pnext[4], ynext[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        pnext[e] = nin ? q_at(qp[u], e) : 0.0;
+        ynext[e] = (kStable && nin) ? q_at(qy[u], e) : 0.0;
+      }
+      opnd(qp[u], qy[u], nin, next);
+      double yold[4], rold[4], bv[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        yold[e] = kStable ? ycur[e] : q_at(cy[u], e);
+        rold[e] = kStable ? 0.0 : q_at(cr[u], e);
+        bv[e] = q_at(cb[u], e);
+      }
+      {
+        const int il = clampi(i + 1 + kQ4, 0, imax);
+        qp[u] = ld4(P, ny, il, cc);
+        if (kStable) qy[u] = ld4(Y, ny, il, cc);
+        const int ic = clampi(i + kQ4, 0, own);
+        if (!kStable) {
+          cy[u] = ld4(Y, ny, ic, cc);
+          cr[u] = ld4(Rv, ny, ic, cc);
+        }
+        if (!skipped) cb[u] = ld4(B, ny, ic, cc);
+      }
+      const int buf = u & 1;
+      eL[buf][t] = cur[0];
+      eR[buf][t] = cur[3];
+      lds_barrier();
+      if (live && cin) {
+        const double west = t > 0 ? eR[buf][t - 1] : 0.0;
+        const double east = t + 1 < nt ? eL[buf][t + 1] : 0.0;
+        double av[4];
+        stencil4(k, prev, cur, next, west, east, av);
+        QY yo;
+        QV ro;
+        double ys[4], rs[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double d = alpha * pcur[e];
+          const double yn = yold[e] + d;
+          ys[e] = yn;
+          if (prm.has_scale) n2 = norm_acc(kind, n2, norm_elem(kind, d / (prm.atol + prm.rtol * fabs(yn))));
+          const double rn = kStable ? bv[e] - av[e] : rold[e] - alpha * av[e];
+          rs[e] = static_cast<double>(static_cast<TV>(rn));
+          rr = fma(rs[e], rs[e], rr);
+          if (prm.has_scale && !skipped)
+            n1 = norm_acc(kind, n1, norm_elem(kind, rs[e] / (prm.atol + prm.rtol * fabs(bv[e]))));
+        }
+        yo.x = static_cast<TY>(ys[0]);
+        yo.y = static_cast<TY>(ys[1]);
+        yo.z = static_cast<TY>(ys[2]);
+        yo.w = static_cast<TY>(ys[3]);
+        ro.x = static_cast<TV>(rs[0]);
+        ro.y = static_cast<TV>(rs[1]);
+        ro.z = static_cast<TV>(rs[2]);
+        ro.w = static_cast<TV>(rs[3]);
+        const int64_t o = static_cast<int64_t>(i) * ny + c0;
+        *reinterpret_cast<QY*>(YO + o) = yo;
+        *reinterpret_cast<QV*>(Rv + o) = ro;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        prev[e] = cur[e];
+        cur[e] = next[e];
+        pcur[e] = pnext[e];
+        ycur[e] = ynext[e];
+      }
+      if (can_skip && !skipped && __any(n2 > 1.0)) skipped = true;
+    }
+  }
+  if (skipped && n2 != n2) n1 = INFINITY;
+  block_reduce3_n(kind, n2, rr, n1, red, nt);
+  if (t == 0) {
+    double* pr = part + static_cast<int64_t>(row) * 3 * gridDim.x;
+    pr[blockIdx.x] = n2;
+    pr[gridDim.x + blockIdx.x] = rr;
+    pr[2 * gridDim.x + blockIdx.x] = n1;
+  }
+}
+
+bool use_vec4(int64_t ny) { return ny % 4 == 0 && ny <= 4 * kMaxThreads4; }
+int threads4(int64_t ny) { return static_cast<int>(((ny / 4 + 63) / 64) * 64); }
+int tiles4(int64_t nx) { return static_cast<int>((nx + kTH4 - 1) / kTH4); }
+int tiles4_dir(int64_t nx) { return static_cast<int>((nx + kTH4Dir - 1) / kTH4Dir); }
+
 int tiles_of(int64_t nx, int64_t ny) {
   return static_cast<int>(((nx + kTH - 1) / kTH) * ((ny + kTW - 1) / kTW));
 }
@@ -450,9 +781,14 @@ int check_args(int64_t rows, int64_t nx, int64_t ny, const double* coeffs, const
 template <bool S, typename TY, typename TV, typename TB>
 void launch_update(hipStream_t s, dim3 grid, int nx, int ny, int tj, St5 k, const void* p, const void* yi, void* yo,
                    void* r, const void* b, int64_t vs, CgP c, const lxa_krylov_row* st, double* part) {
-  hipLaunchKernelGGL((cgs_update_kernel<S, TY, TV, TB>), grid, dim3(kTW), 0, s, nx, ny, tj, k,
-                     static_cast<const TV*>(p), static_cast<const TY*>(yi), static_cast<TY*>(yo), static_cast<TV*>(r),
-                     static_cast<const TB*>(b), vs, c, st, part);
+  if (use_vec4(ny))
+    hipLaunchKernelGGL((cgs_update4_kernel<S, TY, TV, TB>), grid, dim3(static_cast<unsigned>(threads4(ny))), 0, s, nx,
+                       ny, k, static_cast<const TV*>(p), static_cast<const TY*>(yi), static_cast<TY*>(yo),
+                       static_cast<TV*>(r), static_cast<const TB*>(b), vs, c, st, part);
+  else
+    hipLaunchKernelGGL((cgs_update_kernel<S, TY, TV, TB>), grid, dim3(kTW), 0, s, nx, ny, tj, k,
+                       static_cast<const TV*>(p), static_cast<const TY*>(yi), static_cast<TY*>(yo),
+                       static_cast<TV*>(r), static_cast<const TB*>(b), vs, c, st, part);
 }
 
 template <typename TY, typename TV, typename TB>
@@ -470,7 +806,10 @@ void launch_update2(int stable, hipStream_t s, dim3 grid, int nx, int ny, int tj
 
 extern "C" int lxa_cg_stencil_workspace(int64_t rows, int64_t nx, int64_t ny, size_t* partial_bytes) {
   LXA_REQUIRE(rows >= 1 && nx >= 1 && ny >= 1 && partial_bytes, "lxa_cg_stencil_workspace: bad argument");
-  *partial_bytes = static_cast<size_t>(rows) * 3 * static_cast<size_t>(lxa::tiles_of(nx, ny)) * sizeof(double);
+  int64_t tiles = lxa::tiles_of(nx, ny);
+  if (lxa::tiles4(nx) > tiles) tiles = lxa::tiles4(nx);
+  if (lxa::tiles4_dir(nx) > tiles) tiles = lxa::tiles4_dir(nx);
+  *partial_bytes = static_cast<size_t>(rows) * 3 * static_cast<size_t>(tiles) * sizeof(double);
   return LXA_OK;
 }
 
@@ -484,11 +823,24 @@ extern "C" int lxa_cg_stencil_direction(void* stream, int vdtype, int64_t rows, 
   hipStream_t s = static_cast<hipStream_t>(stream);
   const lxa::St5 k{coeffs[0], coeffs[1], coeffs[2], coeffs[3], coeffs[4]};
   const int tj = static_cast<int>((ny + lxa::kTW - 1) / lxa::kTW);
-  const int nblk = lxa::tiles_of(nx, ny);
+  const bool v4 = lxa::use_vec4(ny);
+  const int nblk = v4 ? lxa::tiles4_dir(nx) : lxa::tiles_of(nx, ny);
   const dim3 grid(static_cast<unsigned>(nblk), static_cast<unsigned>(rows));
   const int64_t vs = nx * ny;
   const lxa::CgP c = lxa::cgp_of(prm);
-  if (vdtype == LXA_F32)
+  if (v4) {
+    const dim3 blk(static_cast<unsigned>(lxa::threads4(ny)));
+    if (vdtype == LXA_F32)
+      hipLaunchKernelGGL(lxa::cgs_direction4_kernel<float>, grid, blk, 0, s, static_cast<int>(nx), static_cast<int>(ny),
+                         k, static_cast<const float*>(r), static_cast<const float*>(p_old), static_cast<float*>(p_new),
+                         first, vs, state, partials);
+    else if (vdtype == LXA_F64)
+      hipLaunchKernelGGL(lxa::cgs_direction4_kernel<double>, grid, blk, 0, s, static_cast<int>(nx),
+                         static_cast<int>(ny), k, static_cast<const double*>(r), static_cast<const double*>(p_old),
+                         static_cast<double*>(p_new), first, vs, state, partials);
+    else
+      return lxa::set_error(LXA_EINVAL, "lxa_cg_stencil_direction: bad dtype");
+  } else if (vdtype == LXA_F32)
     hipLaunchKernelGGL(lxa::cgs_direction_kernel<float>, grid, dim3(lxa::kTW), 0, s, static_cast<int>(nx),
                        static_cast<int>(ny), tj, k, static_cast<const float*>(r), static_cast<const float*>(p_old),
                        static_cast<float*>(p_new), first, vs, state, partials);
@@ -515,7 +867,7 @@ extern "C" int lxa_cg_stencil_update(void* stream, int ydtype, int vdtype, int b
   hipStream_t s = static_cast<hipStream_t>(stream);
   const lxa::St5 k{coeffs[0], coeffs[1], coeffs[2], coeffs[3], coeffs[4]};
   const int tj = static_cast<int>((ny + lxa::kTW - 1) / lxa::kTW);
-  const int nblk = lxa::tiles_of(nx, ny);
+  const int nblk = lxa::use_vec4(ny) ? lxa::tiles4(nx) : lxa::tiles_of(nx, ny);
   const dim3 grid(static_cast<unsigned>(nblk), static_cast<unsigned>(rows));
   const int64_t vs = nx * ny;
   const lxa::CgP c = lxa::cgp_of(prm);
