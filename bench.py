@@ -141,6 +141,7 @@ def main():
                     help="large_lu: Schur products")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: arrivals per second")
     ap.add_argument("--requests", type=int, default=400, help="serving: stream length")
+    ap.add_argument("--refine-steps", type=int, default=2, help="serving: FP64 refinement steps (chosen form)")
     ap.add_argument("--max-iters", type=int, default=None, help="krylov_cg: cap on CG iterations (default: lineax's)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy 5 times (scales rocprofv3 FETCH/WRITE_SIZE)")
@@ -580,6 +581,33 @@ def bench_krylov_gmres(args):
     return _krylov_leg(args, "gmres")
 
 
+def _serving_errors(reqs, ops):
+    """FP64 backward errors of every completed request, on the device.
+    Cholesky: eta = |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf) (Rigal-Gaches).
+    Least squares: eta_LS = |A^T (b - A x)|_inf / (|A|_1 (|A|_inf |x|_inf + |b|_inf)),
+    the normal-equations residual, which is ~ m u_f64 at the exact solution."""
+    import torch
+
+    errs = {"qr": [], "chol": []}
+    for r in reqs:
+        if r.done is None:
+            continue
+        a = (r.a if r.a is not None else ops[r.op_id]).double()
+        x, b = r.x.double(), r.b.double()
+        res = b - a @ x
+        den = a.abs().sum(1).max() * x.abs().max() + b.abs().max()
+        if r.kind == "qr":
+            errs["qr"].append((a.T @ res).abs().max() / (a.abs().sum(0).max() * den))
+        else:
+            errs["chol"].append(res.abs().max() / den)
+    out = {}
+    for k, v in errs.items():
+        if v:
+            t = torch.stack(v).cpu().numpy()
+            out[k] = {"max": float(t.max()), "p50": float(np.median(t)), "count": int(t.size)}
+    return out
+
+
 def bench_serving(args):
     """Poisson request stream (50% least-squares QR, 50% SPD Cholesky, 10% reuse)."""
     import torch
@@ -589,10 +617,14 @@ def bench_serving(args):
     out = {}
     for form in ("chosen", "plain"):
         reqs, ops = S.make_request_stream(args.rate, args.requests, seed=5, device="cuda:0")
-        server = S.SolveServer("cuda:0", form=form, hbm_budget=32 << 30, resident_budget=8 << 30)
+        server = S.SolveServer("cuda:0", form=form, hbm_budget=32 << 30, resident_budget=8 << 30,
+                               refine_steps=args.refine_steps)
         stats = server.run(reqs, ops, time_limit=300.0)
         flops = sum(S.request_flops(r) for r in reqs if r.done is not None)
         stats["achieved_tflops"] = flops / stats["makespan_s"] / 1e12
+        stats["backward_error"] = _serving_errors(reqs, ops)
+        if form == "chosen":
+            stats["refine_steps"] = args.refine_steps
         out[form] = stats
         del reqs, ops, server
         torch.cuda.empty_cache()
@@ -601,7 +633,7 @@ def bench_serving(args):
                  "solves/s", c["makespan_s"] * 1e3, args,
                  {"workload": "serving", "rate_per_s": args.rate, "requests": args.requests, "seed": 5,
                   "mix": "50% QR m~U{64..4096} n~U{m/4..m}, 50% Cholesky n~U{16..1024}, 10% reuse"},
-                 "f32",
+                 "f32 factors + f64 refinement",
                  {"bound": "mfma", "achieved": c["achieved_tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                   "frac": c["achieved_tflops"] / FP32_PEAK_TFLOPS, "traffic": None,
                   "kernel": "whole request loop (QR panel + trailing kernels dominate)"},

@@ -239,6 +239,19 @@ int lxa_cg_finish(void* stream, int vdtype, int64_t rows, int64_t n, const void*
                   const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
 int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state);
 
+/* Batched FP64 products with float32 operators of uneven shapes, for FP64
+ * refinement residuals: per system, trans = 0: y = b - A x (y = A x when b
+ * is 0), trans = 1: y = A^T x.  A float32 row-major with leading dimension
+ * lda; x, y float64; b float32.  Addresses are device pointers.  lineax
+ * does not refine, so this has no single counterpart there: it is the matmul
+ * of lineax/_operator.py:265-269 evaluated as an FP64 residual, for the
+ * request loop's refinement. */
+typedef struct {
+  int64_t a, x, b, y;
+  int32_t m, n, lda, pad;
+} lxa_gemv_desc; /* 48 bytes */
+int lxa_gemv_uneven(void* stream, int64_t count, const lxa_gemv_desc* descs, int trans, int max_m, int max_n);
+
 /* Fused CG for a constant-coefficient five-point stencil operator (the
  * matrix-free FunctionLinearOperator of lineax/_operator.py:664-732 whose mv
  * is lxa_stencil5_apply) with the identity preconditioner.  One iteration of

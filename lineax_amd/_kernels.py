@@ -56,6 +56,7 @@ _SIGNATURES = {
     "lxa_cg_finish": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "lxa_cg_result": [_vp, _i64, _vp, _vp],
     "lxa_cg_stencil_workspace": [_i64, _i64, _i64, _szp],
+    "lxa_gemv_uneven": [_vp, _i64, _vp, _int, _int, _int],
     "lxa_gmres_stencil_workspace": [_i64, _i64, _i64, _szp, _szp],
     "lxa_gmres_stencil_step": [_vp, _int, _i64, _i64, _i64, _vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp, _vp],
@@ -545,12 +546,36 @@ def qr_work_elems(dtype: torch.dtype, m: int, n: int) -> int:
     return int(out.value)
 
 
+# lxa_gemv_desc (48 bytes): device addresses of A (float32), x (float64),
+# b (float32, or 0) and y (float64), then m, n, lda.
+GEMV_DESC = np.dtype([("a", "<i8"), ("x", "<i8"), ("b", "<i8"), ("y", "<i8"), ("m", "<i4"), ("n", "<i4"),
+                      ("lda", "<i4"), ("pad", "<i4")])
+assert GEMV_DESC.itemsize == 48
+
+
+def gemv_uneven_raw(descs: torch.Tensor, count: int, trans: bool, max_m: int, max_n: int, stream_of: torch.Tensor):
+    """lxa_gemv_uneven on device descriptors (uint8 tensor of GEMV_DESC records)."""
+    _gpu(descs, "descs")
+    _check(load().lxa_gemv_uneven(_stream(stream_of), int(count), _ptr(descs), int(bool(trans)), int(max_m),
+                                  int(max_n)), "lxa_gemv_uneven")
+
+
 def descs_to_device(descs: np.ndarray, device) -> torch.Tensor:
     """Copies a QR_DESC array to device memory (uint8 tensor)."""
-    if descs.dtype != QR_DESC:
-        raise TypeError("descriptor array must have dtype QR_DESC")
+    if descs.dtype not in (QR_DESC, GEMV_DESC):
+        raise TypeError("descriptor array must have dtype QR_DESC or GEMV_DESC")
     raw = torch.from_numpy(np.ascontiguousarray(descs).view(np.uint8).reshape(-1).copy())
-    return raw.to(device)
+    return host_to_device(raw, device)
+
+
+def host_to_device(host: torch.Tensor, device) -> torch.Tensor:
+    """Copies a small host tensor to the device without waiting for the
+    current stream: the data goes through pinned memory (torch's caching host
+    allocator keeps the pinned block until the copy has run)."""
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        return host.to(dev)
+    return host.pin_memory().to(dev, non_blocking=True)
 
 
 def qr_factor_raw(descs: torch.Tensor, count: int, max_m: int, max_n: int, max_k: int, a: torch.Tensor,

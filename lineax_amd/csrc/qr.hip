@@ -443,6 +443,144 @@ __global__ __launch_bounds__(kBigThreads) void qr_apply_kernel(const lxa_qr_desc
 }
 
 // ---------------------------------------------------------------------------
+// The same product for systems of up to kApplyMaxRows rows, one vector per
+// workgroup: the vector lives in LDS and NB lanes share a row (lane j reads
+// column j of the panel, so a wave load covers 64 / NB contiguous row
+// segments instead of 64 scattered ones).  RPT > 0: each lane keeps its
+// RPT panel entries in registers between the reduction and the update (V is
+// read once per panel); RPT == 0 reads them twice.  Per panel:
+//   w = V^T x (lanes of a column, then waves, summed in a fixed order);
+//   w = T^T w or T w (wave 0, shuffles);  x -= V w (NB-lane row sums).
+// ---------------------------------------------------------------------------
+constexpr int kApplyMaxRows = 16384;
+
+template <typename T, int RPT>
+__global__ __launch_bounds__(kBigThreads) void qr_apply_rows_kernel(const lxa_qr_desc* __restrict__ descs,
+                                                                      const T* __restrict__ a,
+                                                                      const T* __restrict__ tblk,
+                                                                      T* __restrict__ b, int trans) {
+  constexpr int NB = QrCfg<T>::NB;
+  constexpr int G = kWave / NB;       // rows per wave load
+  constexpr int RP = kBigWaves * G;   // rows per pass of the workgroup
+  extern __shared__ __align__(16) unsigned char apply_smem[];
+  T* xs = reinterpret_cast<T*>(apply_smem);
+  __shared__ T s_part[kBigWaves][NB];
+  __shared__ T s_w[NB];
+  const lxa_qr_desc d = descs[blockIdx.x];
+  const int m = d.m, lda = d.lda;
+  const int kmax = min(m, d.n);
+  if (kmax <= 0) return;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  const int j = lane % NB, rg = wv * G + lane / NB;
+  const T* A = a + d.a_offset;
+  T* x = b + d.b_offset + (int64_t)blockIdx.y * d.ldb;
+  for (int i = tid; i < m; i += kBigThreads) xs[i] = x[i];
+  __syncthreads();
+  const int panels = (kmax + NB - 1) / NB;
+  T v[RPT > 0 ? RPT : 1];
+  for (int q = 0; q < panels; ++q) {
+    const int p = trans ? q : panels - 1 - q;
+    const int k = p * NB;
+    const int nb = min(NB, kmax - k);
+    const int jc = min(j, nb - 1);  // clamped column: loads stay inside the panel
+    const T* Ak = A + k + jc;
+    T acc = T(0);
+    if constexpr (RPT > 0) {
+#pragma unroll
+      for (int t = 0; t < RPT; ++t) {
+        const int row = k + rg + t * RP;
+        const T st = Ak[(int64_t)min(row, m - 1) * lda];
+        v[t] = (row < m && j < nb) ? vpick(row, k + j, st) : T(0);
+      }
+#pragma unroll
+      for (int t = 0; t < RPT; ++t) {
+        const int row = k + rg + t * RP;
+        if (row < m) acc += v[t] * xs[row];
+      }
+    } else {
+      for (int row = k + rg; row < m; row += RP) {
+        const T st = Ak[(int64_t)row * lda];
+        if (j < nb) acc += vpick(row, k + j, st) * xs[row];
+      }
+    }
+#pragma unroll
+    for (int off = NB; off < kWave; off <<= 1) acc += __shfl_xor(acc, off, kWave);
+    if (lane < NB) s_part[wv][lane] = acc;
+    __syncthreads();
+    if (wv == 0) {
+      T wsum = T(0);
+      if (lane < NB) {
+#pragma unroll
+        for (int w = 0; w < kBigWaves; ++w) wsum += s_part[w][lane];
+      }
+      const T* Tp = tblk + d.t_offset + (int64_t)p * NB * NB;
+      T s = T(0);
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const T wi = __shfl(wsum, i, kWave);
+        if (lane < NB && i < nb && lane < nb) {
+          if (trans) {  // (T^T w)_lane = sum_{i <= lane} T[i][lane] w_i
+            if (i <= lane) s += Tp[i * NB + lane] * wi;
+          } else {  // (T w)_lane = sum_{i >= lane} T[lane][i] w_i
+            if (i >= lane) s += Tp[lane * NB + i] * wi;
+          }
+        }
+      }
+      if (lane < NB) s_w[lane] = s;
+    }
+    __syncthreads();
+    const T wj = s_w[j];
+    if constexpr (RPT > 0) {
+#pragma unroll
+      for (int t = 0; t < RPT; ++t) {
+        const int row = k + rg + t * RP;
+        T s = v[t] * wj;
+#pragma unroll
+        for (int off = 1; off < NB; off <<= 1) s += __shfl_xor(s, off, kWave);
+        if (j == 0 && row < m) xs[row] -= s;
+      }
+    } else {
+      for (int row = k + rg; row < m; row += RP) {
+        const T st = Ak[(int64_t)row * lda];
+        T s = (j < nb) ? vpick(row, k + j, st) * wj : T(0);
+#pragma unroll
+        for (int off = 1; off < NB; off <<= 1) s += __shfl_xor(s, off, kWave);
+        if (j == 0) xs[row] -= s;
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < m; i += kBigThreads) x[i] = xs[i];
+}
+
+template <typename T, int RPT>
+int launch_apply_rows(hipStream_t st, dim3 grid, int max_m, const lxa_qr_desc* desc, const void* a, const void* tblk,
+                      void* b, int trans) {
+  const size_t smem = sizeof(T) * (size_t)max_m;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&qr_apply_rows_kernel<T, RPT>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(qr_apply_rows)");
+  qr_apply_rows_kernel<T, RPT><<<grid, kBigThreads, smem, st>>>(desc, static_cast<const T*>(a),
+                                                                 static_cast<const T*>(tblk), static_cast<T*>(b),
+                                                                 trans);
+  LXA_LAUNCH_CHECK("qr_apply_rows_kernel");
+  return LXA_OK;
+}
+
+template <typename T>
+int apply_rows(hipStream_t st, dim3 grid, int max_m, const lxa_qr_desc* desc, const void* a, const void* tblk,
+               void* b, int trans) {
+  constexpr int RP = kBigWaves * (kWave / QrCfg<T>::NB);
+  constexpr int kMaxRpt = sizeof(T) == 4 ? 32 : 16;  // no spills under the 128-VGPR cap of 1024 threads
+  const int rpt = (max_m + RP - 1) / RP;
+  if (rpt <= 4) return launch_apply_rows<T, 4>(st, grid, max_m, desc, a, tblk, b, trans);
+  if (rpt <= 8) return launch_apply_rows<T, 8>(st, grid, max_m, desc, a, tblk, b, trans);
+  if (rpt <= 16) return launch_apply_rows<T, 16>(st, grid, max_m, desc, a, tblk, b, trans);
+  if (rpt <= kMaxRpt) return launch_apply_rows<T, kMaxRpt>(st, grid, max_m, desc, a, tblk, b, trans);
+  return launch_apply_rows<T, 0>(st, grid, max_m, desc, a, tblk, b, trans);
+}
+
+// ---------------------------------------------------------------------------
 // Triangular solve with an upper triangular R, R[i][j] = a[i*rs + j*cs]:
 // back substitution for R x = y (tr = 0), forward for R^T x = y (tr = 1).
 // One workgroup per (system, right-hand side); x lives in LDS; 64-row
@@ -484,20 +622,34 @@ __global__ __launch_bounds__(kBigThreads) void trsv_kernel(const lxa_qr_desc* __
     }
     if (!tr) {
       // z_i = y_i - sum_{c >= r1} R[i][c] x_c: wave wv takes rows r0 + wv + 16 t
+      // (eight independent partial sums keep eight loads in flight per lane)
       for (int i = wv; i < bs; i += kBigWaves) {
-        T s = T(0);
+        T s8[8] = {};
         const T* Ri = A + (int64_t)(r0 + i) * rs;
-        for (int c = r1 + lane; c < kk; c += kWave) s += Ri[(int64_t)c * cs] * xs[c];
+        int c = r1 + lane;
+        for (; c + 7 * kWave < kk; c += 8 * kWave) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) s8[u] += Ri[(int64_t)(c + u * kWave) * cs] * xs[c + u * kWave];
+        }
+        for (; c < kk; c += kWave) s8[0] += Ri[(int64_t)c * cs] * xs[c];
+        T s = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
         s = wave_sum(s);
         if (lane == 0) red[i] = s;
       }
     } else {
       // z_i = y_i - sum_{c < r0} R[c][i] x_c: thread (column i, group g)
       const int i = lane, g = wv;
-      T s = T(0);
-      if (i < bs)
-        for (int c = g; c < r0; c += kBigWaves) s += A[(int64_t)c * rs + (int64_t)(r0 + i) * cs] * xs[c];
-      red[g * kWave + i] = s;
+      T s8[8] = {};
+      if (i < bs) {
+        const T* Ai = A + (int64_t)(r0 + i) * cs;
+        int c = g;
+        for (; c + 7 * kBigWaves < r0; c += 8 * kBigWaves) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) s8[u] += Ai[(int64_t)(c + u * kBigWaves) * rs] * xs[c + u * kBigWaves];
+        }
+        for (; c < r0; c += kBigWaves) s8[0] += Ai[(int64_t)c * rs] * xs[c];
+      }
+      red[g * kWave + i] = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
     }
     __syncthreads();
     if (wv == 0) {
@@ -640,6 +792,10 @@ extern "C" int lxa_qr_apply(void* stream, int dtype, int64_t count, const lxa_qr
   LXA_REQUIRE(a && tblk && b, "a, tblk and b must be non-NULL");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 grid((unsigned)count, (unsigned)nrhs);
+  if (max_m <= kApplyMaxRows) {
+    if (dtype == LXA_F32) return apply_rows<float>(st, grid, max_m, desc, a, tblk, b, trans ? 1 : 0);
+    return apply_rows<double>(st, grid, max_m, desc, a, tblk, b, trans ? 1 : 0);
+  }
   if (dtype == LXA_F32)
     qr_apply_kernel<float><<<grid, kBigThreads, 0, st>>>(desc, static_cast<const float*>(a),
                                                           static_cast<const float*>(tblk), static_cast<float*>(b),

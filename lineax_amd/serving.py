@@ -15,18 +15,26 @@ The chosen form (``SolveServer(form="chosen")``) runs in ticks:
    ``lxa_qr_apply`` + ``lxa_trsv`` launch pair);
 4. newly admitted Cholesky systems are factored and solved in one uneven
    launch pair (``lxa_chol_blocked_factor`` / ``_solve``);
-5. the tick ends with one synchronisation; requests completed in it get that
-   completion time.  Finished factorisations stay resident (up to
+5. the float32 solutions of the tick are refined in FP64 (``refine_steps``
+   steps, batched over the tick's requests): Cholesky requests by classical
+   refinement (y = b - A x in FP64, correction with the float32 factor),
+   least-squares requests by the corrected semi-normal equations
+   (x += R^-1 R^-T A^T (b - A x), products in FP64, triangular solves with
+   the float32 R), which contract the error by about kappa(A) u_f32 per step;
+6. completions are found by polling events; requests completed in a tick get
+   that completion time.  Finished factorisations stay resident (up to
    ``resident_budget`` bytes, least recently used evicted) for reuse
    requests.
 
-The plain form runs the same kernels one request at a time in arrival order
-(no batching, no reuse: a reuse request factors its operator again).
+The plain form runs the same float32 kernels one request at a time in arrival
+order (no batching, no reuse: a reuse request factors its operator again; no
+refinement, so its answers are float32).
 
 Semantically each request is ``lx.linear_solve(lx.MatrixLinearOperator(A),
 b, lx.QR())`` (lineax/_solver/qr.py:55-95; least squares for m >= n) or
 ``lx.linear_solve(lx.MatrixLinearOperator(A, positive_semidefinite_tag), b,
-lx.Cholesky())`` (lineax/_solver/cholesky.py:43-78), in float32.
+lx.Cholesky())`` (lineax/_solver/cholesky.py:43-78) on float32 data; the
+chosen form returns the refined solution in float64.
 """
 
 from __future__ import annotations
@@ -124,11 +132,16 @@ class _Resident:
 
 class SolveServer:
     def __init__(self, device, *, form: str = "chosen", hbm_budget: int = 8 << 30,
-                 resident_budget: int = 2 << 30):
+                 resident_budget: int = 2 << 30, refine_steps: int = 2):
         if form not in ("chosen", "plain"):
             raise ValueError("form must be 'chosen' or 'plain'")
+        if refine_steps < 0:
+            raise ValueError("refine_steps must be >= 0")
         self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         self.form = form
+        self.refine_steps = int(refine_steps) if form == "chosen" else 0
         self.arena = Arena(hbm_budget // 4, self.device)
         self.resident_budget = resident_budget // 4
         self.resident: dict[int, _Resident] = {}
@@ -208,6 +221,15 @@ class SolveServer:
         max_k = max(min(it[0], it[1]) for it in items)
         K.trsv_raw(dd, len(items), max_k, buf, 1, buf, lower=False, trans=False, unit_diag=False)
 
+    def _footprint(self, r: Request) -> dict:
+        """Arena elements of a new request: in the chosen form QR factors the
+        augmented [A | b] and keeps an n-vector for the solution."""
+        if self.form == "chosen" and r.kind == "qr":
+            fp = qr_footprint(r.m, r.n + 1, self.nb)
+            fp["b"] = r.n
+            return fp
+        return qr_footprint(r.m, r.n, self.nb) if r.kind == "qr" else chol_footprint(r.n)
+
     def _chol_batch(self, items, factor: bool):
         if not items:
             return
@@ -218,6 +240,86 @@ class SolveServer:
             K.chol_blocked_factor_raw(dd, len(items), buf, info)
         K.chol_blocked_solve_raw(dd, len(items), buf, buf)
 
+    def _solutions(self, batch):
+        """The answers of a tick's solves, batch = [(request, item, rhs blocks)]
+        with the float32 solution in the first n entries of each item's b
+        block.  Without refinement: float32 copies.  With refinement: FP64
+        solutions after ``refine_steps`` batched steps, all on the current
+        stream with no host synchronisation:
+          y = b - A x (FP64, every request);
+          Cholesky:       x += (U^T U)^-1 y;
+          least squares:  x += R^-1 R^-T (A^T y)  (corrected semi-normal
+                          equations with the float32 R; A^T y in FP64).
+        Each step contracts the error (in the norm |R e|) by about
+        2 kappa(A) u_f32 plus the triangular solves' rounding, so Gaussian
+        least-squares problems with kappa up to ~1e5 reach the FP64 level in
+        two to three steps without applying Q."""
+        buf = self.arena.buf
+        if self.refine_steps == 0:
+            return [buf[it[3]["b"][0]: it[3]["b"][0] + r.n].clone() for r, it, _ in batch]
+        dev = self.device
+        order = ([i for i, e in enumerate(batch) if e[0].kind == "qr"]
+                 + [i for i, e in enumerate(batch) if e[0].kind != "qr"])
+        ents = [batch[i] for i in order]
+        nq = sum(1 for e in ents if e[0].kind == "qr")
+        cnt = len(ents)
+        m = np.array([e[0].m for e in ents], dtype=np.int64)
+        n = np.array([e[0].n for e in ents], dtype=np.int64)
+        offx = np.concatenate(([0], np.cumsum(n)[:-1]))
+        offm = np.concatenate(([0], np.cumsum(m)[:-1]))
+        nt, mt = int(n.sum()), int(m.sum())
+        nq_el, mq_el = int(n[:nq].sum()), int(m[:nq].sum())
+        bo = np.array([e[1][3]["b"][0] for e in ents], dtype=np.int64)
+        # arena positions of the solutions: the first n entries of each b block
+        idx = K.host_to_device(torch.from_numpy(np.arange(nt) + np.repeat(bo - offx, n)), dev)
+        x = buf[idx].double()
+        y = torch.empty(mt, dtype=torch.float64, device=dev)
+        g = torch.empty(nq_el, dtype=torch.float64, device=dev)
+        # FP64 products: y = b - A x over all requests; g = A^T y over the QR ones
+        gd = np.zeros(cnt + nq, dtype=K.GEMV_DESC)
+        ops = [e[0].a if e[0].a is not None else self._operator_data[e[0].op_id] for e in ents]
+        aptr = np.array([a.data_ptr() for a in ops], dtype=np.int64)
+        gd["a"][:cnt] = aptr
+        gd["x"][:cnt] = x.data_ptr() + 8 * offx
+        gd["b"][:cnt] = [e[0].b.data_ptr() for e in ents]
+        gd["y"][:cnt] = y.data_ptr() + 8 * offm
+        gd["m"][:cnt], gd["n"][:cnt], gd["lda"][:cnt] = m, n, n
+        gd["a"][cnt:] = aptr[:nq]
+        gd["x"][cnt:] = y.data_ptr() + 8 * offm[:nq]
+        gd["y"][cnt:] = g.data_ptr() + 8 * offx[:nq]
+        gd["m"][cnt:], gd["n"][cnt:], gd["lda"][cnt:] = m[:nq], n[:nq], n[:nq]
+        # factor descriptors (QR first, then Cholesky), right-hand side in the b block
+        qd = np.zeros(cnt, dtype=K.QR_DESC)
+        for i, (r, (mi, ni, lda, blk), _) in enumerate(ents):
+            qd[i]["a_offset"] = blk["a"][0]
+            if "tau" in blk:
+                qd[i]["tau_offset"], qd[i]["t_offset"] = blk["tau"][0], blk["t"][0]
+            qd[i]["b_offset"] = blk["b"][0]
+            qd[i]["m"], qd[i]["n"], qd[i]["lda"], qd[i]["ldb"] = mi, ni, lda, mi
+        raw = K.host_to_device(torch.from_numpy(np.concatenate((gd.view(np.uint8), qd.view(np.uint8)))), dev)
+        g0, g1 = raw[: cnt * 48], raw[cnt * 48: (cnt + nq) * 48]
+        d_q = raw[(cnt + nq) * 48: (cnt + nq) * 48 + nq * 56]
+        d_c = raw[(cnt + nq) * 48 + nq * 56:]
+        max_m, max_n = int(m.max()), int(n.max())
+        max_mq = int(m[:nq].max()) if nq else 0
+        max_kq = int(n[:nq].max()) if nq else 0
+        for _ in range(self.refine_steps):
+            K.gemv_uneven_raw(g0, cnt, False, max_m, max_n, buf)
+            if nq:
+                K.gemv_uneven_raw(g1, nq, True, max_mq, max_kq, buf)
+            # right-hand sides of the corrections, in the solution slots
+            buf[idx] = torch.cat((g, y[mq_el:])).float() if nq else y.float()
+            if nq:
+                K.trsv_raw(d_q, nq, max_kq, buf, 1, buf, lower=False, trans=True, unit_diag=False)
+                K.trsv_raw(d_q, nq, max_kq, buf, 1, buf, lower=False, trans=False, unit_diag=False)
+            if cnt > nq:
+                K.chol_blocked_solve_raw(d_c, cnt - nq, buf, buf)
+            x += buf[idx].double()
+        xs = [None] * cnt
+        for j, i in enumerate(order):
+            xs[i] = x[int(offx[j]): int(offx[j]) + int(n[j])]
+        return xs
+
     # ------------------------------------------------------------------ run
     def run(self, requests: list[Request], operators: dict, *, time_limit: float = 120.0) -> dict:
         """Serves the stream in real time; returns latency and throughput figures.
@@ -227,9 +329,16 @@ class SolveServer:
         self._operator_data = operators
         reqs = sorted(requests, key=lambda r: r.arrival)
         for r in reqs:
+            a = r.a if r.a is not None else operators.get(r.op_id)
+            if a is None or a.shape != (r.m, r.n) or a.dtype != torch.float32 or not a.is_contiguous():
+                raise ValueError(f"request {r.rid}: the operator must be a contiguous float32 {r.m}x{r.n} tensor")
+            if r.b.shape != (r.m,) or r.b.dtype != torch.float32 or not r.b.is_contiguous():
+                raise ValueError(f"request {r.rid}: b must be a contiguous float32 vector of length {r.m}")
+            if a.device != self.device or r.b.device != self.device:
+                raise ValueError(f"request {r.rid}: operator and b must live on {self.device}")
             if r.kind == "qr" and r.m < r.n:
                 raise ValueError("the request loop serves least-squares QR requests (m >= n)")
-            fp = qr_footprint(r.m, r.n, self.nb) if r.kind == "qr" else chol_footprint(r.n)
+            fp = self._footprint(r)
             if sum(fp.values()) + len(fp) * Arena.ALIGN > self.arena.size:
                 raise MemoryError(f"request {r.rid} ({r.kind} {r.m}x{r.n}) exceeds the HBM arena")
         torch.cuda.synchronize(self.device)
@@ -358,19 +467,14 @@ class SolveServer:
                         res.last_use = self.ticks
                         res.busy += 1
                         merged = dict(res.blocks)
-                        merged["b"] = blocks["b"]
+                        merged.update(blocks)
                         solve_reuse.append((r, (r.m, r.n, res.lda, merged), blocks))
                         continue
                     if r.reuse and r.op_id in owner_pending:
                         still.append(r)
                         continue
                     a = r.a if r.a is not None else self._operator_data[r.op_id]
-                    if r.kind == "qr":
-                        fp = qr_footprint(r.m, r.n + 1, self.nb)
-                        fp["b"] = r.n
-                    else:
-                        fp = chol_footprint(r.n)
-                    blocks = self._alloc_blocks(fp)
+                    blocks = self._alloc_blocks(self._footprint(r))
                     if blocks is None:
                         blocked = True
                         still.append(r)
@@ -420,16 +524,17 @@ class SolveServer:
                         items.append((r.m, r.n, r.n + 1, blk))
                     self._trsv_batch(items)
                     for (r, blk, _), item in zip(qr_done, items):
-                        batch.append((r, self.arena.view(blk["b"][0], r.n).clone(), item, None))
+                        batch.append((r, item, None))
                 qr_reuse = [(r, it, rb) for r, it, rb in solve_reuse if r.kind == "qr"]
                 ch_reuse = [(r, it, rb) for r, it, rb in solve_reuse if r.kind != "qr"]
                 if qr_reuse:
                     self._qr_solve_batch([it for _, it, _ in qr_reuse])
                 self._chol_batch([it for _, it, _ in chol_new], factor=True)
                 self._chol_batch([it for _, it, _ in ch_reuse], factor=False)
-                for r, item, rb in qr_reuse + chol_new + ch_reuse:
-                    batch.append((r, self.arena.view(item[3]["b"][0], r.n).clone(), item, rb))
+                batch += qr_reuse + chol_new + ch_reuse
                 if batch:
+                    xs = self._solutions(batch)
+                    batch = [(r, x, item, rb) for (r, item, rb), x in zip(batch, xs)]
                     ev = torch.cuda.Event()
                     ev.record(sstream)
                     inflight.append((ev, batch))
@@ -452,9 +557,9 @@ class SolveServer:
         owner_pending.pop(r.op_id, None)
         # keep what a reuse solve needs: the factor (and tau, T blocks for QR)
         keep = {k: v for k, v in blk.items() if k not in ("b", "w")}
-        self.arena.release(*blk["b"])
-        if "w" in blk:
-            self.arena.release(*blk["w"])
+        for k in ("b", "w"):
+            if k in blk:
+                self.arena.release(*blk[k])
         elems = sum(e for _, e in keep.values())
         if elems <= self.resident_budget:
             self._evict_until(elems)

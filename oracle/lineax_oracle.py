@@ -210,6 +210,22 @@ def normwise_backward_error(a, x, b):
     return float(np.abs(r).max() / denom) if denom > 0 else float(np.abs(r).max())
 
 
+def lstsq_backward_error(a, x, b):
+    """eta_LS = |A^T (b - A x)|_inf / (|A|_1 (|A|_inf |x|_inf + |b|_inf)), in
+    float64: the normal-equations residual of a least-squares solution
+    (zero at the exact minimiser, about m u_f64 at its FP64 rounding), scaled
+    so that eta_LS <= 1 for any x; as |A^T r|_inf <= |A|_1 |r|_inf, it never
+    exceeds the normwise backward error of A x = b.
+    The accuracy measure for the least-squares half of the request loop, in
+    the spirit of LAPACK's xQRT17 ratio |r^T A| / (|A| |B| max(m, n) eps)."""
+    a = np.asarray(a, np.float64)
+    x = np.asarray(x, np.float64)
+    b = np.asarray(b, np.float64)
+    g = a.T @ (b - a @ x)
+    denom = np.abs(a).sum(axis=0).max() * (np.abs(a).sum(axis=-1).max() * np.abs(x).max() + np.abs(b).max())
+    return float(np.abs(g).max() / denom) if denom > 0 else float(np.abs(g).max())
+
+
 def relative_error(x, x_ref):
     x = np.asarray(x, np.float64)
     x_ref = np.asarray(x_ref, np.float64)

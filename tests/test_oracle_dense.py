@@ -197,3 +197,28 @@ def test_backward_error_metric():
     eta = O.normwise_backward_error(a, np.array([1.0, 1.5]), b)
     # r = [0, -2]; |A|inf = 4, |x|inf = 1.5, |b|inf = 4 -> 2 / 10
     assert abs(eta - 0.2) < 1e-15
+
+
+def test_lstsq_backward_error_metric():
+    # A = [[1, 0], [0, 1], [1, 1]], b = [1, 2, 0]: A^T A = [[2, 1], [1, 2]],
+    # A^T b = [1, 2] -> x* = [0, 1]; residual r = [1, 1, -1], A^T r = [0, 0]
+    a = np.array([[1.0, 0.0], [0.0, 1.0], [1.0, 1.0]])
+    b = np.array([1.0, 2.0, 0.0])
+    assert O.lstsq_backward_error(a, np.array([0.0, 1.0]), b) == 0.0
+    # x = [1, 1]: r = [0, 1, -2], A^T r = [-2, -1]; |A|_1 = 2, |A|_inf = 2,
+    # |x|_inf = 1, |b|_inf = 2 -> 2 / (2 * (2 + 2)) = 0.25
+    assert abs(O.lstsq_backward_error(a, np.array([1.0, 1.0]), b) - 0.25) < 1e-15
+    # the float64 lstsq minimiser of a random inconsistent problem sits at
+    # rounding level, a float32-rounded one well above it
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((200, 60))
+    b = rng.standard_normal(200)
+    x = np.linalg.lstsq(a, b, rcond=None)[0]
+    assert O.lstsq_backward_error(a, x, b) < 1e-14
+    assert O.lstsq_backward_error(a, x.astype(np.float32), b) > 1e-10
+    # a transposed-operand slip (A r in place of A^T r) cannot pass: shapes differ
+    # unless square; for square A the two differ on a non-symmetric example
+    sq = np.array([[1.0, 2.0], [0.0, 1.0]])
+    bb = np.array([1.0, 1.0])
+    xx = np.array([0.0, 0.0])  # r = b; A^T r = [1, 3], A r = [3, 1]
+    assert abs(O.lstsq_backward_error(sq, xx, bb) - 3.0 / (3.0 * 1.0)) < 1e-15
