@@ -498,9 +498,16 @@ __global__ __launch_bounds__(kBigThreads) void qr_apply_rows_kernel(const lxa_qr
         if (row < m) acc += v[t] * xs[row];
       }
     } else {
-      for (int row = k + rg; row < m; row += RP) {
-        const T st = Ak[(int64_t)row * lda];
-        if (j < nb) acc += vpick(row, k + j, st) * xs[row];
+      // eight rows per round, loads issued together (clamped rows, masked)
+      for (int row0 = k + rg; row0 < m; row0 += 8 * RP) {
+        T st[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) st[u] = Ak[(int64_t)min(row0 + u * RP, m - 1) * lda];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int row = row0 + u * RP;
+          if (row < m && j < nb) acc += vpick(row, k + j, st[u]) * xs[row];
+        }
       }
     }
 #pragma unroll
@@ -540,12 +547,18 @@ __global__ __launch_bounds__(kBigThreads) void qr_apply_rows_kernel(const lxa_qr
         if (j == 0 && row < m) xs[row] -= s;
       }
     } else {
-      for (int row = k + rg; row < m; row += RP) {
-        const T st = Ak[(int64_t)row * lda];
-        T s = (j < nb) ? vpick(row, k + j, st) * wj : T(0);
+      for (int row0 = k + rg; row0 < m; row0 += 8 * RP) {
+        T st[8];
 #pragma unroll
-        for (int off = 1; off < NB; off <<= 1) s += __shfl_xor(s, off, kWave);
-        if (j == 0) xs[row] -= s;
+        for (int u = 0; u < 8; ++u) st[u] = Ak[(int64_t)min(row0 + u * RP, m - 1) * lda];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int row = row0 + u * RP;
+          T s = (row < m && j < nb) ? vpick(row, k + j, st[u]) * wj : T(0);
+#pragma unroll
+          for (int off = 1; off < NB; off <<= 1) s += __shfl_xor(s, off, kWave);
+          if (j == 0 && row < m) xs[row] -= s;
+        }
       }
     }
     __syncthreads();
@@ -590,6 +603,17 @@ int apply_rows(hipStream_t st, dim3 grid, int max_m, const lxa_qr_desc* desc, co
 constexpr int kTrsvBlock = 64;
 constexpr int kTrsvMaxK = 8192;
 
+// v from lane j of the wave (j uniform): v_readlane into a scalar register.
+__device__ __forceinline__ float lane_bcast(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_bcast(double v, int j) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), j);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), j);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBigThreads) void trsv_kernel(const lxa_qr_desc* __restrict__ descs,
                                                              const T* __restrict__ a, T* __restrict__ b, int lower,
@@ -598,6 +622,7 @@ __global__ __launch_bounds__(kBigThreads) void trsv_kernel(const lxa_qr_desc* __
   T* xs = reinterpret_cast<T*>(trsv_smem);                      // [kTrsvMaxK] (only k used)
   T* tile = xs + ((kTrsvMaxK + 1) & ~1);                        // [64][65]
   T* red = tile + kTrsvBlock * (kTrsvBlock + 1);                // [16][64]
+  T* rinv = red + kBigWaves * kWave;                            // [64] reciprocal diagonal
   const lxa_qr_desc d = descs[blockIdx.x];
   const int kk = min(d.m, d.n);
   if (kk <= 0) return;
@@ -614,11 +639,16 @@ __global__ __launch_bounds__(kBigThreads) void trsv_kernel(const lxa_qr_desc* __
     const int r0 = bi * kTrsvBlock;
     const int r1 = min(kk, r0 + kTrsvBlock);
     const int bs = r1 - r0;
-    // diagonal block copy: tile[i][j] = R[r0 + i][r0 + j]
-    for (int idx = tid; idx < kTrsvBlock * kTrsvBlock; idx += kBigThreads) {
+    // diagonal block tile[i][j] = R[r0 + i][r0 + j]: loads issued here (clamped
+    // indices, masked values), stored to LDS after the product pass
+    constexpr int kTileVals = kTrsvBlock * kTrsvBlock / kBigThreads;
+    T tv[kTileVals];
+#pragma unroll
+    for (int u = 0; u < kTileVals; ++u) {
+      const int idx = tid + u * kBigThreads;
       const int i = idx >> 6, j = idx & 63;
-      tile[i * (kTrsvBlock + 1) + j] =
-          (i < bs && j < bs && j >= i) ? A[(int64_t)(r0 + i) * rs + (int64_t)(r0 + j) * cs] : T(0);
+      const T v = A[(int64_t)(r0 + min(i, bs - 1)) * rs + (int64_t)(r0 + min(j, bs - 1)) * cs];
+      tv[u] = (i < bs && j < bs && j >= i) ? v : T(0);
     }
     if (!tr) {
       // z_i = y_i - sum_{c >= r1} R[i][c] x_c: wave wv takes rows r0 + wv + 16 t
@@ -651,6 +681,13 @@ __global__ __launch_bounds__(kBigThreads) void trsv_kernel(const lxa_qr_desc* __
       }
       red[g * kWave + i] = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
     }
+#pragma unroll
+    for (int u = 0; u < kTileVals; ++u) {
+      const int idx = tid + u * kBigThreads;
+      const int i = idx >> 6, j = idx & 63;
+      tile[i * (kTrsvBlock + 1) + j] = tv[u];
+      if (i == j) rinv[i] = (unit_diag || i >= bs) ? T(1) : T(1) / tv[u];
+    }
     __syncthreads();
     if (wv == 0) {
       T z = T(0);
@@ -664,17 +701,27 @@ __global__ __launch_bounds__(kBigThreads) void trsv_kernel(const lxa_qr_desc* __
           z = xs[r0 + lane] - s;
         }
       }
+      // Substitution through the block, one unknown per step: lane j scales
+      // by the reciprocal pivot and readlane broadcasts it (a register read,
+      // not an LDS round trip).  Fully unrolled, so the LDS reads of the
+      // multipliers do not wait on the chain.  Steps j >= bs are no-ops
+      // (zero padding, x_j = 0).
+      const T rl = rinv[lane];
       if (!tr) {
-        for (int j = bs - 1; j >= 0; --j) {
-          if (lane == j && !unit_diag) z = z / tile[j * (kTrsvBlock + 1) + j];
-          const T xj = __shfl(z, j, kWave);
-          if (lane < j) z -= tile[lane * (kTrsvBlock + 1) + j] * xj;
+#pragma unroll
+        for (int j = kTrsvBlock - 1; j >= 0; --j) {
+          const T mult = tile[lane * (kTrsvBlock + 1) + j];
+          if (lane == j) z *= rl;
+          const T xj = j < bs ? lane_bcast(z, j) : T(0);
+          if (lane < j) z -= mult * xj;
         }
       } else {
-        for (int j = 0; j < bs; ++j) {
-          if (lane == j && !unit_diag) z = z / tile[j * (kTrsvBlock + 1) + j];
-          const T xj = __shfl(z, j, kWave);
-          if (lane > j && lane < bs) z -= tile[j * (kTrsvBlock + 1) + lane] * xj;
+#pragma unroll
+        for (int j = 0; j < kTrsvBlock; ++j) {
+          const T mult = tile[j * (kTrsvBlock + 1) + lane];
+          if (lane == j) z *= rl;
+          const T xj = j < bs ? lane_bcast(z, j) : T(0);
+          if (lane > j) z -= mult * xj;
         }
       }
       if (lane < bs) xs[r0 + lane] = z;
@@ -686,7 +733,7 @@ __global__ __launch_bounds__(kBigThreads) void trsv_kernel(const lxa_qr_desc* __
 
 size_t trsv_smem_bytes(int dtype) {
   const size_t e = dtype == LXA_F64 ? 8 : 4;
-  return e * (((kTrsvMaxK + 1) & ~1) + kTrsvBlock * (kTrsvBlock + 1) + kBigWaves * kWave);
+  return e * (((kTrsvMaxK + 1) & ~1) + kTrsvBlock * (kTrsvBlock + 1) + kBigWaves * kWave + kTrsvBlock);
 }
 
 // One panel step for every system: column k_all for all (kcol == nullptr)

@@ -49,6 +49,7 @@ import torch
 from . import _kernels as K
 
 QR_ELEM = 4  # float32 bytes
+SOLVE_STREAMS = 2  # with the factor stream and the default stream: the 4 hardware queues of a process
 
 
 @dataclasses.dataclass
@@ -400,20 +401,23 @@ class SolveServer:
             self.ticks += 1
 
     def _run_chosen(self, reqs, t0, time_limit):
-        """Continuous batching on two streams.
+        """Continuous batching on a factor stream and SOLVE_STREAMS solve streams.
 
         Factor stream: one batched panel step per tick for every QR system being
         factored.  A new QR request carries its right-hand side as an extra
         column of A (leading dimension n + 1), so the panel updates leave Q^T b
-        in that column and only the triangular solve remains.  Solve stream:
-        triangular solves of finished QR systems (after an event on the factor
-        stream), Cholesky factor + solve of new requests, and reuse solves.
+        in that column and only the triangular solve remains.  Solve streams
+        (one per tick, in turn): triangular solves of finished QR systems
+        (after an event on the factor stream), Cholesky factor + solve of new
+        requests, reuse solves, and the refinement of all of them.  A tick's
+        solve chain is latency-bound (one workgroup per system), so chains of
+        consecutive ticks run side by side on different compute units.
         Completions are found by polling events; arena blocks are returned only
         after the work that uses them has finished.  At most `depth` ticks are
         in flight on the factor stream."""
         dev = self.device
         fstream = torch.cuda.Stream(device=dev)
-        sstream = torch.cuda.Stream(device=dev)
+        sstreams = [torch.cuda.Stream(device=dev) for _ in range(SOLVE_STREAMS)]
         depth = 2
         pending = list(reqs)
         waiting: list[Request] = []
@@ -425,14 +429,18 @@ class SolveServer:
         buf = self.arena.buf
 
         def finish_batches(block: bool) -> None:
-            while inflight and (block or inflight[0][0].query()):
-                ev, batch = inflight.pop(0)
-                ev.synchronize()
-                t_done = self._now(t0)
-                for r, x, item, rhs_blocks in batch:
+            # chains on different solve streams may finish out of order
+            if block and inflight:
+                inflight[0][0].synchronize()
+            done = [e for e in inflight if e[0].query()]
+            if not done:
+                return
+            t_done = self._now(t0)
+            for e in done:
+                inflight.remove(e)
+                for r, x, item, rhs_blocks in e[1]:
                     r.x, r.done = x, t_done
                     self._retire(r, item, rhs_blocks, owner_pending)
-                block = False
 
         while True:
             now = self._now(t0)
@@ -450,6 +458,7 @@ class SolveServer:
                 continue
             qr_done, chol_new, solve_reuse = [], [], []
             still, blocked = [], False
+            sstream = sstreams[self.ticks % len(sstreams)]
             with torch.cuda.stream(fstream):
                 for r in waiting:
                     if blocked:
