@@ -239,6 +239,33 @@ int lxa_cg_finish(void* stream, int vdtype, int64_t rows, int64_t n, const void*
                   const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
 int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state);
 
+/* BiCGStab with right preconditioning (lineax/_solver/bicgstab.py:78-205;
+ * its body 143-167, cond_fun 121-140, RESULTS 185-197).  Sequence:
+ * [caller: t = A y0] begin (r = r0 = b - t, the initial carry alpha = omega
+ * = rho = 1); per iteration: direction (p = r + beta (p - omega v); first = 1
+ * on the first iteration, where p = r), [caller: x = M p, v = A x], alpha,
+ * s (s = r - alpha v), [caller: z = M s, t = A z], omega, update (y += alpha
+ * x + omega z, r = s - omega t, step, tolerance norms and cond_fun); finally
+ * result.  Row state: gamma = rho, alpha, ip = omega, beta = <r0, r> of the
+ * current residual.  prm->eps selects the breakdown test: 0 tests for exact
+ * zeros of omega, alpha, rho (lineax with jax_enable_x64), > 0 for values
+ * below it (lineax's 1e-16 otherwise).  Partials as lxa_krylov_partials_size
+ * with restart 0. */
+int lxa_bicgstab_begin(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* b, const void* t,
+                       void* r, void* r0, const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
+int lxa_bicgstab_direction(void* stream, int vdtype, int64_t rows, int64_t n, const void* r, void* p, const void* v,
+                           int first, lxa_krylov_row* state);
+int lxa_bicgstab_alpha(void* stream, int vdtype, int64_t rows, int64_t n, const void* r0, const void* v,
+                       const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
+int lxa_bicgstab_s(void* stream, int vdtype, int64_t rows, int64_t n, const void* r, const void* v, void* s,
+                   lxa_krylov_row* state);
+int lxa_bicgstab_omega(void* stream, int vdtype, int64_t rows, int64_t n, const void* s, const void* t,
+                       const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
+int lxa_bicgstab_update(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* x, const void* z,
+                        const void* s, const void* t, const void* r0, void* y, void* r, const void* b,
+                        const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
+int lxa_bicgstab_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state);
+
 /* Batched FP64 products with float32 operators of uneven shapes, for FP64
  * refinement residuals: per system, trans = 0: y = b - A x (y = A x when b
  * is 0), trans = 1: y = A^T x.  A float32 row-major with leading dimension

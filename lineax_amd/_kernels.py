@@ -55,6 +55,13 @@ _SIGNATURES = {
     "lxa_cg_precond_gamma": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "lxa_cg_finish": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "lxa_cg_result": [_vp, _i64, _vp, _vp],
+    "lxa_bicgstab_begin": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lxa_bicgstab_direction": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _int, _vp],
+    "lxa_bicgstab_alpha": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "lxa_bicgstab_s": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp],
+    "lxa_bicgstab_omega": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "lxa_bicgstab_update": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lxa_bicgstab_result": [_vp, _i64, _vp, _vp],
     "lxa_cg_stencil_workspace": [_i64, _i64, _i64, _szp],
     "lxa_gemv_uneven": [_vp, _i64, _vp, _int, _int, _int],
     "lxa_gmres_stencil_workspace": [_i64, _i64, _i64, _szp, _szp],

@@ -1,11 +1,13 @@
+from .bicgstab import BiCGStab
 from .cg import CG
 from .cholesky import Cholesky
 from .diagonal import Diagonal
 from .gmres import GMRES
 from .lu import LU
+from .normal import Normal
 from .qr import QR
 from .triangular import Triangular
-from .unsupported import LSMR, SVD, BiCGStab, Normal, Tridiagonal
+from .unsupported import LSMR, SVD, Tridiagonal
 
 __all__ = [
     "BiCGStab", "CG", "Cholesky", "Diagonal", "GMRES", "LSMR", "LU", "Normal", "QR", "SVD", "Triangular",

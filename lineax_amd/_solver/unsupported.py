@@ -51,14 +51,3 @@ class Tridiagonal(_Unsupported):
 
 class LSMR(_Unsupported):
     _reference = "lineax/_solver/lsmr.py"
-
-
-class BiCGStab(_Unsupported):
-    _reference = "lineax/_solver/bicgstab.py"
-
-    def assume_full_rank(self):
-        return True
-
-
-class Normal(_Unsupported):
-    _reference = "lineax/_solver/normal.py"

@@ -314,6 +314,223 @@ __global__ void cg_result_kernel(int64_t rows, CgParams prm, lxa_krylov_row* __r
 }
 
 // ---------------------------------------------------------------------------
+// BiCGStab (lineax/_solver/bicgstab.py:78-205), right preconditioning.
+// Row state: gamma = rho, alpha = alpha, ip = omega (lineax's carry),
+// beta = <r0, r> of the current residual (the next iteration's rho_new),
+// norm_r / norm_diff the two tolerance norms.  Iteration k:
+//   direction: p = r + beta_k (p - omega v), beta_k = (rho_new / rho)(alpha / omega)
+//   [caller: x = M p, v = A x]
+//   alpha:     alpha = rho_new / <r0, v>, rho = rho_new
+//   s:         s = r - alpha v  [caller: z = M s, t = A z]
+//   omega:     omega = <s, t> / <t, t>
+//   update:    diff = alpha x + omega z, y += diff, r = s - omega t, the
+//              next rho_new = <r0, r>, the norms, step += 1 and cond_fun
+//              (bicgstab.py:113-140) on the carry (omega, alpha, rho).
+// ---------------------------------------------------------------------------
+
+// Breakdown test (bicgstab.py:113-119): exact zeros when threshold == 0 (a
+// float64 solve, as with jax_enable_x64), else value < threshold.
+__device__ __forceinline__ bool bcg_breakdown(double omega, double alpha, double rho, double threshold) {
+  if (threshold == 0.0) return omega == 0.0 || alpha == 0.0 || rho == 0.0;
+  return omega < threshold || alpha < threshold || rho < threshold;
+}
+
+// r = r0 = b - t; partials [row][nblk] of <r0, r0> (the stored values).
+template <typename TY, typename TV>
+__global__ __launch_bounds__(kB) void bcg_begin_kernel(int64_t n, const TY* __restrict__ b, const TY* __restrict__ t,
+                                                       TV* __restrict__ r, TV* __restrict__ r0,
+                                                       double* __restrict__ part) {
+  __shared__ double red[4];
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const int64_t o = row * n;
+  double acc = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const TV rv = static_cast<TV>(static_cast<double>(b[o + i]) - static_cast<double>(t[o + i]));
+    r[o + i] = rv;
+    r0[o + i] = rv;
+    acc = fma(static_cast<double>(rv), static_cast<double>(rv), acc);
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = acc;
+}
+
+// The initial carry (bicgstab.py:169-180): alpha = omega = rho = 1, diff =
+// inf, so the first cond_fun is step < max_steps.
+__global__ void bcg_begin_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part, CgParams prm,
+                                        lxa_krylov_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  lxa_krylov_row s{};
+  s.gamma = 1.0;
+  s.alpha = 1.0;
+  s.ip = 1.0;
+  s.beta = sum_partials(part + row * nblk, nblk);
+  s.norm_r = INFINITY;
+  s.norm_diff = INFINITY;
+  s.step = 0;
+  s.active = prm.max_steps > 0;
+  st[row] = s;
+}
+
+// p = r (first) or p = r + beta_k (p - omega v).
+template <typename TV>
+__global__ __launch_bounds__(kB) void bcg_direction_kernel(int64_t n, const TV* __restrict__ r, TV* __restrict__ p,
+                                                           const TV* __restrict__ v, int first,
+                                                           const lxa_krylov_row* __restrict__ st) {
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const lxa_krylov_row s = st[row];
+  const double beta = (s.beta / s.gamma) * (s.alpha / s.ip);
+  const double omega = s.ip;
+  const int64_t o = row * n;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double rv = static_cast<double>(r[o + i]);
+    p[o + i] = first ? static_cast<TV>(rv)
+                     : static_cast<TV>(rv + beta * (static_cast<double>(p[o + i]) -
+                                                    omega * static_cast<double>(v[o + i])));
+  }
+}
+
+// alpha = rho_new / <r0, v>; the carry's rho becomes rho_new.
+__global__ void bcg_alpha_kernel(int64_t rows, int nblk, const double* __restrict__ part,
+                                 lxa_krylov_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  const double rho_new = st[row].beta;
+  st[row].alpha = rho_new / sum_partials(part + row * nblk, nblk);
+  st[row].gamma = rho_new;
+}
+
+// s = r - alpha v.
+template <typename TV>
+__global__ __launch_bounds__(kB) void bcg_s_kernel(int64_t n, const TV* __restrict__ r, const TV* __restrict__ v,
+                                                   TV* __restrict__ s, const lxa_krylov_row* __restrict__ st) {
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const double alpha = st[row].alpha;
+  const int64_t o = row * n;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB)
+    s[o + i] = static_cast<TV>(static_cast<double>(r[o + i]) - alpha * static_cast<double>(v[o + i]));
+}
+
+// Partials [row][2][nblk]: <s, t> and <t, t> in one pass.
+template <typename TV>
+__global__ __launch_bounds__(kB) void bcg_st_kernel(int64_t n, const TV* __restrict__ s, const TV* __restrict__ t,
+                                                    const lxa_krylov_row* __restrict__ st,
+                                                    double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const int64_t o = row * n;
+  double a = 0.0, q = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double tv = static_cast<double>(t[o + i]);
+    a = fma(static_cast<double>(s[o + i]), tv, a);
+    q = fma(tv, tv, q);
+  }
+  a = block_sum(a, red);
+  q = block_sum(q, red);
+  if (threadIdx.x == 0) {
+    part[(row * 2) * gridDim.x + blockIdx.x] = a;
+    part[(row * 2 + 1) * gridDim.x + blockIdx.x] = q;
+  }
+}
+
+__global__ void bcg_omega_kernel(int64_t rows, int nblk, const double* __restrict__ part,
+                                 lxa_krylov_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  const double* pr = part + row * 2 * nblk;
+  st[row].ip = sum_partials(pr, nblk) / sum_partials(pr + nblk, nblk);
+}
+
+// diff = alpha x + omega z; y += diff; r = s - omega t.  Partials
+// [row][3][nblk]: [0] <r0, r>, [1] |r / b_scale|, [2] |diff / y_scale|.
+template <typename TY, typename TV>
+__global__ __launch_bounds__(kB) void bcg_update_kernel(int64_t n, const TV* __restrict__ x, const TV* __restrict__ z,
+                                                        const TV* __restrict__ s, const TV* __restrict__ t,
+                                                        const TV* __restrict__ r0, TY* __restrict__ y,
+                                                        TV* __restrict__ r, const TY* __restrict__ b, CgParams prm,
+                                                        const lxa_krylov_row* __restrict__ st,
+                                                        double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!row_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const double alpha = st[row].alpha, omega = st[row].ip;
+  const int kind = prm.norm_kind;
+  const int64_t o = row * n;
+  double rr = 0.0, n1 = 0.0, n2 = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double d = alpha * static_cast<double>(x[o + i]) + omega * static_cast<double>(z[o + i]);
+    const double yn = static_cast<double>(y[o + i]) + d;
+    y[o + i] = static_cast<TY>(yn);
+    const TV rs = static_cast<TV>(static_cast<double>(s[o + i]) - omega * static_cast<double>(t[o + i]));
+    r[o + i] = rs;
+    const double rv = static_cast<double>(rs);
+    rr = fma(static_cast<double>(r0[o + i]), rv, rr);
+    if (prm.has_scale) {
+      n1 = norm_accum(kind, n1, norm_elem(kind, rv / (prm.atol + prm.rtol * fabs(static_cast<double>(b[o + i])))));
+      n2 = norm_accum(kind, n2, norm_elem(kind, d / (prm.atol + prm.rtol * fabs(static_cast<double>(y[o + i])))));
+    }
+  }
+  rr = block_sum(rr, red);
+  n1 = norm_block(kind, n1, red);
+  n2 = norm_block(kind, n2, red);
+  if (threadIdx.x == 0) {
+    const int64_t base = (row * 3) * gridDim.x + blockIdx.x;
+    part[base] = rr;
+    part[base + gridDim.x] = n1;
+    part[base + 2 * gridDim.x] = n2;
+  }
+}
+
+// step += 1, the next rho_new, the norms and cond_fun on the carry.
+__global__ void bcg_update_finish_kernel(int64_t rows, int nblk, int64_t n, const double* __restrict__ part,
+                                         CgParams prm, double threshold, lxa_krylov_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  const double* pr = part + row * 3 * nblk;
+  const int kind = prm.norm_kind;
+  double rr = 0.0, n1 = 0.0, n2 = 0.0;
+  for (int k = 0; k < nblk; ++k) {
+    rr += pr[k];
+    n1 = norm_accum(kind, n1, pr[nblk + k]);
+    n2 = norm_accum(kind, n2, pr[2 * nblk + k]);
+  }
+  lxa_krylov_row s = st[row];
+  s.beta = rr;
+  s.norm_r = norm_final(kind, n1, n);
+  s.norm_diff = norm_final(kind, n2, n);
+  s.step += 1;
+  const bool not_converged = prm.has_scale ? (s.norm_r > 1.0 || s.norm_diff > 1.0) : true;
+  s.active = !bcg_breakdown(s.ip, s.alpha, s.gamma, threshold) && not_converged && s.step < prm.max_steps;
+  st[row] = s;
+}
+
+// RESULTS (bicgstab.py:185-197): the step cap as for CG, then breakdown
+// when the last carry broke down without converging.
+__global__ void bcg_result_kernel(int64_t rows, CgParams prm, double threshold, lxa_krylov_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  const lxa_krylov_row s = st[row];
+  int res = LXA_RESULT_SUCCESSFUL;
+  const bool capped = s.step == prm.max_steps;
+  if (!prm.max_steps_given) {
+    if (capped) res = LXA_RESULT_SINGULAR;
+  } else if (prm.has_scale) {
+    if (capped) res = LXA_RESULT_MAX_STEPS_REACHED;
+  }
+  const bool not_converged = prm.has_scale ? (s.norm_r > 1.0 || s.norm_diff > 1.0) : true;
+  if (bcg_breakdown(s.ip, s.alpha, s.gamma, threshold) && not_converged) res = LXA_RESULT_BREAKDOWN;
+  st[row].result = res;
+}
+
+// ---------------------------------------------------------------------------
 // GMRES (lineax/_solver/gmres.py:106-413)
 // ---------------------------------------------------------------------------
 
@@ -957,6 +1174,126 @@ extern "C" int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(lxa::cg_result_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, cg_params(prm), state);
   LXA_LAUNCH_CHECK("lxa_cg_result");
+  return LXA_OK;
+}
+
+// ----------------------------------------------------------------------------- BiCGStab
+
+extern "C" int lxa_bicgstab_begin(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* b,
+                                  const void* t, void* r, void* r0, const lxa_krylov_params* prm,
+                                  lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(b && t && r && r0, "lxa_bicgstab_begin: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  LXA_DISPATCH_YV(ydtype, vdtype,
+                  hipLaunchKernelGGL((lxa::bcg_begin_kernel<TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                                     static_cast<const TY*>(b), static_cast<const TY*>(t), static_cast<TV*>(r),
+                                     static_cast<TV*>(r0), partials));
+  hipLaunchKernelGGL(lxa::bcg_begin_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials,
+                     cg_params(prm), state);
+  LXA_LAUNCH_CHECK("lxa_bicgstab_begin");
+  return LXA_OK;
+}
+
+extern "C" int lxa_bicgstab_direction(void* stream, int vdtype, int64_t rows, int64_t n, const void* r, void* p,
+                                      const void* v, int first, lxa_krylov_row* state) {
+  LXA_REQUIRE(rows >= 1 && rows <= 65535 && n >= 1 && state && r && p && v, "lxa_bicgstab_direction: bad argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  if (vdtype == LXA_F64)
+    hipLaunchKernelGGL((lxa::bcg_direction_kernel<double>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const double*>(r), static_cast<double*>(p), static_cast<const double*>(v), first,
+                       state);
+  else if (vdtype == LXA_F32)
+    hipLaunchKernelGGL((lxa::bcg_direction_kernel<float>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const float*>(r), static_cast<float*>(p), static_cast<const float*>(v), first,
+                       state);
+  else
+    return lxa::set_error(LXA_EINVAL, "lxa_bicgstab_direction: bad dtype");
+  LXA_LAUNCH_CHECK("lxa_bicgstab_direction");
+  return LXA_OK;
+}
+
+extern "C" int lxa_bicgstab_alpha(void* stream, int vdtype, int64_t rows, int64_t n, const void* r0, const void* v,
+                                  const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(r0 && v, "lxa_bicgstab_alpha: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  if (vdtype == LXA_F64) launch_dot<double>(s, rows, n, nb, r0, v, state, 1, partials, nb, 0);
+  else if (vdtype == LXA_F32) launch_dot<float>(s, rows, n, nb, r0, v, state, 1, partials, nb, 0);
+  else return lxa::set_error(LXA_EINVAL, "lxa_bicgstab_alpha: bad dtype");
+  hipLaunchKernelGGL(lxa::bcg_alpha_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials, state);
+  LXA_LAUNCH_CHECK("lxa_bicgstab_alpha");
+  return LXA_OK;
+}
+
+extern "C" int lxa_bicgstab_s(void* stream, int vdtype, int64_t rows, int64_t n, const void* r, const void* v,
+                              void* s_out, lxa_krylov_row* state) {
+  LXA_REQUIRE(rows >= 1 && rows <= 65535 && n >= 1 && state && r && v && s_out, "lxa_bicgstab_s: bad argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  if (vdtype == LXA_F64)
+    hipLaunchKernelGGL((lxa::bcg_s_kernel<double>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const double*>(r), static_cast<const double*>(v), static_cast<double*>(s_out),
+                       state);
+  else if (vdtype == LXA_F32)
+    hipLaunchKernelGGL((lxa::bcg_s_kernel<float>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const float*>(r), static_cast<const float*>(v), static_cast<float*>(s_out), state);
+  else
+    return lxa::set_error(LXA_EINVAL, "lxa_bicgstab_s: bad dtype");
+  LXA_LAUNCH_CHECK("lxa_bicgstab_s");
+  return LXA_OK;
+}
+
+extern "C" int lxa_bicgstab_omega(void* stream, int vdtype, int64_t rows, int64_t n, const void* s_in, const void* t,
+                                  const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(s_in && t, "lxa_bicgstab_omega: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  if (vdtype == LXA_F64)
+    hipLaunchKernelGGL((lxa::bcg_st_kernel<double>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const double*>(s_in), static_cast<const double*>(t), state, partials);
+  else if (vdtype == LXA_F32)
+    hipLaunchKernelGGL((lxa::bcg_st_kernel<float>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const float*>(s_in), static_cast<const float*>(t), state, partials);
+  else
+    return lxa::set_error(LXA_EINVAL, "lxa_bicgstab_omega: bad dtype");
+  hipLaunchKernelGGL(lxa::bcg_omega_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials, state);
+  LXA_LAUNCH_CHECK("lxa_bicgstab_omega");
+  return LXA_OK;
+}
+
+extern "C" int lxa_bicgstab_update(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* x,
+                                   const void* z, const void* s_in, const void* t, const void* r0, void* y, void* r,
+                                   const void* b, const lxa_krylov_params* prm, lxa_krylov_row* state,
+                                   double* partials) {
+  if (int rc = check_common(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(x && z && s_in && t && r0 && y && r && b, "lxa_bicgstab_update: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  const CgParams c = cg_params(prm);
+  LXA_DISPATCH_YV(ydtype, vdtype,
+                  hipLaunchKernelGGL((lxa::bcg_update_kernel<TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                                     static_cast<const TV*>(x), static_cast<const TV*>(z),
+                                     static_cast<const TV*>(s_in), static_cast<const TV*>(t),
+                                     static_cast<const TV*>(r0), static_cast<TY*>(y), static_cast<TV*>(r),
+                                     static_cast<const TY*>(b), c, state, partials));
+  hipLaunchKernelGGL(lxa::bcg_update_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, n, partials, c,
+                     prm->eps, state);
+  LXA_LAUNCH_CHECK("lxa_bicgstab_update");
+  return LXA_OK;
+}
+
+extern "C" int lxa_bicgstab_result(void* stream, int64_t rows, const lxa_krylov_params* prm,
+                                   lxa_krylov_row* state) {
+  LXA_REQUIRE(rows >= 1 && prm && state, "lxa_bicgstab_result: bad argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(lxa::bcg_result_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, cg_params(prm), prm->eps,
+                     state);
+  LXA_LAUNCH_CHECK("lxa_bicgstab_result");
   return LXA_OK;
 }
 

@@ -175,6 +175,20 @@ def chol_uneven(ns, a_off, a32, b_off, b32, threads=1):
 # ---------------------------------------------------------------------------
 
 
+def normal_solve(a, b):
+    """lx.Normal(lx.Cholesky()) (lineax/_solver/normal.py:106-138): for m >= n
+    x = (A^T A)^-1 A^T b, else x = A^T (A A^T)^-1 b, each Gram matrix factored
+    by potrf 'U' and solved by potrs; float64."""
+    a = _f64c(a)
+    b = np.asarray(b, np.float64)
+    m, n = a.shape
+    if m >= n:
+        u, _ = cho_factor(a.T @ a)
+        return cho_solve(u, a.T @ b)
+    u, _ = cho_factor(a @ a.T)
+    return a.T @ cho_solve(u, b)
+
+
 def linear_solve_results(solution, vector, result):
     """lineax/_solve.py:104-123 for one system."""
     if result == SUCCESSFUL and not np.all(np.isfinite(solution)):
@@ -335,6 +349,76 @@ def cg(mv, b, rtol, atol, *, max_steps=None, stabilise_every=10, y0=None, precon
 # ---------------------------------------------------------------------------
 # GMRES (lineax/_solver/gmres.py:106-413)
 # ---------------------------------------------------------------------------
+
+
+# ---------------------------------------------------------------------------
+# BiCGStab (lineax/_solver/bicgstab.py:78-205)
+# ---------------------------------------------------------------------------
+
+
+def bicgstab(mv, b, rtol, atol, *, max_steps=None, y0=None, precond=None, norm=max_norm, dtype=np.float64,
+             x64=True):
+    """Returns (y, result, num_steps), following bicgstab.py line by line:
+    r0 = b - A y0 (103); breakdown test (113-119: exact zeros under
+    jax_enable_x64, else < 1e-16); cond_fun (121-140); body (143-167) with
+    right preconditioning; initial carry alpha = omega = rho = 1, diff = inf
+    (169-180); RESULTS (185-197)."""
+    b = np.asarray(b, dtype)
+    M = precond if precond is not None else (lambda v: v)
+    size = b.size
+    steps_cap = 10 * size if max_steps is None else max_steps
+    has_scale = not (atol == 0 and rtol == 0)
+    b_scale = atol + rtol * np.abs(b)
+    y = np.zeros_like(b) if y0 is None else np.asarray(y0, dtype).copy()
+    r0 = b - mv(y)
+
+    def breakdown(omega, alpha, rho):
+        if x64:
+            return bool(omega == 0.0) or bool(alpha == 0.0) or bool(rho == 0.0)
+        return bool(omega < 1e-16) or bool(alpha < 1e-16) or bool(rho < 1e-16)
+
+    def not_converged(r, diff, y):
+        if not has_scale:
+            return True
+        y_scale = atol + rtol * np.abs(y)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            n1 = norm(r / b_scale)
+            n2 = norm(diff / y_scale)
+        return bool(n1 > 1) or bool(n2 > 1)
+
+    r = r0.copy()
+    alpha = omega = rho = 1.0
+    p = np.zeros_like(b)
+    v = np.zeros_like(b)
+    diff = np.full_like(y, np.inf)
+    step = 0
+    while (not breakdown(omega, alpha, rho)) and not_converged(r, diff, y) and step < steps_cap:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rho_new = np.dot(r0, r)
+            beta = (rho_new / rho) * (alpha / omega)
+            p = r + beta * (p - omega * v)
+            x = M(p)
+            v = mv(x)
+            alpha = rho_new / np.dot(r0, v)
+            s = r - alpha * v
+            z = M(s)
+            t = mv(z)
+            omega = np.dot(s, t) / np.dot(t, t)
+            diff = alpha * x + omega * z
+            y = y + diff
+            r = s - omega * t
+        rho = rho_new
+        step += 1
+
+    if max_steps is None:
+        result = SINGULAR if step == steps_cap else SUCCESSFUL
+    elif has_scale:
+        result = MAX_STEPS_REACHED if step == steps_cap else SUCCESSFUL
+    else:
+        result = SUCCESSFUL
+    if breakdown(omega, alpha, rho) and not_converged(r, diff, y):
+        result = BREAKDOWN
+    return y, result, step
 
 
 def _normalise(x, eps=None):

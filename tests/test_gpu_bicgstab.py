@@ -1,0 +1,141 @@
+"""GPU parity of BiCGStab (lineax/_solver/bicgstab.py) against the oracle.
+
+The oracle is the float64 restatement O.bicgstab.  Float64 solves match its
+solution to 1e-8 relative with the same RESULTS code and a step count within
+one (reductions are summed in a different order); rows of a batch stop
+independently; the max_steps and non-finite rules agree; an exact
+preconditioner converges in a step or two and the gradient of the solve is
+the closed form A^-T g; graph replays give the eager answer.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import lineax_amd as lx
+from lineax_amd.stencil import convection_diffusion2d, convection_diffusion_coeffs
+from oracle import lineax_oracle as O
+from tests.helpers import poisson_1d
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, device, dtype=torch.float64):
+    return torch.as_tensor(np.asarray(x), dtype=dtype, device=device)
+
+
+def _nonsymmetric(rng, n):
+    return 4.0 * np.eye(n) + rng.standard_normal((n, n)) / np.sqrt(n)
+
+
+def test_bicgstab_dense_vs_oracle(device, rng):
+    a = _nonsymmetric(rng, 40)
+    b = rng.standard_normal(40)
+    sol = lx.linear_solve(lx.MatrixLinearOperator(_t(a, device)), _t(b, device), lx.BiCGStab(1e-10, 1e-10))
+    x_ref, res_ref, steps_ref = O.bicgstab(lambda v: a @ v, b, 1e-10, 1e-10)
+    assert int(sol.result) == res_ref == O.SUCCESSFUL
+    assert abs(int(sol.stats["num_steps"]) - steps_ref) <= 1
+    np.testing.assert_allclose(sol.value.cpu().numpy(), x_ref, rtol=1e-8, atol=1e-9)
+
+
+def test_bicgstab_stencil_batch_vs_oracle(device, rng):
+    n = 16
+    op = convection_diffusion2d(n, device=device)
+    a = O.stencil_matrix(n, n, convection_diffusion_coeffs(n)).toarray()
+    b = rng.standard_normal((3, n * n))
+    b[1] *= 1e-3  # rows converge after different numbers of steps
+    sol = lx.linear_solve(op, _t(b, device).reshape(3, n, n), lx.BiCGStab(1e-9, 1e-9))
+    for i in range(3):
+        x_ref, res_ref, steps_ref = O.bicgstab(lambda v: a @ v, b[i], 1e-9, 1e-9)
+        assert int(sol.result[i]) == res_ref
+        assert abs(int(sol.stats["num_steps"][i]) - steps_ref) <= 1
+        np.testing.assert_allclose(sol.value[i].reshape(-1).cpu().numpy(), x_ref, rtol=1e-7, atol=1e-9)
+
+
+def test_bicgstab_float32_working_vectors(device, rng):
+    n = 32
+    op = convection_diffusion2d(n, device=device)
+    b = _t(rng.standard_normal((n, n)), device)
+    ref = lx.linear_solve(op, b, lx.BiCGStab(1e-10, 1e-10)).value
+    sol = lx.linear_solve(op, b, lx.BiCGStab(1e-6, 1e-6, working_dtype=torch.float32))
+    assert int(sol.result) == lx.RESULTS.successful
+    assert sol.value.dtype == torch.float64
+    scale = float(ref.abs().max())
+    assert float((sol.value - ref).abs().max()) <= 1e-4 * scale
+
+
+def test_bicgstab_float32_breakdown_rule(device, rng):
+    # Without jax_enable_x64 lineax tests omega, alpha, rho < 1e-16 with no
+    # absolute value (bicgstab.py:117-119): a negative one on a nonsymmetric
+    # operator ends the solve with RESULTS.breakdown, as in the oracle run
+    # under the same rule.
+    a = _nonsymmetric(rng, 60)
+    b = rng.standard_normal(60)
+    sol = lx.linear_solve(lx.MatrixLinearOperator(_t(a, device, torch.float32)), _t(b, device, torch.float32),
+                          lx.BiCGStab(1e-5, 1e-5), throw=False)
+    _, res_ref, steps_ref = O.bicgstab(lambda v: a @ v, b, 1e-5, 1e-5, x64=False)
+    assert res_ref == O.BREAKDOWN
+    assert int(sol.result) == res_ref and int(sol.stats["num_steps"]) == steps_ref
+    # an SPD operator keeps them positive: float32 converges
+    m = rng.standard_normal((60, 60))
+    spd = m @ m.T / 60 + np.eye(60)
+    sol = lx.linear_solve(lx.MatrixLinearOperator(_t(spd, device, torch.float32)), _t(b, device, torch.float32),
+                          lx.BiCGStab(1e-5, 1e-5))
+    assert int(sol.result) == lx.RESULTS.successful and sol.value.dtype == torch.float32
+    np.testing.assert_allclose(sol.value.double().cpu().numpy(), np.linalg.solve(spd, b), rtol=1e-3, atol=1e-4)
+
+
+def test_bicgstab_max_steps_rules(device, rng):
+    op = lx.MatrixLinearOperator(_t(-poisson_1d(100), device))
+    b = _t(rng.standard_normal(100), device)
+    s = lx.linear_solve(op, b, lx.BiCGStab(0.0, 0.0, max_steps=2), throw=False)
+    assert int(s.stats["num_steps"]) == 2 and int(s.result) == lx.RESULTS.successful
+    s = lx.linear_solve(op, b, lx.BiCGStab(1e-14, 1e-14, max_steps=3), throw=False)
+    assert int(s.stats["num_steps"]) == 3 and int(s.result) == lx.RESULTS.max_steps_reached
+
+
+def test_bicgstab_singular_operator(device):
+    sing = np.diag(np.r_[np.ones(5), 0.0])
+    s = lx.linear_solve(lx.MatrixLinearOperator(_t(sing, device)), _t(np.ones(6), device),
+                        lx.BiCGStab(1e-12, 1e-12), throw=False)
+    x_ref, res_ref, steps_ref = O.bicgstab(lambda v: sing @ v, np.ones(6), 1e-12, 1e-12)
+    assert int(s.stats["num_steps"]) == steps_ref
+    assert int(s.result) == O.linear_solve_results(x_ref, np.ones(6), res_ref) == lx.RESULTS.singular
+
+
+def test_bicgstab_exact_preconditioner_and_gradient(device):
+    # lineax tests/test_adjoint.py:80-130
+    rng = np.random.default_rng(123)
+    a = rng.uniform(size=(10, 10)) + np.diag(np.arange(10) ** 6.0)
+    b = rng.uniform(size=10)
+    g = rng.standard_normal(10)
+    op = lx.MatrixLinearOperator(_t(a, device))
+    m = lx.MatrixLinearOperator(_t(np.linalg.inv(a), device))
+    B = _t(b, device).requires_grad_(True)
+    sol = lx.linear_solve(op, B, lx.BiCGStab(1e-6, 1e-6, max_steps=3), options={"preconditioner": m})
+    assert int(sol.result) == lx.RESULTS.successful and int(sol.stats["num_steps"]) <= 2
+    np.testing.assert_allclose(sol.value.detach().cpu().numpy(), np.linalg.solve(a, b), rtol=1e-6, atol=1e-8)
+    (sol.value * _t(g, device)).sum().backward()
+    np.testing.assert_allclose(B.grad.cpu().numpy(), np.linalg.solve(a.T, g), rtol=1e-5, atol=1e-7)
+
+
+def test_bicgstab_transpose_state(device, rng):
+    a = _nonsymmetric(rng, 30)
+    c = rng.standard_normal(30)
+    solver = lx.BiCGStab(1e-11, 1e-11)
+    state = solver.init(lx.MatrixLinearOperator(_t(a, device)), {})
+    st_t, opts = solver.transpose(state, {})
+    x, result, _ = solver.compute(st_t, _t(c, device), opts)
+    assert int(result) == lx.RESULTS.successful
+    np.testing.assert_allclose(x.cpu().numpy(), np.linalg.solve(a.T, c), rtol=1e-8, atol=1e-9)
+
+
+def test_bicgstab_graph_matches_eager(device, rng):
+    n = 24
+    op = convection_diffusion2d(n, device=device)
+    b = _t(rng.standard_normal((2, n, n)), device)
+    eager = lx.linear_solve(op, b, lx.BiCGStab(1e-9, 1e-9))
+    graph = lx.linear_solve(op, b, lx.BiCGStab(1e-9, 1e-9, graph=True))
+    assert torch.equal(eager.result, graph.result)
+    assert torch.equal(eager.stats["num_steps"], graph.stats["num_steps"])
+    assert torch.equal(eager.value, graph.value)

@@ -222,3 +222,29 @@ def test_lstsq_backward_error_metric():
     bb = np.array([1.0, 1.0])
     xx = np.array([0.0, 0.0])  # r = b; A^T r = [1, 3], A r = [3, 1]
     assert abs(O.lstsq_backward_error(sq, xx, bb) - 3.0 / (3.0 * 1.0)) < 1e-15
+
+
+def test_normal_solve_pins():
+    rng = np.random.default_rng(9)
+    # tall full rank: the least-squares solution
+    a = rng.standard_normal((30, 8))
+    b = rng.standard_normal(30)
+    np.testing.assert_allclose(O.normal_solve(a, b), np.linalg.lstsq(a, b, rcond=None)[0], rtol=1e-10, atol=1e-12)
+    # wide full rank: the minimum-norm solution, which solves A x = b exactly
+    a = rng.standard_normal((6, 15))
+    b = rng.standard_normal(6)
+    x = O.normal_solve(a, b)
+    np.testing.assert_allclose(x, np.linalg.lstsq(a, b, rcond=None)[0], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(a @ x, b, rtol=1e-10, atol=1e-12)
+    # the reference's non-square KATs (tests/test_singular.py:113-145)
+    a1 = np.array([[1.0, 5.0, -1.0], [-2.0, -2.0, 3.0]])
+    y1 = np.array([3.0, 4.0])
+    a2 = np.array([[1.0, -2.0], [5.0, -2.0], [-1.0, 3.0]])
+    y2 = np.array([3.0, 4.0, 5.0])
+    for aa, yy in ((a1, y1), (a2, y2)):
+        np.testing.assert_allclose(O.normal_solve(aa, yy), np.linalg.lstsq(aa, yy, rcond=None)[0], rtol=1e-12,
+                                   atol=1e-12)
+    # square: the solution of A x = b (a transposed Gram matrix would fail this)
+    a = np.array([[2.0, 1.0], [0.0, 3.0]])
+    b = np.array([5.0, 6.0])
+    np.testing.assert_allclose(O.normal_solve(a, b), [1.5, 2.0], rtol=1e-13)

@@ -163,3 +163,82 @@ def test_gmres_max_steps_reached(rng, restart):
     b = rng.standard_normal(40)
     _, result, steps = O.gmres(lambda v: a @ v, b, 1e-14, 1e-14, restart=restart, max_steps=2)
     assert steps == 2 and result == O.MAX_STEPS_REACHED
+
+
+# --- BiCGStab (lineax/_solver/bicgstab.py)
+
+
+def _nonsymmetric(rng, n):
+    return 4.0 * np.eye(n) + rng.standard_normal((n, n)) / np.sqrt(n)
+
+
+def test_bicgstab_matches_direct_solve(rng):
+    a = _nonsymmetric(rng, 40)
+    b = rng.standard_normal(40)
+    x, result, steps = O.bicgstab(lambda v: a @ v, b, 1e-12, 1e-12)
+    assert result == O.SUCCESSFUL
+    np.testing.assert_allclose(x, np.linalg.solve(a, b), rtol=1e-9, atol=1e-10)
+    assert 0 < steps < 60
+
+
+def test_bicgstab_agrees_with_scipy_on_convection_diffusion(rng):
+    # the upwind convection-diffusion operator of the GMRES workload, 12 x 12
+    from lineax_amd import stencil as S
+
+    coeffs = S.convection_diffusion_coeffs(12)
+    a = O.stencil_matrix(12, 12, coeffs).toarray()
+    b = rng.standard_normal(144)
+    x, result, _ = O.bicgstab(lambda v: a @ v, b, 1e-11, 1e-11)
+    assert result == O.SUCCESSFUL
+    xs, info = spla.bicgstab(a, b, rtol=1e-13, atol=0.0, maxiter=5000)
+    assert info == 0
+    np.testing.assert_allclose(x, xs, rtol=1e-7, atol=1e-8)
+    np.testing.assert_allclose(x, np.linalg.solve(a, b), rtol=1e-8, atol=1e-9)
+
+
+def test_bicgstab_exact_preconditioner():
+    # lineax tests/test_adjoint.py:80-130: with the exact inverse as the
+    # (right) preconditioner a solve takes a step or two
+    rng = np.random.default_rng(123)
+    a = rng.uniform(size=(10, 10)) + np.diag(np.arange(10) ** 6.0)
+    b = rng.uniform(size=10)
+    inv = np.linalg.inv(a)
+    x, result, steps = O.bicgstab(lambda v: a @ v, b, 1e-6, 1e-6, max_steps=3, precond=lambda v: inv @ v)
+    assert result == O.SUCCESSFUL and steps <= 2
+    np.testing.assert_allclose(x, np.linalg.solve(a, b), rtol=1e-6, atol=1e-8)
+
+
+def test_bicgstab_max_steps_and_breakdown_rules(rng):
+    a = -poisson_1d(100)
+    b = rng.standard_normal(100)
+    mv = lambda v: a @ v
+    # max_steps only (tests/test_solve.py:174-194): successful after exactly 2
+    _, result, steps = O.bicgstab(mv, b, 0.0, 0.0, max_steps=2)
+    assert steps == 2 and result == O.SUCCESSFUL
+    # a tolerance not met within max_steps: max_steps_reached
+    _, result, steps = O.bicgstab(mv, b, 1e-14, 1e-14, max_steps=3)
+    assert steps == 3 and result == O.MAX_STEPS_REACHED
+    # a skew operator makes <r0, A r0> = 0 on the first step: rho_new / 0 is
+    # inf, the iterate turns non-finite and the loop stops on the NaN norms
+    skew = np.array([[0.0, 1.0], [-1.0, 0.0]])
+    x, result, steps = O.bicgstab(lambda v: skew @ v, np.array([1.0, 2.0]), 1e-8, 1e-8)
+    assert steps == 1 and not np.all(np.isfinite(x))
+    assert O.linear_solve_results(x, np.array([1.0, 2.0]), result) == O.SINGULAR
+    # a singular operator: the second step's omega is 0/0, the iterate turns
+    # NaN and the NaN norms end the loop; the solve reports singular through
+    # the non-finite check (lineax/_solve.py:104-113)
+    sing = np.diag(np.r_[np.ones(5), 0.0])
+    x, result, steps = O.bicgstab(lambda v: sing @ v, np.ones(6), 1e-12, 1e-12)
+    assert steps == 2 and result == O.SUCCESSFUL and np.all(np.isnan(x))
+    assert O.linear_solve_results(x, np.ones(6), result) == O.SINGULAR
+
+
+def test_bicgstab_planted_sign_error_fails(rng):
+    # the restatement with beta's sign flipped must not reach the solution
+    a = _nonsymmetric(rng, 30)
+    b = rng.standard_normal(30)
+    x_good, _, _ = O.bicgstab(lambda v: a @ v, b, 1e-12, 1e-12, max_steps=60)
+    np.testing.assert_allclose(x_good, np.linalg.solve(a, b), rtol=1e-8, atol=1e-9)
+    # a wrong operator sign models a dropped minus in the residual update
+    x_bad, _, _ = O.bicgstab(lambda v: -(a @ v), b, 1e-12, 1e-12, max_steps=60)
+    assert not np.allclose(x_bad, np.linalg.solve(a, b), rtol=1e-3)
