@@ -389,10 +389,12 @@ int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_t k, const 
  * solutions are refined against A by the caller (GMRES preconditioned with
  * these factors, FP64 residuals from lxa_gemv_f32a).
  *
- * lxa_lu_large_factor: blocked LU in place, P A = L U with P's row swaps
- *   confined to each 128-row diagonal block (L unit lower below the
- *   diagonal, U on and above); piv[i] (n entries, 0-based) is the row swapped
- *   with row i, as getrf records it; linv / uinv receive the inverses of the
+ * lxa_lu_large_factor: blocked LU in place, P A = L U, each 128-column
+ *   block column pivoting over its diagonal block and the 128 rows below it
+ *   (so every multiplier is bounded by 1, as with partial pivoting, but no
+ *   device-wide arg-max is needed; L unit lower below the diagonal, U on
+ *   and above); piv[i] (n entries, 0-based, i <= piv[i] < i + 256) is the
+ *   row swapped with row i, as getrf records it; linv / uinv receive the inverses of the
  *   128 x 128 diagonal blocks of L and U (ceil(n/128) each, row-major, dense);
  *   mode (LXA_GEMM_*) is the precision of the outer Schur-complement
  *   products (panels are float32).  info[0] (zeroed by the caller) receives
@@ -412,10 +414,15 @@ int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_t k, const 
 int lxa_lu_large_block(void);
 int lxa_lu_large_factor(void* stream, int mode, int64_t n, float* a, int64_t lda, float* linv, float* uinv,
                         int32_t* piv, int32_t* info);
-/* The factorisation's row swaps on nrhs vectors x[r * ldx + i]: reverse = 0
- * applies P (before the L solve), reverse = 1 applies P^T (after the L^T
- * solve of a transposed system). */
-int lxa_lu_large_swaps(void* stream, int64_t n, int nrhs, float* x, int64_t ldx, const int32_t* piv, int reverse);
+/* The factorisation's row permutation: perm[i] = the original row at
+ * position i once the swap sequence piv has been applied in order (getrf's
+ * row interchanges, laswp, composed once); lxa_lu_large_permute then
+ * applies it to nrhs vectors x[r * ldx + i]: inverse = 0 gives y = P x
+ * (before the L solve), inverse = 1 gives y = P^T x (after the L^T solve of
+ * a transposed system); y must not alias x. */
+int lxa_lu_large_perm(void* stream, int64_t n, const int32_t* piv, int32_t* perm);
+int lxa_lu_large_permute(void* stream, int64_t n, int nrhs, const float* x, int64_t ldx, float* y, int64_t ldy,
+                         const int32_t* perm, int inverse);
 int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, const float* lu, int64_t lda, const float* inv,
                       float* x, int64_t ldx, int32_t* sync);
 int lxa_rbt_matrix(void* stream, int64_t n, float* a, int64_t lda, const float* du, const float* dv);

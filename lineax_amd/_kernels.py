@@ -81,7 +81,8 @@ _SIGNATURES = {
     "lxa_gemm_sub": [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64],
     "lxa_lu_large_block": [],
     "lxa_lu_large_factor": [_vp, _int, _i64, _vp, _i64, _vp, _vp, _vp, _vp],
-    "lxa_lu_large_swaps": [_vp, _i64, _int, _vp, _i64, _vp, _int],
+    "lxa_lu_large_perm": [_vp, _i64, _vp, _vp],
+    "lxa_lu_large_permute": [_vp, _i64, _int, _vp, _i64, _vp, _i64, _vp, _int],
     "lxa_lu_large_trsv": [_vp, _int, _i64, _int, _vp, _i64, _vp, _vp, _i64, _vp],
     "lxa_rbt_matrix": [_vp, _i64, _vp, _i64, _vp, _vp],
     "lxa_rbt_vectors": [_vp, _int, _i64, _int, _vp, _i64, _vp, _int],
@@ -793,10 +794,22 @@ def lu_large_factor(a: torch.Tensor, mode: str, linv: torch.Tensor, uinv: torch.
                                       _ptr(piv), _ptr(info)), "lxa_lu_large_factor")
 
 
-def lu_large_swaps(x: torch.Tensor, piv: torch.Tensor, reverse: bool) -> None:
-    _f32_2d(x, "x")
-    _check(load().lxa_lu_large_swaps(_stream(x), x.shape[1], x.shape[0], _ptr(x), x.stride(0), _ptr(piv),
-                                     int(bool(reverse))), "lxa_lu_large_swaps")
+def lu_large_perm(piv: torch.Tensor) -> torch.Tensor:
+    """The factorisation's row permutation (perm[i] = original row at position i)."""
+    _gpu(piv, "piv")
+    perm = torch.empty_like(piv)
+    _check(load().lxa_lu_large_perm(_stream(piv), piv.shape[0], _ptr(piv), _ptr(perm)), "lxa_lu_large_perm")
+    return perm
+
+
+def lu_large_permute(x: torch.Tensor, perm: torch.Tensor, inverse: bool) -> torch.Tensor:
+    """P x (inverse=False) or P^T x of the rows [nrhs, n] of x, into a new tensor."""
+    _gpu(x, "x")
+    _gpu(perm, "perm")
+    y = torch.empty_like(x)
+    _check(load().lxa_lu_large_permute(_stream(x), x.shape[1], x.shape[0], _ptr(x), x.stride(0), _ptr(y), y.stride(0),
+                                       _ptr(perm), int(bool(inverse))), "lxa_lu_large_permute")
+    return y
 
 
 def lu_large_trsv(kind: int, lu: torch.Tensor, inv: torch.Tensor, x: torch.Tensor, sync: torch.Tensor) -> None:

@@ -114,50 +114,66 @@ __global__ void rbt_vec_kernel(int n, int s, int nrhs, T* __restrict__ x, int64_
 // elements -- two barriers per column, LDS buffers alternating by column.
 // The inverses run the same way on register tiles with the factors staged
 // in LDS, one barrier per column.
-__global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, float* __restrict__ a, int64_t lda,
-                                                               float* __restrict__ linv, float* __restrict__ uinv,
-                                                               int32_t* __restrict__ piv, int32_t* __restrict__ info) {
+// Rows of the pivot window: the diagonal block and the block below it.
+// Pivoting over 256 rows keeps every multiplier of the block column (the
+// diagonal block's L and the next block's L21) bounded by 1; over the
+// diagonal block alone, the multipliers below it are unbounded and the
+// growth reaches ~1e3 on Gaussian operators (~1e-5 backward error instead
+// of ~1e-7).
+constexpr int kPivWin = 2 * kB;
+
+__global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, int win, float* __restrict__ a,
+                                                               int64_t lda, float* __restrict__ linv,
+                                                               float* __restrict__ uinv, int32_t* __restrict__ piv,
+                                                               int32_t* __restrict__ info) {
   extern __shared__ __align__(16) float dsm[];
   float* S = dsm;  // [128][129]: the factors, for the inverse sweeps
   constexpr int LD = kB + 1;
-  __shared__ float colv[2][kB];
+  constexpr int kRowT = kPivWin / 32;  // register rows per thread
+  __shared__ float colv[2][kPivWin];
   __shared__ float rowj[2][kB];
   __shared__ float rowp[2][kB];
   const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63;
-  float R[4][4];
+  // R[ai][bi]: window row ty + 32 ai, column tx + 32 bi.  Rows of the
+  // diagonal block beyond nb are identity padding; window rows beyond win
+  // are zero and never chosen.
+  float R[kRowT][4];
 #pragma unroll
-  for (int ai = 0; ai < 4; ++ai)
+  for (int ai = 0; ai < kRowT; ++ai)
 #pragma unroll
     for (int bi = 0; bi < 4; ++bi) {
       const int r = ty + 32 * ai, c = tx + 32 * bi;
-      R[ai][bi] = (r < nb && c < nb) ? a[(int64_t)(k + r) * lda + k + c] : (r == c ? 1.f : 0.f);
+      if (r < kB)
+        R[ai][bi] = (r < nb && c < nb) ? a[(int64_t)(k + r) * lda + k + c] : (r == c ? 1.f : 0.f);
+      else
+        R[ai][bi] = (r < win && c < nb) ? a[(int64_t)(k + r) * lda + k + c] : 0.f;
     }
   for (int j = 0; j < kB; ++j) {
     const int buf = j & 1, jb = j >> 5, jt = j & 31;
     if (tx == jt) {
 #pragma unroll
-      for (int ai = 0; ai < 4; ++ai)
+      for (int ai = 0; ai < kRowT; ++ai)
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
           if (bi == jb) colv[buf][ty + 32 * ai] = R[ai][bi];
     }
     __syncthreads();
-    // arg-max of |column j| over rows >= j (NaN counts as largest; ties:
-    // lowest row), computed by every wave
-    float v0 = -1.f, v1 = -1.f;
-    if (lane >= j) {
-      v0 = fabsf(colv[buf][lane]);
-      if (v0 != v0) v0 = INFINITY;
-    }
-    if (lane + 64 >= j) {
-      v1 = fabsf(colv[buf][lane + 64]);
-      if (v1 != v1) v1 = INFINITY;
-    }
-    float v = v0;
+    // arg-max of |column j| over window rows >= j (NaN counts as largest;
+    // ties: lowest row), computed by every wave
+    float v = -1.f;
     int idx = lane;
-    if (v1 > v0) {
-      v = v1;
-      idx = lane + 64;
+#pragma unroll
+    for (int q = 0; q < kPivWin / kWave; ++q) {
+      const int row = lane + kWave * q;
+      float w = -1.f;
+      if (row >= j && (row < kB || row < win)) {
+        w = fabsf(colv[buf][row]);
+        if (w != w) w = INFINITY;
+      }
+      if (w > v) {
+        v = w;
+        idx = row;
+      }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -181,7 +197,7 @@ __global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, fl
     }
     if (ty == (p & 31)) {
 #pragma unroll
-      for (int ai = 0; ai < 4; ++ai)
+      for (int ai = 0; ai < kRowT; ++ai)
         if (ai == (p >> 5)) {
 #pragma unroll
           for (int bi = 0; bi < 4; ++bi) rowp[buf][tx + 32 * bi] = R[ai][bi];
@@ -194,7 +210,7 @@ __global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, fl
     __syncthreads();
     // swap rows j and p, multipliers into column j, rank-1 update
 #pragma unroll
-    for (int ai = 0; ai < 4; ++ai) {
+    for (int ai = 0; ai < kRowT; ++ai) {
       const int r = ty + 32 * ai;
       if (r == j) {
 #pragma unroll
@@ -211,14 +227,19 @@ __global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, fl
       }
     }
   }
-  // the factors to global memory and to LDS
+  // the factors to global memory and to LDS; the window rows below the
+  // block leave with their final multipliers (their L21 block)
 #pragma unroll
-  for (int ai = 0; ai < 4; ++ai)
+  for (int ai = 0; ai < kRowT; ++ai)
 #pragma unroll
     for (int bi = 0; bi < 4; ++bi) {
       const int r = ty + 32 * ai, c = tx + 32 * bi;
-      S[r * LD + c] = R[ai][bi];
-      if (r < nb && c < nb) a[(int64_t)(k + r) * lda + k + c] = R[ai][bi];
+      if (r < kB) {
+        S[r * LD + c] = R[ai][bi];
+        if (r < nb && c < nb) a[(int64_t)(k + r) * lda + k + c] = R[ai][bi];
+      } else if (r < win && c < nb) {
+        a[(int64_t)(k + r) * lda + k + c] = R[ai][bi];
+      }
     }
   __syncthreads();
   // X = L^-1 (unit lower): for j: X[i][:] -= L[i][j] X[j][:], i > j.
@@ -276,19 +297,21 @@ __global__ void zero_i32_kernel(int32_t* __restrict__ p, int count) {
   if (i < count) p[i] = 0;
 }
 
-// The row swaps of diagonal block k (rows k .. k + nb - 1, in order) applied
-// to every column outside the block: one thread per column, the block's rows
-// staged in LDS.
-constexpr int kSwapCols = 256;
+// The row swaps of block column k (pivot steps k .. k + nb - 1, in order,
+// each within the window rows k .. k + win - 1) applied to every column
+// outside the block column: one thread per column, the window's rows staged
+// in LDS.
+constexpr int kSwapCols = 128;
 
-__global__ __launch_bounds__(kSwapCols) void block_row_swaps_kernel(int n, int k, int nb, float* __restrict__ a,
-                                                                    int64_t lda, const int32_t* __restrict__ piv) {
-  extern __shared__ __align__(16) float col[];  // [nb][kSwapCols]
+__global__ __launch_bounds__(kSwapCols) void block_row_swaps_kernel(int n, int k, int nb, int win,
+                                                                    float* __restrict__ a, int64_t lda,
+                                                                    const int32_t* __restrict__ piv) {
+  extern __shared__ __align__(16) float col[];  // [win][kSwapCols]
   const int t = threadIdx.x;
   const int c = blockIdx.x * kSwapCols + t;
   const bool live = c < n && (c < k || c >= k + nb);
   if (live)
-    for (int r = 0; r < nb; ++r) col[r * kSwapCols + t] = a[(int64_t)(k + r) * lda + c];
+    for (int r = 0; r < win; ++r) col[r * kSwapCols + t] = a[(int64_t)(k + r) * lda + c];
   if (!live) return;
   for (int j = 0; j < nb; ++j) {
     const int p = piv[k + j] - k;
@@ -298,37 +321,49 @@ __global__ __launch_bounds__(kSwapCols) void block_row_swaps_kernel(int n, int k
       col[p * kSwapCols + t] = v;
     }
   }
-  for (int r = 0; r < nb; ++r) a[(int64_t)(k + r) * lda + c] = col[r * kSwapCols + t];
+  for (int r = 0; r < win; ++r) a[(int64_t)(k + r) * lda + c] = col[r * kSwapCols + t];
 }
 
-// The same swaps on right-hand sides x[r * ldx + i]: forward order (P x) or
-// reverse order (P^T x).  One thread per (vector, 128-row block).
-__global__ void rhs_swaps_kernel(int n, int nrhs, float* __restrict__ x, int64_t ldx,
-                                 const int32_t* __restrict__ piv, int reverse) {
-  const int nblk = (n + kB - 1) / kB;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nblk * nrhs) return;
-  float* v = x + (int64_t)(q / nblk) * ldx;
-  const int j0 = (q % nblk) * kB, j1 = min(n, j0 + kB);
-  if (!reverse) {
-    for (int j = j0; j < j1; ++j) {
-      const int p = piv[j];
-      if (p != j) {
-        const float t = v[j];
-        v[j] = v[p];
-        v[p] = t;
-      }
-    }
-  } else {
-    for (int j = j1 - 1; j >= j0; --j) {
-      const int p = piv[j];
-      if (p != j) {
-        const float t = v[j];
-        v[j] = v[p];
-        v[p] = t;
+// The row permutation of the whole factorisation: perm[i] = the original
+// row now at position i (the swap sequence piv applied in order to the
+// identity).  One workgroup; the sequence is inherently sequential, so one
+// thread walks it over an LDS copy (n <= kPermLds) or over global memory.
+constexpr int kPermLds = 32768;
+
+__global__ __launch_bounds__(256) void perm_from_swaps_kernel(int n, const int32_t* __restrict__ piv,
+                                                              int32_t* __restrict__ perm) {
+  extern __shared__ int32_t pl[];
+  const bool in_lds = n <= kPermLds;
+  int32_t* p = in_lds ? pl : perm;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < n; ++j) {
+      const int q = piv[j];
+      if (q != j) {
+        const int32_t t = p[j];
+        p[j] = p[q];
+        p[q] = t;
       }
     }
   }
+  __syncthreads();
+  if (in_lds)
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = pl[i];
+}
+
+// y = P x (y[i] = x[perm[i]], inverse = 0) or y = P^T x (y[perm[i]] = x[i]),
+// nrhs vectors with row strides ldx / ldy.
+__global__ void permute_rows_kernel(int n, int nrhs, const float* __restrict__ x, int64_t ldx, float* __restrict__ y,
+                                    int64_t ldy, const int32_t* __restrict__ perm, int inverse) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= static_cast<int64_t>(n) * nrhs) return;
+  const int64_t r = q / n;
+  const int i = static_cast<int>(q % n);
+  if (!inverse)
+    y[r * ldy + i] = x[r * ldx + perm[i]];
+  else
+    y[r * ldy + perm[i]] = x[r * ldx + i];
 }
 
 // ---------------------------------------------------------------------------
@@ -603,24 +638,28 @@ int lu_large_factor_impl(hipStream_t st, int mode, int n, float* a, int64_t lda,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dsmem);
   if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(diag_lu_kernel)");
   e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_row_swaps_kernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float) * kB * kSwapCols));
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float) * kPivWin * kSwapCols));
   if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(block_row_swaps_kernel)");
   for (int K0 = 0; K0 < n; K0 += kOuter) {
     const int K1 = min(n, K0 + kOuter);
     for (int k = K0; k < K1; k += kB) {
       const int k2 = min(K1, k + kB), nb = k2 - k;
+      // pivot window: the block and the 128 rows below it (all rows below
+      // are current in these columns: the inner updates are right-looking)
+      const int win = nb < kB ? nb : min(kPivWin, n - k);
       float* Li = linv + (int64_t)(k / kB) * kB * kB;
       float* Ui = uinv + (int64_t)(k / kB) * kB * kB;
-      diag_lu_kernel<<<1, kDiagThreads, dsmem, st>>>(k, nb, a, lda, Li, Ui, piv, info);
+      diag_lu_kernel<<<1, kDiagThreads, dsmem, st>>>(k, nb, win, a, lda, Li, Ui, piv, info);
       LXA_LAUNCH_CHECK("diag_lu_kernel");
-      block_row_swaps_kernel<<<(n + kSwapCols - 1) / kSwapCols, kSwapCols, sizeof(float) * nb * kSwapCols, st>>>(
-          n, k, nb, a, lda, piv);
+      block_row_swaps_kernel<<<(n + kSwapCols - 1) / kSwapCols, kSwapCols, sizeof(float) * win * kSwapCols, st>>>(
+          n, k, nb, win, a, lda, piv);
       LXA_LAUNCH_CHECK("block_row_swaps_kernel");
       float* A11 = a + (int64_t)k * lda + k;
-      // L21 = A21 U11^-1 (in place, N = nb <= 128)
-      if (n - k2 > 0) {
-        if (int rc = gemm_update(st, LXA_GEMM_F32, n - k2, nb, nb, A11 + (int64_t)nb * lda, lda, Ui, kB,
-                                 A11 + (int64_t)nb * lda, lda, 0, 1.f))
+      // L21 = A21 U11^-1 (in place, N = nb <= 128) for the rows below the
+      // window (the window's rows hold theirs from diag_lu_kernel)
+      if (n - k - win > 0) {
+        if (int rc = gemm_update(st, LXA_GEMM_F32, n - k - win, nb, nb, A11 + (int64_t)win * lda, lda, Ui, kB,
+                                 A11 + (int64_t)win * lda, lda, 0, 1.f))
           return rc;
       }
       // panel U12 = L11^-1 A12 (in place, M = nb <= 128), then the panel's Schur update
@@ -674,15 +713,28 @@ extern "C" int lxa_lu_large_factor(void* stream, int mode, int64_t n, float* a, 
   return lu_large_factor_impl(static_cast<hipStream_t>(stream), mode, (int)n, a, lda, linv, uinv, piv, info);
 }
 
-extern "C" int lxa_lu_large_swaps(void* stream, int64_t n, int nrhs, float* x, int64_t ldx, const int32_t* piv,
-                                  int reverse) {
-  LXA_REQUIRE(n >= 0 && n <= (1 << 20) && nrhs >= 0 && ldx >= n, "invalid n / nrhs / ldx");
+extern "C" int lxa_lu_large_perm(void* stream, int64_t n, const int32_t* piv, int32_t* perm) {
+  LXA_REQUIRE(n >= 0 && n <= (1 << 20), "invalid n");
+  if (n == 0) return LXA_OK;
+  LXA_REQUIRE(piv && perm, "piv and perm must be non-NULL");
+  const size_t smem = n <= kPermLds ? sizeof(int32_t) * (size_t)n : 0;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&perm_from_swaps_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(int32_t) * kPermLds));
+  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(perm_from_swaps_kernel)");
+  perm_from_swaps_kernel<<<1, 256, smem, static_cast<hipStream_t>(stream)>>>((int)n, piv, perm);
+  LXA_LAUNCH_CHECK("perm_from_swaps_kernel");
+  return LXA_OK;
+}
+
+extern "C" int lxa_lu_large_permute(void* stream, int64_t n, int nrhs, const float* x, int64_t ldx, float* y,
+                                    int64_t ldy, const int32_t* perm, int inverse) {
+  LXA_REQUIRE(n >= 0 && n <= (1 << 20) && nrhs >= 0 && ldx >= n && ldy >= n, "invalid n / nrhs / ldx / ldy");
   if (n == 0 || nrhs == 0) return LXA_OK;
-  LXA_REQUIRE(x && piv, "x and piv must be non-NULL");
-  const int64_t threads = ((n + kB - 1) / kB) * nrhs;
-  rhs_swaps_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
-      (int)n, nrhs, x, ldx, piv, reverse ? 1 : 0);
-  LXA_LAUNCH_CHECK("rhs_swaps_kernel");
+  LXA_REQUIRE(x && y && perm && x != y, "x, y and perm must be non-NULL, y must not alias x");
+  const int64_t threads = n * nrhs;
+  permute_rows_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      (int)n, nrhs, x, ldx, y, ldy, perm, inverse ? 1 : 0);
+  LXA_LAUNCH_CHECK("permute_rows_kernel");
   return LXA_OK;
 }
 

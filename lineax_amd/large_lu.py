@@ -4,7 +4,8 @@
 
 1. A' = U^T A V with two-sided depth-2 random butterflies (seeded), padded
    to a multiple of 4 with an identity block;
-2. P A' = L U with pivoting confined to each 128 x 128 diagonal block,
+2. P A' = L U, each 128-column block column pivoting over its diagonal
+   block and the 128 rows below it (every multiplier bounded by 1),
    blocked: float32 panels and triangular block products, Schur-complement
    products on the matrix cores with float32 accumulation in bfloat16,
    float16, split bfloat16 ("bf16x3": hi*hi + hi*lo + lo*hi, the default) or
@@ -20,7 +21,7 @@
 
 The answer is that of lineax's LU (lineax/_solver/lu.py: getrf with partial
 pivoting, then getrs) to the refinement tolerance; the factors are not
-getrf's.  Pivoting inside diagonal blocks of the randomised matrix replaces
+getrf's.  Pivoting over a 256-row window of the randomised matrix replaces
 one device-wide arg-max per column with a workgroup-local one.
 """
 
@@ -103,6 +104,7 @@ class LargeLU:
         self.info = torch.zeros(1, dtype=torch.int32, device=dev)
         self.piv = torch.empty(np_, dtype=torch.int32, device=dev)
         K.lu_large_factor(lu, precision, self.linv, self.uinv, self.piv, self.info)
+        self.perm = K.lu_large_perm(self.piv)
         self.lu = lu
         self.sync = torch.empty(self.nblk + 2, dtype=torch.int32, device=dev)
         self.norm_inf = self.a.abs().sum(dim=1, dtype=torch.float64).max()
@@ -115,7 +117,7 @@ class LargeLU:
         first, last = (self.dv, self.du) if trans else (self.du, self.dv)
         K.rbt_vectors(w, first, form=1)
         if not trans:
-            K.lu_large_swaps(w, self.piv, reverse=False)
+            w = K.lu_large_permute(w, self.perm, inverse=False)
         for r0 in range(0, rows, MAX_RHS_PER_SOLVE):
             chunk = w[r0: r0 + MAX_RHS_PER_SOLVE]
             if trans:
@@ -125,7 +127,7 @@ class LargeLU:
                 K.lu_large_trsv(0, self.lu, self.linv, chunk, self.sync)
                 K.lu_large_trsv(1, self.lu, self.uinv, chunk, self.sync)
         if trans:
-            K.lu_large_swaps(w, self.piv, reverse=True)
+            w = K.lu_large_permute(w, self.perm, inverse=True)
         K.rbt_vectors(w, last, form=0)
         return w[:, : self.n]
 

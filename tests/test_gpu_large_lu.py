@@ -5,7 +5,9 @@
   differs, so the tolerance is tight and a wrong MFMA lane layout fails.
 * rbt_matrix / rbt_vectors against the butterfly matrices formed explicitly.
 * LargeLU: L U reproduces the row-swapped randomised matrix (float32
-  products; swaps stay inside 128-row diagonal blocks); the
+  products; swaps stay inside each block column's 256-row pivot window,
+  multipliers stay small); the factors alone reach float32-level backward
+  error on Gaussian operators (the pivot growth stays small); the
   refined solve reaches normwise backward error <= 1e-12 against the float64
   oracle matrix and matches numpy's float64 solve; the transposed solve and
   the LU solver's large path agree with float64 solves; an all-zero matrix
@@ -124,12 +126,16 @@ def test_large_lu_factors_reproduce_randomised_matrix(device, rng, n):
     piv = fac.piv.cpu().numpy()
     for i in range(fac.np):  # the factorisation's row swaps, in order
         if piv[i] != i:
-            assert i // 128 == piv[i] // 128  # swaps stay inside a diagonal block
+            assert i < piv[i] < (i // 128) * 128 + 256  # swaps stay inside the block's pivot window
             target[[i, piv[i]]] = target[[piv[i], i]]
     err = np.max(np.abs(L @ U - target)) / np.max(np.abs(target))
     assert err <= 1e-4
     assert int(fac.info.item()) == 0
-    assert np.max(np.abs(np.tril(lu, -1))) <= 1.0 + 1e-6 or n > 128  # |l| <= 1 inside one block
+    # every multiplier of the pivot windows is bounded by 1 (partial pivoting
+    # within the window); the rows below a window get theirs from a product
+    # with the inverse of U11, bounded only in practice
+    assert np.max(np.abs(np.tril(lu, -1))) <= 1.0 + 1e-6 or n > 256
+    assert np.max(np.abs(np.tril(lu, -1))) <= 50.0
 
 
 def test_large_lu_block_inverses_and_solves_on_gaussian(device, rng):
@@ -226,3 +232,20 @@ def test_large_lu_zero_matrix_is_singular(device):
     sol = lx.linear_solve(op, torch.ones(n, dtype=torch.float32, device=device), lx.LU(large_threshold=128),
                           throw=False)
     assert int(sol.result) == lx.RESULTS.singular
+
+
+def test_large_lu_factor_accuracy_gaussian(device, rng):
+    # pivoting over 256-row windows keeps the growth small: the unrefined
+    # solve through the factors is float32-accurate (backward error ~1e-7,
+    # against ~1e-5 with pivoting confined to 128-row blocks)
+    n = 2048
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    b = rng.standard_normal((1, n)).astype(np.float32)
+    fac = LargeLU(torch.as_tensor(a, device=device), precision="fp32", seed=1)
+    x = fac.apply_inverse(torch.as_tensor(b, device=device)).cpu().numpy().astype(np.float64)[0]
+    a64 = a.astype(np.float64)
+    r = b[0] - a64 @ x
+    eta = np.max(np.abs(r)) / (np.max(np.abs(a64).sum(1)) * np.max(np.abs(x)) + np.max(np.abs(b)))
+    assert eta <= 3e-6, eta
+    lu = fac.lu.cpu().numpy()
+    assert np.max(np.abs(np.tril(lu, -1))) <= 50.0
