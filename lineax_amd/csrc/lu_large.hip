@@ -482,9 +482,20 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
   // loaded while block `step` waits for its x and is applied
   float4 pre[kStageVec];
   auto block_col = [&](int step) { return forward ? step : nblk - 1 - step; };
+  // the inverse of the diagonal block, staged like an operator block
+  const float* D = inv + (int64_t)bi * kB * kB;
+  auto load_inv = [&]() {
+#pragma unroll
+    for (int u = 0; u < kStageVec; ++u) {
+      const int idx = tid + u * kTrsvThreads;
+      pre[u] = *reinterpret_cast<const float4*>(D + (idx >> 5) * kB + 4 * (idx & 31));
+    }
+  };
   if (t > 0) {
     const int bj = block_col(0);
     tile_load(pre, lu, lda, kind, bi, bj, rows, min(kB, n - bj * kB), tid);
+  } else {
+    load_inv();
   }
   for (int step = 0; step < t; ++step) {
     const int bj = block_col(step);
@@ -493,6 +504,8 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
     if (step + 1 < t) {
       const int bn = block_col(step + 1);
       tile_load(pre, lu, lda, kind, bi, bn, rows, min(kB, n - bn * kB), tid);
+    } else {
+      load_inv();  // in flight while the last dependency is awaited
     }
     if (tid == 0) {
       int spin = 0;
@@ -521,16 +534,8 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
     __syncthreads();
   }
   // y = inv(D) acc, with D staged like an operator block (transposed kinds
-  // use inv(D)^T)
-  {
-    const float* D = inv + (int64_t)bi * kB * kB;
-#pragma unroll
-    for (int u = 0; u < kStageVec; ++u) {
-      const int idx = tid + u * kTrsvThreads;
-      pre[u] = *reinterpret_cast<const float4*>(D + (idx >> 5) * kB + 4 * (idx & 31));
-    }
-    tile_store(tile, pre, tr, tid);
-  }
+  // use inv(D)^T); its loads were issued before the last wait
+  tile_store(tile, pre, tr, tid);
   __syncthreads();
   float part[R];
   tile_product<R>(tile, acc, row, hcol, part);
