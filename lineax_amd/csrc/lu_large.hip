@@ -114,6 +114,41 @@ __global__ void rbt_vec_kernel(int n, int s, int nrhs, T* __restrict__ x, int64_
 // elements -- two barriers per column, LDS buffers alternating by column.
 // The inverses run the same way on register tiles with the factors staged
 // in LDS, one barrier per column.
+// One step of a wave arg-max (value, lowest index on ties) with a DPP
+// partner: a register-crossbar move, no LDS round trip.
+template <int kCtrl>
+__device__ __forceinline__ void amax_dpp(float& v, int& idx) {
+  const float ov = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kCtrl, 0xF, 0xF, false));
+  const int oi = __builtin_amdgcn_mov_dpp(idx, kCtrl, 0xF, 0xF, false);
+  if (ov > v || (ov == v && oi < idx)) {
+    v = ov;
+    idx = oi;
+  }
+}
+
+// Arg-max over the wave: quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror and row_mirror leave each 16-lane row's arg-max in all of
+// its lanes; the four rows are combined from scalar reads.  The result is
+// wave-uniform.
+__device__ __forceinline__ int wave_argmax(float v, int idx) {
+  amax_dpp<0xB1>(v, idx);
+  amax_dpp<0x4E>(v, idx);
+  amax_dpp<0x141>(v, idx);
+  amax_dpp<0x140>(v, idx);
+  float bv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  int bi = __builtin_amdgcn_readlane(idx, 0);
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    const float rv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16 * r));
+    const int ri = __builtin_amdgcn_readlane(idx, 16 * r);
+    if (rv > bv || (rv == bv && ri < bi)) {
+      bv = rv;
+      bi = ri;
+    }
+  }
+  return bi;
+}
+
 // Rows of the pivot window: the diagonal block and the block below it.
 // Pivoting over 256 rows keeps every multiplier of the block column (the
 // diagonal block's L and the next block's L21) bounded by 1; over the
@@ -175,16 +210,7 @@ __global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, in
         idx = row;
       }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const float ov = __shfl_xor(v, off, kWave);
-      const int oi = __shfl_xor(idx, off, kWave);
-      if (ov > v || (ov == v && oi < idx)) {
-        v = ov;
-        idx = oi;
-      }
-    }
-    const int p = idx;
+    const int p = wave_argmax(v, idx);
     const float pv = colv[buf][p];
     // rows j and p to LDS (before the swap)
     if (ty == jt) {
