@@ -14,6 +14,7 @@ from ._operator import (
     DivLinearOperator,
     FunctionLinearOperator,
     IdentityLinearOperator,
+    JacobianLinearOperator,
     MatrixLinearOperator,
     MulLinearOperator,
     NegLinearOperator,

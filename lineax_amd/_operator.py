@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import abc
 import functools as ft
+import warnings
 from typing import Callable
 
 import torch
@@ -416,6 +417,132 @@ class FunctionLinearOperator(AbstractLinearOperator):
         return self._device
 
 
+class _PerVector:
+    """Applies a function of one unbatched vector (pytree) to a vector with
+    leading batch dimensions, through torch.func.vmap over the flattened batch."""
+
+    def __init__(self, fn, in_struct):
+        self.fn = fn
+        self.in_struct = in_struct
+
+    def __call__(self, vector):
+        batch = batch_shape_of(vector, self.in_struct)
+        if not batch:
+            return self.fn(vector)
+        nb = 1
+        for d in batch:
+            nb *= d
+        flat = tree_map(lambda t, s: t.reshape(nb, *s.shape), vector, self.in_struct)
+        out = torch.func.vmap(self.fn)(flat)
+        return tree_map(lambda t: t.reshape(*batch, *t.shape[1:]), out)
+
+
+class JacobianLinearOperator(AbstractLinearOperator):
+    """The Jacobian (d fn / d x)(x, args) of ``fn: (x, args) -> y`` at ``x``,
+    never materialised unless asked (lineax/_operator.py:539-660).
+
+    ``mv`` is a Jacobian-vector product: ``torch.func.jvp`` for ``jac=None``
+    or ``"fwd"``; for ``"bwd"`` the transpose of ``torch.func.vjp`` (a
+    vector-Jacobian product of the linear map u -> J^T u), which needs only
+    reverse-mode derivatives of ``fn`` (lineax/_operator.py:621-641), or the
+    vjp itself when the operator is tagged symmetric.  ``transpose`` is the
+    vjp as a FunctionLinearOperator (lineax/_operator.py:646-654).
+    ``linearise`` caches the primal pass (``torch.func.linearize``) and
+    ``materialise`` forms the Jacobian with ``jacfwd`` / ``jacrev`` by
+    lineax's heuristic (lineax/_misc.py:41-55).  ``fn`` must be built from
+    differentiable torch operations.  Derivatives of a solve with respect to
+    the point ``x`` or ``args`` are not propagated (they are constants of the
+    operator); derivatives with respect to the right-hand side are.
+    """
+
+    def __init__(self, fn: Callable, x, args=None, tags=(), jac=None):
+        if jac not in (None, "fwd", "bwd"):
+            raise ValueError(
+                "`jac` argument of `JacobianLinearOperator` should be either `'fwd'`, `'bwd'`, or `None`."
+            )
+        self.fn = fn
+        self.x = tree_map(_inexact_leaf, x)
+        self.args = args
+        self.tags = as_tag_set(tags)
+        self.jac = jac
+        self._in = structure_of(self.x)
+        self._out = None
+        self._check_square_structures()
+
+    def _f(self, x):
+        return self.fn(x, self.args)
+
+    def _jvp(self, v):
+        return torch.func.jvp(self._f, (self.x,), (v,))[1]
+
+    def _jv_from_vjp(self, v):
+        y, vjp_fn = torch.func.vjp(self._f, self.x)
+        if is_symmetric(self):
+            return vjp_fn(v)[0]
+        zeros = tree_map(torch.zeros_like, y)
+        return torch.func.vjp(lambda u: vjp_fn(u)[0], zeros)[1](v)[0]
+
+    def mv(self, vector):
+        one = self._jv_from_vjp if self.jac == "bwd" else self._jvp
+        return _PerVector(one, self.in_structure())(vector)
+
+    def as_matrix(self):
+        return _jacobian_matrix(self)
+
+    def transpose(self):
+        if is_symmetric(self):
+            return self
+        _, vjp_fn = torch.func.vjp(self._f, self.x)
+        return FunctionLinearOperator(
+            _PerVector(lambda y: vjp_fn(y)[0], self.out_structure()),
+            self.out_structure(),
+            transpose_tags(self.tags),
+            output_structure=self.in_structure(),
+            device=self.device,
+        )
+
+    def in_structure(self):
+        return self._in
+
+    def out_structure(self):
+        if self._out is None:
+            with torch.no_grad():
+                self._out = structure_of(self._f(self.x))
+        return self._out
+
+    @property
+    def device(self):
+        leaves = tree_leaves(self.x)
+        return leaves[0].device if leaves else None
+
+    def __repr__(self):
+        return f"JacobianLinearOperator(in={self._in}, jac={self.jac!r}, tags={set(self.tags)})"
+
+
+def _inexact_leaf(t):
+    t = torch.as_tensor(t)
+    if not (t.is_floating_point() or t.is_complex()):
+        t = t.to(default_floating_dtype())
+    return t.detach()
+
+
+def _jacobian_matrix(op: "JacobianLinearOperator") -> torch.Tensor:
+    """[out_size, in_size] Jacobian of op.fn at op.x on flattened vectors."""
+    in_struct, out_struct = op.in_structure(), op.out_structure()
+    in_size, out_size = op.in_size(), op.out_size()
+
+    def flat_fn(xf):
+        return ravel(op._f(unravel(xf, in_struct, ())), out_struct, ())
+
+    x_flat = ravel(op.x, in_struct, ())
+    if op.jac is None:
+        fwd = in_size < 100 or in_size <= 1.5 * out_size
+    else:
+        fwd = op.jac == "fwd"
+    jac = (torch.func.jacfwd if fwd else torch.func.jacrev)(flat_fn)(x_flat)
+    return jac.reshape(out_size, in_size)
+
+
 class _LinearTranspose:
     """y -> (d fn / d x)^T y for linear fn: the vector-Jacobian product at zero.
 
@@ -753,7 +880,7 @@ def is_negative_semidefinite(operator) -> bool:
     raise NotImplementedError(f"`is_negative_semidefinite` has not been implemented for {type(operator)}")
 
 
-_LEAF_OPS = (MatrixLinearOperator, FunctionLinearOperator)
+_LEAF_OPS = (MatrixLinearOperator, FunctionLinearOperator, JacobianLinearOperator)
 
 
 def _register_leaf(check, fn):
@@ -908,6 +1035,26 @@ def _(operator):
     return TaggedLinearOperator(linearise(operator.operator), operator.tags)
 
 
+@linearise.register(JacobianLinearOperator)
+def _(operator):
+    # the primal pass runs once (lineax/_operator.py:1240-1257: jax.linearize,
+    # or the transposed vjp for jac="bwd")
+    if operator.jac == "bwd":
+        y, vjp_fn = torch.func.vjp(operator._f, operator.x)
+        if is_symmetric(operator):
+            one = lambda v: vjp_fn(v)[0]
+        else:
+            zeros = tree_map(torch.zeros_like, y)
+            one = lambda v: torch.func.vjp(lambda u: vjp_fn(u)[0], zeros)[1](v)[0]
+    else:
+        with warnings.catch_warnings():
+            # torch.func.linearize's graph folding reports an internal detail
+            warnings.filterwarnings("ignore", message="Attempted to insert a get_attr Node")
+            _, one = torch.func.linearize(operator._f, operator.x)
+    return FunctionLinearOperator(_PerVector(one, operator.in_structure()), operator.in_structure(), operator.tags,
+                                  output_structure=operator.out_structure(), device=operator.device)
+
+
 @ft.singledispatch
 def materialise(operator):
     """Operator backed by an explicit matrix (lineax/_operator.py:1263-1315)."""
@@ -919,6 +1066,24 @@ def materialise(operator):
 @materialise.register(TaggedLinearOperator)
 def _(operator):
     return TaggedLinearOperator(materialise(operator.operator), operator.tags)
+
+
+@materialise.register(JacobianLinearOperator)
+def _(operator):
+    # lineax/_operator.py:1353-1366; a single 1-D input and output gives a
+    # MatrixLinearOperator, other structures a FunctionLinearOperator over
+    # the flattened matrix
+    jac = _jacobian_matrix(operator)
+    ins, outs = operator.in_structure(), operator.out_structure()
+    if isinstance(ins, TensorStruct) and isinstance(outs, TensorStruct) and len(ins.shape) == 1 \
+            and len(outs.shape) == 1:
+        return MatrixLinearOperator(jac, operator.tags)
+
+    def apply(v):
+        batch = batch_shape_of(v, ins)
+        return unravel(matvec(jac, ravel(v, ins, batch)), outs, batch)
+
+    return FunctionLinearOperator(apply, ins, operator.tags, output_structure=outs, device=operator.device)
 
 
 def conj(operator):
