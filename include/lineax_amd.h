@@ -239,6 +239,19 @@ int lxa_cg_finish(void* stream, int vdtype, int64_t rows, int64_t n, const void*
                   const lxa_krylov_params* prm, lxa_krylov_row* state, double* partials);
 int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state);
 
+/* ---------------------------------------------------------------------------
+ * Thin singular value decomposition of a batch of dense matrices, A = U
+ * diag(s) V^T with s descending (jax.scipy.linalg.svd(full_matrices=False)
+ * in lineax/_solver/svd.py:43), by one-sided Jacobi rotations: one
+ * workgroup per matrix.  a: batch matrices m x n row-major (a_stride
+ * elements apart); u: [batch][m][k], s: [batch][k], vt: [batch][k][n],
+ * k = min(m, n) <= 512; work: lxa_svd_workspace bytes; sweeps[b] (may be
+ * NULL) receives the Jacobi sweeps used (max_sweeps when not converged).
+ * ------------------------------------------------------------------------- */
+int lxa_svd_workspace(int dtype, int64_t batch, int64_t m, int64_t n, size_t* bytes);
+int lxa_svd(void* stream, int dtype, int64_t batch, int64_t m, int64_t n, const void* a, int64_t a_stride, void* u,
+            void* s, void* vt, void* work, int max_sweeps, int32_t* sweeps);
+
 /* BiCGStab with right preconditioning (lineax/_solver/bicgstab.py:78-205;
  * its body 143-167, cond_fun 121-140, RESULTS 185-197).  Sequence:
  * [caller: t = A y0] begin (r = r0 = b - t, the initial carry alpha = omega

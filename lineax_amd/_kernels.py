@@ -64,6 +64,8 @@ _SIGNATURES = {
     "lxa_bicgstab_result": [_vp, _i64, _vp, _vp],
     "lxa_cg_stencil_workspace": [_i64, _i64, _i64, _szp],
     "lxa_gemv_uneven": [_vp, _i64, _vp, _int, _int, _int],
+    "lxa_svd_workspace": [_int, _i64, _i64, _i64, _szp],
+    "lxa_svd": [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _int, _vp],
     "lxa_gmres_stencil_workspace": [_i64, _i64, _i64, _szp, _szp],
     "lxa_gmres_stencil_step": [_vp, _int, _i64, _i64, _i64, _vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp, _vp],
@@ -373,6 +375,35 @@ def gemv(matrix: torch.Tensor, x: torch.Tensor, trans: bool = False) -> torch.Te
         "lxa_gemv",
     )
     return y
+
+
+SVD_MAX_SWEEPS = 60
+
+
+def svd(matrix: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Thin SVD of [*batch, m, n]: (u [*batch, m, k], s [*batch, k] descending,
+    vt [*batch, k, n], sweeps [*batch] int32), k = min(m, n) (one-sided
+    Jacobi, lxa_svd)."""
+    _gpu(matrix, "matrix")
+    dt = _dtype_code(matrix.dtype)
+    m, n = matrix.shape[-2:]
+    k = min(m, n)
+    lead = tuple(matrix.shape[:-2])
+    batch = math.prod(lead)
+    dev = matrix.device
+    u = torch.empty((*lead, m, k), dtype=matrix.dtype, device=dev)
+    s = torch.empty((*lead, k), dtype=matrix.dtype, device=dev)
+    vt = torch.empty((*lead, k, n), dtype=matrix.dtype, device=dev)
+    sweeps = torch.zeros(lead, dtype=torch.int32, device=dev)
+    if batch == 0 or k == 0:
+        return u, s, vt, sweeps
+    a = matrix.reshape(batch, m, n).contiguous()
+    need = ctypes.c_size_t(0)
+    _check(load().lxa_svd_workspace(dt, batch, m, n, ctypes.byref(need)), "lxa_svd_workspace")
+    work = torch.empty(max(1, int(need.value) // matrix.element_size()), dtype=matrix.dtype, device=dev)
+    _check(load().lxa_svd(_stream(a), dt, batch, m, n, _ptr(a), m * n, _ptr(u), _ptr(s), _ptr(vt), _ptr(work),
+                          SVD_MAX_SWEEPS, _ptr(sweeps)), "lxa_svd")
+    return u, s, vt, sweeps
 
 
 def solve_results(solution: torch.Tensor, vector: torch.Tensor, result: torch.Tensor) -> torch.Tensor:

@@ -6,8 +6,9 @@ from .gmres import GMRES
 from .lu import LU
 from .normal import Normal
 from .qr import QR
+from .svd import SVD
 from .triangular import Triangular
-from .unsupported import LSMR, SVD, Tridiagonal
+from .unsupported import LSMR, Tridiagonal
 
 __all__ = [
     "BiCGStab", "CG", "Cholesky", "Diagonal", "GMRES", "LSMR", "LU", "Normal", "QR", "SVD", "Triangular",

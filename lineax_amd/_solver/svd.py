@@ -1,0 +1,57 @@
+"""SVD solver (lineax/_solver/svd.py) on the one-sided Jacobi SVD kernel."""
+
+from __future__ import annotations
+
+import torch
+
+from .. import _kernels as K
+from .._misc import resolve_rcond
+from .._operator import matvec
+from .._solution import RESULTS
+from .._solve import AbstractLinearSolver
+from .misc import pack_structures, ravel_vector, transpose_packed_structures, unravel_solution
+
+
+class SVD(AbstractLinearSolver):
+    """Pseudoinverse solve through a thin SVD, for any operator (nonsquare or
+    singular), as scipy.linalg.lstsq.
+
+    ``init`` factors the operator's matrix (lineax/_solver/svd.py:41-45);
+    ``compute`` keeps the singular values above rcond * s_max (rcond defaults
+    to 2 eps max(m, n); lineax/_solver/svd.py:47-71) and returns
+    V diag(1/s) U^T b with the rank in the stats; ``transpose`` swaps the
+    factors (svd.py:73-80).  State: ``((u, s, vt), packed_structures)``.
+    """
+
+    def __init__(self, rcond: float | None = None):
+        self.rcond = rcond
+
+    def init(self, operator, options):
+        u, s, vt, _ = K.svd(operator.as_matrix())
+        return (u, s, vt), pack_structures(operator)
+
+    def compute(self, state, vector, options):
+        (u, s, vt), packed = state
+        flat, batch = ravel_vector(vector, packed)
+        flat = flat.to(u.dtype)
+        m, n = u.shape[-2], vt.shape[-1]
+        rcond = resolve_rcond(self.rcond, n, m, s.dtype)
+        cutoff = rcond * s[..., :1] if s.shape[-1] > 0 else torch.zeros_like(s[..., :1])
+        # strictly greater, so an all-zero matrix has rank 0 (svd.py:60-61)
+        mask = s > cutoff
+        rank = mask.sum(dim=-1)
+        s_inv = torch.where(mask, 1.0 / torch.where(mask, s, torch.ones_like(s)), torch.zeros_like(s))
+        utb = matvec(u.mT, flat)
+        solution = matvec(vt.mT, s_inv * utb)
+        return unravel_solution(solution, packed, tuple(solution.shape[:-1])), RESULTS.successful, {"rank": rank}
+
+    def transpose(self, state, options):
+        (u, s, vt), packed = state
+        return ((vt.mT, s, u.mT), transpose_packed_structures(packed)), {}
+
+    def conj(self, state, options):
+        (u, s, vt), packed = state
+        return ((u.conj(), s, vt.conj()), packed), {}
+
+    def assume_full_rank(self):
+        return False

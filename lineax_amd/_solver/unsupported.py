@@ -38,10 +38,6 @@ class _Unsupported(AbstractLinearSolver):
         return False
 
 
-class SVD(_Unsupported):
-    _reference = "lineax/_solver/svd.py"
-
-
 class Tridiagonal(_Unsupported):
     _reference = "lineax/_solver/tridiagonal.py"
 

@@ -189,6 +189,21 @@ def normal_solve(a, b):
     return a.T @ cho_solve(u, b)
 
 
+def svd_solve(a, b, rcond=None):
+    """lx.SVD() (lineax/_solver/svd.py:41-71): thin SVD, singular values kept
+    when s > rcond * s_max (rcond default 2 eps max(m, n),
+    lineax/_misc.py:30-38), x = V diag(1/s) U^T b; returns (x, rank)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    m, n = a.shape
+    u, s, vt = np.linalg.svd(a, full_matrices=False)
+    rc = 2 * np.finfo(np.float64).eps * max(m, n) if rcond is None else rcond
+    cutoff = rc * s[0] if s.size else 0.0
+    mask = s > cutoff
+    s_inv = np.where(mask, 1.0 / np.where(mask, s, 1.0), 0.0)
+    return vt.T @ (s_inv * (u.T @ b)), int(mask.sum())
+
+
 def linear_solve_results(solution, vector, result):
     """lineax/_solve.py:104-123 for one system."""
     if result == SUCCESSFUL and not np.all(np.isfinite(solution)):

@@ -109,7 +109,13 @@ def test_identity_operator_short_circuit():
 
 
 def test_unsupported_solvers_raise():
-    with pytest.raises(NotImplementedError, match="svd.py"):
+    with pytest.raises(NotImplementedError, match="lsmr.py"):
+        lx.LSMR(1e-6, 1e-6).init(lx.MatrixLinearOperator(torch.eye(2)), {})
+
+
+def test_solvers_refuse_host_tensors():
+    # no CPU fallback: a host operator is an error, not a silent CPU solve
+    with pytest.raises(ValueError, match="GPU"):
         lx.SVD().init(lx.MatrixLinearOperator(torch.eye(2)), {})
 
 

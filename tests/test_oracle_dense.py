@@ -248,3 +248,27 @@ def test_normal_solve_pins():
     a = np.array([[2.0, 1.0], [0.0, 3.0]])
     b = np.array([5.0, 6.0])
     np.testing.assert_allclose(O.normal_solve(a, b), [1.5, 2.0], rtol=1e-13)
+
+
+def test_svd_solve_pins():
+    rng = np.random.default_rng(11)
+    # full rank, tall and wide: lstsq / minimum norm
+    for shape in ((20, 7), (7, 20), (9, 9)):
+        a = rng.standard_normal(shape)
+        b = rng.standard_normal(shape[0])
+        x, rank = O.svd_solve(a, b)
+        assert rank == min(shape)
+        np.testing.assert_allclose(x, np.linalg.lstsq(a, b, rcond=None)[0], rtol=1e-10, atol=1e-12)
+    # rank deficient: the pseudoinverse solution (numpy.linalg.pinv)
+    a = rng.standard_normal((12, 3)) @ rng.standard_normal((3, 8))
+    b = rng.standard_normal(12)
+    x, rank = O.svd_solve(a, b)
+    assert rank == 3
+    np.testing.assert_allclose(x, np.linalg.pinv(a) @ b, rtol=1e-8, atol=1e-10)
+    # the zero matrix has rank 0 and solution 0 (the strict > cutoff)
+    x, rank = O.svd_solve(np.zeros((3, 4)), np.ones(3))
+    assert rank == 0 and np.all(x == 0)
+    # the reference's non-square KATs (tests/test_singular.py:100-145)
+    a1 = np.array([[1.0, 5.0, -1.0], [-2.0, -2.0, 3.0]])
+    np.testing.assert_allclose(O.svd_solve(a1, [3.0, 4.0])[0], np.linalg.lstsq(a1, [3.0, 4.0], rcond=None)[0],
+                               rtol=1e-12, atol=1e-12)
