@@ -240,6 +240,48 @@ int lxa_cg_finish(void* stream, int vdtype, int64_t rows, int64_t n, const void*
 int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state);
 
 /* ---------------------------------------------------------------------------
+ * LSMR (lineax/_solver/lsmr.py:92-362, damp 0, two_norm) on device-resident
+ * per-row state, so rows stop independently.  u and v are stored unscaled:
+ * the bidiagonalisation's unit vectors are u / beta and v / alpha (beta,
+ * alpha of the row state; a zero norm divides by 1, as lsmr.py does).
+ * Sequence: [caller: t = A x0] begin (u = b - t, beta, |b|), [caller:
+ * z = A^T u] init (v, alpha, the recurrence's initial scalars, h = v /
+ * alpha, hbar = 0); per iteration: [caller: w = A v] step_u (u = w / alpha
+ * - alpha u / beta, the new beta), [caller: z = A^T u] step_v (when beta is
+ * nonzero v = z / beta - beta v / alpha and the new alpha; the Givens
+ * recurrence of lsmr.py:229-306 and the estimates of |r|, |A|, cond(A),
+ * |A^T r|), step_x (hbar, x, h updates and |x|, then the stopping rules of
+ * lsmr.py:316-325); finally result (lsmr.py:347-362).  Partials: as
+ * lxa_krylov_partials_size(rows, max(m, n), 0).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  double alpha, beta;
+  double zetabar, alphabar, rho, rhobar, cbar, sbar;
+  double betadd, betad, rhodold, tautildeold, thetatilde, zeta, delta;
+  double normA2, maxrbar, minrbar, condA, normr, normAr, normb, normx, normA;
+  double c_hbar, c_x, c_h; /* coefficients of this step's vector updates */
+  int32_t itn, istop, active, result;
+} lxa_lsmr_row; /* 232 bytes */
+
+typedef struct {
+  double rtol, atol, conlim;
+  int64_t max_steps; /* min(m, n) * 10 when max_steps_given == 0 */
+  int32_t max_steps_given, pad;
+} lxa_lsmr_params;
+
+int lxa_lsmr_begin(void* stream, int ydtype, int vdtype, int64_t rows, int64_t m, const void* b, const void* t,
+                   void* u, const lxa_lsmr_params* prm, lxa_lsmr_row* state, double* partials);
+int lxa_lsmr_init(void* stream, int vdtype, int64_t rows, int64_t n, const void* z, void* v, void* h, void* hbar,
+                  const lxa_lsmr_params* prm, lxa_lsmr_row* state, double* partials);
+int lxa_lsmr_step_u(void* stream, int vdtype, int64_t rows, int64_t m, const void* w, void* u,
+                    const lxa_lsmr_params* prm, lxa_lsmr_row* state, double* partials);
+int lxa_lsmr_step_v(void* stream, int vdtype, int64_t rows, int64_t n, const void* z, void* v,
+                    const lxa_lsmr_params* prm, lxa_lsmr_row* state, double* partials);
+int lxa_lsmr_step_x(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* v, void* h,
+                    void* hbar, void* x, const lxa_lsmr_params* prm, lxa_lsmr_row* state, double* partials);
+int lxa_lsmr_result(void* stream, int64_t rows, const lxa_lsmr_params* prm, lxa_lsmr_row* state);
+
+/* ---------------------------------------------------------------------------
  * Thin singular value decomposition of a batch of dense matrices, A = U
  * diag(s) V^T with s descending (jax.scipy.linalg.svd(full_matrices=False)
  * in lineax/_solver/svd.py:43), by one-sided Jacobi rotations: one

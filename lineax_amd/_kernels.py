@@ -62,6 +62,12 @@ _SIGNATURES = {
     "lxa_bicgstab_omega": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "lxa_bicgstab_update": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "lxa_bicgstab_result": [_vp, _i64, _vp, _vp],
+    "lxa_lsmr_begin": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lxa_lsmr_init": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lxa_lsmr_step_u": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "lxa_lsmr_step_v": [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "lxa_lsmr_step_x": [_vp, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lxa_lsmr_result": [_vp, _i64, _vp, _vp],
     "lxa_cg_stencil_workspace": [_i64, _i64, _i64, _szp],
     "lxa_gemv_uneven": [_vp, _i64, _vp, _int, _int, _int],
     "lxa_svd_workspace": [_int, _i64, _i64, _i64, _szp],
@@ -508,6 +514,44 @@ class RowState:
         return self.i32[:, 14]
 
 
+class LsmrParams(ctypes.Structure):
+    _fields_ = [
+        ("rtol", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("conlim", ctypes.c_double),
+        ("max_steps", ctypes.c_int64),
+        ("max_steps_given", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+# lxa_lsmr_row: 27 doubles, then itn, istop, active, result as int32
+LSMR_ROW_BYTES = 232
+LSMR_F64 = {name: i for i, name in enumerate(
+    "alpha beta zetabar alphabar rho rhobar cbar sbar betadd betad rhodold tautildeold thetatilde zeta delta "
+    "normA2 maxrbar minrbar condA normr normAr normb normx normA c_hbar c_x c_h".split())}
+LSMR_I32 = {"itn": 54, "istop": 55, "active": 56, "result": 57}
+
+
+class LsmrState:
+    """Device array of per-row LSMR state (lxa_lsmr_row)."""
+
+    def __init__(self, rows: int, device):
+        self.raw = torch.zeros((rows, LSMR_ROW_BYTES), dtype=torch.uint8, device=device)
+        self.f64 = self.raw.view(torch.float64)  # [rows, 29]
+        self.i32 = self.raw.view(torch.int32)  # [rows, 58]
+
+    @property
+    def ptr(self):
+        return self.raw.data_ptr()
+
+    def scalar(self, name):
+        return self.f64[:, LSMR_F64[name]]
+
+    def int_field(self, name):
+        return self.i32[:, LSMR_I32[name]]
+
+
 def partials_buffer(rows: int, n: int, restart: int, device) -> torch.Tensor:
     need = ctypes.c_size_t(0)
     _check(load().lxa_krylov_partials_size(rows, n, restart, ctypes.byref(need)), "lxa_krylov_partials_size")
@@ -545,9 +589,9 @@ def call(name: str, *args) -> None:
     for a in args:
         if isinstance(a, torch.Tensor):
             conv.append(a.data_ptr())
-        elif isinstance(a, RowState):
+        elif isinstance(a, (RowState, LsmrState)):
             conv.append(a.ptr)
-        elif isinstance(a, KrylovParams):
+        elif isinstance(a, (KrylovParams, LsmrParams)):
             conv.append(ctypes.cast(ctypes.pointer(a), ctypes.c_void_p))
         else:
             conv.append(a)

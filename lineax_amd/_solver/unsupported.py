@@ -43,7 +43,3 @@ class Tridiagonal(_Unsupported):
 
     def assume_full_rank(self):
         return True
-
-
-class LSMR(_Unsupported):
-    _reference = "lineax/_solver/lsmr.py"

@@ -531,6 +531,328 @@ __global__ void bcg_result_kernel(int64_t rows, CgParams prm, double threshold, 
 }
 
 // ---------------------------------------------------------------------------
+// LSMR (lineax/_solver/lsmr.py:92-362), damp 0, two_norm.  Vectors of a row
+// are stored unscaled; u / sel(beta) and v / sel(alpha) are the unit
+// vectors of the bidiagonalisation (sel(z) = z, or 1 when z is 0, as
+// lax.select in lsmr.py:145,162,207).
+// ---------------------------------------------------------------------------
+
+struct LsParams {
+  double rtol, atol, conlim;
+  int64_t max_steps;
+  int has_scale, max_steps_given;
+};
+
+__device__ __forceinline__ double sel1(double z) { return z == 0.0 ? 1.0 : z; }
+__device__ __forceinline__ double sgn(double z) { return z > 0.0 ? 1.0 : (z < 0.0 ? -1.0 : 0.0); }
+__device__ __forceinline__ double nanmin(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a < b ? a : b;
+}
+
+// c, s, r with [c -s; s c] [a; b] = [r; 0] (lsmr.py:373-415)
+__device__ void ls_givens(double a, double b, double& c, double& s, double& r) {
+  if (a == 0.0 || b == 0.0) {
+    if (b == 0.0) {
+      c = sgn(a);
+      s = 0.0;
+      r = fabs(a);
+    } else {
+      c = 0.0;
+      s = sgn(b);
+      r = fabs(b);
+    }
+  } else if (fabs(b) > fabs(a)) {
+    const double tau = a / b;
+    s = sgn(b) / sqrt(1.0 + tau * tau);
+    c = s * tau;
+    r = b / s;
+  } else {
+    const double tau = b / a;
+    c = sgn(a) / sqrt(1.0 + tau * tau);
+    s = c * tau;
+    r = a / c;
+  }
+}
+
+__device__ __forceinline__ bool ls_active(const lxa_lsmr_row* st) { return st[blockIdx.y].active != 0; }
+
+// u = b - t; partials [row][2][nblk]: sum u^2, sum b^2.
+template <typename TY, typename TV>
+__global__ __launch_bounds__(kB) void ls_begin_kernel(int64_t m, const TY* __restrict__ b, const TY* __restrict__ t,
+                                                      TV* __restrict__ u, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(m);
+  const int64_t o = row * m;
+  double uu = 0.0, bb = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double bv = static_cast<double>(b[o + i]);
+    const TV uv = static_cast<TV>(bv - static_cast<double>(t[o + i]));
+    u[o + i] = uv;
+    uu = fma(static_cast<double>(uv), static_cast<double>(uv), uu);
+    bb = fma(bv, bv, bb);
+  }
+  uu = block_sum(uu, red);
+  bb = block_sum(bb, red);
+  if (threadIdx.x == 0) {
+    part[(row * 2) * gridDim.x + blockIdx.x] = uu;
+    part[(row * 2 + 1) * gridDim.x + blockIdx.x] = bb;
+  }
+}
+
+__global__ void ls_begin_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part,
+                                       lxa_lsmr_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  const double* pr = part + row * 2 * nblk;
+  lxa_lsmr_row s{};
+  s.beta = sqrt(sum_partials(pr, nblk));
+  s.normb = sqrt(sum_partials(pr + nblk, nblk));
+  s.active = 1;
+  st[row] = s;
+}
+
+// v = z / beta (beta nonzero) or 0; partials [row][nblk]: sum v^2.
+template <typename TV>
+__global__ __launch_bounds__(kB) void ls_v0_kernel(int64_t n, const TV* __restrict__ z, TV* __restrict__ v,
+                                                   const lxa_lsmr_row* __restrict__ st, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const double beta = st[row].beta;
+  const int64_t o = row * n;
+  double vv = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const TV x = beta != 0.0 ? static_cast<TV>(static_cast<double>(z[o + i]) / beta) : TV(0);
+    v[o + i] = x;
+    vv = fma(static_cast<double>(x), static_cast<double>(x), vv);
+  }
+  vv = block_sum(vv, red);
+  if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = vv;
+}
+
+// alpha and the loop state of lsmr.py:165-187.
+__global__ void ls_init_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part,
+                                      lxa_lsmr_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  lxa_lsmr_row s = st[row];
+  const double alpha = s.beta != 0.0 ? sqrt(sum_partials(part + row * nblk, nblk)) : 0.0;
+  s.alpha = alpha;
+  s.zetabar = alpha * s.beta;
+  s.alphabar = alpha;
+  s.rho = 1.0;
+  s.rhobar = 1.0;
+  s.cbar = 1.0;
+  s.sbar = 0.0;
+  s.betadd = s.beta;
+  s.betad = 0.0;
+  s.rhodold = 1.0;
+  s.tautildeold = 0.0;
+  s.thetatilde = 0.0;
+  s.zeta = 0.0;
+  s.delta = 0.0;
+  s.normA2 = alpha * alpha;
+  s.maxrbar = 0.0;
+  s.minrbar = 1.79769313486231570815e308;  // the largest double (jnp.finfo(dtype).max in float64)
+  s.condA = 1.0;
+  s.normr = s.beta;
+  s.normAr = alpha * s.beta;
+  s.normx = 0.0;
+  s.itn = 0;
+  s.istop = 0;
+  if (alpha == 0.0) s.istop = 2;  // x0 solves the least-squares problem
+  if (s.beta == 0.0) s.istop = 1;  // x0 solves A x = b
+  s.active = s.istop == 0;
+  st[row] = s;
+}
+
+// h = v / alpha, hbar = 0.
+template <typename TV>
+__global__ __launch_bounds__(kB) void ls_h0_kernel(int64_t n, const TV* __restrict__ v, TV* __restrict__ h,
+                                                   TV* __restrict__ hbar, const lxa_lsmr_row* __restrict__ st) {
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const double ia = 1.0 / sel1(st[row].alpha);
+  const int64_t o = row * n;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    h[o + i] = static_cast<TV>(static_cast<double>(v[o + i]) * ia);
+    hbar[o + i] = TV(0);
+  }
+}
+
+// u = w / alpha - alpha u / beta (w = A v unscaled; lsmr.py:198-200);
+// partials [row][nblk]: sum u^2.
+template <typename TV>
+__global__ __launch_bounds__(kB) void ls_u_kernel(int64_t m, const TV* __restrict__ w, TV* __restrict__ u,
+                                                  const lxa_lsmr_row* __restrict__ st, double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!ls_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(m);
+  const double alpha = st[row].alpha;
+  const double ia = 1.0 / sel1(alpha), ab = alpha / sel1(st[row].beta);
+  const int64_t o = row * m;
+  double uu = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const TV x = static_cast<TV>(static_cast<double>(w[o + i]) * ia - ab * static_cast<double>(u[o + i]));
+    u[o + i] = x;
+    uu = fma(static_cast<double>(x), static_cast<double>(x), uu);
+  }
+  uu = block_sum(uu, red);
+  if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = uu;
+}
+
+__global__ void ls_u_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part,
+                                   lxa_lsmr_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  st[row].beta = sqrt(sum_partials(part + row * nblk, nblk));
+}
+
+// When beta is nonzero: v = z / beta - beta v / alpha (z = A^T u unscaled;
+// lsmr.py:202-209); partials [row][nblk]: sum v^2.
+template <typename TV>
+__global__ __launch_bounds__(kB) void ls_v_kernel(int64_t n, const TV* __restrict__ z, TV* __restrict__ v,
+                                                  const lxa_lsmr_row* __restrict__ st, double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!ls_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const double beta = st[row].beta;
+  if (beta == 0.0) return;  // lsmr.py:211-212: u, v and alpha unchanged (uniform per row)
+  const Chunk c = chunk_of(n);
+  const double ib = 1.0 / beta, ba = beta / sel1(st[row].alpha);
+  const int64_t o = row * n;
+  double vv = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const TV x = static_cast<TV>(static_cast<double>(z[o + i]) * ib - ba * static_cast<double>(v[o + i]));
+    v[o + i] = x;
+    vv = fma(static_cast<double>(x), static_cast<double>(x), vv);
+  }
+  vv = block_sum(vv, red);
+  if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = vv;
+}
+
+// The new alpha, then the rotations and estimates of lsmr.py:224-306.
+__global__ void ls_v_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part,
+                                   lxa_lsmr_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  lxa_lsmr_row s = st[row];
+  s.itn += 1;
+  if (s.beta != 0.0) s.alpha = sqrt(sum_partials(part + row * nblk, nblk));
+  const double alpha = s.alpha, beta = s.beta;
+  // Qhat_{k,2k+1} (damp = 0), then Q_i turning B_i into R_i
+  double chat, shat, alphahat;
+  ls_givens(s.alphabar, 0.0, chat, shat, alphahat);
+  const double rhoold = s.rho;
+  double c, sn, rho;
+  ls_givens(alphahat, beta, c, sn, rho);
+  s.rho = rho;
+  const double thetanew = sn * alpha;
+  s.alphabar = c * alpha;
+  // Qbar_i turning R_i^T into R_i^bar
+  const double rhobarold = s.rhobar, zetaold = s.zeta;
+  const double thetabar = s.sbar * rho, rhotemp = s.cbar * rho;
+  double cbar, sbar, rhobar;
+  ls_givens(s.cbar * rho, thetanew, cbar, sbar, rhobar);
+  s.cbar = cbar;
+  s.sbar = sbar;
+  s.rhobar = rhobar;
+  s.zeta = cbar * s.zetabar;
+  s.zetabar = -sbar * s.zetabar;
+  // coefficients of the h, hbar and x updates (lsmr.py:253-262)
+  s.c_hbar = -(thetabar * rho / (rhoold * rhobarold));
+  s.c_x = s.zeta / (rho * rhobar);
+  s.c_h = -(thetanew / rho);
+  // estimate of |r| (lsmr.py:264-292)
+  const double betaacute = chat * s.betadd, betacheck = -shat * s.betadd;
+  const double betahat = c * betaacute;
+  s.betadd = -sn * betaacute;
+  const double thetatildeold = s.thetatilde;
+  double ctildeold, stildeold, rhotildeold;
+  ls_givens(s.rhodold, thetabar, ctildeold, stildeold, rhotildeold);
+  s.thetatilde = stildeold * rhobar;
+  s.rhodold = ctildeold * rhobar;
+  s.betad = -stildeold * s.betad + ctildeold * betahat;
+  s.tautildeold = (zetaold - thetatildeold * s.tautildeold) / rhotildeold;
+  const double taud = (s.zeta - s.thetatilde * s.tautildeold) / s.rhodold;
+  s.delta = s.delta + betacheck * betacheck;
+  s.normr = sqrt(s.delta + (s.betad - taud) * (s.betad - taud) + s.betadd * s.betadd);
+  // |A| and cond(A) (lsmr.py:294-306)
+  s.normA2 = s.normA2 + beta * beta;
+  s.normA = sqrt(s.normA2);
+  s.normA2 = s.normA2 + alpha * alpha;
+  s.maxrbar = nanmax(s.maxrbar, rhobarold);
+  if (s.itn > 1) s.minrbar = nanmin(s.minrbar, rhobarold);
+  s.condA = nanmax(s.maxrbar, rhotemp) / nanmin(s.minrbar, rhotemp);
+  s.normAr = fabs(s.zetabar);
+  st[row] = s;
+}
+
+// hbar = h + c_hbar hbar; x += c_x hbar; h = v / alpha + c_h h; partials
+// [row][nblk]: sum x^2.
+template <typename TY, typename TV>
+__global__ __launch_bounds__(kB) void ls_x_kernel(int64_t n, const TV* __restrict__ v, TV* __restrict__ h,
+                                                  TV* __restrict__ hbar, TY* __restrict__ x,
+                                                  const lxa_lsmr_row* __restrict__ st, double* __restrict__ part) {
+  __shared__ double red[4];
+  if (!ls_active(st)) return;
+  const int64_t row = blockIdx.y;
+  const Chunk c = chunk_of(n);
+  const lxa_lsmr_row& s = st[row];
+  const double chb = s.c_hbar, cx = s.c_x, ch = s.c_h, ia = 1.0 / sel1(s.alpha);
+  const int64_t o = row * n;
+  double xx = 0.0;
+  for (int64_t i = c.begin + threadIdx.x; i < c.end; i += kB) {
+    const double hv = static_cast<double>(h[o + i]);
+    const TV hb = static_cast<TV>(hv + chb * static_cast<double>(hbar[o + i]));
+    hbar[o + i] = hb;
+    const TY xn = static_cast<TY>(static_cast<double>(x[o + i]) + cx * static_cast<double>(hb));
+    x[o + i] = xn;
+    h[o + i] = static_cast<TV>(static_cast<double>(v[o + i]) * ia + ch * hv);
+    xx = fma(static_cast<double>(xn), static_cast<double>(xn), xx);
+  }
+  xx = block_sum(xx, red);
+  if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = xx;
+}
+
+// |x| and the stopping rules (lsmr.py:309-325).
+__global__ void ls_x_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part, LsParams prm,
+                                   lxa_lsmr_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows || !st[row].active) return;
+  lxa_lsmr_row s = st[row];
+  s.normx = sqrt(sum_partials(part + row * nblk, nblk));
+  const double well_posed_tol = prm.atol + prm.rtol * (s.normA * s.normx + s.normb);
+  const double least_squares_tol = prm.atol + prm.rtol * (s.normA * s.normr);
+  if (s.itn >= prm.max_steps) s.istop = 4;
+  if (s.condA > prm.conlim) s.istop = 3;
+  if (s.normAr < least_squares_tol) s.istop = 2;
+  if (s.normr < well_posed_tol) s.istop = 1;
+  s.active = s.istop == 0;
+  st[row] = s;
+}
+
+__global__ void ls_result_kernel(int64_t rows, LsParams prm, lxa_lsmr_row* __restrict__ st) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  const lxa_lsmr_row s = st[row];
+  int res = LXA_RESULT_SUCCESSFUL;
+  const bool capped = s.itn == prm.max_steps;
+  if (!prm.max_steps_given) {
+    if (capped) res = LXA_RESULT_SINGULAR;
+  } else if (prm.has_scale) {
+    if (capped) res = LXA_RESULT_MAX_STEPS_REACHED;
+  }
+  if (s.istop < 3) res = LXA_RESULT_SUCCESSFUL;
+  if (s.istop == 3) res = LXA_RESULT_CONLIM;
+  st[row].result = res;
+}
+
+// ---------------------------------------------------------------------------
 // GMRES (lineax/_solver/gmres.py:106-413)
 // ---------------------------------------------------------------------------
 
@@ -1294,6 +1616,126 @@ extern "C" int lxa_bicgstab_result(void* stream, int64_t rows, const lxa_krylov_
   hipLaunchKernelGGL(lxa::bcg_result_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, cg_params(prm), prm->eps,
                      state);
   LXA_LAUNCH_CHECK("lxa_bicgstab_result");
+  return LXA_OK;
+}
+
+// ----------------------------------------------------------------------------- LSMR
+
+static_assert(sizeof(lxa_lsmr_row) == 232, "lxa_lsmr_row layout is part of the C ABI");
+static_assert(sizeof(lxa_lsmr_params) == 40, "lxa_lsmr_params layout is part of the C ABI");
+
+static lxa::LsParams ls_params(const lxa_lsmr_params* p) {
+  lxa::LsParams l;
+  l.rtol = p->rtol;
+  l.atol = p->atol;
+  l.conlim = p->conlim;
+  l.max_steps = p->max_steps;
+  l.has_scale = !(p->rtol == 0.0 && p->atol == 0.0);
+  l.max_steps_given = p->max_steps_given;
+  return l;
+}
+
+static int check_lsmr(int64_t rows, int64_t len, const void* prm, const void* st, const void* part) {
+  LXA_REQUIRE(rows >= 1 && rows <= 65535, "LSMR: rows must be in [1, 65535]");
+  LXA_REQUIRE(len >= 1, "LSMR: vector length must be positive");
+  LXA_REQUIRE(prm && st && part, "LSMR: null pointer");
+  return LXA_OK;
+}
+
+#define LXA_DISPATCH_V(vdt, ...)                                                     \
+  do {                                                                               \
+    if ((vdt) == LXA_F64) {                                                          \
+      using TV = double;                                                             \
+      __VA_ARGS__;                                                                   \
+    } else if ((vdt) == LXA_F32) {                                                   \
+      using TV = float;                                                              \
+      __VA_ARGS__;                                                                   \
+    } else {                                                                         \
+      return ::lxa::set_error(LXA_EINVAL, "unsupported vector dtype %d", vdt);       \
+    }                                                                                \
+  } while (0)
+
+extern "C" int lxa_lsmr_begin(void* stream, int ydtype, int vdtype, int64_t rows, int64_t m, const void* b,
+                              const void* t, void* u, const lxa_lsmr_params* prm, lxa_lsmr_row* state,
+                              double* partials) {
+  if (int rc = check_lsmr(rows, m, prm, state, partials)) return rc;
+  LXA_REQUIRE(b && t && u, "lxa_lsmr_begin: null vector");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(m, rows);
+  LXA_DISPATCH_YV(ydtype, vdtype,
+                  hipLaunchKernelGGL((lxa::ls_begin_kernel<TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, m,
+                                     static_cast<const TY*>(b), static_cast<const TY*>(t), static_cast<TV*>(u),
+                                     partials));
+  hipLaunchKernelGGL(lxa::ls_begin_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials, state);
+  LXA_LAUNCH_CHECK("lxa_lsmr_begin");
+  return LXA_OK;
+}
+
+extern "C" int lxa_lsmr_init(void* stream, int vdtype, int64_t rows, int64_t n, const void* z, void* v, void* h,
+                             void* hbar, const lxa_lsmr_params* prm, lxa_lsmr_row* state, double* partials) {
+  if (int rc = check_lsmr(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(z && v && h && hbar, "lxa_lsmr_init: null vector");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  LXA_DISPATCH_V(vdtype, {
+    hipLaunchKernelGGL((lxa::ls_v0_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const TV*>(z), static_cast<TV*>(v), state, partials);
+    hipLaunchKernelGGL(lxa::ls_init_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials, state);
+    hipLaunchKernelGGL((lxa::ls_h0_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                       static_cast<const TV*>(v), static_cast<TV*>(h), static_cast<TV*>(hbar), state);
+  });
+  LXA_LAUNCH_CHECK("lxa_lsmr_init");
+  return LXA_OK;
+}
+
+extern "C" int lxa_lsmr_step_u(void* stream, int vdtype, int64_t rows, int64_t m, const void* w, void* u,
+                               const lxa_lsmr_params* prm, lxa_lsmr_row* state, double* partials) {
+  if (int rc = check_lsmr(rows, m, prm, state, partials)) return rc;
+  LXA_REQUIRE(w && u, "lxa_lsmr_step_u: null vector");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(m, rows);
+  LXA_DISPATCH_V(vdtype, hipLaunchKernelGGL((lxa::ls_u_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, m,
+                                            static_cast<const TV*>(w), static_cast<TV*>(u), state, partials));
+  hipLaunchKernelGGL(lxa::ls_u_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials, state);
+  LXA_LAUNCH_CHECK("lxa_lsmr_step_u");
+  return LXA_OK;
+}
+
+extern "C" int lxa_lsmr_step_v(void* stream, int vdtype, int64_t rows, int64_t n, const void* z, void* v,
+                               const lxa_lsmr_params* prm, lxa_lsmr_row* state, double* partials) {
+  if (int rc = check_lsmr(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(z && v, "lxa_lsmr_step_v: null vector");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  LXA_DISPATCH_V(vdtype, hipLaunchKernelGGL((lxa::ls_v_kernel<TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                                            static_cast<const TV*>(z), static_cast<TV*>(v), state, partials));
+  hipLaunchKernelGGL(lxa::ls_v_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials, state);
+  LXA_LAUNCH_CHECK("lxa_lsmr_step_v");
+  return LXA_OK;
+}
+
+extern "C" int lxa_lsmr_step_x(void* stream, int ydtype, int vdtype, int64_t rows, int64_t n, const void* v, void* h,
+                               void* hbar, void* x, const lxa_lsmr_params* prm, lxa_lsmr_row* state,
+                               double* partials) {
+  if (int rc = check_lsmr(rows, n, prm, state, partials)) return rc;
+  LXA_REQUIRE(v && h && hbar && x, "lxa_lsmr_step_x: null vector");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = lxa::nblocks_for(n, rows);
+  LXA_DISPATCH_YV(ydtype, vdtype,
+                  hipLaunchKernelGGL((lxa::ls_x_kernel<TY, TV>), lxa::grid2(nb, rows), dim3(lxa::kB), 0, s, n,
+                                     static_cast<const TV*>(v), static_cast<TV*>(h), static_cast<TV*>(hbar),
+                                     static_cast<TY*>(x), state, partials));
+  hipLaunchKernelGGL(lxa::ls_x_finish_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, nb, partials,
+                     ls_params(prm), state);
+  LXA_LAUNCH_CHECK("lxa_lsmr_step_x");
+  return LXA_OK;
+}
+
+extern "C" int lxa_lsmr_result(void* stream, int64_t rows, const lxa_lsmr_params* prm, lxa_lsmr_row* state) {
+  LXA_REQUIRE(rows >= 1 && prm && state, "lxa_lsmr_result: bad argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(lxa::ls_result_kernel, lxa::grid_rows(rows), dim3(64), 0, s, rows, ls_params(prm), state);
+  LXA_LAUNCH_CHECK("lxa_lsmr_result");
   return LXA_OK;
 }
 

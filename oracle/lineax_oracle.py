@@ -436,6 +436,132 @@ def bicgstab(mv, b, rtol, atol, *, max_steps=None, y0=None, precond=None, norm=m
     return y, result, step
 
 
+# ---------------------------------------------------------------------------
+# LSMR (lineax/_solver/lsmr.py:92-396)
+# ---------------------------------------------------------------------------
+
+
+def _givens(a, b):
+    """lsmr.py:373-415: c, s, r with [c -s; s c] [a; b] = [r; 0]."""
+    if a == 0.0 or b == 0.0:
+        if b == 0.0:
+            return float(np.sign(a)), 0.0, abs(a)
+        return 0.0, float(np.sign(b)), abs(b)
+    if abs(b) > abs(a):
+        tau = a / b
+        s = np.sign(b) / np.sqrt(1.0 + tau * tau)
+        c = s * tau
+        return c, s, b / s
+    tau = b / a
+    c = np.sign(a) / np.sqrt(1.0 + tau * tau)
+    s = c * tau
+    return c, s, a / c
+
+
+def lsmr(mv, rmv, b, n, rtol, atol, *, max_steps=None, conlim=1e8, x0=None, stats=False):
+    """Returns (x, result, num_steps, istop), following lsmr.py line by line
+    (damp = 0, two_norm): initialisation 92-187, the loop body 189-333,
+    RESULTS 347-362.  ``mv``/``rmv`` apply A and A^T.  With ``stats`` a
+    fifth element holds the estimates of lsmr.py:335-343 (norm_r, norm_Ar,
+    norm_A = sqrt(normA2), cond_A)."""
+    b = np.asarray(b, np.float64)
+    m = b.size
+    nrm = lambda z: float(np.sqrt(np.dot(z, z)))
+    has_scale = not (atol == 0 and rtol == 0)
+    steps_cap = min(m, n) * 10 if max_steps is None else max_steps
+    x = np.zeros(n) if x0 is None else np.asarray(x0, np.float64).copy()
+    u = b - mv(x)
+    normb = nrm(b)
+    beta = nrm(u)
+    if beta != 0.0:
+        u = u / beta
+        v = rmv(u)
+        alpha = nrm(v)
+    else:
+        v = np.zeros(n)
+        alpha = 0.0
+    v = v / (1.0 if alpha == 0.0 else alpha)
+    h = v.copy()
+    hbar = np.zeros(n)
+    itn = 0
+    zetabar, alphabar, rho, rhobar, cbar, sbar = alpha * beta, alpha, 1.0, 1.0, 1.0, 0.0
+    betadd, betad, rhodold, tautildeold, thetatilde, zeta, delta = beta, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0
+    normA2, maxrbar, minrbar, condA = alpha ** 2, 0.0, np.finfo(np.float64).max, 1.0
+    istop, normr, normAr = 0, beta, alpha * beta
+    if alpha == 0.0:
+        istop = 2
+    if beta == 0.0:
+        istop = 1
+    while istop == 0:
+        itn += 1
+        u = -alpha * u + mv(v)
+        beta = nrm(u)
+        if beta != 0.0:
+            u = u / beta
+            v = -beta * v + rmv(u)
+            alpha = nrm(v)
+            v = v / (1.0 if alpha == 0.0 else alpha)
+        chat, shat, alphahat = _givens(alphabar, 0.0)
+        rhoold = rho
+        c, s, rho = _givens(alphahat, beta)
+        thetanew = s * alpha
+        alphabar = c * alpha
+        rhobarold = rhobar
+        zetaold = zeta
+        thetabar = sbar * rho
+        rhotemp = cbar * rho
+        cbar, sbar, rhobar = _givens(cbar * rho, thetanew)
+        zeta = cbar * zetabar
+        zetabar = -sbar * zetabar
+        hbar = hbar * -(thetabar * rho / (rhoold * rhobarold)) + h
+        x = x + (zeta / (rho * rhobar)) * hbar
+        h = h * -(thetanew / rho) + v
+        betaacute = chat * betadd
+        betacheck = -shat * betadd
+        betahat = c * betaacute
+        betadd = -s * betaacute
+        thetatildeold = thetatilde
+        ctildeold, stildeold, rhotildeold = _givens(rhodold, thetabar)
+        thetatilde = stildeold * rhobar
+        rhodold = ctildeold * rhobar
+        betad = -stildeold * betad + ctildeold * betahat
+        tautildeold = (zetaold - thetatildeold * tautildeold) / rhotildeold
+        taud = (zeta - thetatilde * tautildeold) / rhodold
+        delta = delta + betacheck ** 2
+        normr = np.sqrt(delta + (betad - taud) ** 2 + betadd ** 2)
+        normA2 = normA2 + beta ** 2
+        normA = np.sqrt(normA2)
+        normA2 = normA2 + alpha ** 2
+        maxrbar = max(maxrbar, rhobarold)
+        if itn > 1:
+            minrbar = min(minrbar, rhobarold)
+        condA = max(maxrbar, rhotemp) / min(minrbar, rhotemp)
+        normAr = abs(zetabar)
+        normx = nrm(x)
+        if itn >= steps_cap:
+            istop = 4
+        if condA > conlim:
+            istop = 3
+        if normAr < atol + rtol * (normA * normr):
+            istop = 2
+        if normr < atol + rtol * (normA * normx + normb):
+            istop = 1
+    if max_steps is None:
+        result = SINGULAR if itn == steps_cap else SUCCESSFUL
+    elif has_scale:
+        result = MAX_STEPS_REACHED if itn == steps_cap else SUCCESSFUL
+    else:
+        result = SUCCESSFUL
+    if istop < 3:
+        result = SUCCESSFUL
+    if istop == 3:
+        result = CONLIM
+    if stats:
+        return x, result, itn, istop, {"norm_r": normr, "norm_Ar": normAr, "norm_A": float(np.sqrt(normA2)),
+                                       "cond_A": condA}
+    return x, result, itn, istop
+
+
 def _normalise(x, eps=None):
     nrm = two_norm(x)
     if eps is None:

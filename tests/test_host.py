@@ -109,8 +109,8 @@ def test_identity_operator_short_circuit():
 
 
 def test_unsupported_solvers_raise():
-    with pytest.raises(NotImplementedError, match="lsmr.py"):
-        lx.LSMR(1e-6, 1e-6).init(lx.MatrixLinearOperator(torch.eye(2)), {})
+    with pytest.raises(NotImplementedError, match="tridiagonal.py"):
+        lx.Tridiagonal().init(lx.MatrixLinearOperator(torch.eye(2), lx.tridiagonal_tag), {})
 
 
 def test_solvers_refuse_host_tensors():

@@ -242,3 +242,72 @@ def test_bicgstab_planted_sign_error_fails(rng):
     # a wrong operator sign models a dropped minus in the residual update
     x_bad, _, _ = O.bicgstab(lambda v: -(a @ v), b, 1e-12, 1e-12, max_steps=60)
     assert not np.allclose(x_bad, np.linalg.solve(a, b), rtol=1e-3)
+
+
+# --- LSMR (lineax/_solver/lsmr.py)
+
+
+def test_lsmr_least_squares_and_min_norm(rng):
+    for shape in ((40, 15), (15, 40), (25, 25)):
+        a = rng.standard_normal(shape)
+        b = rng.standard_normal(shape[0])
+        x, result, steps, istop = O.lsmr(lambda v: a @ v, lambda u: a.T @ u, b, shape[1], 1e-12, 1e-12)
+        assert result == O.SUCCESSFUL and istop in (1, 2)
+        np.testing.assert_allclose(x, np.linalg.lstsq(a, b, rcond=None)[0], rtol=1e-7, atol=1e-8)
+
+
+def test_lsmr_agrees_with_scipy(rng):
+    a = rng.standard_normal((60, 30))
+    b = rng.standard_normal(60)
+    x, _, steps, _ = O.lsmr(lambda v: a @ v, lambda u: a.T @ u, b, 30, 1e-10, 1e-10)
+    res = spla.lsmr(a, b, atol=1e-10, btol=1e-10, maxiter=300)
+    np.testing.assert_allclose(x, res[0], rtol=1e-7, atol=1e-8)
+    assert abs(steps - res[2]) <= 2  # same recurrence; stopping tests differ slightly
+
+
+def test_lsmr_rank_deficient_gives_pseudoinverse(rng):
+    a = rng.standard_normal((20, 3)) @ rng.standard_normal((3, 12))
+    b = rng.standard_normal(20)
+    x, result, _, _ = O.lsmr(lambda v: a @ v, lambda u: a.T @ u, b, 12, 1e-12, 1e-12)
+    assert result == O.SUCCESSFUL
+    np.testing.assert_allclose(x, np.linalg.pinv(a) @ b, rtol=1e-6, atol=1e-8)
+
+
+def test_lsmr_rules():
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal((30, 30))
+    b = rng.standard_normal(30)
+    mv, rmv = (lambda v: a @ v), (lambda u: a.T @ u)
+    # b = 0: beta = 0, x0 solves it at once (istop 1, no iterations)
+    x, result, steps, istop = O.lsmr(mv, rmv, np.zeros(30), 30, 1e-8, 1e-8)
+    assert steps == 0 and istop == 1 and np.all(x == 0) and result == O.SUCCESSFUL
+    # max_steps with tolerances: istop 4 -> max_steps_reached
+    _, result, steps, istop = O.lsmr(mv, rmv, b, 30, 1e-14, 1e-14, max_steps=3)
+    assert steps == 3 and istop == 4 and result == O.MAX_STEPS_REACHED
+    # max_steps only: successful (tests/test_solve.py:174-194)
+    _, result, steps, _ = O.lsmr(mv, rmv, b, 30, 0.0, 0.0, max_steps=2)
+    assert steps == 2 and result == O.SUCCESSFUL
+    # an ill-conditioned operator with a small conlim: conlim
+    ill = np.diag(np.logspace(0, -9, 30))
+    _, result, _, istop = O.lsmr(lambda v: ill @ v, lambda u: ill.T @ u, b, 30, 1e-14, 1e-14, conlim=1e3)
+    assert istop == 3 and result == O.CONLIM
+
+
+def test_lsmr_estimates_pinned(rng):
+    # After a few steps (orthogonality intact) the estimates of lsmr.py:264-306
+    # are the true |r_k| and |A^T r_k| (Fong and Saunders 2011, sections 3-4)
+    # and agree with scipy's LSMR after the same number of steps; |A| is the
+    # Frobenius norm of the bidiagonal B_k, bounded by |A|_F.
+    a = rng.standard_normal((60, 30))
+    b = rng.standard_normal(60)
+    x, _, steps, istop, st = O.lsmr(lambda v: a @ v, lambda u: a.T @ u, b, 30, 0.0, 0.0, max_steps=6, stats=True)
+    assert steps == 6 and istop == 4
+    r = b - a @ x
+    np.testing.assert_allclose(st["norm_r"], np.linalg.norm(r), rtol=1e-10)
+    np.testing.assert_allclose(st["norm_Ar"], np.linalg.norm(a.T @ r), rtol=1e-8)
+    ref = spla.lsmr(a, b, atol=0.0, btol=0.0, conlim=0.0, maxiter=6)
+    assert ref[2] == 6
+    np.testing.assert_allclose(x, ref[0], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose([st["norm_r"], st["norm_Ar"], st["cond_A"]], [ref[3], ref[4], ref[6]], rtol=1e-10)
+    # scipy reports |A| before the last alpha^2 is added; lineax after it
+    assert ref[5] < st["norm_A"] <= np.linalg.norm(a)
