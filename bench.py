@@ -134,7 +134,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", default="uneven_cholesky",
-                    choices=["uneven_cholesky", "small_lu", "large_lu", "krylov_cg", "krylov_gmres", "serving"],
+                    choices=["uneven_cholesky", "small_lu", "large_lu", "krylov_cg", "krylov_gmres", "serving",
+                             "krylov_bicgstab", "lsmr", "svd", "triangular", "normal"],
                     help="default: the uneven-batch Cholesky configuration (the headline metric)")
     ap.add_argument("--n", type=int, default=16384, help="large_lu: order (8192 or 16384)")
     ap.add_argument("--precision", default="bf16x3", choices=["bf16", "fp16", "bf16x3", "fp32"],
@@ -173,6 +174,11 @@ def main():
         "krylov_cg": bench_krylov_cg,
         "krylov_gmres": bench_krylov_gmres,
         "serving": bench_serving,
+        "krylov_bicgstab": bench_krylov_bicgstab,
+        "lsmr": bench_lsmr,
+        "svd": bench_svd,
+        "triangular": bench_triangular,
+        "normal": bench_normal,
     }
     out = legs[args.workload](args)
     print(json.dumps(out))
@@ -641,6 +647,286 @@ def bench_serving(args):
                  {"p50_ms": c["p50_ms"], "p99_ms": c["p99_ms"], "chosen": c, "plain_form": p,
                   "speedup_vs_plain": c["solves_per_s"] / p["solves_per_s"] if p["solves_per_s"] else None})
 
+
+def _blas_threads():
+    return max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+
+
+def bench_krylov_bicgstab(args):
+    """BiCGStab on the GMRES operator (1024^2 upwind convection-diffusion, 16
+    right-hand sides): float64 iterate and b, float32 working vectors, FP64
+    inner products, graph-captured blocks, timed over a fixed number of
+    iterations (unpreconditioned it needs thousands to reach 1e-6)."""
+    import torch
+
+    import lineax_amd as lx
+    from lineax_amd.stencil import convection_diffusion2d
+    from oracle import lineax_oracle as O
+
+    dev, nx, rows = "cuda:0", 1024, 16
+    iters = args.max_iters or 300
+    op = convection_diffusion2d(nx, dtype=torch.float64, device=dev)
+    gen = torch.Generator(device="cpu").manual_seed(4)
+    b32 = torch.randn((rows, nx, nx), generator=gen, dtype=torch.float32)
+    b = b32.to(torch.float64).to(dev)
+
+    def make(steps):
+        return lx.BiCGStab(rtol=1e-6, atol=1e-6, max_steps=steps, working_dtype=torch.float32, graph=True)
+
+    lx.linear_solve(op, b, make(40), throw=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sol = lx.linear_solve(op, b, make(iters), throw=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    steps = int(sol.stats["num_steps"].max())
+    per_ms = wall * 1e3 / steps
+    # 23 vector passes per iteration, float32-equivalent (the direction
+    # update 4, two stencil applications 2 + 2, <r0, v> 2, s 3, <t, s> and
+    # <t, t> 2, the update of y and r with the next rho and both norms 8)
+    bytes_per_iter = 23 * 4 * nx * nx * rows
+    achieved = bytes_per_iter / (per_ms / 1e3) / 1e9
+    # residual of the returned iterate, float64 on the device, against |b|
+    r = b - op.mv(sol.value)
+    rel = float((torch.linalg.vector_norm(r.reshape(rows, -1), dim=1)
+                 / torch.linalg.vector_norm(b.reshape(rows, -1), dim=1)).max())
+    cpu = None
+    if not args.no_cpu_baseline:
+        coeffs = op.coeffs
+        b0 = b32[0].double().numpy()
+        mv = lambda v: O.stencil_apply(v.reshape(nx, nx), coeffs).ravel()
+        block = 10
+        c0 = time.perf_counter()
+        O.bicgstab(mv, b0.ravel(), 1e-6, 1e-6, max_steps=block)
+        cpu_per = (time.perf_counter() - c0) / block
+        cpu = {"value": 1.0 / cpu_per, "unit": "iterations x RHS / s", "cores": 1, "kind": "port",
+               "sample": f"{block} BiCGStab iterations on one {nx}^2 grid (oracle, float64, one thread)"}
+    return _line("iterations x right-hand sides per second", steps * rows / wall, "iterations x RHS / s",
+                 wall * 1e3, args,
+                 {"workload": "krylov_bicgstab", "grid": [nx, nx], "rhs": rows, "iterations": steps,
+                  "rtol": 1e-6, "atol": 1e-6, "seed": 4},
+                 "f32 working vectors, f64 iterate / inner products",
+                 {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                  "kernel": "one BiCGStab iteration, all launches (bcg_* + stencil5 + copies)",
+                  "algorithmic_bytes_per_iteration": bytes_per_iter},
+                 cpu,
+                 {"ms_per_iteration": per_ms, "max_relative_residual": rel,
+                  "results": sol.result.cpu().numpy().tolist()})
+
+
+def bench_lsmr(args):
+    """LSMR on one dense float32 least-squares problem, 32768 x 4096: N(0, 1/n)
+    entries with the columns scaled by 10^(-2 j / n) (singular values over
+    about two decades), to rtol = atol = 1e-6, graph-captured iteration
+    blocks.  A v and A^T u are the GEMV kernel on the same storage (no
+    transposed copy); they are the dominant kernels and are timed on their
+    own with HIP events for the roofline."""
+    import torch
+
+    import lineax_amd as lx
+    from lineax_amd import _kernels as K
+    from oracle import lineax_oracle as O
+
+    dev, m, n = "cuda:0", 32768, 4096
+    gen = torch.Generator(device="cpu").manual_seed(6)
+    scale = torch.logspace(0.0, -2.0, n, dtype=torch.float64).to(torch.float32)
+    a_h = torch.randn((m, n), generator=gen, dtype=torch.float32) * (scale / float(np.sqrt(n)))
+    b_h = torch.randn((m,), generator=gen, dtype=torch.float32)
+    a, b = a_h.to(dev), b_h.to(dev)
+    op = lx.MatrixLinearOperator(a)
+    solver = lx.LSMR(1e-6, 1e-6, graph=True)
+    res = {}
+
+    def step():
+        res["sol"] = lx.linear_solve(op, b, solver, throw=False)
+
+    ms, _ = _time_steps(step, args.steps, args.warmup)
+    t = float(np.mean(ms))
+    sol = res["sol"]
+    iters = int(sol.stats["num_steps"])
+    # per iteration: A v and A^T u read A (8mn); the vector passes of the
+    # GEMVs and the three LSMR kernels (u: 5m, v: 5n, x/h/hbar: 7n; 4 B each)
+    bytes_per_iter = 8 * m * n + 4 * (5 * m + 13 * n)
+    iter_gbs = bytes_per_iter * iters / (t / 1e3) / 1e9
+    # the two products alone, HIP events on the launch stream
+    v = torch.randn((n,), device=dev)
+    u = torch.randn((m,), device=dev)
+    kms, _ = _time_steps(lambda: (K.gemv(a, v), K.gemv(a.mT, u)), 20, 3)
+    kms = float(np.mean(kms))
+    achieved = 8 * m * n / (kms / 1e3) / 1e9
+    # least-squares optimality of the answer, float64 on the device
+    a64 = a.double()
+    x64 = sol.value.double()
+    r64 = b.double() - a64 @ x64
+    opt = float(torch.linalg.vector_norm(a64.T @ r64) / (torch.linalg.norm(a64) * torch.linalg.vector_norm(r64)))
+    del a64
+    cpu = None
+    threads = _blas_threads()
+    if not args.no_cpu_baseline:
+        from threadpoolctl import threadpool_limits
+
+        a_c, b_c = a_h.double().numpy(), b_h.double().numpy()
+        block = 10
+        with threadpool_limits(limits=threads):
+            c0 = time.perf_counter()
+            O.lsmr(lambda w: a_c @ w, lambda z: a_c.T @ z, b_c, n, 1e-6, 1e-6, max_steps=block)
+            per = (time.perf_counter() - c0) / block
+        cpu = {"value": 1.0 / (per * iters), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"{block} iterations of the oracle's LSMR (float64 numpy, BLAS threads) on the same "
+                         f"problem, time scaled to the {iters} iterations of the GPU solve"}
+    return _line("solves/s", 1e3 / t, "solves/s", t, args,
+                 {"workload": "lsmr", "m": m, "n": n, "rtol": 1e-6, "atol": 1e-6, "seed": 6,
+                  "column_scaling": "10^(-2 j / n)"}, "f32 (f64 scalars)",
+                 {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                  "kernel": "gemv_n4_kernel (A v) + gemv_t_part_kernel / gemv_t_reduce_kernel (A^T u), one read "
+                            "of A each",
+                  "ms_per_pair": kms, "algorithmic_bytes_per_pair": 8 * m * n},
+                 cpu,
+                 {"iterations": iters, "result": int(sol.result), "istop": int(sol.stats["istop"]),
+                  "ms_per_iteration": t / max(iters, 1), "iteration_gbs": iter_gbs,
+                  "iteration_frac": iter_gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_iteration": bytes_per_iter,
+                  "ls_optimality_AtR_over_A_R": opt})
+
+
+def bench_svd(args):
+    """SVD solves of 512 float32 least-squares problems of 96 x 64 (one-sided
+    Jacobi SVD, one workgroup per matrix, then the pseudoinverse product)."""
+    import torch
+
+    import lineax_amd as lx
+    from lineax_amd import _kernels as K
+    from oracle import lineax_oracle as O
+
+    dev, batch, m, n = "cuda:0", 512, 96, 64
+    rng = np.random.default_rng(8)
+    a_h = (rng.standard_normal((batch, m, n)) / np.sqrt(n)).astype(np.float32)
+    b_h = rng.standard_normal((batch, m)).astype(np.float32)
+    a, b = torch.as_tensor(a_h, device=dev), torch.as_tensor(b_h, device=dev)
+    op = lx.MatrixLinearOperator(a)
+    res = {}
+
+    def step():
+        res["sol"] = lx.linear_solve(op, b, lx.SVD(), throw=False)
+
+    ms, _ = _time_steps(step, args.steps, args.warmup)
+    t = float(np.mean(ms))
+    sweeps = K.svd(a)[3].double().cpu().numpy()
+    # per sweep n(n-1)/2 pairs, each 3 inner products over m (6m), the
+    # rotation of two G columns (6m) and two V columns (6n); then the solve
+    flops = float(np.sum(sweeps * (n * (n - 1) / 2) * (12 * m + 6 * n))) + batch * (4 * m * n + 2 * n * n)
+    tflops = flops / (t / 1e3) / 1e12
+    from threadpoolctl import threadpool_limits
+
+    sample = 64
+    with threadpool_limits(limits=1):
+        c0 = time.perf_counter()
+        refs = [O.svd_solve(a_h[i].astype(np.float64), b_h[i].astype(np.float64))[0] for i in range(sample)]
+        cpu_s = time.perf_counter() - c0
+    x = res["sol"].value.double().cpu().numpy()
+    err = max(O.relative_error(x[i], refs[i]) for i in range(sample))
+    return _line("solves/s", batch / (t / 1e3), "solves/s", t, args,
+                 {"workload": "svd", "batch": batch, "m": m, "n": n, "seed": 8}, "f32 (f64 inner products)",
+                 {"bound": "mfma", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                  "frac": tflops / FP32_PEAK_TFLOPS, "traffic": None,
+                  "kernel": "svd_jacobi_kernel<float> (vector ALU; one workgroup per matrix, latency-bound)",
+                  "algorithmic_flops_per_step": flops, "mean_sweeps": float(sweeps.mean())},
+                 {"value": sample / cpu_s, "unit": "solves/s", "cores": 1, "kind": "port",
+                  "sample": f"{sample} of the {batch} problems, float64 LAPACK SVD + pseudoinverse, one thread"},
+                 {"max_rel_err_vs_fp64_oracle": err})
+
+
+def bench_triangular(args):
+    """256 upper-triangular float32 solves of order 2048 (Triangular solver,
+    one workgroup per system, blocked substitution)."""
+    import torch
+
+    import lineax_amd as lx
+    from oracle import lineax_oracle as O
+
+    dev, batch, n = "cuda:0", 256, 2048
+    gen = torch.Generator(device=dev).manual_seed(9)
+    a = torch.triu(torch.randn((batch, n, n), generator=gen, dtype=torch.float32, device=dev), 1)
+    a *= 0.5 / float(np.sqrt(n))
+    a += torch.diag_embed(1.0 + torch.rand((batch, n), generator=gen, dtype=torch.float32, device=dev))
+    b = torch.randn((batch, n), generator=gen, dtype=torch.float32, device=dev)
+    op = lx.MatrixLinearOperator(a, lx.upper_triangular_tag)
+    res = {}
+
+    def step():
+        res["sol"] = lx.linear_solve(op, b, lx.Triangular(), throw=False)
+
+    ms, _ = _time_steps(step, args.steps, args.warmup)
+    t = float(np.mean(ms))
+    # the upper triangle once (2 n^2 B) and the vector in and out
+    per = 2 * n * n + 8 * n
+    achieved = batch * per / (t / 1e3) / 1e9
+    import scipy.linalg as sl
+    from threadpoolctl import threadpool_limits
+
+    sample = 32
+    u64 = a[:sample].double().cpu().numpy()
+    b64 = b[:sample].double().cpu().numpy()
+    with threadpool_limits(limits=1):
+        c0 = time.perf_counter()
+        refs = [sl.solve_triangular(u64[i], b64[i], lower=False) for i in range(sample)]
+        cpu_s = time.perf_counter() - c0
+    x = res["sol"].value[:sample].double().cpu().numpy()
+    err = max(O.relative_error(x[i], refs[i]) for i in range(sample))
+    return _line("solves/s", batch / (t / 1e3), "solves/s", t, args,
+                 {"workload": "triangular", "batch": batch, "n": n, "seed": 9}, "f32",
+                 {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "trsv_kernel (step)",
+                  "algorithmic_bytes_per_launch": batch * per},
+                 {"value": sample / cpu_s, "unit": "solves/s", "cores": 1, "kind": "reference",
+                  "sample": f"{sample} of the {batch} systems, float64 LAPACK trtrs (scipy solve_triangular), "
+                            "one thread"},
+                 {"max_rel_err_vs_fp64_lapack": err})
+
+
+def bench_normal(args):
+    """Normal(Cholesky()) on 64 float32 least-squares problems of 2048 x 256:
+    A^T A as one batched device GEMM, the Cholesky kernels, A^T b and the
+    two triangular solves."""
+    import torch
+
+    import lineax_amd as lx
+    from oracle import lineax_oracle as O
+
+    dev, batch, m, n = "cuda:0", 64, 2048, 256
+    rng = np.random.default_rng(10)
+    a_h = (rng.standard_normal((batch, m, n)) / np.sqrt(n)).astype(np.float32)
+    b_h = rng.standard_normal((batch, m)).astype(np.float32)
+    a, b = torch.as_tensor(a_h, device=dev), torch.as_tensor(b_h, device=dev)
+    op = lx.MatrixLinearOperator(a)
+    res = {}
+
+    def step():
+        res["sol"] = lx.linear_solve(op, b, lx.Normal(lx.Cholesky()), throw=False)
+
+    ms, _ = _time_steps(step, args.steps, args.warmup)
+    t = float(np.mean(ms))
+    flops = batch * (2.0 * m * n * n + n ** 3 / 3.0 + 2.0 * m * n + 2.0 * n * n)
+    tflops = flops / (t / 1e3) / 1e12
+    from threadpoolctl import threadpool_limits
+
+    sample = 16
+    with threadpool_limits(limits=1):
+        c0 = time.perf_counter()
+        refs = [O.normal_solve(a_h[i].astype(np.float64), b_h[i].astype(np.float64)) for i in range(sample)]
+        cpu_s = time.perf_counter() - c0
+    x = res["sol"].value.double().cpu().numpy()
+    err = max(O.relative_error(x[i], refs[i]) for i in range(sample))
+    return _line("solves/s", batch / (t / 1e3), "solves/s", t, args,
+                 {"workload": "normal", "batch": batch, "m": m, "n": n, "seed": 10}, "f32",
+                 {"bound": "mfma", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                  "frac": tflops / FP32_PEAK_TFLOPS, "traffic": None,
+                  "kernel": "step: Gram GEMM (library) + chol_factor/chol_solve kernels + gemv",
+                  "algorithmic_flops_per_step": flops},
+                 {"value": sample / cpu_s, "unit": "solves/s", "cores": 1, "kind": "port",
+                  "sample": f"{sample} of the {batch} problems, float64 normal equations (oracle), one thread"},
+                 {"max_rel_err_vs_fp64_oracle": err})
 
 if __name__ == "__main__":
     main()

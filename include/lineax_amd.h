@@ -134,13 +134,18 @@ int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n,
  * Operators and result mapping
  * ---------------------------------------------------------------------- */
 
-/* y = A x (trans = 0, A is [m, n]) or y = A^T x (trans = 1), batched.
- * Replaces jnp.matmul(self.matrix, vector, precision=HIGHEST) in
- * MatrixLinearOperator.mv, lineax/_operator.py:265-269. */
+/* y = A x (trans = 0, A is [m, n]) or y = A^T x (trans = 1), batched,
+ * FP64 accumulation.  Replaces jnp.matmul(self.matrix, vector,
+ * precision=HIGHEST) in MatrixLinearOperator.mv, lineax/_operator.py:265-269
+ * (trans = 1 serves the transposed operator, _operator.py:271-274, on the
+ * same storage).  trans = 1 splits the rows over workgroups and adds the
+ * per-chunk sums in a fixed order; it needs lxa_gemv_workspace_size bytes
+ * of device scratch (0 for trans = 0 and for small problems). */
+int lxa_gemv_workspace_size(int dtype, int64_t batch, int64_t m, int64_t n, int trans, size_t* bytes);
 int lxa_gemv(void* stream, int dtype, int64_t batch, int64_t m, int64_t n,
              const void* a, int64_t a_batch_stride, const void* x,
              int64_t x_batch_stride, void* y, int64_t y_batch_stride,
-             int trans);
+             int trans, void* work, size_t work_bytes);
 
 /* The RESULTS post-processing of lineax/_solve.py:104-123, per system:
  * result[i] == successful and any non-finite entry in solution row i

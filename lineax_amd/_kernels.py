@@ -44,7 +44,8 @@ _SIGNATURES = {
         _vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _int, _int, _dbl,
         _vp, _vp, _vp, _vp, ctypes.c_size_t,
     ],
-    "lxa_gemv": [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int],
+    "lxa_gemv_workspace_size": [_int, _i64, _i64, _i64, _int, _szp],
+    "lxa_gemv": [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int, _vp, ctypes.c_size_t],
     "lxa_solve_results": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp],
     "lxa_stencil5_apply": [_vp, _int, _int, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64],
     "lxa_krylov_partials_size": [_i64, _i64, _int, _szp],
@@ -363,6 +364,10 @@ def gemv(matrix: torch.Tensor, x: torch.Tensor, trans: bool = False) -> torch.Te
         raise TypeError(f"vector dtype {x.dtype} differs from matrix dtype {matrix.dtype}")
     dt = _dtype_code(matrix.dtype)
     m, n = matrix.shape[-2:]
+    if m > 1 and n > 1 and matrix.stride(-2) == 1 and matrix.stride(-1) == m:
+        # a transposed view (MatrixLinearOperator.transpose()): run the other
+        # product on the contiguous storage instead of copying the matrix
+        return gemv(matrix.mT, x, trans=not trans)
     kin, kout = (m, n) if trans else (n, m)
     if x.shape[-1] != kin:
         raise ValueError(f"vector of size {x.shape[-1]} does not match operator input size {kin}")
@@ -376,8 +381,16 @@ def gemv(matrix: torch.Tensor, x: torch.Tensor, trans: bool = False) -> torch.Te
     av, _, as_ = _flat_batch(matrix.expand(*lead, m, n), 2)
     xv, _, xs = _flat_batch(x.expand(*lead, kin), 1)
     lib = load()
+    work, wbytes = None, 0
+    if trans:
+        need = ctypes.c_size_t(0)
+        _check(lib.lxa_gemv_workspace_size(dt, batch, m, n, 1, ctypes.byref(need)), "lxa_gemv_workspace_size")
+        wbytes = int(need.value)
+        if wbytes:
+            work = torch.empty(wbytes // 8, dtype=torch.float64, device=matrix.device)
     _check(
-        lib.lxa_gemv(_stream(y), dt, batch, m, n, _ptr(av), as_, _ptr(xv), xs, _ptr(y), kout, int(bool(trans))),
+        lib.lxa_gemv(_stream(y), dt, batch, m, n, _ptr(av), as_, _ptr(xv), xs, _ptr(y), kout, int(bool(trans)),
+                     _ptr(work) if work is not None else None, wbytes),
         "lxa_gemv",
     )
     return y

@@ -9,7 +9,7 @@ from .._norm import two_norm
 from .._operator import conj, linearise
 from .._solve import AbstractLinearSolver
 from .._structure import batch_shape_of, ravel, tree_leaves, unravel
-from .cg import MAX_CHECK_BLOCK, FlatOperator
+from .cg import GRAPH_BLOCK, MAX_CHECK_BLOCK, FlatOperator
 
 
 class LSMR(AbstractLinearSolver):
@@ -27,10 +27,13 @@ class LSMR(AbstractLinearSolver):
     Only the two-norm is provided (lineax's default and the norm the
     recurrence estimates).  ``working_dtype`` (not in lineax) stores u, v, h
     and hbar in a narrower dtype than x; the scalars are float64 throughout.
+    ``graph`` (not in lineax) captures blocks of iterations in a HIP graph
+    and replays them, checking the rows' active flags on the host between
+    replays; the operator must then be capturable (device work only).
     """
 
     def __init__(self, rtol, atol, norm=two_norm, max_steps: int | None = None, conlim: float = 1e8,
-                 working_dtype: torch.dtype | None = None):
+                 working_dtype: torch.dtype | None = None, graph: bool = False):
         for value in (rtol, atol, conlim):
             if isinstance(value, (int, float)) and value < 0:
                 raise ValueError("Tolerances must be non-negative.")
@@ -44,6 +47,7 @@ class LSMR(AbstractLinearSolver):
         self.max_steps = max_steps
         self.conlim = conlim
         self.working_dtype = working_dtype
+        self.graph = graph
 
     def init(self, operator, options):
         return linearise(operator)
@@ -85,17 +89,33 @@ class LSMR(AbstractLinearSolver):
         # v = A^T u / beta, alpha = |v|, h = v / alpha, hbar = 0, istop for beta = 0 or alpha = 0
         K.call("lxa_lsmr_init", stream, vd, rows, n, At(u, vdt).contiguous(), v, h, hbar, prm, st, part)
 
+        def iteration():
+            stream = K.current_stream(b)  # the capture stream while a graph is recorded
+            K.call("lxa_lsmr_step_u", stream, vd, rows, m, A(v, vdt).contiguous(), u, prm, st, part)
+            K.call("lxa_lsmr_step_v", stream, vd, rows, n, At(u, vdt).contiguous(), v, prm, st, part)
+            K.call("lxa_lsmr_step_x", stream, yd, vd, rows, n, v, h, hbar, x, prm, st, part)
+
+        # Rows that have stopped skip every kernel (their active flag is
+        # read on the device), so whole blocks may run past a row's stop.
         it = 0
         block = 1
+        period = GRAPH_BLOCK if (self.graph and dev.type == "cuda") else 0
+        graph = None
         while it < max_steps and bool(st.int_field("active").any()):
-            for _ in range(block):
-                stream = K.current_stream(b)
-                K.call("lxa_lsmr_step_u", stream, vd, rows, m, A(v, vdt).contiguous(), u, prm, st, part)
-                K.call("lxa_lsmr_step_v", stream, vd, rows, n, At(u, vdt).contiguous(), v, prm, st, part)
-                K.call("lxa_lsmr_step_x", stream, yd, vd, rows, n, v, h, hbar, x, prm, st, part)
-                it += 1
-                if it >= max_steps:
-                    break
+            if graph is None and period and it >= period and it + period <= max_steps:
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    for _ in range(period):
+                        iteration()
+            if graph is not None and it + period <= max_steps:
+                graph.replay()
+                it += period
+            else:
+                for _ in range(block):
+                    iteration()
+                    it += 1
+                    if it >= max_steps or (period and it % period == 0):
+                        break
             block = min(block * 2, MAX_CHECK_BLOCK)
         K.call("lxa_lsmr_result", K.current_stream(b), rows, prm, st)
 

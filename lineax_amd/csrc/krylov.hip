@@ -102,16 +102,57 @@ __global__ __launch_bounds__(kB) void dot_partial_kernel(int64_t n, const TA* __
   if (threadIdx.x == 0) part[row * pstride + poff + blockIdx.x] = acc;
 }
 
+// Sum / NaN-propagating max of a row's block partials, in index order.  The
+// loads go out sixteen at a time (independent of the running sum), so the
+// one-thread-per-row finish kernels wait for nblk / 16 memory round trips,
+// not nblk; the order of the additions is unchanged.
 __device__ double sum_partials(const double* p, int nblk) {
   double s = 0.0;
-  for (int k = 0; k < nblk; ++k) s += p[k];
+  int k = 0;
+  for (; k + 16 <= nblk; k += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = p[k + u];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += v[u];
+  }
+  for (; k < nblk; ++k) s += p[k];
   return s;
 }
 
 __device__ double max_partials(const double* p, int nblk) {
   double s = 0.0;
-  for (int k = 0; k < nblk; ++k) s = nanmax(s, p[k]);
+  int k = 0;
+  for (; k + 16 <= nblk; k += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = p[k + u];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s = nanmax(s, v[u]);
+  }
+  for (; k < nblk; ++k) s = nanmax(s, p[k]);
   return s;
+}
+
+// Three reductions over a row's [3][nblk] partials, each in index order:
+// acc[q] accumulates with norm kind kinds[q] (0 NaN-propagating max, else
+// sum).  Loads go out eight entries of each at a time.
+__device__ void reduce3_partials(const double* pr, int nblk, const int (&kinds)[3], double (&acc)[3]) {
+  int k = 0;
+  for (; k + 8 <= nblk; k += 8) {
+    double v[3][8];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[q][u] = pr[q * nblk + k + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) acc[q] = norm_accum(kinds[q], acc[q], v[q][u]);
+  }
+  for (; k < nblk; ++k)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc[q] = norm_accum(kinds[q], acc[q], pr[q * nblk + k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -266,12 +307,9 @@ __global__ void cg_finish_kernel(int64_t rows, int nblk, int64_t n, const double
   if (row >= rows || !st[row].active) return;
   const double* pr = part + row * 3 * nblk;
   const int kind = prm.norm_kind;
-  double n2 = 0.0, rr = 0.0, n1 = 0.0;
-  for (int k = 0; k < nblk; ++k) {
-    n2 = norm_accum(kind, n2, pr[k]);
-    rr += pr[nblk + k];
-    n1 = norm_accum(kind, n1, pr[2 * nblk + k]);
-  }
+  double acc[3] = {0.0, 0.0, 0.0};
+  reduce3_partials(pr, nblk, {kind, 1, kind}, acc);
+  const double n2 = acc[0], rr = acc[1], n1 = acc[2];
   lxa_krylov_row s = st[row];
   const double gamma_prev = s.gamma;
   s.gamma = rr;
@@ -496,12 +534,9 @@ __global__ void bcg_update_finish_kernel(int64_t rows, int nblk, int64_t n, cons
   if (row >= rows || !st[row].active) return;
   const double* pr = part + row * 3 * nblk;
   const int kind = prm.norm_kind;
-  double rr = 0.0, n1 = 0.0, n2 = 0.0;
-  for (int k = 0; k < nblk; ++k) {
-    rr += pr[k];
-    n1 = norm_accum(kind, n1, pr[nblk + k]);
-    n2 = norm_accum(kind, n2, pr[2 * nblk + k]);
-  }
+  double acc[3] = {0.0, 0.0, 0.0};
+  reduce3_partials(pr, nblk, {1, kind, kind}, acc);
+  const double rr = acc[0], n1 = acc[1], n2 = acc[2];
   lxa_krylov_row s = st[row];
   s.beta = rr;
   s.norm_r = norm_final(kind, n1, n);
@@ -1226,12 +1261,9 @@ __global__ void gm_cycle_end_kernel(int64_t rows, int nblk, int64_t n, const dou
   if (row >= rows || !st[row].active) return;
   const int kind = prm.norm_kind;
   const double* pr = part + row * 3 * nblk;
-  double n2 = 0.0, nr = 0.0, n1 = 0.0;
-  for (int k = 0; k < nblk; ++k) {
-    if (!first_pass) n2 = norm_accum(kind, n2, pr[k]);
-    nr = norm_accum(kind, nr, pr[nblk + k]);
-    n1 = norm_accum(kind, n1, pr[2 * nblk + k]);
-  }
+  double acc[3] = {0.0, 0.0, 0.0};
+  reduce3_partials(pr, nblk, {kind, kind, kind}, acc);
+  const double n2 = first_pass ? 0.0 : acc[0], nr = acc[1], n1 = acc[2];
   lxa_krylov_row s = st[row];
   lxa_gmres_row g = gs[row];
   const double r_new_norm = norm_final(kind, nr, n);

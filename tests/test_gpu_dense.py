@@ -164,7 +164,9 @@ def test_cholesky_reads_upper_triangle_only(device, rng):
 
 
 def test_gemv_matches(device, rng):
-    for m, n in [(1, 1), (5, 7), (64, 64), (300, 129)]:
+    # (5000, 300) and (4099, 37) split the transposed product over row chunks
+    # (the 16-byte path and the scalar path); the rest run in one chunk
+    for m, n in [(1, 1), (5, 7), (64, 64), (300, 129), (5000, 300), (4099, 37)]:
         a = rng.standard_normal((3, m, n))
         x = rng.standard_normal((3, n))
         y = K.gemv(_t(a, device), _t(x, device)).cpu().numpy()
@@ -172,6 +174,15 @@ def test_gemv_matches(device, rng):
         xt = rng.standard_normal((3, m))
         yt = K.gemv(_t(a, device), _t(xt, device), trans=True).cpu().numpy()
         np.testing.assert_allclose(yt, np.einsum("bij,bi->bj", a, xt), rtol=1e-12, atol=1e-12)
+        # transposed views run the other product on the same storage, no copy
+        ta = _t(a, device)
+        yv = K.gemv(ta.mT, _t(xt, device)).cpu().numpy()
+        np.testing.assert_allclose(yv, np.einsum("bij,bi->bj", a, xt), rtol=1e-12, atol=1e-12)
+        yvt = K.gemv(ta.mT, _t(x, device), trans=True).cpu().numpy()
+        np.testing.assert_allclose(yvt, np.einsum("bij,bj->bi", a, x), rtol=1e-12, atol=1e-12)
+        op_t = lx.MatrixLinearOperator(ta[0]).transpose()
+        np.testing.assert_allclose(op_t.mv(_t(xt[0], device)).cpu().numpy(), a[0].T @ xt[0], rtol=1e-12,
+                                   atol=1e-12)
 
 
 # --- differentiation: lineax's JVP (x' = A^{-1}(b' - A' x)) and transpose rules.

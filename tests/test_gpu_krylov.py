@@ -11,7 +11,7 @@ import pytest
 import torch
 
 import lineax_amd as lx
-from lineax_amd.stencil import convection_diffusion2d, poisson2d
+from lineax_amd.stencil import convection_diffusion2d, convection_diffusion_coeffs, poisson2d
 from oracle import lineax_oracle as O
 from tests.helpers import poisson_1d, well_conditioned
 
@@ -154,6 +154,24 @@ def test_stencil_apply_matches_oracle(device, rng):
     yt = op.T.mv(_t(x, device)).cpu().numpy()
     A = O.stencil_matrix(13, 13, op.coeffs)
     np.testing.assert_allclose(yt[0].ravel(), A.T @ x[0].ravel(), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("nx,ny", [(13, 13), (5, 8), (37, 64), (19, 1028), (64, 2048)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_stencil_operator_shapes(device, rng, nx, ny, dtype):
+    # widths that are a multiple of 4 take the four-column kernel (row blocks
+    # of 16, column strips of 1024), the others one thread per point
+    from lineax_amd.stencil import StencilOperator
+
+    coeffs = convection_diffusion_coeffs(max(nx, ny))
+    op = StencilOperator(nx, ny, coeffs, dtype=dtype, device=device)
+    x = rng.standard_normal((3, nx, ny))
+    if dtype == torch.float32:
+        x = x.astype(np.float32).astype(np.float64)
+    y = op.mv(_t(x, device, dtype)).double().cpu().numpy()
+    tol = 1e-13 if dtype == torch.float64 else 1e-6
+    for i in range(3):
+        np.testing.assert_allclose(y[i], O.stencil_apply(x[i], coeffs), rtol=tol, atol=tol)
 
 
 def test_cg_grad_matches_closed_form(device, rng):

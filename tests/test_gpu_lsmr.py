@@ -208,3 +208,23 @@ def test_lsmr_function_operator_and_transpose(device, rng):
     c = rng.standard_normal(20)
     sol_t = lx.linear_solve(lx.MatrixLinearOperator(at).transpose(), _t(c, device), lx.LSMR(1e-10, 1e-10))
     np.testing.assert_allclose(sol_t.value.cpu().numpy(), np.linalg.pinv(a.T) @ c, rtol=1e-7, atol=1e-8)
+
+
+def test_lsmr_graph_replays_match_eager(device, rng):
+    # blocks of 30 iterations captured once and replayed: the same kernels in
+    # the same order as the eager loop, so the answers agree bit for bit
+    # (columns scaled over a decade: about 117 steps, several blocks)
+    a = rng.standard_normal((300, 120)) * np.logspace(0, -1, 120)
+    b = rng.standard_normal((2, 300))
+    eager = _solve(a, b, lx.LSMR(1e-10, 1e-10), device)
+    graph = _solve(a, b, lx.LSMR(1e-10, 1e-10, graph=True), device)
+    steps = graph.stats["num_steps"].cpu().numpy()
+    print(f"graph: steps {steps.tolist()}", flush=True)
+    assert steps.min() > 60
+    assert torch.equal(eager.stats["num_steps"], graph.stats["num_steps"])
+    assert torch.equal(eager.value, graph.value)
+    assert torch.equal(eager.result, graph.result)
+    x_ref, res_ref, steps_ref, _ = _oracle(a, b[0], 1e-10, 1e-10)
+    assert int(graph.result[0]) == res_ref
+    assert abs(int(steps[0]) - steps_ref) <= 3
+    np.testing.assert_allclose(graph.value[0].cpu().numpy(), x_ref, rtol=1e-6, atol=1e-8)
