@@ -456,10 +456,13 @@ def solve_results(solution: torch.Tensor, vector: torch.Tensor, result: torch.Te
 # ---------------------------------------------------------------------------
 
 
-def stencil5(x: torch.Tensor, nx: int, ny: int, coeffs, out_dtype=None) -> torch.Tensor:
-    """Five-point stencil on grid functions: x is [..., nx, ny] or [..., nx*ny]."""
+def stencil5(x: torch.Tensor, nx: int, ny: int, coeffs, out_dtype=None, out: torch.Tensor | None = None) -> torch.Tensor:
+    """Five-point stencil on grid functions: x is [..., nx, ny] or [..., nx*ny].
+
+    ``out`` (optional): a contiguous tensor of x's element count in the output
+    dtype that receives the result (no allocation)."""
     _gpu(x, "vector")
-    out_dtype = out_dtype or x.dtype
+    out_dtype = out.dtype if out is not None else (out_dtype or x.dtype)
     size = nx * ny
     if x.numel() % size:
         raise ValueError(f"vector of shape {tuple(x.shape)} is not a batch of {nx}x{ny} grids")
@@ -470,7 +473,13 @@ def stencil5(x: torch.Tensor, nx: int, ny: int, coeffs, out_dtype=None) -> torch
     if xv.stride(1) != 1 or (rows > 1 and xv.stride(0) < size):
         xv = xv.contiguous()
     xstride = xv.stride(0) if rows > 1 else size
-    y = torch.empty((rows, size), dtype=out_dtype, device=x.device)
+    if out is not None:
+        _gpu(out, "out")
+        if out.numel() != x.numel() or not out.is_contiguous() or out.device != x.device:
+            raise ValueError("out must be a contiguous tensor of the vector's size on its device")
+        y = out.view(rows, size)
+    else:
+        y = torch.empty((rows, size), dtype=out_dtype, device=x.device)
     if rows == 0:
         return y.reshape(x.shape)
     c = (ctypes.c_double * 5)(*[float(v) for v in coeffs])

@@ -1102,6 +1102,21 @@ def diagonal(operator) -> torch.Tensor:
     return torch.diagonal(operator.as_matrix(), dim1=-2, dim2=-1)
 
 
+def mv_into(operator, v: torch.Tensor, struct, out: torch.Tensor) -> torch.Tensor:
+    """operator.mv on a flat [rows, n] tensor, written into ``out`` (flat,
+    contiguous, in the storage dtype).  Operators with an ``mv_into`` method
+    (the stencil operators) write there directly; others are copied in."""
+    if isinstance(operator, TaggedLinearOperator):
+        return mv_into(operator.operator, v, struct, out)
+    direct = getattr(operator, "mv_into", None)
+    if direct is not None and isinstance(struct, TensorStruct):
+        direct(v, out)
+        return out
+    if isinstance(operator, NegLinearOperator):
+        return mv_into(operator.operator, v, struct, out).neg_()
+    return out.copy_(mv_as(operator, v, struct, out.dtype))
+
+
 def mv_as(operator, v: torch.Tensor, struct, out_dtype: torch.dtype) -> torch.Tensor:
     """operator.mv on a flat [rows, n] tensor, returned flat in ``out_dtype``.
 

@@ -88,7 +88,7 @@ class BiCGStab(AbstractLinearSolver):
             stream = K.current_stream(b)  # the capture stream while a graph is recorded
             K.call("lxa_bicgstab_direction", stream, vd, rows, size, r, p, v, int(i == 0), st)
             x = M(p, vdt) if precond else p
-            v.copy_(A(x, vdt))
+            A.into(x, v)
             K.call("lxa_bicgstab_alpha", stream, vd, rows, size, r0, v, prm, st, part)
             K.call("lxa_bicgstab_s", stream, vd, rows, size, r, v, s, st)
             z = M(s, vdt) if precond else s

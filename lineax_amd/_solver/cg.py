@@ -13,6 +13,7 @@ from .._operator import (
     TaggedLinearOperator,
     conj,
     mv_as,
+    mv_into,
     is_negative_semidefinite,
     is_positive_semidefinite,
     linearise,
@@ -69,10 +70,15 @@ class FlatOperator:
         self.identity = isinstance(operator, IdentityLinearOperator)
 
     def __call__(self, v: torch.Tensor, out_dtype: torch.dtype) -> torch.Tensor:
-        rows = v.shape[0]
         if self.identity:
             return v if v.dtype == out_dtype else v.to(out_dtype)
         return mv_as(self.op, v, self.struct, out_dtype)
+
+    def into(self, v: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """The product written into ``out`` (flat, contiguous, storage dtype)."""
+        if self.identity:
+            return out.copy_(v)
+        return mv_into(self.op, v, self.struct, out)
 
 
 class CG(AbstractLinearSolver):

@@ -46,6 +46,10 @@ class StencilOperator(FunctionLinearOperator):
         """The product in another storage dtype (FP64 accumulation either way)."""
         return self._apply(x, out_dtype=out_dtype)
 
+    def mv_into(self, x: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """The product written into ``out`` (its dtype is the storage dtype)."""
+        return K.stencil5(x, self.nx, self.ny, self.coeffs, out=out)
+
     def as_matrix(self):
         return _materialise_by_columns(self)
 
