@@ -3,11 +3,12 @@
 // Three tiers, chosen by order n:
 //  * n <= 64: one lane per row in registers, 64/LPS systems per wave; the
 //    scaled pivot row is broadcast through a double-buffered LDS row.
-//  * n <= 256 (float) / 181 (double): one 256-thread workgroup per system
-//    with the upper triangle resident in LDS in row-packed form (131.6 KB
-//    at n = 256); right-looking factorisation, triangular solves by one
-//    wave with the vector in registers.  (Uneven batches have their own
-//    register-blocked kernel in chol_uneven.hip.)
+//  * n <= 256 (float) / 181 (double): one 256-thread workgroup per system.
+//    Float factors run the register-tiled 16-column panel factorisation of
+//    chol_uneven.hip (chol_factor_regs_f32); double factors, and all solves,
+//    keep the upper triangle resident in LDS in row-packed form (131.6 KB
+//    at n = 256): right-looking factorisation, triangular solves by one
+//    wave with the vector in registers.
 //  * larger n: one workgroup per system in global memory (correctness path).
 //
 // Failure semantics follow jax.scipy.linalg.cho_factor as lineax uses it
@@ -409,7 +410,10 @@ int chol_factor_t(hipStream_t s, int64_t batch, int64_t n64, T* a, int64_t strid
   else if (n <= 16) launch_chol_small<T, 16>(s, batch, n, a, stride, negate, info);
   else if (n <= 32) launch_chol_small<T, 32>(s, batch, n, a, stride, negate, info);
   else if (n <= 64) launch_chol_small<T, 64>(s, batch, n, a, stride, negate, info);
-  else if (n <= lds_max_n<T>()) {
+  else if (sizeof(T) == 4 && n <= 256) {
+    // register-tiled 16-column panels (chol_uneven.hip)
+    return chol_factor_regs_f32(s, batch, n, reinterpret_cast<float*>(a), stride, negate, info);
+  } else if (n <= lds_max_n<T>()) {
     const size_t shmem = static_cast<size_t>(n) * (n + 1) / 2 * sizeof(T);
     if (int rc = allow_lds(chol_factor_lds_kernel<T>, shmem)) return rc;
     hipLaunchKernelGGL((chol_factor_lds_kernel<T>), dim3(static_cast<unsigned>(batch)), dim3(kWG), shmem, s, n, a,

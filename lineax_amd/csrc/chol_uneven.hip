@@ -540,6 +540,49 @@ void chol_uneven_kernel(int64_t count, int max_n, int n_lo, int n_hi,
   }
 }
 
+// Factor only, for uniform float32 batches (lxa_chol_factor, orders 65..256):
+// the register-tiled factorisation above on A's upper triangle (negated when
+// `negate`), then U (row-major, lower triangle zero) overwrites A.  A failed
+// factorisation leaves the whole matrix NaN and info the 1-based index of the
+// first non-positive pivot (chol.hip's convention for every tier).
+template <int KR>
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(KR == 8 ? LXA_SMALL_WAVES : 1)))
+void chol_factor_regs_kernel(int n, float* __restrict__ a, int64_t a_stride, int negate,
+                             int32_t* __restrict__ info) {
+  extern __shared__ __align__(16) unsigned char lxa_smem[];
+  const Lds s = carve(lxa_smem);
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  float* A = a + static_cast<int64_t>(blockIdx.x) * a_stride;
+  const int N16 = round16(n);
+  const int nk = N16 >> 4;
+  float R[KR][KR];
+#pragma unroll
+  for (int aa = 0; aa < KR; ++aa) {
+#pragma unroll
+    for (int bb = aa; bb < KR; ++bb) {
+      R[aa][bb] = 0.0f;
+      if (aa < nk && bb < nk) {
+        const int i = ty + 16 * aa, j = tx + 16 * bb;
+        if (i < n && j < n) {
+          if (j >= i) {
+            const float v = A[static_cast<int64_t>(i) * n + j];
+            R[aa][bb] = negate ? -v : v;
+          }
+        } else if (i == j) {
+          R[aa][bb] = 1.0f;
+        }
+      }
+    }
+  }
+  const int fail = factor_registers<KR>(R, n, s);
+  const int64_t nn = static_cast<int64_t>(n) * n;
+  for (int64_t t = tid; t < nn; t += kT) {
+    const int i = static_cast<int>(t / n), j = static_cast<int>(t % n);
+    A[t] = fail ? quiet_nan<float>() : (j < i ? 0.0f : s.T[pku(N16, i, j)]);
+  }
+  if (tid == 0) info[blockIdx.x] = fail;
+}
+
 __global__ void zero_u32_kernel(unsigned int* p) { *p = 0u; }
 
 template <typename K>
@@ -583,6 +626,26 @@ int launch_class(hipStream_t s, int64_t count, int max_n, int n_lo, int n_hi, in
 }
 
 }  // namespace
+
+template <int KR>
+static int launch_factor_regs(hipStream_t s, int64_t batch, int n, float* a, int64_t stride, int negate,
+                              int32_t* info) {
+  const size_t shmem = uneven_lds_bytes(n);
+  if (int rc = allow_lds(chol_factor_regs_kernel<KR>, shmem)) return rc;
+  hipLaunchKernelGGL((chol_factor_regs_kernel<KR>), dim3(static_cast<unsigned>(batch)), dim3(kT), shmem, s, n, a,
+                     stride, negate, info);
+  LXA_LAUNCH_CHECK("lxa_chol_factor (register tiles) launch");
+  return LXA_OK;
+}
+
+int chol_factor_regs_f32(hipStream_t s, int64_t batch, int n, float* a, int64_t stride, int negate,
+                         int32_t* info) {
+  if (n < 1 || n > kMaxN) return set_error(LXA_EINVAL, "chol_factor_regs_f32: n=%d outside [1, %d]", n, kMaxN);
+  if (batch > kMaxGrid) return set_error(LXA_EINVAL, "chol_factor_regs_f32: batch too large");
+  return n <= 128 ? launch_factor_regs<8>(s, batch, n, a, stride, negate, info)
+                  : launch_factor_regs<16>(s, batch, n, a, stride, negate, info);
+}
+
 }  // namespace lxa
 
 extern "C" int lxa_chol_uneven_workspace_size(int64_t count, size_t* bytes) {

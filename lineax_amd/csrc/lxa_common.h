@@ -109,6 +109,11 @@ __device__ __forceinline__ int packed_upper(int n, int i, int j) {
   return i * n - ((i * (i - 1)) >> 1) + (j - i);
 }
 
+// Register-tiled Cholesky factor of a uniform float32 batch, orders 1..256
+// (chol_uneven.hip); same layout, NaN-on-failure and info conventions as
+// lxa_chol_factor.
+int chol_factor_regs_f32(hipStream_t s, int64_t batch, int n, float* a, int64_t stride, int negate, int32_t* info);
+
 }  // namespace lxa
 
 #define LXA_REQUIRE(cond, ...)                                   \

@@ -132,14 +132,17 @@ def test_cholesky_matches_oracle(device, rng, n, dtype):
         assert np.all(lower == 0)
 
 
-@pytest.mark.parametrize("n", [6, 70, 300])
-def test_cholesky_failure_is_singular(device, rng, n):
+@pytest.mark.parametrize("n", [6, 70, 200, 300])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_cholesky_failure_is_singular(device, rng, n, dtype):
+    # float32 orders 65..256 take the register-tiled factor, the rest the
+    # lane-per-row, LDS and global-memory tiers
     a = well_conditioned(rng, n, kind="psd")
     a[n // 2, n // 2] = -5.0
-    op = lx.MatrixLinearOperator(_t(a, device), lx.positive_semidefinite_tag)
-    sol = lx.linear_solve(op, _t(rng.standard_normal(n), device), lx.Cholesky(), throw=False)
+    op = lx.MatrixLinearOperator(_t(a, device, dtype), lx.positive_semidefinite_tag)
+    sol = lx.linear_solve(op, _t(rng.standard_normal(n), device, dtype), lx.Cholesky(), throw=False)
     assert int(sol.result) == lx.RESULTS.singular
-    u, info = K.chol_factor(_t(a, device))
+    u, info = K.chol_factor(_t(a, device, dtype))
     assert int(info) > 0 and torch.all(torch.isnan(u))
 
 
@@ -147,6 +150,10 @@ def test_cholesky_failure_is_singular(device, rng, n):
 def test_cholesky_nsd(device, rng, n):
     a = well_conditioned(rng, n, kind="nsd")
     b = rng.standard_normal(n)
+    op32 = lx.MatrixLinearOperator(_t(a, device, torch.float32), lx.negative_semidefinite_tag)
+    x32 = lx.linear_solve(op32, _t(b, device, torch.float32), lx.Cholesky()).value.double().cpu().numpy()
+    a32 = a.astype(np.float32).astype(np.float64)
+    assert O.normwise_backward_error(a32, x32, b.astype(np.float32).astype(np.float64)) < 64 * n * EPS32
     op = lx.MatrixLinearOperator(_t(a, device), lx.negative_semidefinite_tag)
     sol = lx.linear_solve(op, _t(b, device), lx.Cholesky())
     np.testing.assert_allclose(sol.value.cpu().numpy(), np.linalg.solve(a, b), rtol=1e-10, atol=1e-10)
@@ -154,12 +161,13 @@ def test_cholesky_nsd(device, rng, n):
     np.testing.assert_allclose(auto.value.cpu().numpy(), np.linalg.solve(a, b), rtol=1e-10, atol=1e-10)
 
 
-def test_cholesky_reads_upper_triangle_only(device, rng):
-    a = well_conditioned(rng, 40, kind="psd")
+@pytest.mark.parametrize("n,dtype", [(40, torch.float64), (150, torch.float32), (240, torch.float32)])
+def test_cholesky_reads_upper_triangle_only(device, rng, n, dtype):
+    a = well_conditioned(rng, n, kind="psd")
     junk = a.copy()
-    junk[np.tril_indices(40, -1)] = 1e30
-    u1, _ = K.chol_factor(_t(a, device))
-    u2, _ = K.chol_factor(_t(junk, device))
+    junk[np.tril_indices(n, -1)] = 1e30
+    u1, _ = K.chol_factor(_t(a, device, dtype))
+    u2, _ = K.chol_factor(_t(junk, device, dtype))
     assert torch.equal(u1, u2)
 
 
