@@ -169,24 +169,56 @@ __device__ int factor_registers(float (&R)[KR][KR], int n, const Lds& s) {
 
     // (c) rank-16 update of the register slots below the panel.  The p loop
     // stays rolled: unrolled, the per-panel code would outgrow the
-    // instruction cache.
+    // instruction cache.  K is a runtime value, so a guard per register
+    // tile is a branch per tile.  With 16 x 16 tiles (136 of them) the
+    // branches cost more than the arithmetic: the panel values of rows and
+    // columns outside (K, nk) are selected to zero instead (PT rows hold
+    // stale values there, so a select, not a multiplication) and only whole
+    // groups of four tile rows are skipped; tiles in a group's inactive
+    // rows or columns subtract an exact zero from entries that are never
+    // read again (already copied to T, or padding).  The 8 x 8 tiles keep
+    // the per-tile guards, which measure faster there.
 #pragma unroll 1
     for (int pc = 0; pc < 16; pc += 2) {
+      const float2 zero = make_float2(0.0f, 0.0f);
       float2 uj[KR];
+      if constexpr (KR > 8) {
 #pragma unroll
-      for (int b = 0; b < KR; ++b) {
-        uj[b] = make_float2(0.0f, 0.0f);
-        if (b > K && b < nk) uj[b] = *reinterpret_cast<const float2*>(PT + (tx + 16 * b) * kPTStride + pc);
-      }
+        for (int b = 0; b < KR; ++b) {
+          const float2 t = *reinterpret_cast<const float2*>(PT + (tx + 16 * b) * kPTStride + pc);
+          uj[b] = (b > K && b < nk) ? t : zero;
+        }
 #pragma unroll
-      for (int a = 0; a < KR; ++a) {
-        if (a > K && a < nk) {
-          const float2 ui = *reinterpret_cast<const float2*>(PT + (ty + 16 * a) * kPTStride + pc);
+        for (int ag = 0; ag < KR; ag += 4) {
+          if (ag + 3 > K && ag < nk) {
 #pragma unroll
-          for (int b = a; b < KR; ++b) {
-            if (b < nk) {
-              R[a][b] -= ui.x * uj[b].x;
-              R[a][b] -= ui.y * uj[b].y;
+            for (int a = ag; a < ag + 4 && a < KR; ++a) {
+              const float2 t = *reinterpret_cast<const float2*>(PT + (ty + 16 * a) * kPTStride + pc);
+              const float2 ui = (a > K && a < nk) ? t : zero;
+#pragma unroll
+              for (int b = a; b < KR; ++b) {
+                R[a][b] -= ui.x * uj[b].x;
+                R[a][b] -= ui.y * uj[b].y;
+              }
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < KR; ++b) {
+          uj[b] = zero;
+          if (b > K && b < nk) uj[b] = *reinterpret_cast<const float2*>(PT + (tx + 16 * b) * kPTStride + pc);
+        }
+#pragma unroll
+        for (int a = 0; a < KR; ++a) {
+          if (a > K && a < nk) {
+            const float2 ui = *reinterpret_cast<const float2*>(PT + (ty + 16 * a) * kPTStride + pc);
+#pragma unroll
+            for (int b = a; b < KR; ++b) {
+              if (b < nk) {
+                R[a][b] -= ui.x * uj[b].x;
+                R[a][b] -= ui.y * uj[b].y;
+              }
             }
           }
         }
