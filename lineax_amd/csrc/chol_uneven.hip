@@ -607,10 +607,12 @@ void chol_factor_regs_kernel(int n, float* __restrict__ a, int64_t a_stride, int
     }
   }
   const int fail = factor_registers<KR>(R, n, s);
-  const int64_t nn = static_cast<int64_t>(n) * n;
-  for (int64_t t = tid; t < nn; t += kT) {
-    const int i = static_cast<int>(t / n), j = static_cast<int>(t % n);
-    A[t] = fail ? quiet_nan<float>() : (j < i ? 0.0f : s.T[pku(N16, i, j)]);
+  // rows by wave, columns by lane (no index division)
+  const int w = tid >> 6, l = tid & 63;
+  for (int i = w; i < n; i += kT / 64) {
+    float* Ai = A + static_cast<int64_t>(i) * n;
+    const int ri = pku(N16, i, i) - i;
+    for (int j = l; j < n; j += 64) Ai[j] = fail ? quiet_nan<float>() : (j < i ? 0.0f : s.T[ri + j]);
   }
   if (tid == 0) info[blockIdx.x] = fail;
 }
