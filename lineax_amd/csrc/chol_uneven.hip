@@ -261,6 +261,10 @@ __device__ __forceinline__ void solve_load_col(const Lds& s, int N16, int l, int
 }
 
 constexpr int kG = 1;  // steps per prefetch group (1 keeps the refined kernel free of scratch)
+#ifndef LXA_RES_ROWS
+#define LXA_RES_ROWS 4
+#endif
+constexpr int kRB = LXA_RES_ROWS;  // rows per residual batch (per wave)
 
 template <int NC>
 __device__ void wave_solve(int n, const Lds& s, float (&v)[NC]) {
@@ -358,12 +362,12 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
     acc[c] = aab[c] = 0.0;
     xi[c] = (l + 64 * c < n) ? s.x[l + 64 * c] : 0.0;
   }
-  // Rows j0 + 4u (u < 8) per batch; the next batch's loads are issued before
+  // Rows j0 + 4u (u < kRB) per batch; the next batch's loads are issued before
   // this batch is reduced (the eight row reductions are independent, so their
   // shuffles interleave).
-  auto load_rows = [&](int j0, float (&av)[8][NC]) {
+  auto load_rows = [&](int j0, float (&av)[kRB][NC]) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < kRB; ++u) {
       const int j = j0 + 4 * u;
       const float* Aj = A + static_cast<int64_t>(j) * n;
 #pragma unroll
@@ -373,14 +377,14 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
       }
     }
   };
-  float av[8][NC];
+  float av[kRB][NC];
   load_rows(w, av);
-  for (int j0 = w; j0 < n; j0 += 32) {
-    float nv[8][NC];
-    load_rows(j0 + 32, nv);  // rows past n load nothing (zero)
-    double red[16];
+  for (int j0 = w; j0 < n; j0 += 4 * kRB) {
+    float nv[kRB][NC];
+    load_rows(j0 + 4 * kRB, nv);  // rows past n load nothing (zero)
+    double red[2 * kRB];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < kRB; ++u) {
       const int j = j0 + 4 * u;
       const double xj = j < n ? s.x[j] : 0.0;
       double dot = 0.0, rab = 0.0;
@@ -396,20 +400,20 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
         }
       }
       red[u] = dot;
-      red[8 + u] = rab;
+      red[kRB + u] = rab;
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < kRB; ++u) {
       const int j = j0 + 4 * u;
       const double dot = wave_sum(red[u]);
-      const double rab = want_abs ? wave_sum(red[8 + u]) : 0.0;
+      const double rab = want_abs ? wave_sum(red[kRB + u]) : 0.0;
       if (l == 0 && j < n) {
         s.rowpart[j] = dot;
         if (want_abs) s.rowabs[j] = rab;
       }
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < kRB; ++u)
 #pragma unroll
       for (int c = 0; c < NC; ++c) av[u][c] = nv[u][c];
   }
