@@ -260,7 +260,7 @@ __device__ __forceinline__ void solve_load_col(const Lds& s, int N16, int l, int
   }
 }
 
-constexpr int kG = 1;  // steps per prefetch group (1 keeps the refined kernel free of scratch)
+constexpr int kG = 4;  // steps per prefetch group (4 fits both tile classes without scratch)
 #ifndef LXA_RES_ROWS
 #define LXA_RES_ROWS 4
 #endif
