@@ -165,7 +165,8 @@ int lxa_solve_results(void* stream, int dtype, int64_t rows, int64_t len,
 /* y = A x for the five-point stencil with constant coefficients
  * coeffs = {center, west (j-1), east (j+1), south (i-1), north (i+1)} on an
  * nx x ny grid (j fastest) with zero Dirichlet boundary, for `rows` grid
- * functions.  in/out dtypes may differ (FP64 accumulation). */
+ * functions.  in/out dtypes may differ (FP64 accumulation).  x and y must
+ * not overlap (rows are read ahead of the rows written). */
 int lxa_stencil5_apply(void* stream, int in_dtype, int out_dtype, int64_t rows, int64_t nx, int64_t ny,
                        const double* coeffs, const void* x, int64_t x_row_stride, void* y,
                        int64_t y_row_stride);
