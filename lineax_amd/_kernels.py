@@ -477,6 +477,10 @@ def stencil5(x: torch.Tensor, nx: int, ny: int, coeffs, out_dtype=None, out: tor
         _gpu(out, "out")
         if out.numel() != x.numel() or not out.is_contiguous() or out.device != x.device:
             raise ValueError("out must be a contiguous tensor of the vector's size on its device")
+        x0, x1 = xv.data_ptr(), xv.data_ptr() + xv.numel() * xv.element_size()
+        o0, o1 = out.data_ptr(), out.data_ptr() + out.numel() * out.element_size()
+        if x0 < o1 and o0 < x1:
+            raise ValueError("out must not overlap the vector (the stencil reads rows ahead of those it writes)")
         y = out.view(rows, size)
     else:
         y = torch.empty((rows, size), dtype=out_dtype, device=x.device)
