@@ -30,7 +30,7 @@ namespace {
 
 constexpr int kT = 256;
 #ifndef LXA_SMALL_WAVES
-#define LXA_SMALL_WAVES 2
+#define LXA_SMALL_WAVES 3
 #endif
 constexpr int kMaxN = 256;
 constexpr int kPTStride = 18;  // 16 panel values + 2 pad: b64 reads hit distinct banks
