@@ -34,15 +34,23 @@ constexpr int kT = 256;
 #endif
 constexpr int kMaxN = 256;
 constexpr int kPTStride = 18;  // 16 panel values + 2 pad: b64 reads hit distinct banks
-constexpr size_t kUnionBytes = 256 * kPTStride * sizeof(float);  // >= (8 * 256 + 8) doubles
+constexpr size_t kDoubleBytes = (2048 + 16) * sizeof(double);  // x, r, rowpart, rowabs, acc[4], red
 constexpr size_t kRinvBytes = 256 * sizeof(float);
 constexpr size_t kItemBytes = 16;
 
 __host__ __device__ inline int round16(int n) { return (n + 15) & ~15; }
 
+// The panel copy PT (rows up to the largest order of the launch) and the
+// refinement's doubles share the front of LDS.  Sized by the launch's largest
+// order, so that orders up to 176 fit two workgroups in 160 KB.
+__host__ __device__ inline size_t union_bytes(int max_n) {
+  const size_t pt = static_cast<size_t>(round16(max_n)) * kPTStride * sizeof(float);
+  return pt > kDoubleBytes ? pt : kDoubleBytes;
+}
+
 __host__ __device__ inline size_t uneven_lds_bytes(int max_n) {
   const size_t n16 = static_cast<size_t>(round16(max_n));
-  return kUnionBytes + kRinvBytes + kItemBytes + n16 * (n16 + 1) / 2 * sizeof(float);
+  return union_bytes(max_n) + kRinvBytes + kItemBytes + n16 * (n16 + 1) / 2 * sizeof(float);
 }
 
 __device__ __forceinline__ float bcast(float v, int lane) {
@@ -64,7 +72,9 @@ struct Lds {
   float* T;          // packed U, order N16
 };
 
-__device__ Lds carve(unsigned char* base) {
+// lds_n: the largest order the launch's LDS was sized for (uneven_lds_bytes).
+__device__ Lds carve(unsigned char* base, int lds_n) {
+  const size_t ub = union_bytes(lds_n);
   Lds s;
   s.PT = reinterpret_cast<float*>(base);
   double* d = reinterpret_cast<double*>(base);
@@ -74,9 +84,9 @@ __device__ Lds carve(unsigned char* base) {
   s.rowabs = d + 768;
   s.acc = d + 1024;
   s.red = d + 2048;
-  s.rinv = reinterpret_cast<float*>(base + kUnionBytes);
-  s.item = reinterpret_cast<int*>(base + kUnionBytes + kRinvBytes);
-  s.T = reinterpret_cast<float*>(base + kUnionBytes + kRinvBytes + kItemBytes);
+  s.rinv = reinterpret_cast<float*>(base + ub);
+  s.item = reinterpret_cast<int*>(base + ub + kRinvBytes);
+  s.T = reinterpret_cast<float*>(base + ub + kRinvBytes + kItemBytes);
   return s;
 }
 
@@ -446,7 +456,7 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
 // skips the rest, so two launches with different tiles split one batch.
 // report_invalid: this launch marks orders outside [1, max_n].
 template <int KR, bool kRefine, typename TX>
-__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(KR == 8 ? LXA_SMALL_WAVES : 1)))
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(KR == 8 ? LXA_SMALL_WAVES : (KR < 16 ? 2 : 1))))
 void chol_uneven_kernel(int64_t count, int max_n, int n_lo, int n_hi,
                                                          int report_invalid, const int32_t* __restrict__ ns,
                                                          const int64_t* __restrict__ a_off,
@@ -458,7 +468,7 @@ void chol_uneven_kernel(int64_t count, int max_n, int n_lo, int n_hi,
                                                          int32_t* __restrict__ info, int32_t* __restrict__ iters,
                                                          double* __restrict__ eta) {
   extern __shared__ __align__(16) unsigned char lxa_smem[];
-  const Lds s = carve(lxa_smem);
+  const Lds s = carve(lxa_smem, n_hi < 16 * KR ? n_hi : 16 * KR);  // launch_class's lds_n
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int tx = tid & 15, ty = tid >> 4;
   int64_t next = blockIdx.x;
@@ -488,7 +498,7 @@ void chol_uneven_kernel(int64_t count, int max_n, int n_lo, int n_hi,
     TX* X = x + b_off[sys];
     const int nk = round16(n) >> 4;
 
-    constexpr int NC = KR / 4;  // 64-entry chunks of a vector (orders <= 64 NC)
+    constexpr int NC = (KR + 3) / 4;  // 64-entry chunks of a vector (orders <= 64 NC)
     float R[KR][KR];
 #pragma unroll
     for (int aa = 0; aa < KR; ++aa) {
@@ -586,7 +596,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(KR == 8 ? LX
 void chol_factor_regs_kernel(int n, float* __restrict__ a, int64_t a_stride, int negate,
                              int32_t* __restrict__ info) {
   extern __shared__ __align__(16) unsigned char lxa_smem[];
-  const Lds s = carve(lxa_smem);
+  const Lds s = carve(lxa_smem, n);  // launch_factor_regs sizes LDS for n
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   float* A = a + static_cast<int64_t>(blockIdx.x) * a_stride;
   const int N16 = round16(n);
@@ -709,12 +719,13 @@ extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, con
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool refine = max_refine > 0;
   unsigned int* queue = static_cast<unsigned int*>(workspace);
-  // Two register-tile classes: orders <= 128 run with 8 x 8 tiles (a quarter
-  // of the registers, so several systems share a compute unit), larger ones
-  // with 16 x 16.  A persistent solve is one launch per class over the same
-  // order (the large class first, each with its own queue counter); one
-  // workgroup per system uses the class that covers max_n.
-  const int split = 128;
+  // Three register-tile classes: orders <= 128 run with 8 x 8 tiles (three
+  // systems per compute unit), 129..176 with 11 x 11 (two per compute unit:
+  // their LDS fits twice in 160 KB), larger ones with 16 x 16 (one).  A
+  // persistent solve is one launch per class over the same order (the large
+  // classes first, each with its own queue counter); one workgroup per
+  // system uses the class that covers max_n.
+  const int split = 128, split2 = 176;
   int rc = LXA_OK;
 #define LXA_UNEVEN_CLASS(KR, LO, HI, REPORT, Q)                                                                    \
   do {                                                                                                           \
@@ -729,10 +740,17 @@ extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, con
                                                order, persistent, Q, 0, tol, info, iters, eta);                 \
   } while (0)
   if (!persistent) {
-    if (max_n > split) LXA_UNEVEN_CLASS(16, 0, max_n, 1, queue);
+    if (max_n > split2) LXA_UNEVEN_CLASS(16, 0, max_n, 1, queue);
+    else if (max_n > split) LXA_UNEVEN_CLASS(11, 0, max_n, 1, queue);
     else LXA_UNEVEN_CLASS(8, 0, max_n, 1, queue);
+  } else if (max_n > split2) {
+    LXA_UNEVEN_CLASS(16, split2, max_n, 1, queue);
+    if (rc) return rc;
+    LXA_UNEVEN_CLASS(11, split, split2, 0, queue + 1);
+    if (rc) return rc;
+    LXA_UNEVEN_CLASS(8, 0, split, 0, queue + 2);
   } else if (max_n > split) {
-    LXA_UNEVEN_CLASS(16, split, max_n, 1, queue);
+    LXA_UNEVEN_CLASS(11, split, max_n, 1, queue);
     if (rc) return rc;
     LXA_UNEVEN_CLASS(8, 0, split, 0, queue + 1);
   } else {
