@@ -245,16 +245,23 @@ __device__ int factor_registers(float (&R)[KR][KR], int n, const Lds& s) {
 // next group of steps (a row of U forward, a column backward, and the
 // reciprocal pivots) are loaded while the current group runs, so the LDS
 // latency stays off the chain.  (Groups of kG steps.)
+// The loads are unconditional (an out-of-range entry reads element 0 and is
+// selected away): with no branch around them the compiler knows how many
+// LDS reads are in flight and waits only for the group it is about to use,
+// so the next group's reads stay in flight during this group's steps.
 template <int NC>
 __device__ __forceinline__ void solve_load_row(const Lds& s, int N16, int n, int l, int k, bool live, float (&t)[NC],
                                                float& r) {
   const float* T = s.T;
-  r = live ? s.rinv[k] : 0.0f;
+  const float rv = s.rinv[live ? k : 0];
+  r = live ? rv : 0.0f;
   const int rk = pku(N16, k, k);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int jj = l + 64 * c;
-    t[c] = (live && jj > k && jj < n) ? T[rk + (jj - k)] : 0.0f;
+    const bool in = live && jj > k && jj < n;
+    const float v = T[in ? rk + (jj - k) : 0];
+    t[c] = in ? v : 0.0f;
   }
 }
 
@@ -262,11 +269,14 @@ template <int NC>
 __device__ __forceinline__ void solve_load_col(const Lds& s, int N16, int l, int k, bool live, float (&t)[NC],
                                                float& r) {
   const float* T = s.T;
-  r = live ? s.rinv[k] : 0.0f;
+  const float rv = s.rinv[live ? k : 0];
+  r = live ? rv : 0.0f;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int i = l + 64 * c;
-    t[c] = (live && i < k) ? T[pku(N16, i, k)] : 0.0f;
+    const bool in = live && i < k;
+    const float v = T[in ? pku(N16, i, k) : 0];
+    t[c] = in ? v : 0.0f;
   }
 }
 
