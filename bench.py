@@ -4,8 +4,10 @@ Default workload: the uneven-batch Cholesky configuration — 20,000 SPD
 float32 systems, n ~ U{8..256}, A = M M^T / n + 0.1 I, b ~ N(0, 1), seed 1 —
 solved as one step per pass over the whole batch, inputs resident in HBM.
 
-  chosen form: one persistent launch, FP32 factorisation in LDS, FP64
-               refinement to normwise backward error <= 1e-12 (float64 x).
+  chosen form: one persistent launch per register-tile class (three:
+               orders 177..256, 129..176, <= 128), FP32 factorisation in
+               registers and LDS, FP64 refinement to normwise backward error
+               <= 1e-12 (float64 x).
   plain form:  the kernels lx.Cholesky runs (factor + solve launch pair) on
                one order bucket at a time (249 buckets), FP32, no
                refinement (float32 x).  The per-bucket schedule on the
@@ -292,8 +294,8 @@ def bench_uneven_cholesky(args):
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic,
-        "kernel": "chol_uneven_kernel<16|8, true, double>: one persistent launch per register-tile class "
-                  "(orders > 128, orders <= 128) per step; achieved over the step's two launches",
+        "kernel": "chol_uneven_kernel<16|11|8, true, double>: one persistent launch per register-tile class "
+                  "(orders 177..256, 129..176, <= 128) per step; achieved over the step's three launches",
         "algorithmic_bytes_per_launch": batch.algorithmic_bytes,
         "fp32_flop_view": {
             "achieved_tflops": batch.algorithmic_flops / (mean_ms / 1e3) / 1e12,

@@ -3,9 +3,10 @@
 lineax can batch only systems of one shape (``jax.vmap``), so an uneven mix
 becomes one vmap per size bucket (the "plain form" here: one FP32 launch per
 bucket).  ``UnevenBatch`` keeps the whole mix resident in HBM, packed back to
-back, and ``cholesky_solve`` runs it as one persistent launch: workgroups pull
-systems from a device queue sorted largest-first, factor in FP32 in LDS and
-refine the solution in FP64 until the normwise backward error meets ``tol``.
+back, and ``cholesky_solve`` runs it as one persistent launch per
+register-tile class: workgroups pull systems from a device queue sorted
+largest-first, factor in FP32 (registers and LDS) and refine the solution in
+FP64 until the normwise backward error meets ``tol``.
 
 Each system i means ``linear_solve(MatrixLinearOperator(A_i,
 positive_semidefinite_tag), b_i, Cholesky())`` (lineax/_solver/cholesky.py)
@@ -109,7 +110,7 @@ class UnevenBatch:
                        out: UnevenSolution | None = None) -> UnevenSolution:
         """Solves every system.
 
-        ``form="chosen"``: one persistent launch, FP32 factorisation, FP64
+        ``form="chosen"``: persistent launches (one per tile class), FP32 factorisation, FP64
         refinement to ``tol``; float64 solutions.
         ``form="plain"``: one non-persistent FP32 launch per order bucket, no
         refinement; float32 solutions.
