@@ -383,17 +383,24 @@ __device__ void residual_sym_upper(int n, const float* __restrict__ A, const flo
     xi[c] = (l + 64 * c < n) ? s.x[l + 64 * c] : 0.0;
   }
   // Rows j0 + 4u (u < kRB) per batch; the next batch's loads are issued before
-  // this batch is reduced (the eight row reductions are independent, so their
-  // shuffles interleave).
+  // this batch is reduced (the row reductions are independent, so their
+  // shuffles interleave).  The loads are unconditional, so the compiler
+  // waits only for the batch it uses and the next batch's loads stay in
+  // flight: a lane left of the diagonal reads the row's diagonal element
+  // (a cache line the row loads anyway), a row past n reads row n - 1, and
+  // both are selected away.
   auto load_rows = [&](int j0, float (&av)[kRB][NC]) {
 #pragma unroll
     for (int u = 0; u < kRB; ++u) {
       const int j = j0 + 4 * u;
-      const float* Aj = A + static_cast<int64_t>(j) * n;
+      const int jr = j < n ? j : n - 1;
+      const float* Aj = A + static_cast<int64_t>(jr) * n;
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         const int i = l + 64 * c;
-        av[u][c] = (j < n && i >= j && i < n) ? Aj[i] : 0.0f;
+        const int ic = i < jr ? jr : (i < n ? i : n - 1);
+        const float v = Aj[ic];
+        av[u][c] = (j < n && i >= j && i < n) ? v : 0.0f;
       }
     }
   };
