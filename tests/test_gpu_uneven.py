@@ -1,6 +1,7 @@
 """GPU parity of the uneven-batch Cholesky launch.
 
-Chosen form (one persistent launch, FP32 factor + FP64 refinement): each
+Chosen form (persistent launches, one per register-tile class; FP32 factor
++ FP64 refinement): each
 float64 solution must match the oracle's float64 Cholesky solve of the same
 (exact, upcast) data to 1e-10 relative, and have normwise backward error
 <= 1e-12 both as reported by the kernel and as recomputed on the host.
@@ -98,3 +99,34 @@ def test_persistent_queue_reuse(device, host):
     x1 = s1.x.clone()
     s2 = batch.cholesky_solve(form="chosen", tol=1e-12)
     assert torch.equal(x1, s2.x)
+
+
+@pytest.mark.parametrize("orders", [
+    [127, 128, 129, 130, 175, 176, 177, 178, 255, 256],  # all three tile classes, both sides of each split
+    [100, 129, 150, 176, 176],                            # largest order 176: the 11x11 and 8x8 classes only
+])
+def test_tile_class_boundaries(device, orders):
+    # Orders <= 128 run 8 x 8 register tiles, 129..176 run 11 x 11 (two
+    # workgroups per compute unit, LDS sized by the class's largest order),
+    # 177..256 run 16 x 16; each class is its own persistent launch, and the
+    # plain form launches one class per order bucket.
+    rng = np.random.default_rng(23)
+    mats, rhss = [], []
+    for n in orders:
+        m = rng.standard_normal((n, n))
+        mats.append((m @ m.T / n + 0.1 * np.eye(n)).astype(np.float32))
+        rhss.append(rng.standard_normal(n).astype(np.float32))
+    host = W.pack_uneven(orders, mats, rhss)
+    batch = UnevenBatch.from_host(host, device)
+    chosen = batch.cholesky_solve(form="chosen", tol=1e-12)
+    plain = batch.cholesky_solve(form="plain")
+    x_ref, info_ref = O.chol_uneven(host.n, host.a_offset, host.a, host.b_offset, host.b, threads=4)
+    assert np.all(info_ref == 0) and torch.all(chosen.info == 0) and torch.all(plain.info == 0)
+    for i, n in enumerate(orders):
+        a = host.matrix(i).astype(np.float64)
+        b = host.rhs(i).astype(np.float64)
+        xi = _x(chosen, host, i)
+        ref = x_ref[host.b_offset[i]: host.b_offset[i] + n]
+        assert O.relative_error(xi, ref) < 1e-10, (i, n)
+        assert O.normwise_backward_error(a, xi, b) <= 1e-12, (i, n)
+        assert O.normwise_backward_error(a, _x(plain, host, i), b) < 64 * n * EPS32, (i, n)
