@@ -137,7 +137,7 @@ int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n,
 /* y = A x (trans = 0, A is [m, n]) or y = A^T x (trans = 1), batched,
  * FP64 accumulation.  Replaces jnp.matmul(self.matrix, vector,
  * precision=HIGHEST) in MatrixLinearOperator.mv, lineax/_operator.py:265-269
- * (trans = 1 serves the transposed operator, _operator.py:271-274, on the
+ * (trans = 1 serves the transposed operator, _operator.py:274-277, on the
  * same storage).  trans = 1 splits the rows over workgroups and adds the
  * per-chunk sums in a fixed order; it needs lxa_gemv_workspace_size bytes
  * of device scratch (0 for trans = 0 and for small problems). */
@@ -246,7 +246,7 @@ int lxa_cg_finish(void* stream, int vdtype, int64_t rows, int64_t n, const void*
 int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_krylov_row* state);
 
 /* ---------------------------------------------------------------------------
- * LSMR (lineax/_solver/lsmr.py:92-362, damp 0, two_norm) on device-resident
+ * LSMR (lineax/_solver/lsmr.py:94-359, damp 0, two_norm) on device-resident
  * per-row state, so rows stop independently.  u and v are stored unscaled:
  * the bidiagonalisation's unit vectors are u / beta and v / alpha (beta,
  * alpha of the row state; a zero norm divides by 1, as lsmr.py does).
@@ -255,9 +255,9 @@ int lxa_cg_result(void* stream, int64_t rows, const lxa_krylov_params* prm, lxa_
  * alpha, hbar = 0); per iteration: [caller: w = A v] step_u (u = w / alpha
  * - alpha u / beta, the new beta), [caller: z = A^T u] step_v (when beta is
  * nonzero v = z / beta - beta v / alpha and the new alpha; the Givens
- * recurrence of lsmr.py:229-306 and the estimates of |r|, |A|, cond(A),
+ * recurrence of lsmr.py:240-312 and the estimates of |r|, |A|, cond(A),
  * |A^T r|), step_x (hbar, x, h updates and |x|, then the stopping rules of
- * lsmr.py:316-325); finally result (lsmr.py:347-362).  Partials: as
+ * lsmr.py:316-329); finally result (lsmr.py:344-357).  Partials: as
  * lxa_krylov_partials_size(rows, max(m, n), 0).
  * ------------------------------------------------------------------------- */
 typedef struct {

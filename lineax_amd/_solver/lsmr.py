@@ -15,7 +15,7 @@ from .cg import GRAPH_BLOCK, MAX_CHECK_BLOCK, FlatOperator
 class LSMR(AbstractLinearSolver):
     """Least-squares solver of Fong and Saunders for A x = b, any shape of A.
 
-    Semantics of lineax/_solver/lsmr.py:52-370 with damp = 0: the
+    Semantics of lineax/_solver/lsmr.py:55-441 with damp = 0: the
     Golub-Kahan bidiagonalisation, the rotations Qhat, Q and Qbar, the
     estimates of |r|, |A^T r|, |A| and cond(A) (224-306), the stopping rules
     in their order (309-325), max_steps = 10 * min(m, n) by default and the

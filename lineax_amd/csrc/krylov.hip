@@ -566,10 +566,10 @@ __global__ void bcg_result_kernel(int64_t rows, CgParams prm, double threshold, 
 }
 
 // ---------------------------------------------------------------------------
-// LSMR (lineax/_solver/lsmr.py:92-362), damp 0, two_norm.  Vectors of a row
+// LSMR (lineax/_solver/lsmr.py:94-359), damp 0, two_norm.  Vectors of a row
 // are stored unscaled; u / sel(beta) and v / sel(alpha) are the unit
 // vectors of the bidiagonalisation (sel(z) = z, or 1 when z is 0, as
-// lax.select in lsmr.py:145,162,207).
+// lax.select in lsmr.py:142,153,219,223).
 // ---------------------------------------------------------------------------
 
 struct LsParams {
@@ -586,7 +586,7 @@ __device__ __forceinline__ double nanmin(double a, double b) {
   return a < b ? a : b;
 }
 
-// c, s, r with [c -s; s c] [a; b] = [r; 0] (lsmr.py:373-415)
+// c, s, r with [c -s; s c] [a; b] = [r; 0] (lsmr.py:361-409)
 __device__ void ls_givens(double a, double b, double& c, double& s, double& r) {
   if (a == 0.0 || b == 0.0) {
     if (b == 0.0) {
@@ -668,7 +668,7 @@ __global__ __launch_bounds__(kB) void ls_v0_kernel(int64_t n, const TV* __restri
   if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = vv;
 }
 
-// alpha and the loop state of lsmr.py:165-187.
+// alpha and the loop state of lsmr.py:137-200.
 __global__ void ls_init_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part,
                                       lxa_lsmr_row* __restrict__ st) {
   const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -718,7 +718,7 @@ __global__ __launch_bounds__(kB) void ls_h0_kernel(int64_t n, const TV* __restri
   }
 }
 
-// u = w / alpha - alpha u / beta (w = A v unscaled; lsmr.py:198-200);
+// u = w / alpha - alpha u / beta (w = A v unscaled; lsmr.py:214-219);
 // partials [row][nblk]: sum u^2.
 template <typename TV>
 __global__ __launch_bounds__(kB) void ls_u_kernel(int64_t m, const TV* __restrict__ w, TV* __restrict__ u,
@@ -748,7 +748,7 @@ __global__ void ls_u_finish_kernel(int64_t rows, int nblk, const double* __restr
 }
 
 // When beta is nonzero: v = z / beta - beta v / alpha (z = A^T u unscaled;
-// lsmr.py:202-209); partials [row][nblk]: sum v^2.
+// lsmr.py:220-223); partials [row][nblk]: sum v^2.
 template <typename TV>
 __global__ __launch_bounds__(kB) void ls_v_kernel(int64_t n, const TV* __restrict__ z, TV* __restrict__ v,
                                                   const lxa_lsmr_row* __restrict__ st, double* __restrict__ part) {
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(kB) void ls_v_kernel(int64_t n, const TV* __restric
   if (!ls_active(st)) return;
   const int64_t row = blockIdx.y;
   const double beta = st[row].beta;
-  if (beta == 0.0) return;  // lsmr.py:211-212: u, v and alpha unchanged (uniform per row)
+  if (beta == 0.0) return;  // lsmr.py:226-230: u, v and alpha unchanged (uniform per row)
   const Chunk c = chunk_of(n);
   const double ib = 1.0 / beta, ba = beta / sel1(st[row].alpha);
   const int64_t o = row * n;
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(kB) void ls_v_kernel(int64_t n, const TV* __restric
   if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = vv;
 }
 
-// The new alpha, then the rotations and estimates of lsmr.py:224-306.
+// The new alpha, then the rotations and estimates of lsmr.py:240-312.
 __global__ void ls_v_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part,
                                    lxa_lsmr_row* __restrict__ st) {
   const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -798,11 +798,11 @@ __global__ void ls_v_finish_kernel(int64_t rows, int nblk, const double* __restr
   s.rhobar = rhobar;
   s.zeta = cbar * s.zetabar;
   s.zetabar = -sbar * s.zetabar;
-  // coefficients of the h, hbar and x updates (lsmr.py:253-262)
+  // coefficients of the h, hbar and x updates (lsmr.py:260-272)
   s.c_hbar = -(thetabar * rho / (rhoold * rhobarold));
   s.c_x = s.zeta / (rho * rhobar);
   s.c_h = -(thetanew / rho);
-  // estimate of |r| (lsmr.py:264-292)
+  // estimate of |r| (lsmr.py:274-300)
   const double betaacute = chat * s.betadd, betacheck = -shat * s.betadd;
   const double betahat = c * betaacute;
   s.betadd = -sn * betaacute;
@@ -816,7 +816,7 @@ __global__ void ls_v_finish_kernel(int64_t rows, int nblk, const double* __restr
   const double taud = (s.zeta - s.thetatilde * s.tautildeold) / s.rhodold;
   s.delta = s.delta + betacheck * betacheck;
   s.normr = sqrt(s.delta + (s.betad - taud) * (s.betad - taud) + s.betadd * s.betadd);
-  // |A| and cond(A) (lsmr.py:294-306)
+  // |A| and cond(A) (lsmr.py:302-312)
   s.normA2 = s.normA2 + beta * beta;
   s.normA = sqrt(s.normA2);
   s.normA2 = s.normA2 + alpha * alpha;
@@ -854,7 +854,7 @@ __global__ __launch_bounds__(kB) void ls_x_kernel(int64_t n, const TV* __restric
   if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = xx;
 }
 
-// |x| and the stopping rules (lsmr.py:309-325).
+// |x| and the stopping rules (lsmr.py:313-329).
 __global__ void ls_x_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part, LsParams prm,
                                    lxa_lsmr_row* __restrict__ st) {
   const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -437,12 +437,12 @@ def bicgstab(mv, b, rtol, atol, *, max_steps=None, y0=None, precond=None, norm=m
 
 
 # ---------------------------------------------------------------------------
-# LSMR (lineax/_solver/lsmr.py:92-396)
+# LSMR (lineax/_solver/lsmr.py:94-409)
 # ---------------------------------------------------------------------------
 
 
 def _givens(a, b):
-    """lsmr.py:373-415: c, s, r with [c -s; s c] [a; b] = [r; 0]."""
+    """lsmr.py:361-409: c, s, r with [c -s; s c] [a; b] = [r; 0]."""
     if a == 0.0 or b == 0.0:
         if b == 0.0:
             return float(np.sign(a)), 0.0, abs(a)
@@ -462,7 +462,7 @@ def lsmr(mv, rmv, b, n, rtol, atol, *, max_steps=None, conlim=1e8, x0=None, stat
     """Returns (x, result, num_steps, istop), following lsmr.py line by line
     (damp = 0, two_norm): initialisation 92-187, the loop body 189-333,
     RESULTS 347-362.  ``mv``/``rmv`` apply A and A^T.  With ``stats`` a
-    fifth element holds the estimates of lsmr.py:335-343 (norm_r, norm_Ar,
+    fifth element holds the estimates of lsmr.py:334-342 (norm_r, norm_Ar,
     norm_A = sqrt(normA2), cond_A)."""
     b = np.asarray(b, np.float64)
     m = b.size

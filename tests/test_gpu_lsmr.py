@@ -6,7 +6,7 @@ its solution to 1e-8 relative with the same istop and RESULTS code and a
 step count within one (inner products are summed in a different order);
 tall, wide, square and rank-deficient operators, batch rows that stop after
 different numbers of steps, an initial guess, every stopping rule and the
-RESULTS mapping of lsmr.py:347-362 are covered.
+RESULTS mapping of lsmr.py:344-357 are covered.
 """
 
 import numpy as np
@@ -49,7 +49,7 @@ def test_lsmr_dense_vs_oracle(device, rng, shape):
 def test_lsmr_estimates(device, rng):
     a = rng.standard_normal((80, 30))
     b = rng.standard_normal(80)
-    # a short run, orthogonality intact: the estimates of lsmr.py:264-306
+    # a short run, orthogonality intact: the estimates of lsmr.py:274-312
     # match the oracle's and are the true |r| and |A^T r|
     sol = _solve(a, b, lx.LSMR(0.0, 0.0, max_steps=6), device)
     _, _, steps_ref, _, st = _oracle(a, b, 0.0, 0.0, max_steps=6, stats=True)
@@ -120,7 +120,7 @@ def test_lsmr_initial_guess(device, rng):
 def test_lsmr_max_steps_rules(device, rng):
     a = rng.standard_normal((40, 40)) + 0.1 * np.eye(40)
     b = rng.standard_normal(40)
-    # max_steps given with tolerances: max_steps_reached (lsmr.py:352-357)
+    # max_steps given with tolerances: max_steps_reached (lsmr.py:348-353)
     sol = _solve(a, b, lx.LSMR(1e-14, 1e-14, max_steps=5), device, throw=False)
     x_ref, res_ref, steps_ref, istop_ref = _oracle(a, b, 1e-14, 1e-14, max_steps=5)
     assert (res_ref, steps_ref, istop_ref) == (O.MAX_STEPS_REACHED, 5, 4)
@@ -136,7 +136,7 @@ def test_lsmr_max_steps_rules(device, rng):
 
 def test_lsmr_default_cap_is_singular(device, rng):
     # without max_steps the cap is 10 min(m, n) and reaching it is reported
-    # as singular (lsmr.py:347-351): singular values down to 1e-12, no conlim
+    # as singular (lsmr.py:344-347): singular values down to 1e-12, no conlim
     # stop and tolerances the finite-precision recurrence does not reach
     n = 10
     q = np.linalg.qr(rng.standard_normal((2 * n, n)))[0]

@@ -294,7 +294,7 @@ def test_lsmr_rules():
 
 
 def test_lsmr_estimates_pinned(rng):
-    # After a few steps (orthogonality intact) the estimates of lsmr.py:264-306
+    # After a few steps (orthogonality intact) the estimates of lsmr.py:274-312
     # are the true |r_k| and |A^T r_k| (Fong and Saunders 2011, sections 3-4)
     # and agree with scipy's LSMR after the same number of steps; |A| is the
     # Frobenius norm of the bidiagonal B_k, bounded by |A|_F.
