@@ -290,7 +290,7 @@ int lxa_lsmr_result(void* stream, int64_t rows, const lxa_lsmr_params* prm, lxa_
 /* ---------------------------------------------------------------------------
  * Thin singular value decomposition of a batch of dense matrices, A = U
  * diag(s) V^T with s descending (jax.scipy.linalg.svd(full_matrices=False)
- * in lineax/_solver/svd.py:43), by one-sided Jacobi rotations: one
+ * in lineax/_solver/svd.py:51), by one-sided Jacobi rotations: one
  * workgroup per matrix.  a: batch matrices m x n row-major (a_stride
  * elements apart); u: [batch][m][k], s: [batch][k], vt: [batch][k][n],
  * k = min(m, n) <= 512; work: lxa_svd_workspace bytes; sweeps[b] (may be

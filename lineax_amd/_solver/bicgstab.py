@@ -12,15 +12,15 @@ from .._structure import batch_shape_of, ravel, structure_equal, unravel
 from .cg import GRAPH_BLOCK, MAX_CHECK_BLOCK, FlatOperator, norm_kind
 from .misc import preconditioner_and_y0
 
-# Breakdown threshold of lineax without jax_enable_x64 (bicgstab.py:117-119);
-# float64 solves test for exact zeros (bicgstab.py:115-116).
+# Breakdown threshold of lineax without jax_enable_x64 (bicgstab.py:112-113);
+# float64 solves test for exact zeros (bicgstab.py:110-111).
 F32_BREAKDOWN = 1e-16
 
 
 class BiCGStab(AbstractLinearSolver):
     """Biconjugate gradient stabilised method with right preconditioning.
 
-    Semantics of lineax/_solver/bicgstab.py:34-213: the iteration of
+    Semantics of lineax/_solver/bicgstab.py:35-222: the iteration of
     jax.scipy.sparse.linalg.bicgstab (body 143-167), the dual stopping test
     in the residual and the update (cond_fun 121-140), the breakdown test on
     omega, alpha and rho, max_steps = 10 * size by default and the RESULTS

@@ -40,10 +40,10 @@ class Normal(AbstractLinearSolver):
     the inner operator's matrix A^H A (or A A^H) is one device GEMM and the
     factorisation runs on the Cholesky kernels; for other inner solvers
     (e.g. CG) the operator is only linearised and the products stay
-    matrix-free (normal.py:106-120).  ``compute`` applies A^H before (tall)
-    or after (wide) the inner solve (normal.py:122-138); ``transpose`` and
+    matrix-free (normal.py:105-120).  ``compute`` applies A^H before (tall)
+    or after (wide) the inner solve (normal.py:122-140); ``transpose`` and
     ``conj`` conjugate the inner state and swap or keep the tall flag
-    (normal.py:140-178).  State: ``(inner_state, tall,
+    (normal.py:142-179).  State: ``(inner_state, tall,
     operator_conj_transpose, inner_options)``.
     """
 

@@ -16,11 +16,11 @@ class SVD(AbstractLinearSolver):
     """Pseudoinverse solve through a thin SVD, for any operator (nonsquare or
     singular), as scipy.linalg.lstsq.
 
-    ``init`` factors the operator's matrix (lineax/_solver/svd.py:41-45);
+    ``init`` factors the operator's matrix (lineax/_solver/svd.py:49-53);
     ``compute`` keeps the singular values above rcond * s_max (rcond defaults
-    to 2 eps max(m, n); lineax/_solver/svd.py:47-71) and returns
+    to 2 eps max(m, n); lineax/_solver/svd.py:55-78) and returns
     V diag(1/s) U^T b with the rank in the stats; ``transpose`` swaps the
-    factors (svd.py:73-80).  State: ``((u, s, vt), packed_structures)``.
+    factors (svd.py:80-93).  State: ``((u, s, vt), packed_structures)``.
     """
 
     def __init__(self, rcond: float | None = None):
@@ -37,7 +37,7 @@ class SVD(AbstractLinearSolver):
         m, n = u.shape[-2], vt.shape[-1]
         rcond = resolve_rcond(self.rcond, n, m, s.dtype)
         cutoff = rcond * s[..., :1] if s.shape[-1] > 0 else torch.zeros_like(s[..., :1])
-        # strictly greater, so an all-zero matrix has rank 0 (svd.py:60-61)
+        # strictly greater, so an all-zero matrix has rank 0 (svd.py:70-71)
         mask = s > cutoff
         rank = mask.sum(dim=-1)
         s_inv = torch.where(mask, 1.0 / torch.where(mask, s, torch.ones_like(s)), torch.zeros_like(s))

@@ -363,10 +363,10 @@ __global__ void cg_result_kernel(int64_t rows, CgParams prm, lxa_krylov_row* __r
 //   omega:     omega = <s, t> / <t, t>
 //   update:    diff = alpha x + omega z, y += diff, r = s - omega t, the
 //              next rho_new = <r0, r>, the norms, step += 1 and cond_fun
-//              (bicgstab.py:113-140) on the carry (omega, alpha, rho).
+//              (bicgstab.py:107-168) on the carry (omega, alpha, rho).
 // ---------------------------------------------------------------------------
 
-// Breakdown test (bicgstab.py:113-119): exact zeros when threshold == 0 (a
+// Breakdown test (bicgstab.py:107-113): exact zeros when threshold == 0 (a
 // float64 solve, as with jax_enable_x64), else value < threshold.
 __device__ __forceinline__ bool bcg_breakdown(double omega, double alpha, double rho, double threshold) {
   if (threshold == 0.0) return omega == 0.0 || alpha == 0.0 || rho == 0.0;
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(kB) void bcg_begin_kernel(int64_t n, const TY* __re
   if (threadIdx.x == 0) part[row * gridDim.x + blockIdx.x] = acc;
 }
 
-// The initial carry (bicgstab.py:169-180): alpha = omega = rho = 1, diff =
+// The initial carry (bicgstab.py:170-183): alpha = omega = rho = 1, diff =
 // inf, so the first cond_fun is step < max_steps.
 __global__ void bcg_begin_finish_kernel(int64_t rows, int nblk, const double* __restrict__ part, CgParams prm,
                                         lxa_krylov_row* __restrict__ st) {
@@ -547,7 +547,7 @@ __global__ void bcg_update_finish_kernel(int64_t rows, int nblk, int64_t n, cons
   st[row] = s;
 }
 
-// RESULTS (bicgstab.py:185-197): the step cap as for CG, then breakdown
+// RESULTS (bicgstab.py:188-203): the step cap as for CG, then breakdown
 // when the last carry broke down without converging.
 __global__ void bcg_result_kernel(int64_t rows, CgParams prm, double threshold, lxa_krylov_row* __restrict__ st) {
   const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;

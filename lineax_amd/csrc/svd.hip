@@ -1,6 +1,6 @@
 // Singular value decomposition of a batch of dense matrices by one-sided
 // (Hestenes) Jacobi rotations: the factorisation behind lineax's SVD solver
-// (lineax/_solver/svd.py:41-45, jax.scipy.linalg.svd with
+// (lineax/_solver/svd.py:49-53, jax.scipy.linalg.svd with
 // full_matrices=False).
 //
 // One workgroup per matrix.  G is A (m >= n) or A^T (m < n), R x C with

@@ -176,7 +176,7 @@ def chol_uneven(ns, a_off, a32, b_off, b32, threads=1):
 
 
 def normal_solve(a, b):
-    """lx.Normal(lx.Cholesky()) (lineax/_solver/normal.py:106-138): for m >= n
+    """lx.Normal(lx.Cholesky()) (lineax/_solver/normal.py:105-140): for m >= n
     x = (A^T A)^-1 A^T b, else x = A^T (A A^T)^-1 b, each Gram matrix factored
     by potrf 'U' and solved by potrs; float64."""
     a = _f64c(a)
@@ -190,7 +190,7 @@ def normal_solve(a, b):
 
 
 def svd_solve(a, b, rcond=None):
-    """lx.SVD() (lineax/_solver/svd.py:41-71): thin SVD, singular values kept
+    """lx.SVD() (lineax/_solver/svd.py:49-78): thin SVD, singular values kept
     when s > rcond * s_max (rcond default 2 eps max(m, n),
     lineax/_misc.py:30-38), x = V diag(1/s) U^T b; returns (x, rank)."""
     a = np.asarray(a, np.float64)

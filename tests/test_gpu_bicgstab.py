@@ -66,7 +66,7 @@ def test_bicgstab_float32_working_vectors(device, rng):
 
 def test_bicgstab_float32_breakdown_rule(device, rng):
     # Without jax_enable_x64 lineax tests omega, alpha, rho < 1e-16 with no
-    # absolute value (bicgstab.py:117-119): a negative one on a nonsymmetric
+    # absolute value (bicgstab.py:112-113): a negative one on a nonsymmetric
     # operator ends the solve with RESULTS.breakdown, as in the oracle run
     # under the same rule.
     a = _nonsymmetric(rng, 60)
