@@ -120,6 +120,9 @@ int lxa_chol_solve(void* stream, int dtype, int64_t batch, int64_t n,
  * b_i, lx.Cholesky()) (lineax/_solver/cholesky.py:43-78), evaluated for the
  * exact float64 value of the float32 data when max_refine > 0. */
 #define LXA_UNEVEN_PERSISTENT 1
+/* Bytes of workspace the persistent launch needs (the work-queue counters,
+ * 16 bytes whatever the count); lxa_chol_uneven_solve refuses a smaller
+ * workspace with LXA_EINVAL. */
 int lxa_chol_uneven_workspace_size(int64_t count, size_t* bytes);
 int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n,
                           const int32_t* n, const int64_t* a_offset,
@@ -129,6 +132,27 @@ int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n,
                           double tol, int32_t* info, int32_t* iters,
                           double* eta, void* workspace,
                           size_t workspace_bytes);
+
+/* lxa_chol_uneven_solve without a workspace argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  It has no counterpart in the
+ * reference.  Same arguments, semantics and results as lxa_chol_uneven_solve;
+ * the work-queue counters live in a 256-byte device array inside the
+ * library (one per GPU, held until the process ends), whose address is
+ * looked up on the first call on each device.  Calls allocate nothing (no
+ * hipMalloc, hipFree or allocating library routine) and refuse a request
+ * whose counters would exceed that array (LXA_EINVAL).  Each persistent
+ * launch clears its counter with a fill kernel on `stream` first.  All calls
+ * on one GPU share the array: calls that overlap on one GPU (different
+ * streams, or host threads without ordering) race, and must be serialised
+ * by the caller. */
+int lxa_chol_uneven_solve_noalloc(void* stream, int64_t count, int max_n,
+                                  const int32_t* n, const int64_t* a_offset,
+                                  const float* a, const int64_t* b_offset,
+                                  const float* b, void* x, int x_dtype,
+                                  const int32_t* order, int flags,
+                                  int max_refine, double tol, int32_t* info,
+                                  int32_t* iters, double* eta);
 
 /* ------------------------------------------------------------------------
  * Operators and result mapping

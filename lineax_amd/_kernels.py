@@ -44,6 +44,9 @@ _SIGNATURES = {
         _vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _int, _int, _dbl,
         _vp, _vp, _vp, _vp, ctypes.c_size_t,
     ],
+    "lxa_chol_uneven_solve_noalloc": [
+        _vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _int, _int, _dbl, _vp, _vp, _vp,
+    ],
     "lxa_gemv_workspace_size": [_int, _i64, _i64, _i64, _int, _szp],
     "lxa_gemv": [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int, _vp, ctypes.c_size_t],
     "lxa_solve_results": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp],
@@ -316,11 +319,14 @@ def chol_uneven_solve(
     iters: torch.Tensor | None = None,
     eta: torch.Tensor | None = None,
     workspace: torch.Tensor | None = None,
+    noalloc: bool = False,
 ) -> None:
     """Raw launch of lxa_chol_uneven_solve (see the header for semantics).
 
     Arguments are device tensors already laid out; ``order`` may list a
-    subset of the systems (one size bucket, for instance).
+    subset of the systems (one size bucket, for instance).  ``noalloc``
+    calls lxa_chol_uneven_solve_noalloc instead (queue counters in the
+    library's own device array; no workspace).
     """
     for t, what in ((n, "n"), (a_offset, "a_offset"), (a, "a"), (b_offset, "b_offset"), (b, "b"),
                     (x, "x"), (order, "order"), (info, "info")):
@@ -334,6 +340,16 @@ def chol_uneven_solve(
     xdt = _dtype_code(x.dtype)
     count = order.numel()
     lib = load()
+    flags = UNEVEN_PERSISTENT if persistent else 0
+    if noalloc:
+        _check(
+            lib.lxa_chol_uneven_solve_noalloc(
+                _stream(a), count, int(max_n), _ptr(n), _ptr(a_offset), _ptr(a), _ptr(b_offset), _ptr(b),
+                _ptr(x), xdt, _ptr(order), flags, int(max_refine), float(tol), _ptr(info), _ptr(iters), _ptr(eta),
+            ),
+            "lxa_chol_uneven_solve_noalloc",
+        )
+        return
     ws_ptr, ws_bytes = None, 0
     if persistent:
         need = ctypes.c_size_t(0)
@@ -344,7 +360,7 @@ def chol_uneven_solve(
     _check(
         lib.lxa_chol_uneven_solve(
             _stream(a), count, int(max_n), _ptr(n), _ptr(a_offset), _ptr(a), _ptr(b_offset), _ptr(b),
-            _ptr(x), xdt, _ptr(order), UNEVEN_PERSISTENT if persistent else 0, int(max_refine), float(tol),
+            _ptr(x), xdt, _ptr(order), flags, int(max_refine), float(tol),
             _ptr(info), _ptr(iters), _ptr(eta), ws_ptr, ws_bytes,
         ),
         "lxa_chol_uneven_solve",

@@ -23,6 +23,8 @@
 // Work arrives either from a device queue (persistent launch: workgroups
 // pull the next system; callers order the queue largest first) or one
 // system per workgroup.
+#include <mutex>
+
 #include "lxa_common.h"
 
 namespace lxa {
@@ -305,7 +307,6 @@ __device__ void wave_solve(int n, const Lds& s, float (&v)[NC]) {
       for (int u = 0; u < kG; ++u) {
         const int kk = g + u;
         if (kk < kend) {
-          const int k = 64 * c0 + kk;
           const float yk = bcast(v[c0], kk) * ra[u];
           if (l == kk) v[c0] = yk;
 #pragma unroll
@@ -713,29 +714,68 @@ int chol_factor_regs_f32(hipStream_t s, int64_t batch, int n, float* a, int64_t 
 
 }  // namespace lxa
 
-extern "C" int lxa_chol_uneven_workspace_size(int64_t count, size_t* bytes) {
-  LXA_REQUIRE(count >= 0 && bytes, "lxa_chol_uneven_workspace_size: bad argument");
-  *bytes = 256;  // work-queue counter
+namespace lxa {
+
+namespace {
+
+// Work-queue counters of the persistent launches: one per register-tile class.
+constexpr int kQueueCounters = 3;
+constexpr size_t kUnevenWorkspaceBytes = 16;  // kQueueCounters 4-byte counters, rounded up
+static_assert(kQueueCounters * sizeof(unsigned int) <= kUnevenWorkspaceBytes, "workspace holds the queue counters");
+
+// Device scratch of lxa_chol_uneven_solve_noalloc: the queue counters live in
+// this file-scope array instead of a caller's workspace.  Each GPU the
+// library runs on holds its own copy until the process ends.  Host code uses
+// the address hipGetSymbolAddress returns for the current device, never the
+// array's name (a host-side stand-in, not a device address).
+constexpr size_t kNoAllocScratchBytes = 256;
+static_assert(kUnevenWorkspaceBytes <= kNoAllocScratchBytes, "queue counters exceed the no-alloc scratch");
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept; 0 on success.
+int noalloc_scratch(unsigned int** out) {
+  static std::mutex mu;
+  static void* ptr[kMaxDevices] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_error(e, "lxa_chol_uneven_solve_noalloc: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(LXA_EUNSUPPORTED, "lxa_chol_uneven_solve_noalloc: device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_error(e, "lxa_chol_uneven_solve_noalloc: hipGetSymbolAddress(scratch)");
+    if (!p) return set_error(LXA_EHIP, "lxa_chol_uneven_solve_noalloc: null scratch address");
+    ptr[dev] = p;
+  }
+  *out = static_cast<unsigned int*>(ptr[dev]);
   return LXA_OK;
 }
 
-extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, const int32_t* n,
-                                     const int64_t* a_offset, const float* a, const int64_t* b_offset,
-                                     const float* b, void* x, int x_dtype, const int32_t* order, int flags,
-                                     int max_refine, double tol, int32_t* info, int32_t* iters, double* eta,
-                                     void* workspace, size_t workspace_bytes) {
-  LXA_REQUIRE(count >= 0 && count <= lxa::kMaxGrid, "lxa_chol_uneven_solve: bad count");
-  LXA_REQUIRE(max_n >= 1 && max_n <= lxa::kMaxN, "lxa_chol_uneven_solve: max_n must be in [1, %d]", lxa::kMaxN);
+int uneven_check(int64_t count, int max_n, const int32_t* n, const int64_t* a_offset, const float* a,
+                 const int64_t* b_offset, const float* b, void* x, int x_dtype, const int32_t* order,
+                 int max_refine, int32_t* info) {
+  LXA_REQUIRE(count >= 0 && count <= kMaxGrid, "lxa_chol_uneven_solve: bad count");
+  LXA_REQUIRE(max_n >= 1 && max_n <= kMaxN, "lxa_chol_uneven_solve: max_n must be in [1, %d]", kMaxN);
   LXA_REQUIRE(n && a_offset && a && b_offset && b && x && order && info, "lxa_chol_uneven_solve: null pointer");
   LXA_REQUIRE(x_dtype == LXA_F32 || x_dtype == LXA_F64, "lxa_chol_uneven_solve: bad x_dtype");
   LXA_REQUIRE(max_refine >= 0 && max_refine <= 64, "lxa_chol_uneven_solve: max_refine in [0, 64]");
   LXA_REQUIRE(max_refine == 0 || x_dtype == LXA_F64, "lxa_chol_uneven_solve: refinement needs x_dtype F64");
+  return LXA_OK;
+}
+
+// The launches of one uneven solve; `queue` holds kQueueCounters counters
+// (persistent launches only; each launch clears its own with a fill kernel).
+int uneven_launch(hipStream_t s, int64_t count, int max_n, const int32_t* n, const int64_t* a_offset,
+                  const float* a, const int64_t* b_offset, const float* b, void* x, int x_dtype,
+                  const int32_t* order, int flags, int max_refine, double tol, int32_t* info, int32_t* iters,
+                  double* eta, unsigned int* queue) {
   const bool persistent = (flags & LXA_UNEVEN_PERSISTENT) != 0;
-  LXA_REQUIRE(!persistent || (workspace && workspace_bytes >= 8), "lxa_chol_uneven_solve: workspace required");
-  if (count == 0) return LXA_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
   const bool refine = max_refine > 0;
-  unsigned int* queue = static_cast<unsigned int*>(workspace);
   // Three register-tile classes: orders <= 128 run with 8 x 8 tiles (three
   // systems per compute unit), 129..176 with 11 x 11 (two per compute unit:
   // their LDS fits twice in 160 KB), larger ones with 16 x 16 (one).  A
@@ -744,17 +784,17 @@ extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, con
   // system uses the class that covers max_n.
   const int split = 128, split2 = 176;
   int rc = LXA_OK;
-#define LXA_UNEVEN_CLASS(KR, LO, HI, REPORT, Q)                                                                    \
+#define LXA_UNEVEN_CLASS(KR, LO, HI, REPORT, Q)                                                                  \
   do {                                                                                                           \
     if (refine)                                                                                                  \
-      rc = lxa::launch_class<KR, true, double>(s, count, max_n, LO, HI, REPORT, n, a_offset, a, b_offset, b, x,    \
-                                               order, persistent, Q, max_refine, tol, info, iters, eta);        \
+      rc = launch_class<KR, true, double>(s, count, max_n, LO, HI, REPORT, n, a_offset, a, b_offset, b, x,        \
+                                          order, persistent, Q, max_refine, tol, info, iters, eta);             \
     else if (x_dtype == LXA_F64)                                                                                 \
-      rc = lxa::launch_class<KR, false, double>(s, count, max_n, LO, HI, REPORT, n, a_offset, a, b_offset, b, x,   \
-                                                order, persistent, Q, 0, tol, info, iters, eta);                \
+      rc = launch_class<KR, false, double>(s, count, max_n, LO, HI, REPORT, n, a_offset, a, b_offset, b, x,       \
+                                           order, persistent, Q, 0, tol, info, iters, eta);                     \
     else                                                                                                         \
-      rc = lxa::launch_class<KR, false, float>(s, count, max_n, LO, HI, REPORT, n, a_offset, a, b_offset, b, x,    \
-                                               order, persistent, Q, 0, tol, info, iters, eta);                 \
+      rc = launch_class<KR, false, float>(s, count, max_n, LO, HI, REPORT, n, a_offset, a, b_offset, b, x,        \
+                                          order, persistent, Q, 0, tol, info, iters, eta);                      \
   } while (0)
   if (!persistent) {
     if (max_n > split2) LXA_UNEVEN_CLASS(16, 0, max_n, 1, queue);
@@ -777,4 +817,48 @@ extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, con
   if (rc) return rc;
   LXA_LAUNCH_CHECK("lxa_chol_uneven_solve launch");
   return LXA_OK;
+}
+
+}  // namespace
+}  // namespace lxa
+
+extern "C" int lxa_chol_uneven_workspace_size(int64_t count, size_t* bytes) {
+  LXA_REQUIRE(count >= 0 && bytes, "lxa_chol_uneven_workspace_size: bad argument");
+  *bytes = lxa::kUnevenWorkspaceBytes;  // work-queue counters
+  return LXA_OK;
+}
+
+extern "C" int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n, const int32_t* n,
+                                     const int64_t* a_offset, const float* a, const int64_t* b_offset,
+                                     const float* b, void* x, int x_dtype, const int32_t* order, int flags,
+                                     int max_refine, double tol, int32_t* info, int32_t* iters, double* eta,
+                                     void* workspace, size_t workspace_bytes) {
+  if (int rc = lxa::uneven_check(count, max_n, n, a_offset, a, b_offset, b, x, x_dtype, order, max_refine, info))
+    return rc;
+  const bool persistent = (flags & LXA_UNEVEN_PERSISTENT) != 0;
+  LXA_REQUIRE(!persistent || (workspace && workspace_bytes >= lxa::kUnevenWorkspaceBytes),
+              "lxa_chol_uneven_solve: the persistent launch needs a workspace of %zu bytes (got %zu)",
+              lxa::kUnevenWorkspaceBytes, workspace ? workspace_bytes : static_cast<size_t>(0));
+  if (count == 0) return LXA_OK;
+  return lxa::uneven_launch(static_cast<hipStream_t>(stream), count, max_n, n, a_offset, a, b_offset, b, x, x_dtype,
+                            order, flags, max_refine, tol, info, iters, eta, static_cast<unsigned int*>(workspace));
+}
+
+extern "C" int lxa_chol_uneven_solve_noalloc(void* stream, int64_t count, int max_n, const int32_t* n,
+                                             const int64_t* a_offset, const float* a, const int64_t* b_offset,
+                                             const float* b, void* x, int x_dtype, const int32_t* order, int flags,
+                                             int max_refine, double tol, int32_t* info, int32_t* iters,
+                                             double* eta) {
+  if (int rc = lxa::uneven_check(count, max_n, n, a_offset, a, b_offset, b, x, x_dtype, order, max_refine, info))
+    return rc;
+  size_t need = 0;
+  if (int rc = lxa_chol_uneven_workspace_size(count, &need)) return rc;
+  LXA_REQUIRE(need <= lxa::kNoAllocScratchBytes,
+              "lxa_chol_uneven_solve_noalloc: needs %zu bytes of scratch, more than its %zu", need,
+              lxa::kNoAllocScratchBytes);
+  unsigned int* queue = nullptr;
+  if (int rc = lxa::noalloc_scratch(&queue)) return rc;
+  if (count == 0) return LXA_OK;
+  return lxa::uneven_launch(static_cast<hipStream_t>(stream), count, max_n, n, a_offset, a, b_offset, b, x, x_dtype,
+                            order, flags, max_refine, tol, info, iters, eta, queue);
 }
