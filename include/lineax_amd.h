@@ -318,7 +318,10 @@ int lxa_lsmr_result(void* stream, int64_t rows, const lxa_lsmr_params* prm, lxa_
  * workgroup per matrix.  a: batch matrices m x n row-major (a_stride
  * elements apart); u: [batch][m][k], s: [batch][k], vt: [batch][k][n],
  * k = min(m, n) <= 512; work: lxa_svd_workspace bytes; sweeps[b] (may be
- * NULL) receives the Jacobi sweeps used (max_sweeps when not converged).
+ * NULL) receives the Jacobi sweeps used, or max_sweeps + 1 when the last
+ * allowed sweep still rotated (not converged).  Singular values are sorted
+ * descending with NaN above every number and ties by column, so every
+ * output slot is written.
  * ------------------------------------------------------------------------- */
 int lxa_svd_workspace(int dtype, int64_t batch, int64_t m, int64_t n, size_t* bytes);
 int lxa_svd(void* stream, int dtype, int64_t batch, int64_t m, int64_t n, const void* a, int64_t a_stride, void* u,

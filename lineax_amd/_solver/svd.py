@@ -20,18 +20,23 @@ class SVD(AbstractLinearSolver):
     ``compute`` keeps the singular values above rcond * s_max (rcond defaults
     to 2 eps max(m, n); lineax/_solver/svd.py:55-78) and returns
     V diag(1/s) U^T b with the rank in the stats; ``transpose`` swaps the
-    factors (svd.py:80-93).  State: ``((u, s, vt), packed_structures)``.
+    factors (svd.py:80-93).  State: ``((u, s, vt), packed_structures,
+    converged)``; a Jacobi iteration that did not converge within its sweep
+    budget (lineax's LAPACK SVD reports nothing) gives
+    ``RESULTS.max_steps_reached`` rather than passing inexact factors off as
+    successful.
     """
 
     def __init__(self, rcond: float | None = None):
         self.rcond = rcond
 
     def init(self, operator, options):
-        u, s, vt, _ = K.svd(operator.as_matrix())
-        return (u, s, vt), pack_structures(operator)
+        u, s, vt, sweeps = K.svd(operator.as_matrix())
+        converged = sweeps <= K.SVD_MAX_SWEEPS
+        return (u, s, vt), pack_structures(operator), converged
 
     def compute(self, state, vector, options):
-        (u, s, vt), packed = state
+        (u, s, vt), packed, converged = state
         flat, batch = ravel_vector(vector, packed)
         flat = flat.to(u.dtype)
         m, n = u.shape[-2], vt.shape[-1]
@@ -43,15 +48,16 @@ class SVD(AbstractLinearSolver):
         s_inv = torch.where(mask, 1.0 / torch.where(mask, s, torch.ones_like(s)), torch.zeros_like(s))
         utb = matvec(u.mT, flat)
         solution = matvec(vt.mT, s_inv * utb)
-        return unravel_solution(solution, packed, tuple(solution.shape[:-1])), RESULTS.successful, {"rank": rank}
+        result = torch.where(converged, RESULTS.successful, RESULTS.max_steps_reached).to(torch.int32)
+        return unravel_solution(solution, packed, tuple(solution.shape[:-1])), result, {"rank": rank}
 
     def transpose(self, state, options):
-        (u, s, vt), packed = state
-        return ((vt.mT, s, u.mT), transpose_packed_structures(packed)), {}
+        (u, s, vt), packed, converged = state
+        return ((vt.mT, s, u.mT), transpose_packed_structures(packed), converged), {}
 
     def conj(self, state, options):
-        (u, s, vt), packed = state
-        return ((u.conj(), s, vt.conj()), packed), {}
+        (u, s, vt), packed, converged = state
+        return ((u.conj(), s, vt.conj()), packed, converged), {}
 
     def assume_full_rank(self):
         return False

@@ -10,8 +10,8 @@
 // LAPACK's getrf output format, so jax.scipy's lu_solve semantics apply.
 //
 // Larger orders use a one-workgroup-per-system right-looking kernel working
-// in global memory (correctness path for moderate n; the blocked MFMA path
-// in lu_blocked.hip serves large single systems).
+// in global memory (correctness path for moderate n; single large float32
+// systems take the blocked path of lu_large.hip and gemm.hip instead).
 #include <climits>
 
 #include "lxa_common.h"

@@ -120,10 +120,19 @@ __global__ __launch_bounds__(kSvdThreads) void svd_jacobi_kernel(int m, int n, c
     if (tl == 0) s_norm[c] = sqrt(nn);
   }
   __syncthreads();
+  // Descending order with NaN above every number and ties broken by column
+  // index: a total order, so the ranks are a permutation and every output
+  // slot is written.
   for (int c = tid; c < C; c += kSvdThreads) {
     const double v = s_norm[c];
+    const bool vnan = isnan(v);
     int rk = 0;
-    for (int j = 0; j < C; ++j) rk += (s_norm[j] > v || (s_norm[j] == v && j < c)) ? 1 : 0;
+    for (int j = 0; j < C; ++j) {
+      const double w = s_norm[j];
+      const bool wnan = isnan(w);
+      const bool above = wnan ? (!vnan || j < c) : (!vnan && (w > v || (w == v && j < c)));
+      rk += above ? 1 : 0;
+    }
     s_rank[c] = rk;
     sv[mat * C + rk] = static_cast<T>(v);
   }
@@ -151,7 +160,8 @@ __global__ __launch_bounds__(kSvdThreads) void svd_jacobi_kernel(int m, int n, c
     else
       Um[(int64_t)j * C + k] = V[idx];
   }
-  if (tid == 0 && sweeps_out) sweeps_out[mat] = sweep;
+  // sweeps run, or max_sweeps + 1 when the last sweep still rotated (not converged)
+  if (tid == 0 && sweeps_out) sweeps_out[mat] = s_rot ? max_sweeps + 1 : sweep;
 }
 
 }  // namespace

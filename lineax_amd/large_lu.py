@@ -107,6 +107,9 @@ class LargeLU:
         self.perm = K.lu_large_perm(self.piv)
         self.lu = lu
         self.sync = torch.empty(self.nblk + 2, dtype=torch.int32, device=dev)
+        # set on the device when a block triangular solve gave up waiting for a
+        # dependency (the kernel's bounded spin); checked once per solve
+        self.trsv_failed = torch.zeros(1, dtype=torch.int32, device=dev)
         self.norm_inf = self.a.abs().sum(dim=1, dtype=torch.float64).max()
 
     def apply_inverse(self, v: torch.Tensor, trans: bool = False) -> torch.Tensor:
@@ -120,12 +123,9 @@ class LargeLU:
             w = K.lu_large_permute(w, self.perm, inverse=False)
         for r0 in range(0, rows, MAX_RHS_PER_SOLVE):
             chunk = w[r0: r0 + MAX_RHS_PER_SOLVE]
-            if trans:
-                K.lu_large_trsv(2, self.lu, self.uinv, chunk, self.sync)
-                K.lu_large_trsv(3, self.lu, self.linv, chunk, self.sync)
-            else:
-                K.lu_large_trsv(0, self.lu, self.linv, chunk, self.sync)
-                K.lu_large_trsv(1, self.lu, self.uinv, chunk, self.sync)
+            for kind, inv in (((2, self.uinv), (3, self.linv)) if trans else ((0, self.linv), (1, self.uinv))):
+                K.lu_large_trsv(kind, self.lu, inv, chunk, self.sync)
+                torch.maximum(self.trsv_failed, self.sync[self.nblk + 1:], out=self.trsv_failed)
         if trans:
             w = K.lu_large_permute(w, self.perm, inverse=True)
         K.rbt_vectors(w, last, form=0)
@@ -179,7 +179,15 @@ class LargeLU:
             )
             gmres_steps += int(sol.stats["num_steps"].max())
             x = x + sol.value
+        self.check_trsv()
         return x, eta, {"residuals": residuals, "gmres_steps": gmres_steps, "zero_pivot": int(self.info.item())}
+
+    def check_trsv(self) -> None:
+        """Raises if any block triangular solve since construction timed out
+        waiting for a dependency (its output would be wrong); synchronises."""
+        if int(self.trsv_failed.item()):
+            raise RuntimeError("lxa_lu_large_trsv: a block waited past its spin limit for the block it depends "
+                               "on; the triangular solve is incomplete")
 
 
 def factor_flops(n: int) -> float:

@@ -154,3 +154,32 @@ def test_uneven_batch_rejects_bad_layout():
         UnevenBatch([300], [0], np.zeros(90000, np.float32), [0], np.zeros(300, np.float32), device="cpu")
     with pytest.raises(ValueError, match="run past"):
         UnevenBatch([4], [10], np.zeros(16, np.float32), [0], np.zeros(4, np.float32), device="cpu")
+
+
+def test_uneven_workspace_minimum_refused_on_host():
+    """A persistent uneven solve with a workspace below the reported size is
+    refused with LXA_EINVAL before anything reaches the device."""
+    if not K.LIB_PATH.exists():
+        pytest.skip("kernel library not built")
+    lib = K.load()
+    need = ctypes.c_size_t(0)
+    assert lib.lxa_chol_uneven_workspace_size(10, ctypes.byref(need)) == 0
+    assert need.value >= 12  # three 4-byte queue counters
+    fake = ctypes.c_void_p(4096)  # never dereferenced: validation fails first
+    rc = lib.lxa_chol_uneven_solve(None, 10, 256, fake, fake, fake, fake, fake, fake, K.F64, fake,
+                                   K.UNEVEN_PERSISTENT, 2, 1e-12, fake, None, None, fake, 8)
+    assert rc == 1  # LXA_EINVAL
+    assert "workspace" in lib.lxa_last_error().decode()
+
+
+def test_large_lu_trsv_timeout_flag_raises():
+    """A block triangular solve that gave up waiting (device flag set) makes
+    the large-LU solve raise instead of returning a wrong answer."""
+    from lineax_amd.large_lu import LargeLU
+
+    fac = LargeLU.__new__(LargeLU)
+    fac.trsv_failed = torch.zeros(1, dtype=torch.int32)
+    fac.check_trsv()
+    fac.trsv_failed = torch.ones(1, dtype=torch.int32)
+    with pytest.raises(RuntimeError, match="spin limit"):
+        fac.check_trsv()
