@@ -120,9 +120,14 @@ int lxa_chol_solve(void* stream, int dtype, int64_t batch, int64_t n,
  * b_i, lx.Cholesky()) (lineax/_solver/cholesky.py:43-78), evaluated for the
  * exact float64 value of the float32 data when max_refine > 0. */
 #define LXA_UNEVEN_PERSISTENT 1
-/* Bytes of workspace the persistent launch needs (the work-queue counters,
- * 16 bytes whatever the count); lxa_chol_uneven_solve refuses a smaller
- * workspace with LXA_EINVAL. */
+/* flags & LXA_UNEVEN_SORTED (with LXA_UNEVEN_PERSISTENT): `order` lists the
+ * systems by n descending.  Each tile class's launch then takes its own
+ * contiguous range of the order instead of sweeping all of it.  The claim is
+ * checked on the device first; an order that is not sorted is swept. */
+#define LXA_UNEVEN_SORTED 2
+/* Bytes of workspace the persistent launch needs (three work-queue counters
+ * and a flag: 16 bytes whatever the count); lxa_chol_uneven_solve refuses a
+ * smaller workspace with LXA_EINVAL. */
 int lxa_chol_uneven_workspace_size(int64_t count, size_t* bytes);
 int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n,
                           const int32_t* n, const int64_t* a_offset,
@@ -132,6 +137,24 @@ int lxa_chol_uneven_solve(void* stream, int64_t count, int max_n,
                           double tol, int32_t* info, int32_t* iters,
                           double* eta, void* workspace,
                           size_t workspace_bytes);
+
+/* The autotuner's hook (no counterpart in the reference):
+ * lxa_chol_uneven_solve with the tile-class boundaries given per call
+ * instead of taken from the library's autotune table: orders <= max4
+ * (<= 64) on 4 x 4 blocks of one wave, <= max8 (<= 128) on 8 x 8 blocks of
+ * four waves, larger on 16 x 16 blocks.  lineax_amd/tune_uneven.py times each
+ * class per order bucket with it and writes the table;
+ * lxa_chol_uneven_tuned_classes reports the table the library was built
+ * with. */
+int lxa_chol_uneven_tune_solve(void* stream, int64_t count, int max_n,
+                               const int32_t* n, const int64_t* a_offset,
+                               const float* a, const int64_t* b_offset,
+                               const float* b, void* x, int x_dtype,
+                               const int32_t* order, int flags, int max_refine,
+                               double tol, int32_t* info, int32_t* iters,
+                               double* eta, void* workspace,
+                               size_t workspace_bytes, int max4, int max8);
+int lxa_chol_uneven_tuned_classes(int* max4, int* max8);
 
 /* lxa_chol_uneven_solve without a workspace argument, for callers that
  * cannot allocate device memory (code under HIP graph capture, serving

@@ -21,6 +21,7 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "_lib" / "liblineax_amd.so"
 F32 = 0
 F64 = 1
 UNEVEN_PERSISTENT = 1
+UNEVEN_SORTED = 2
 
 _lib = None
 _lock = threading.Lock()
@@ -44,6 +45,11 @@ _SIGNATURES = {
         _vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _int, _int, _dbl,
         _vp, _vp, _vp, _vp, ctypes.c_size_t,
     ],
+    "lxa_chol_uneven_tune_solve": [
+        _vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _int, _int, _dbl,
+        _vp, _vp, _vp, _vp, ctypes.c_size_t, _int, _int,
+    ],
+    "lxa_chol_uneven_tuned_classes": [_vp, _vp],
     "lxa_chol_uneven_solve_noalloc": [
         _vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _int, _int, _dbl, _vp, _vp, _vp,
     ],
@@ -320,13 +326,18 @@ def chol_uneven_solve(
     eta: torch.Tensor | None = None,
     workspace: torch.Tensor | None = None,
     noalloc: bool = False,
+    sorted_order: bool = False,
+    classes: tuple[int, int] | None = None,
 ) -> None:
     """Raw launch of lxa_chol_uneven_solve (see the header for semantics).
 
     Arguments are device tensors already laid out; ``order`` may list a
     subset of the systems (one size bucket, for instance).  ``noalloc``
     calls lxa_chol_uneven_solve_noalloc instead (queue counters in the
-    library's own device array; no workspace).
+    library's own device array; no workspace).  ``sorted_order`` declares
+    ``order`` sorted by n descending (LXA_UNEVEN_SORTED).  ``classes``
+    = (max4, max8) runs lxa_chol_uneven_tune_solve with those tile-class
+    boundaries (the autotuner's hook).
     """
     for t, what in ((n, "n"), (a_offset, "a_offset"), (a, "a"), (b_offset, "b_offset"), (b, "b"),
                     (x, "x"), (order, "order"), (info, "info")):
@@ -340,7 +351,7 @@ def chol_uneven_solve(
     xdt = _dtype_code(x.dtype)
     count = order.numel()
     lib = load()
-    flags = UNEVEN_PERSISTENT if persistent else 0
+    flags = (UNEVEN_PERSISTENT | (UNEVEN_SORTED if sorted_order else 0)) if persistent else 0
     if noalloc:
         _check(
             lib.lxa_chol_uneven_solve_noalloc(
@@ -351,12 +362,22 @@ def chol_uneven_solve(
         )
         return
     ws_ptr, ws_bytes = None, 0
-    if persistent:
+    if persistent or classes is not None:
         need = ctypes.c_size_t(0)
         _check(lib.lxa_chol_uneven_workspace_size(count, ctypes.byref(need)), "lxa_chol_uneven_workspace_size")
         if workspace is None or workspace.numel() * workspace.element_size() < need.value:
             workspace = torch.empty(int(need.value), dtype=torch.uint8, device=a.device)
         ws_ptr, ws_bytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    if classes is not None:
+        _check(
+            lib.lxa_chol_uneven_tune_solve(
+                _stream(a), count, int(max_n), _ptr(n), _ptr(a_offset), _ptr(a), _ptr(b_offset), _ptr(b),
+                _ptr(x), xdt, _ptr(order), flags, int(max_refine), float(tol),
+                _ptr(info), _ptr(iters), _ptr(eta), ws_ptr, ws_bytes, int(classes[0]), int(classes[1]),
+            ),
+            "lxa_chol_uneven_tune_solve",
+        )
+        return
     _check(
         lib.lxa_chol_uneven_solve(
             _stream(a), count, int(max_n), _ptr(n), _ptr(a_offset), _ptr(a), _ptr(b_offset), _ptr(b),
@@ -365,6 +386,13 @@ def chol_uneven_solve(
         ),
         "lxa_chol_uneven_solve",
     )
+
+
+def uneven_tuned_classes() -> tuple[int, int]:
+    """(max4, max8): the tile-class boundaries of the library's autotune table."""
+    m4, m8 = ctypes.c_int(0), ctypes.c_int(0)
+    _check(load().lxa_chol_uneven_tuned_classes(ctypes.byref(m4), ctypes.byref(m8)), "lxa_chol_uneven_tuned_classes")
+    return int(m4.value), int(m8.value)
 
 
 # ---------------------------------------------------------------------------

@@ -127,7 +127,7 @@ class UnevenBatch:
                 K.chol_uneven_solve(
                     self.n, self.a_offset, self.a, self.b_offset, self.b, x, self.order_desc,
                     max_n=self.max_n, persistent=True, max_refine=max_refine, tol=tol, info=info,
-                    iters=iters, eta=eta, workspace=self._workspace,
+                    iters=iters, eta=eta, workspace=self._workspace, sorted_order=True,
                 )
             else:
                 for order_n, s, e in self.buckets:
