@@ -402,3 +402,47 @@ def test_gmres_fused_graph_replays_match_eager(device, rng):
         assert torch.equal(g.value, eager.value)
         assert torch.equal(g.stats["num_steps"], eager.stats["num_steps"])
         assert torch.equal(g.result, eager.result)
+
+
+# --- RESULTS rules pinned exactly against the oracle
+
+
+@pytest.mark.parametrize("case", ["rotation", "shift1", "shift3"])
+def test_gmres_stagnation_matches_oracle(device, case):
+    # gmres.py:175-179, 228-230: cycles that do not decrease the residual norm
+    # count towards stagnation_iters; these operators make every cycle's
+    # step exactly zero, so the stop is deterministic.
+    if case == "rotation":
+        a = np.array([[0.0, 1.0], [-1.0, 0.0]])
+        b = np.array([1.0, 0.0])
+        restart, stag = 1, 3
+    else:
+        n = 6
+        a = np.roll(np.eye(n), 1, axis=0)
+        b = np.eye(n)[0]
+        restart, stag = (1 if case == "shift1" else 3), 4
+    x_ref, res_ref, steps_ref = O.gmres(lambda v: a @ v, b, 1e-12, 1e-12, restart=restart, stagnation_iters=stag)
+    assert res_ref == lx.RESULTS.stagnation
+    sol = lx.linear_solve(lx.MatrixLinearOperator(_t(a, device)), _t(b, device),
+                          lx.GMRES(rtol=1e-12, atol=1e-12, restart=restart, stagnation_iters=stag), throw=False)
+    assert int(sol.result) == res_ref
+    assert int(sol.stats["num_steps"]) == steps_ref
+    np.testing.assert_array_equal(sol.value.cpu().numpy(), x_ref)
+
+
+@pytest.mark.parametrize("c", [3e-13, 6e-13])
+def test_cg_breakdown_guard_scale_matches_oracle(device, c):
+    # 100 * rcond = 100 * 2 eps * 10 = 4.44e-13 separates the two cases
+    # (lineax/_misc.py:30-38, cg.py:173-178): NaN step below, a solve above.
+    n = 10
+    b = np.linspace(1.0, 2.0, n)
+    x_ref, res_ref, steps_ref = O.cg(lambda v: c * v, b, 1e-10, 1e-10)
+    op = lx.MatrixLinearOperator(_t(c * np.eye(n), device), lx.positive_semidefinite_tag)
+    sol = lx.linear_solve(op, _t(b, device), lx.CG(rtol=1e-10, atol=1e-10), throw=False)
+    assert int(sol.result) == res_ref
+    x = sol.value.cpu().numpy()
+    if np.isnan(x_ref).any():
+        assert np.isnan(x).all()
+    else:
+        np.testing.assert_allclose(x, x_ref, rtol=1e-12)
+        assert abs(int(sol.stats["num_steps"]) - steps_ref) <= 1

@@ -71,6 +71,24 @@ def test_cg_max_steps_rules(rng):
     assert result in (O.SINGULAR,)
 
 
+@pytest.mark.parametrize("c, nan", [(3e-13, True), (6e-13, False)])
+def test_cg_breakdown_guard_scale(c, nan):
+    # lineax's guard NaNs the step unless |<Ap, p>| > 100 rcond |gamma|, with
+    # rcond = 2 eps max(n, m) (lineax/_misc.py:30-38, cg.py:173-178).  On
+    # c I (n = 10, float64) the first step has <Ap, p> / gamma = c, and
+    # 100 * 2 eps * 10 = 4.44e-13 lies between the two cases: a guard scale of
+    # eps, 2 eps, eps n or 2 eps n^2 gives the other answer for one of them.
+    n = 10
+    b = np.linspace(1.0, 2.0, n)
+    x, result, steps = O.cg(lambda v: c * v, b, 1e-10, 1e-10)
+    if nan:
+        assert np.isnan(x).all() and result == O.SINGULAR
+    else:
+        assert result == O.SUCCESSFUL
+        np.testing.assert_allclose(x, b / c, rtol=1e-12)
+    assert 100 * O.resolve_rcond(n, n, np.float64) == pytest.approx(100 * 2 * np.finfo(np.float64).eps * n)
+
+
 def test_cg_breakdown_guard_gives_nan(rng):
     # rcond = 2 eps size (lineax/_misc.py:36): a tiny <Ap,p> relative to
     # gamma NaNs the step, which linear_solve reports as singular.
@@ -150,11 +168,28 @@ def test_gmres_counts_dummy_first_pass(rng):
 
 
 def test_gmres_stagnation_rule():
-    # A residual that never decreases triggers the stagnation counter.
-    a = np.array([[0.0, 1.0], [-1.0, 0.0]])  # rotation: restart=1 makes no progress
+    # A residual that never decreases triggers the stagnation counter
+    # (gmres.py:175-179, 228-230).  The quarter-turn rotation with restart 1:
+    # A r is orthogonal to r, so each cycle's least-squares step is exactly
+    # zero (no breakdown: |A r| = |r|).  Step 0 is lineax's dummy pass, then
+    # three cycles without a decrease: four steps, stagnation, x = 0.
+    a = np.array([[0.0, 1.0], [-1.0, 0.0]])
     b = np.array([1.0, 0.0])
-    _, result, steps = O.gmres(lambda v: a @ v, b, 1e-12, 1e-12, restart=1, stagnation_iters=3)
-    assert result in (O.STAGNATION, O.BREAKDOWN)
+    x, result, steps = O.gmres(lambda v: a @ v, b, 1e-12, 1e-12, restart=1, stagnation_iters=3)
+    assert result == O.STAGNATION and steps == 4
+    assert np.array_equal(x, np.zeros(2))
+    # The cyclic shift with restart < n is GMRES's classic stall: the Krylov
+    # space of e0 never contains e0 until step n.
+    n = 6
+    shift = np.roll(np.eye(n), 1, axis=0)
+    e0 = np.eye(n)[0]
+    for restart in (1, 3, 5):
+        x, result, steps = O.gmres(lambda v: shift @ v, e0, 1e-12, 1e-12, restart=restart, stagnation_iters=4)
+        assert result == O.STAGNATION and steps == 5
+    # restart = n reaches the exact solution in one cycle instead
+    x, result, _ = O.gmres(lambda v: shift @ v, e0, 1e-12, 1e-12, restart=n, stagnation_iters=4)
+    assert result == O.SUCCESSFUL
+    np.testing.assert_allclose(shift @ x, e0, atol=1e-12)
 
 
 @pytest.mark.parametrize("restart", [5, 20])
