@@ -74,6 +74,21 @@ int lxa_lu_solve(void* stream, int dtype, int64_t batch, int64_t n,
                  int64_t piv_batch_stride, void* b, int64_t b_batch_stride,
                  int trans);
 
+/* getrf + getrs fused for small orders (n <= 64), one launch: x = A^-1 b
+ * for `batch` systems, A read-only ([batch, n, n], stride a_batch_stride),
+ * b [batch, n] (stride b_batch_stride), x [batch, n] (stride
+ * x_batch_stride).  The factors and pivots (lu [batch, n, n] contiguous,
+ * piv [batch, n], 0-based as lxa_lu_factor) are written only when lu / piv
+ * are non-null (both or neither); info as lxa_lu_factor (may be null).
+ * Equals lxa_lu_factor followed by lxa_lu_solve (trans = 0) bit for bit.
+ * Replaces the lu_factor / lu_solve pair of lineax/_solver/lu.py:53, 64 for
+ * a vmapped solve that forms its state and solves once (the first compute
+ * of a batch of small operators). */
+int lxa_lu_small_solve(void* stream, int dtype, int64_t batch, int64_t n,
+                       const void* a, int64_t a_batch_stride, const void* b,
+                       int64_t b_batch_stride, void* x, int64_t x_batch_stride,
+                       void* lu, int32_t* piv, int32_t* info);
+
 /* ------------------------------------------------------------------------
  * Cholesky (lineax/_solver/cholesky.py)
  * ---------------------------------------------------------------------- */
@@ -489,6 +504,15 @@ int lxa_gmres_result(void* stream, int64_t rows, const lxa_krylov_params* prm, l
 #define LXA_GEMM_BF16X3 3 /* split bf16: hi*hi + hi*lo + lo*hi, ~16-bit operands */
 int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_t k, const float* a, int64_t lda,
                  const float* b, int64_t ldb, float* c, int64_t ldc);
+/* C_s = A_s B_s for `batch` systems (< 65536; strides in elements, 0
+ * broadcasts an operand), same tiles and modes.  Replaces the materialised
+ * product of lineax's Normal solver (lineax/_solver/normal.py:105-120 via
+ * ComposedLinearOperator.as_matrix, _operator.py:1125-1192) for float32
+ * operators: the Gram matrix A^T A (A A^T when wide). */
+int lxa_gemm_batched(void* stream, int mode, int64_t batch, int64_t m, int64_t n, int64_t k,
+                     const float* a, int64_t lda, int64_t a_stride, const float* b,
+                     int64_t ldb, int64_t b_stride, float* c, int64_t ldc,
+                     int64_t c_stride);
 
 /* ---------------------------------------------------------------------------
  * Large dense LU (one system of order n in the thousands), the factor and

@@ -38,6 +38,7 @@ _szp = ctypes.POINTER(ctypes.c_size_t)
 _SIGNATURES = {
     "lxa_lu_factor": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _vp],
     "lxa_lu_solve": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int],
+    "lxa_lu_small_solve": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp],
     "lxa_chol_factor": [_vp, _int, _i64, _i64, _vp, _i64, _int, _vp],
     "lxa_chol_solve": [_vp, _int, _i64, _i64, _vp, _i64, _vp, _i64, _int],
     "lxa_chol_uneven_workspace_size": [_i64, _szp],
@@ -97,6 +98,7 @@ _SIGNATURES = {
     "lxa_gmres_cycle_end": [_vp, _i64, _i64, _int, _vp, _vp, _vp, _vp],
     "lxa_gmres_result": [_vp, _i64, _vp, _vp, _vp],
     "lxa_gemm_sub": [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64],
+    "lxa_gemm_batched": [_vp, _int, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64],
     "lxa_lu_large_block": [],
     "lxa_lu_large_factor": [_vp, _int, _i64, _vp, _i64, _vp, _vp, _vp, _vp],
     "lxa_lu_large_perm": [_vp, _i64, _vp, _vp],
@@ -257,6 +259,38 @@ def lu_solve(lu: torch.Tensor, piv: torch.Tensor, b: torch.Tensor, trans: int) -
         "lxa_lu_solve",
     )
     return x
+
+
+SMALL_LU_MAX = 64
+
+
+def lu_small_solve(matrix: torch.Tensor, vector: torch.Tensor, want_factors: bool = False):
+    """Fused getrf + getrs for orders <= 64 (lxa_lu_small_solve): returns
+    (x, lu, piv, info); lu / piv are None unless ``want_factors``.  The matrix
+    and vector batches must match (no broadcasting)."""
+    _gpu(matrix, "matrix")
+    _gpu(vector, "vector")
+    *lead, n, n2 = matrix.shape
+    if n != n2 or n > SMALL_LU_MAX:
+        raise ValueError(f"lu_small_solve needs square matrices of order <= {SMALL_LU_MAX}, got {tuple(matrix.shape)}")
+    if tuple(vector.shape) != (*lead, n):
+        raise ValueError(f"vector batch {tuple(vector.shape)} does not match the matrix batch {tuple(matrix.shape)}")
+    if vector.dtype != matrix.dtype:
+        raise TypeError(f"vector dtype {vector.dtype} differs from matrix dtype {matrix.dtype}")
+    dt = _dtype_code(matrix.dtype)
+    a = matrix.contiguous()
+    b = vector.contiguous()
+    batch = math.prod(lead)
+    dev = matrix.device
+    x = torch.empty_like(b)
+    lu = torch.empty_like(a) if want_factors else None
+    piv = torch.empty((*lead, n), dtype=torch.int32, device=dev) if want_factors else None
+    info = torch.empty(tuple(lead), dtype=torch.int32, device=dev)
+    if batch == 0:
+        return x, lu, piv, info
+    _check(load().lxa_lu_small_solve(_stream(a), dt, batch, n, _ptr(a), n * n, _ptr(b), n, _ptr(x), n,
+                                     _ptr(lu), _ptr(piv), _ptr(info)), "lxa_lu_small_solve")
+    return x, lu, piv, info
 
 
 # ---------------------------------------------------------------------------
@@ -920,6 +954,55 @@ def gemm_sub(c: torch.Tensor, a: torch.Tensor, b: torch.Tensor, mode: str = "fp3
         raise ValueError(f"shape mismatch: {tuple(a.shape)} x {tuple(b.shape)} -> {tuple(c.shape)}")
     _check(load().lxa_gemm_sub(_stream(c), GEMM_MODES[mode], m, n, k, _ptr(a), a.stride(0), _ptr(b), b.stride(0),
                                _ptr(c), c.stride(0)), "lxa_gemm_sub")
+    return c
+
+
+GEMM_MAX_BATCH = 65535
+
+
+def gemm(a: torch.Tensor, b: torch.Tensor, mode: str = "fp32") -> torch.Tensor:
+    """A @ B of float32 matrices or batches of them (leading dimensions
+    broadcast as in torch.matmul) on the matrix cores, one launch per 65535
+    systems: the systems of the batch are the grid's third dimension."""
+    for t, what in ((a, "a"), (b, "b")):
+        _gpu(t, what)
+        if t.dtype != torch.float32 or t.dim() < 2:
+            raise TypeError(f"{what} must be a float32 matrix or batch of matrices")
+    m, k = a.shape[-2:]
+    k2, n = b.shape[-2:]
+    if k2 != k:
+        raise ValueError(f"shape mismatch: {tuple(a.shape)} @ {tuple(b.shape)}")
+    batch_shape = torch.broadcast_shapes(a.shape[:-2], b.shape[:-2])
+    count = int(np.prod(batch_shape)) if batch_shape else 1
+    c = torch.empty(*batch_shape, m, n, dtype=torch.float32, device=a.device)
+    if c.numel() == 0:
+        return c
+    if k == 0:
+        return c.zero_()
+
+    def flat(t):
+        # (count, rows, cols) with unit column stride; a broadcast operand
+        # keeps batch stride 0
+        if t.shape[:-2] == batch_shape or not batch_shape:
+            t = t.reshape(-1, *t.shape[-2:]) if t.dim() > 2 else t.unsqueeze(0)
+            if t.stride(-1) != 1 or t.stride(-2) < t.shape[-1]:
+                t = t.contiguous()
+            return t, (t.stride(0) if t.shape[0] > 1 else 0)
+        if t.dim() == 2 or all(d == 1 for d in t.shape[:-2]):
+            t = t.reshape(t.shape[-2:]).contiguous()
+            return t.unsqueeze(0), 0
+        t = t.expand(*batch_shape, *t.shape[-2:]).contiguous().reshape(-1, *t.shape[-2:])
+        return t, t.stride(0)
+
+    af, sa = flat(a)
+    bf, sb = flat(b)
+    cf = c.reshape(-1, m, n)
+    lib = load()
+    for s0 in range(0, count, GEMM_MAX_BATCH):
+        cnt = min(GEMM_MAX_BATCH, count - s0)
+        _check(lib.lxa_gemm_batched(_stream(c), GEMM_MODES[mode], cnt, m, n, k, _ptr(af[s0 if sa else 0]),
+                                    af.stride(1), sa, _ptr(bf[s0 if sb else 0]), bf.stride(1), sb, _ptr(cf[s0]), n,
+                                    m * n), "lxa_gemm_batched")
     return c
 
 

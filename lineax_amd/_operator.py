@@ -808,7 +808,15 @@ class ComposedLinearOperator(AbstractLinearOperator):
         return self.operator1.mv(self.operator2.mv(vector))
 
     def as_matrix(self):
-        return self.operator1.as_matrix() @ self.operator2.as_matrix()
+        m1, m2 = self.operator1.as_matrix(), self.operator2.as_matrix()
+        if (m1.is_cuda and m1.dtype == torch.float32 and m2.dtype == torch.float32
+                and not (m1.requires_grad or m2.requires_grad)):
+            # one matrix-core launch over the whole batch (e.g. Normal's
+            # Gram matrix A^T A)
+            from . import _kernels as K
+
+            return K.gemm(m1, m2)
+        return m1 @ m2
 
     def transpose(self):
         return ComposedLinearOperator(self.operator2.transpose(), self.operator1.transpose())

@@ -11,13 +11,35 @@ from .._solve import AbstractLinearSolver
 from .misc import pack_structures, ravel_vector, transpose_packed_structures, unravel_solution
 
 
+class _PendingFactors:
+    """Factors of a batch of small operators, formed by the first solve.
+
+    ``LU.init`` on operators of order <= 64 defers getrf to the first
+    ``compute``, which factors and solves in one fused launch
+    (``lxa_lu_small_solve``) and keeps the factors here for every later
+    compute, transpose or conj (which form them first if no solve came
+    before).  The factors are lu_factor's bit for bit either way."""
+
+    def __init__(self, matrix):
+        self.matrix = matrix
+        self.lu = None
+        self.piv = None
+
+    def factors(self):
+        if self.lu is None:
+            self.lu, self.piv, _ = K.lu_factor(self.matrix)
+        return self.lu, self.piv
+
+
 class LU(AbstractLinearSolver):
     """LU solver for square nonsingular operators.
 
     ``init`` factors with partial pivoting (lineax/_solver/lu.py:43-54, the
     getrf kernel); ``compute`` solves with ``trans`` taken from the state
     (lu.py:56-66), so ``transpose`` only flips a flag (lu.py:68-81).
-    State: ``((lu, piv), packed_structures, transposed)``.
+    State: ``((lu, piv), packed_structures, transposed)``; for operators of
+    order <= 64 ``("pending", factors)`` whose factors the first compute
+    forms together with the solve (one fused launch).
 
     One unbatched float32 operator of order >= ``large_threshold`` takes the
     large path instead (``lineax_amd.large_lu``): butterfly randomisation,
@@ -49,6 +71,8 @@ class LU(AbstractLinearSolver):
             # lineax/_solver/lu.py:50-51: the diagonal is its own LU factor.
             n = operator.in_size()
             lu = (matrix, torch.arange(n, dtype=torch.int32, device=matrix.device).expand(*matrix.shape[:-1]))
+        elif n <= K.SMALL_LU_MAX and matrix.dtype in (torch.float32, torch.float64):
+            lu = ("pending", _PendingFactors(matrix))
         else:
             lu_mat, piv, _info = K.lu_factor(matrix)
             lu = (lu_mat, piv)
@@ -57,6 +81,16 @@ class LU(AbstractLinearSolver):
     def compute(self, state, vector, options):
         (lu, piv), packed, transpose = state
         flat, batch = ravel_vector(vector, packed)
+        if isinstance(lu, str) and lu == "pending":
+            holder = piv
+            vec = flat.to(holder.matrix.dtype)
+            if (holder.lu is None and not transpose
+                    and tuple(vec.shape[:-1]) == tuple(holder.matrix.shape[:-2])):
+                x, holder.lu, holder.piv, _ = K.lu_small_solve(holder.matrix, vec, want_factors=True)
+            else:
+                lu_mat, pv = holder.factors()
+                x = K.lu_solve(lu_mat, pv, vec, trans=1 if transpose else 0)
+            return unravel_solution(x, packed, tuple(x.shape[:-1])), RESULTS.successful, {}
         if isinstance(lu, str):
             fac = piv
             tol = self.large_tol if self.large_tol is not None else float(torch.finfo(torch.float32).eps)
@@ -76,7 +110,7 @@ class LU(AbstractLinearSolver):
         # As lineax/_solver/lu.py:83-95 (which also flips the transpose flag).
         (lu, piv), packed, transpose = state
         if isinstance(lu, str):
-            return ((lu, piv), packed, not transpose), {}
+            return ((lu, piv), packed, not transpose), {}  # real dtypes: conj is the identity
         return ((lu.conj(), piv), packed, not transpose), {}
 
     def assume_full_rank(self):

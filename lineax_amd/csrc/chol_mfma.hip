@@ -20,7 +20,8 @@
 //      block).
 // Three barriers per panel.  After the factorisation the registers hold U
 // off the diagonal and the inverted diagonal blocks, so the triangular solves
-// are block steps (a 16 x 16 product per block, no scalar chain): forward
+// are block steps (a 16 x 16 product per block, no scalar chain; the
+// inverted diagonal blocks stay in LDS): forward
 // U^T y = c pushes each solved block down its block row; backward U x = y
 // gathers each block row.  With refinement, residuals r = b - A x are formed
 // in FP64 from A's upper triangle in global memory, block by block, and x
@@ -113,6 +114,23 @@ __host__ __device__ constexpr int round16(int n) { return (n + 15) & ~15; }
 // block-ownership tests compile to scalar branches.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
+// Loads of A go through a buffer resource covering the n x n matrix: an
+// offset past its end reads 0 with no fault, so entries outside the upper
+// triangle (or the matrix) are read as 0 without a branch per element.
+constexpr int kOutside = 0x7ffffff0;  // a byte offset beyond any matrix of order <= 256
+constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit buffer, gfx9 resource format
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t matrix_rsrc(const float* A, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(A), 0, n * n * 4, kRsrcWord3);
+}
+
+// A(i, j) for j >= i inside the matrix, else 0.
+__device__ __forceinline__ float upper_at(__amdgpu_buffer_rsrc_t r, int n, int i, int j) {
+  const bool in = i < n && j < n && j >= i;
+  // the builtin returns the 32 bits of the element
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, in ? (i * n + j) * 4 : kOutside, 0, 0));
+}
+
 // LDS carve-up of one workgroup (byte offsets).  The residual's per-wave
 // column sums overlay the panel buffers, which are idle after the
 // factorisation.
@@ -130,8 +148,8 @@ struct Lay {
   static constexpr size_t kColPart = 0;                                 // [WAVES][kN] f64 (overlay)
   static constexpr size_t kColAbs = kColPart + WAVES * kN * 8;          // [WAVES][kN] f64 (overlay)
   static constexpr size_t kVfrag = kPanelEnd;                           // [64][4] f32
-  static constexpr size_t kVm = kVfrag + 256 * 4;                       // [16][16] f32
-  static constexpr size_t kX = kVm + 256 * 4;                           // [kN] f64 solution
+  static constexpr size_t kVall = kVfrag + 256 * 4;                     // [NB][16][16] f32: V_K = U_KK^-1
+  static constexpr size_t kX = kVall + NB * 256 * 4;                    // [kN] f64 solution
   static constexpr size_t kR = kX + kN * 8;                             // [kN] f64 residual
   static constexpr size_t kRowAbs = kR + kN * 8;                        // [kN] f64 row sums of |A|
   static constexpr size_t kC = kRowAbs + kN * 8;                        // [kN] f32 solve vector
@@ -170,7 +188,7 @@ struct Shared {
   float* Praw;
   float* PF;
   float* Vfrag;
-  float* Vm;
+  float* Vall;
   double* X;
   double* R;
   double* RowAbs;
@@ -191,7 +209,7 @@ __device__ __forceinline__ Shared carve(unsigned char* base) {
   s.ColPart = reinterpret_cast<double*>(base + L::kColPart);
   s.ColAbs = reinterpret_cast<double*>(base + L::kColAbs);
   s.Vfrag = reinterpret_cast<float*>(base + L::kVfrag);
-  s.Vm = reinterpret_cast<float*>(base + L::kVm);
+  s.Vall = reinterpret_cast<float*>(base + L::kVall);
   s.X = reinterpret_cast<double*>(base + L::kX);
   s.R = reinterpret_cast<double*>(base + L::kR);
   s.RowAbs = reinterpret_cast<double*>(base + L::kRowAbs);
@@ -214,10 +232,11 @@ __device__ __forceinline__ void wg_barrier() {
 // Factors and inverts the diagonal block in Praw columns col0..col0+15 (one
 // wave; lane q = l % 16 works on column q, the four rows of 16 lanes
 // redundantly).  Writes V = U_KK^-1 as MFMA A-operand fragments of V^T
-// (Vfrag[l][kc] = V[4kc + l/16][l%16]) and row-major (Vm).  Returns the
+// (Vfrag[l][kc] = V[4kc + l/16][l%16]) and row-major (Vm, which the
+// triangular solves read later).  Returns the
 // 1-based index of the first non-positive pivot within the block, or 0.
 template <int LDP>
-__device__ __forceinline__ int diag_factor_invert(const Shared& s, int col0) {
+__device__ __forceinline__ int diag_factor_invert(const Shared& s, int col0, float* Vm) {
   const int l = threadIdx.x & 63, q = l & 15;
   float c[16];
 #pragma unroll
@@ -256,7 +275,7 @@ __device__ __forceinline__ int diag_factor_invert(const Shared& s, int col0) {
     for (int g2 = 0; g2 < 4; ++g2)
       *reinterpret_cast<f4*>(s.Vfrag + (q + 16 * g2) * 4) = f4{c[g2], c[4 + g2], c[8 + g2], c[12 + g2]};
 #pragma unroll
-    for (int p = 0; p < 16; ++p) s.Vm[p * 16 + q] = c[p];
+    for (int p = 0; p < 16; ++p) Vm[p * 16 + q] = c[p];
   }
   return fail;
 }
@@ -267,6 +286,7 @@ __device__ __forceinline__ int diag_factor_invert(const Shared& s, int col0) {
 template <int NB, int WAVES>
 __device__ __forceinline__ void load_tiles(Tiles<NB, WAVES>& T, const float* __restrict__ A, int n, int nk) {
   using L = Lay<NB, WAVES>;
+  const __amdgpu_buffer_rsrc_t rs = matrix_rsrc(A, n);
   const int w = wave_id(), l = threadIdx.x & 63, g = l >> 4, q = l & 15;
 #pragma unroll
   for (int t = 0; t < L::kPPW; ++t) {
@@ -276,15 +296,11 @@ __device__ __forceinline__ void load_tiles(Tiles<NB, WAVES>& T, const float* __r
       int I, J;
       slot_block<NB>(p, k, I, J);
       if (J < nk) {
-        // unconditional loads (an entry outside the upper triangle reads
-        // A[0] and is selected away): no branch per element
         const int j = 16 * J + q;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int i = 16 * I + 4 * g + r;
-          const bool in = i < n && j < n && j >= i;
-          const float av = A[in ? i * n + j : 0];
-          T[t][k][r] = in ? av : ((i == j && i >= n) ? 1.0f : 0.0f);  // identity padding
+          T[t][k][r] = upper_at(rs, n, i, j) + ((i == j && i >= n) ? 1.0f : 0.0f);  // identity padding
         }
       }
     }
@@ -322,7 +338,7 @@ __device__ __forceinline__ int factor_tiles(Tiles<NB, WAVES>& T, int nk, const S
     wg_barrier<WAVES>();
     // 2. diagonal block: factor and invert
     if (w == 0) {
-      const int f = diag_factor_invert<L::kLdp>(s, 16 * K);
+      const int f = diag_factor_invert<L::kLdp>(s, 16 * K, s.Vall + 256 * K);
       if (l == 0) s.Int[1] = f ? 16 * K + f : 0;
     }
     wg_barrier<WAVES>();
@@ -355,10 +371,7 @@ __device__ __forceinline__ int factor_tiles(Tiles<NB, WAVES>& T, int nk, const S
         slot_block<NB>(p, k, I, J);
         if (J < nk) {
           if (I == K) {
-            if (J == K) {
-#pragma unroll
-              for (int r = 0; r < 4; ++r) T[t][k][r] = s.Vm[(4 * g + r) * 16 + q];
-            } else {
+            if (J > K) {  // (the diagonal block's inverse stays in LDS, s.Vall)
               const float* pf = s.PF + J * 256;
 #pragma unroll
               for (int r = 0; r < 4; ++r) T[t][k][r] = pf[back_l + 64 * r];
@@ -384,7 +397,7 @@ __device__ __forceinline__ int factor_tiles(Tiles<NB, WAVES>& T, int nk, const S
 }
 
 // U^T U z = c in place in s.C (float32, padded with zeros to 16 nk), from the
-// factored blocks (U off the diagonal, V = U_KK^-1 on it).
+// factored blocks: U off the diagonal in registers, V_K = U_KK^-1 in LDS.
 template <int NB, int WAVES>
 __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, const Shared& s) {
   using L = Lay<NB, WAVES>;
@@ -395,27 +408,33 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
   for (int I = 0; I < nk; ++I) {
     int col_l = q;  // opaque per step, as in factor_tiles
     asm volatile("" : "+v"(col_l));
+    bool mine = false;
 #pragma unroll
     for (int t = 0; t < L::kPPW; ++t) {
       const int p = pair_of<WAVES>(w, t);
-      f4 yv = {0.0f, 0.0f, 0.0f, 0.0f};
+      mine = mine || p == I || NB - 1 - p == I;
+    }
+    if (mine) {
+      // y_I[q] = sum_p V[p][q] c[p] (rows 4g..4g+3 here, then across groups)
+      const f4 cv = *reinterpret_cast<const f4*>(C + 16 * I + 4 * g);
+      const float* V = s.Vall + 256 * I + (4 * g) * 16 + col_l;
+      float part = V[0] * cv[0] + V[16] * cv[1] + V[32] * cv[2] + V[48] * cv[3];
+      part = group_sum4(part);
+      wave_lds_sync();
+      if (l < 16) C[16 * I + q] = part;
+      wave_lds_sync();
+      const f4 yv = *reinterpret_cast<const f4*>(C + 16 * I + 4 * g);
 #pragma unroll
-      for (int k = 0; k < L::kTPP; ++k) {
-        int Ib, J;
-        slot_block<NB>(p, k, Ib, J);
-        if (Ib == I && J < nk) {
-          if (J == I) {  // slots of a row start at its diagonal block
-            const f4 cv = *reinterpret_cast<const f4*>(C + 16 * I + 4 * g);
-            float part = T[t][k][0] * cv[0] + T[t][k][1] * cv[1] + T[t][k][2] * cv[2] + T[t][k][3] * cv[3];
-            part = group_sum4(part);  // y_I[q]
-            wave_lds_sync();
-            if (l < 16) C[16 * I + q] = part;
-            wave_lds_sync();
-            yv = *reinterpret_cast<const f4*>(C + 16 * I + 4 * g);
-          } else {
-            float part = T[t][k][0] * yv[0] + T[t][k][1] * yv[1] + T[t][k][2] * yv[2] + T[t][k][3] * yv[3];
-            part = group_sum4(part);
-            if (l < 16) C[16 * J + col_l] -= part;
+      for (int t = 0; t < L::kPPW; ++t) {
+        const int p = pair_of<WAVES>(w, t);
+#pragma unroll
+        for (int k = 0; k < L::kTPP; ++k) {
+          int Ib, J;
+          slot_block<NB>(p, k, Ib, J);
+          if (Ib == I && J > I && J < nk) {
+            float pt = T[t][k][0] * yv[0] + T[t][k][1] * yv[1] + T[t][k][2] * yv[2] + T[t][k][3] * yv[3];
+            pt = group_sum4(pt);
+            if (l < 16) C[16 * J + col_l] -= pt;
           }
         }
       }
@@ -427,13 +446,17 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
   for (int K = nk - 1; K >= 0; --K) {
     int col_l = q;
     asm volatile("" : "+v"(col_l));
+    bool mine = false;
 #pragma unroll
     for (int t = 0; t < L::kPPW; ++t) {
       const int p = pair_of<WAVES>(w, t);
-      // is row K in this pair?  (uniform)
-      const bool mine = (p == K) || (NB - 1 - p == K);
-      if (mine) {
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      mine = mine || p == K || NB - 1 - p == K;
+    }
+    if (mine) {
+      float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int t = 0; t < L::kPPW; ++t) {
+        const int p = pair_of<WAVES>(w, t);
 #pragma unroll
         for (int k = 0; k < L::kTPP; ++k) {
           int Ib, J;
@@ -444,25 +467,19 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
             for (int r = 0; r < 4; ++r) acc[r] = fmaf(T[t][k][r], xj, acc[r]);
           }
         }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = row_sum16(acc[r]);  // s[4g + r]
-        float* S = s.S + w * 16;
-        if (q == 0) *reinterpret_cast<f4*>(S + 4 * g) = f4{acc[0], acc[1], acc[2], acc[3]};
-        wave_lds_sync();
-        const float tq = C[16 * K + q] - S[q];
-#pragma unroll
-        for (int k = 0; k < L::kTPP; ++k) {
-          int Ib, J;
-          slot_block<NB>(p, k, Ib, J);
-          if (Ib == K && J == K) {
-            float xr[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) xr[r] = row_sum16(T[t][k][r] * tq);
-            wave_lds_sync();
-            if (q < 4) C[16 * K + 4 * g + q] = q == 0 ? xr[0] : q == 1 ? xr[1] : q == 2 ? xr[2] : xr[3];
-          }
-        }
       }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = row_sum16(acc[r]);  // s[4g + r]
+      float* S = s.S + w * 16;
+      if (q == 0) *reinterpret_cast<f4*>(S + 4 * g) = f4{acc[0], acc[1], acc[2], acc[3]};
+      wave_lds_sync();
+      const float tq = C[16 * K + q] - S[q];
+      const float* V = s.Vall + 256 * K + (4 * g) * 16 + col_l;
+      float xr[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xr[r] = row_sum16(V[16 * r] * tq);
+      wave_lds_sync();
+      if (q < 4) C[16 * K + 4 * g + q] = q == 0 ? xr[0] : q == 1 ? xr[1] : q == 2 ? xr[2] : xr[3];
     }
     wg_barrier<WAVES>();
   }
@@ -496,11 +513,14 @@ __device__ __forceinline__ void block_max4(double (&v)[4], const Shared& s) {
 // its own block rows of the upper triangle once: block (I, J) adds A_IJ x_J
 // to rows of I (a row reduction at the end of the block row) and, off the
 // diagonal, A_IJ^T x_I to rows of J (per-wave column sums in LDS).
+constexpr int kResGroup = 2;  // blocks per residual load group
+
 template <int NB, int WAVES>
 __device__ __forceinline__ void residual_tiles(const float* __restrict__ A, const float* __restrict__ B, int n, int nk,
                                const Shared& s, bool want_abs) {
   using L = Lay<NB, WAVES>;
   const int w = wave_id(), l = threadIdx.x & 63, g = l >> 4, q = l & 15;
+  const __amdgpu_buffer_rsrc_t rs = matrix_rsrc(A, n);
   double* colp = s.ColPart + w * L::kN;
   double* cola = s.ColAbs + w * L::kN;
   for (int i = l; i < 16 * nk; i += 64) {
@@ -515,39 +535,54 @@ __device__ __forceinline__ void residual_tiles(const float* __restrict__ A, cons
     for (int half = 0; half < 2; ++half) {
       const int I = half == 0 ? p : NB - 1 - p;
       if (I < nk) {
-        double racc[4] = {0.0, 0.0, 0.0, 0.0}, rabs[4] = {0.0, 0.0, 0.0, 0.0};
-        double xi[4];
+        // FP64 products; the |A| row sums (for the norm in the backward
+        // error's denominator) in float32, which is plenty for a norm
+        double racc[4] = {0.0, 0.0, 0.0, 0.0};
+        float rabs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        // blocks J = I.. of the row in groups of kResGroup, the next group's
+        // loads issued before the current group is reduced (kResGroup * 4
+        // loads in flight per lane; blocks past the matrix read 0)
+        float cur[kResGroup][4], nxt[kResGroup][4];
+        auto load_group = [&](int J0, float (&dst)[kResGroup][4]) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) xi[r] = s.X[16 * I + 4 * g + r];
-        for (int J = I; J < nk; ++J) {
-          const int j = 16 * J + q;
-          const double xj = s.X[j];
-          float a[4];
+          for (int u = 0; u < kResGroup; ++u)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = 16 * I + 4 * g + r;
-            const bool in = i < n && j < n && j >= i;
-            const float av = A[in ? i * n + j : 0];
-            a[r] = in ? av : 0.0f;
-          }
-          double cs = 0.0, ca = 0.0;
+            for (int r = 0; r < 4; ++r) dst[u][r] = upper_at(rs, n, 16 * I + 4 * g + r, 16 * (J0 + u) + q);
+        };
+        load_group(I, cur);
+        for (int J0 = I; J0 < nk; J0 += kResGroup) {
+          load_group(J0 + kResGroup, nxt);
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = 16 * I + 4 * g + r;
-            const double ad = static_cast<double>(a[r]);
-            racc[r] = fma(ad, xj, racc[r]);
-            rabs[r] += fabs(ad);
-            if (j > i) {  // strictly upper: the symmetric entry of row j
-              cs = fma(ad, xi[r], cs);
-              ca += fabs(ad);
+          for (int u = 0; u < kResGroup; ++u) {
+            const int J = J0 + u;
+            if (J < nk) {
+              const int j = 16 * J + q;
+              const double xj = s.X[j];
+              double cs = 0.0;
+              float ca = 0.0f;
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int i = 16 * I + 4 * g + r;
+                const double ad = static_cast<double>(cur[u][r]);
+                racc[r] = fma(ad, xj, racc[r]);
+                rabs[r] += fabsf(cur[u][r]);
+                if (j > i) {  // strictly upper: the symmetric entry of row j
+                  cs = fma(ad, s.X[i], cs);
+                  ca += fabsf(cur[u][r]);
+                }
+              }
+              cs = group_sum4(cs);
+              if (want_abs) ca = group_sum4(ca);
+              if (l < 16) {
+                colp[j] += cs;
+                if (want_abs) cola[j] += static_cast<double>(ca);
+              }
             }
           }
-          cs = group_sum4(cs);
-          if (want_abs) ca = group_sum4(ca);
-          if (l < 16) {
-            colp[j] += cs;
-            if (want_abs) cola[j] += ca;
-          }
+#pragma unroll
+          for (int u = 0; u < kResGroup; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cur[u][r] = nxt[u][r];
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -557,7 +592,9 @@ __device__ __forceinline__ void residual_tiles(const float* __restrict__ A, cons
         if (q < 4) {
           const double rv = q == 0 ? racc[0] : q == 1 ? racc[1] : q == 2 ? racc[2] : racc[3];
           s.R[16 * I + 4 * g + q] = rv;
-          if (want_abs) s.RowAbs[16 * I + 4 * g + q] = q == 0 ? rabs[0] : q == 1 ? rabs[1] : q == 2 ? rabs[2] : rabs[3];
+          if (want_abs)
+            s.RowAbs[16 * I + 4 * g + q] =
+                static_cast<double>(q == 0 ? rabs[0] : q == 1 ? rabs[1] : q == 2 ? rabs[2] : rabs[3]);
         }
       }
     }

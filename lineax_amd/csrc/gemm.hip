@@ -90,7 +90,12 @@ __device__ __forceinline__ void mfma_block(floatx16& acc, const typename GemmMod
 template <int MODE>
 __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(int M, int N, int K, const float* A, int64_t lda,
                                                             const float* B, int64_t ldb, float* C, int64_t ldc,
-                                                            int beta, float sign) {
+                                                            int beta, float sign, int64_t sA, int64_t sB,
+                                                            int64_t sC) {
+  // system blockIdx.z of a batch (strides in elements; one system: z = 0)
+  A += sA * blockIdx.z;
+  B += sB * blockIdx.z;
+  C += sC * blockIdx.z;
   using E = typename GemmMode<MODE>::E;
   constexpr int LDK = kBK + GemmMode<MODE>::kPad;
   constexpr bool kSplit = MODE == LXA_GEMM_BF16X3;
@@ -217,29 +222,39 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(int M, int N, int K,
 
 }  // namespace
 
-int gemm_update(hipStream_t st, int mode, int M, int N, int K, const float* A, int64_t lda, const float* B,
-                int64_t ldb, float* C, int64_t ldc, int beta, float sign) {
-  if (M <= 0 || N <= 0) return LXA_OK;
+int gemm_update_batched(hipStream_t st, int mode, int batch, int M, int N, int K, const float* A, int64_t lda,
+                        int64_t sA, const float* B, int64_t ldb, int64_t sB, float* C, int64_t ldc, int64_t sC,
+                        int beta, float sign) {
+  if (M <= 0 || N <= 0 || batch <= 0) return LXA_OK;
   if (K <= 0 && beta) return LXA_OK;
-  const dim3 grid((unsigned)((N + kBN - 1) / kBN), (unsigned)((M + kBM - 1) / kBM));
+  const dim3 grid((unsigned)((N + kBN - 1) / kBN), (unsigned)((M + kBM - 1) / kBM), (unsigned)batch);
   switch (mode) {
     case LXA_GEMM_BF16:
-      gemm_kernel<LXA_GEMM_BF16><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign);
+      gemm_kernel<LXA_GEMM_BF16><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign, sA, sB,
+                                                               sC);
       break;
     case LXA_GEMM_F16:
-      gemm_kernel<LXA_GEMM_F16><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign);
+      gemm_kernel<LXA_GEMM_F16><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign, sA, sB,
+                                                              sC);
       break;
     case LXA_GEMM_F32:
-      gemm_kernel<LXA_GEMM_F32><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign);
+      gemm_kernel<LXA_GEMM_F32><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign, sA, sB,
+                                                              sC);
       break;
     case LXA_GEMM_BF16X3:
-      gemm_kernel<LXA_GEMM_BF16X3><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign);
+      gemm_kernel<LXA_GEMM_BF16X3><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign, sA,
+                                                                 sB, sC);
       break;
     default:
       return set_error(LXA_EINVAL, "unknown gemm mode %d", mode);
   }
   LXA_LAUNCH_CHECK("gemm_kernel");
   return LXA_OK;
+}
+
+int gemm_update(hipStream_t st, int mode, int M, int N, int K, const float* A, int64_t lda, const float* B,
+                int64_t ldb, float* C, int64_t ldc, int beta, float sign) {
+  return gemm_update_batched(st, mode, 1, M, N, K, A, lda, 0, B, ldb, 0, C, ldc, 0, beta, sign);
 }
 
 }  // namespace lxa
@@ -254,4 +269,20 @@ extern "C" int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_
               (long long)lda, (long long)ldb, (long long)ldc);
   return lxa::gemm_update(static_cast<hipStream_t>(stream), mode, (int)m, (int)n, (int)k, a, lda, b, ldb, c, ldc, 1,
                           -1.f);
+}
+
+extern "C" int lxa_gemm_batched(void* stream, int mode, int64_t batch, int64_t m, int64_t n, int64_t k,
+                                const float* a, int64_t lda, int64_t a_stride, const float* b, int64_t ldb,
+                                int64_t b_stride, float* c, int64_t ldc, int64_t c_stride) {
+  LXA_REQUIRE(batch >= 0 && batch < 65536 && m >= 0 && n >= 0 && k >= 0 && m < (1 << 30) && n < (1 << 30) &&
+                  k < (1 << 30),
+              "invalid sizes batch=%lld m=%lld n=%lld k=%lld", (long long)batch, (long long)m, (long long)n,
+              (long long)k);
+  if (batch == 0 || m == 0 || n == 0) return LXA_OK;
+  LXA_REQUIRE(a && b && c, "a, b and c must be non-NULL");
+  LXA_REQUIRE(lda >= k && ldb >= n && ldc >= n, "leading dimensions too small (lda=%lld ldb=%lld ldc=%lld)",
+              (long long)lda, (long long)ldb, (long long)ldc);
+  LXA_REQUIRE(a_stride >= 0 && b_stride >= 0 && c_stride >= m * ldc, "bad batch strides");
+  return lxa::gemm_update_batched(static_cast<hipStream_t>(stream), mode, (int)batch, (int)m, (int)n, (int)k, a, lda,
+                                  a_stride, b, ldb, b_stride, c, ldc, c_stride, 0, 1.f);
 }

@@ -248,3 +248,62 @@ def test_poisson_nsd_with_cholesky(device):
     op = lx.MatrixLinearOperator(_t(a, device), (lx.negative_semidefinite_tag, lx.symmetric_tag))
     x = lx.linear_solve(op, _t(b, device), lx.Cholesky()).value
     np.testing.assert_allclose(x.cpu().numpy(), np.linalg.solve(a, b), rtol=1e-9, atol=1e-9)
+
+
+# --- the fused small-LU solve (lxa_lu_small_solve): one launch, bit-equal to
+# lxa_lu_factor + lxa_lu_solve, and the lazy LU state built on it
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("n", [1, 5, 8, 13, 16, 31, 32, 33, 64])
+def test_lu_small_fused_bit_equal(device, rng, dtype, n):
+    batch = 37
+    a = rng.standard_normal((batch, n, n))
+    a[3] = 0.0  # singular: zero pivots, info, non-finite x like the pair
+    a[4, :, 0] = 0.0
+    a[5] = np.round(a[5])  # ties in the pivot search
+    b = rng.standard_normal((batch, n))
+    A, B = _t(a, device).to(dtype), _t(b, device).to(dtype)
+    x, lu, piv, info = K.lu_small_solve(A, B, want_factors=True)
+    lu_ref, piv_ref, info_ref = K.lu_factor(A)
+    x_ref = K.lu_solve(lu_ref, piv_ref, B, trans=0)
+    assert torch.equal(piv, piv_ref) and torch.equal(info, info_ref)
+    assert torch.equal(torch.nan_to_num(lu, nan=7.0), torch.nan_to_num(lu_ref, nan=7.0))
+    assert torch.equal(torch.nan_to_num(x, nan=7.0, posinf=8.0, neginf=9.0),
+                       torch.nan_to_num(x_ref, nan=7.0, posinf=8.0, neginf=9.0))
+    x2, lu2, piv2, _ = K.lu_small_solve(A, B)
+    assert lu2 is None and piv2 is None and torch.equal(torch.nan_to_num(x2, nan=7.0, posinf=8.0, neginf=9.0),
+                                                         torch.nan_to_num(x, nan=7.0, posinf=8.0, neginf=9.0))
+    # and the solves are backward stable (random Gaussian operators: judged by
+    # the normwise backward error, not the forward error)
+    good = [i for i in range(batch) if i not in (3, 4)]
+    xs = x.double().cpu().numpy()
+    eps = float(torch.finfo(dtype).eps)
+    for i in good[:8]:
+        ai = A[i].double().cpu().numpy()
+        bi = B[i].double().cpu().numpy()
+        assert O.normwise_backward_error(ai, xs[i], bi) <= 64 * n * eps
+
+
+def test_lu_lazy_state_reuse_and_transpose(device, rng):
+    n, batch = 24, 6
+    a = np.stack([well_conditioned(rng, n) for _ in range(batch)])
+    b1, b2 = rng.standard_normal((batch, n)), rng.standard_normal((batch, n))
+    op = lx.MatrixLinearOperator(_t(a, device))
+    solver = lx.LU()
+    state = solver.init(op, {})
+    x1, _, _ = solver.compute(state, _t(b1, device), {})  # fused: factors formed here
+    x2, _, _ = solver.compute(state, _t(b2, device), {})  # reuses them
+    for i in range(batch):
+        np.testing.assert_allclose(x1[i].cpu().numpy(), np.linalg.solve(a[i], b1[i]), rtol=1e-10, atol=1e-11)
+        np.testing.assert_allclose(x2[i].cpu().numpy(), np.linalg.solve(a[i], b2[i]), rtol=1e-10, atol=1e-11)
+    # a transposed state before any solve forms the factors itself
+    state = solver.init(op, {})
+    st_t, _ = solver.transpose(state, {})
+    xt, _, _ = solver.compute(st_t, _t(b1, device), {})
+    for i in range(batch):
+        np.testing.assert_allclose(xt[i].cpu().numpy(), np.linalg.solve(a[i].T, b1[i]), rtol=1e-10, atol=1e-11)
+    # the state returned by linear_solve is reusable
+    sol = lx.linear_solve(op, _t(b1, device), solver)
+    again = lx.linear_solve(op, _t(b2, device), solver, state=sol.state)
+    np.testing.assert_allclose(again.value.cpu().numpy(), x2.cpu().numpy(), rtol=0, atol=0)

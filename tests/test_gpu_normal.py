@@ -13,6 +13,7 @@ import pytest
 import torch
 
 import lineax_amd as lx
+from lineax_amd import _kernels as K
 from oracle import lineax_oracle as O
 from tests.helpers import poisson_1d
 
@@ -122,3 +123,38 @@ def test_normal_preconditioned_cg_exact_inverse(device, rng):
                           options={"preconditioner": m})
     assert int(sol.result) == lx.RESULTS.successful
     np.testing.assert_allclose(sol.value.cpu().numpy(), np.linalg.solve(a, b), rtol=1e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize("shape_a,shape_b", [((37, 200), (200, 37)), ((1, 5), (5, 1)), ((130, 129), (129, 257)),
+                                             ((4, 64, 33), (4, 33, 70)), ((3, 20, 9), (9, 20)),
+                                             ((2, 1, 8, 8), (5, 8, 8))])
+def test_batched_gemm_vs_float64(device, rng, shape_a, shape_b):
+    # the Gram product runs on lxa_gemm_batched: float32 inputs, float32
+    # accumulation; each entry within 4 k eps of the float64 product scaled
+    # by sum |a||b|
+    a = rng.standard_normal(shape_a)
+    b = rng.standard_normal(shape_b)
+    c = K.gemm(_t(a, device, torch.float32), _t(b, device, torch.float32)).double().cpu().numpy()
+    a32 = a.astype(np.float32).astype(np.float64)
+    b32 = b.astype(np.float32).astype(np.float64)
+    exact = np.matmul(a32, b32)
+    bound = 4 * shape_a[-1] * np.finfo(np.float32).eps * np.matmul(np.abs(a32), np.abs(b32))
+    assert c.shape == exact.shape
+    assert np.all(np.abs(c - exact) <= bound + 1e-30)
+
+
+def test_normal_float32_gram_on_gemm_batched(device, rng):
+    # tall float32 problems: the Gram matrix comes from the matrix-core GEMM,
+    # the solve from the Cholesky kernels
+    a = np.stack([_matrix(rng, 120, 40) for _ in range(3)])
+    b = rng.standard_normal((3, 120))
+    A = _t(a, device, torch.float32)
+    gram = lx.MatrixLinearOperator(A[0]).transpose() @ lx.MatrixLinearOperator(A[0])
+    g = gram.as_matrix().double().cpu().numpy()
+    a0 = a[0].astype(np.float32).astype(np.float64)
+    bound = 4 * 120 * np.finfo(np.float32).eps * (np.abs(a0).T @ np.abs(a0))
+    assert np.all(np.abs(g - a0.T @ a0) <= bound)
+    for i in range(3):
+        x = lx.linear_solve(lx.MatrixLinearOperator(A[i]), _t(b[i], device, torch.float32),
+                            lx.Normal(lx.Cholesky())).value
+        assert O.lstsq_backward_error(a[i], x.double().cpu().numpy(), b[i]) <= 1e-5
