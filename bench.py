@@ -4,9 +4,10 @@ Default workload: the uneven-batch Cholesky configuration — 20,000 SPD
 float32 systems, n ~ U{8..256}, A = M M^T / n + 0.1 I, b ~ N(0, 1), seed 1 —
 solved as one step per pass over the whole batch, inputs resident in HBM.
 
-  chosen form: one persistent launch per register-tile class (three:
-               orders 177..256, 129..176, <= 128), FP32 factorisation in
-               registers and LDS, FP64 refinement to normwise backward error
+  chosen form: one persistent launch per tile class (three: orders
+               129..256, 65..128, <= 64, from the autotune table), FP32
+               factorisation on the matrix cores with the blocks in
+               registers, FP64 refinement to normwise backward error
                <= 1e-12 (float64 x).
   plain form:  the kernels lx.Cholesky runs (factor + solve launch pair) on
                one order bucket at a time (249 buckets), FP32, no
@@ -16,8 +17,10 @@ solved as one step per pass over the whole batch, inputs resident in HBM.
 ``value`` is the chosen form's solves/s.  Per-step latency percentiles, the
 plain form's figures, accuracy against the float64 oracle (on a sample of
 systems), the roofline of the dominant kernel (timed with HIP events on the
-launch stream) and a CPU baseline (the oracle's float64 Cholesky on host
-cores, bounded sample) are reported on the same JSON line.
+launch stream) and a CPU baseline (LAPACK spotrf + spotrs per system on the
+host cores, bounded sample) are reported on the same JSON line, and, under
+"configs", every other benchmark configuration with its own roofline and
+CPU baseline (``--configs none`` skips them).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -306,6 +309,7 @@ def bench_uneven_cholesky(args):
 
     plain_info = None
     fused_info = None
+    refined_info = None
     if not args.no_plain:
         from lineax_amd import _kernels as K
         from oracle import lineax_oracle as O
@@ -365,12 +369,36 @@ def bench_uneven_cholesky(args):
         fused_info = {
             "value": host.count / (float(np.mean(fms)) / 1e3),
             "ms_per_step": float(np.mean(fms)),
-            "kernels": "chol_uneven_kernel<false,float> per order bucket",
+            "kernels": "chol_mfma_kernel<NB, WAVES, false, float> per order bucket (no refinement)",
             "launches_per_step": len(batch.buckets),
             "max_backward_error_sample": eta_sample(
                 lambda i: fx[int(host.b_offset[i]): int(host.b_offset[i]) + int(host.n[i])]),
         }
         print(f"[bench] fused per-bucket form: {fused_info['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+
+        # The plain form's schedule (one launch per order bucket) at the chosen
+        # form's accuracy: the chosen kernel with FP64 refinement to the same
+        # tolerance, bucket by bucket.  Chosen vs this isolates the batching.
+        xr = torch.empty(batch.x_len, dtype=torch.float64, device="cuda:0")
+        info_r = torch.empty(batch.count, dtype=torch.int32, device="cuda:0")
+        eta_r = torch.empty(batch.count, dtype=torch.float64, device="cuda:0")
+
+        def run_refined_buckets():
+            for order_n, s0, e0 in batch.buckets:
+                K.chol_uneven_solve(batch.n, batch.a_offset, batch.a, batch.b_offset, batch.b, xr,
+                                    batch.order_asc[s0:e0], max_n=order_n, persistent=False, max_refine=10,
+                                    tol=args.tol, info=info_r, eta=eta_r)
+
+        rms, _ = _time_steps(run_refined_buckets, max(3, args.steps // 2), 1)
+        refined_info = {
+            "value": host.count / (float(np.mean(rms)) / 1e3),
+            "ms_per_step": float(np.mean(rms)),
+            "kernels": "chol_mfma_kernel<NB, WAVES, true, double> per order bucket, FP64 refinement to tol",
+            "launches_per_step": len(batch.buckets),
+            "device_eta_max": float(eta_r.max()),
+        }
+        print(f"[bench] per-bucket refined form: {refined_info['ms_per_step']:.3f} ms/step", file=sys.stderr,
+              flush=True)
 
     acc = accuracy_sample(host, sol)
     value = host.count / (mean_ms / 1e3)
@@ -383,8 +411,11 @@ def bench_uneven_cholesky(args):
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic,
-        "kernel": "chol_uneven_kernel<16|11|8, true, double>: one persistent launch per register-tile class "
-                  "(orders 177..256, 129..176, <= 128) per step; achieved over the step's three launches",
+        "traffic_source": "profiles/uneven_cholesky_pmc.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE of one step, "
+                          "copy-calibrated); not measured in this run",
+        "kernel": "chol_mfma_kernel<16|8|4, 4|4|1, true, double>: one persistent launch per tile class (16x16 "
+                  "blocks for orders 129..256, 8x8 for 65..128, 4x4 for <= 64) per step, plus a fill and a "
+                  "prepass launch; achieved over the whole step",
         "algorithmic_bytes_per_launch": batch.algorithmic_bytes,
         "fp32_flop_view": {
             "achieved_tflops": batch.algorithmic_flops / (mean_ms / 1e3) / 1e12,
@@ -428,6 +459,11 @@ def bench_uneven_cholesky(args):
         "plain_form": plain_info,
         "speedup_vs_plain": (value / plain_info["value"]) if plain_info else None,
         "per_bucket_fused_kernel": fused_info,
+        "per_bucket_refined": refined_info,
+        "speedup_vs_per_bucket_refined": (value / refined_info["value"]) if refined_info else None,
+        "accuracy_note": "chosen form: float64 answers refined to device eta <= tol; plain form: float32 answers, "
+                         "unrefined (max_backward_error_sample); per_bucket_refined: the plain form's launch "
+                         "schedule at the chosen form's accuracy",
         "roofline": roofline,
         "cpu_baseline": cpu,
         "wall_s_timed": wall,
@@ -458,7 +494,8 @@ def _line(metric, value, unit, ms, args, config, dtype, roofline, cpu, extra, hi
 
 
 def bench_small_lu(args):
-    """1024 independent 32 x 32 float32 LU solves (factor + solve), one batched launch each."""
+    """1024 independent 32 x 32 float32 LU solves: one fused factor + solve launch
+    (lxa_lu_small_solve), as linear_solve with lx.LU() runs them."""
     import torch
 
     from lineax_amd import _kernels as K
@@ -471,8 +508,10 @@ def bench_small_lu(args):
     res = {}
 
     def step():
-        lu, piv, _ = K.lu_factor(a)
-        res["x"] = K.lu_solve(lu, piv, b, trans=0)
+        # what linear_solve(MatrixLinearOperator(A), b, LU()) runs on a batch of
+        # small operators: one fused getrf + getrs launch that also writes the
+        # factors and pivots (the solver state)
+        res["x"], _, _, _ = K.lu_small_solve(a, b, want_factors=True)
 
     ms, _ = _time_steps(step, args.steps, args.warmup)
     t = float(np.mean(ms))
@@ -486,7 +525,7 @@ def bench_small_lu(args):
                  {"workload": "small_lu", "systems": 1024, "n": 32, "seed": 0}, "f32",
                  {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                  "kernel": "lu_factor_small_kernel + lu_solve_small_kernel (step)",
+                  "kernel": "lu_small_fused_kernel<float, 32> (getrf + getrs, factors and pivots written)",
                   "algorithmic_bytes_per_launch": 1024 * per_bytes},
                  {"value": 1024 / cpu_s, "unit": "solves/s", "cores": 1, "kind": "port",
                   "sample": "all 1024 systems: LAPACK sgetrf+sgetrs per system (scipy, float32), one thread"},
@@ -798,13 +837,14 @@ def bench_serving(args):
     if not _SERVING_WARM:
         # the first requests of a process load every kernel of the loop: warm
         # both forms up on a short stream of their own before timing
-        for form in ("chosen", "plain"):
+        for form in ("chosen", "plain", "fifo"):
             wr, wo = S.make_request_stream(args.rate, 12, seed=99, device="cuda:0", qr_m=(64, 1024),
                                            chol_n=(16, 512))
             S.SolveServer("cuda:0", form=form, hbm_budget=4 << 30, refine_steps=args.refine_steps).run(wr, wo)
         torch.cuda.synchronize()
         _SERVING_WARM.append(True)
-    for form in ("chosen", "plain"):
+    forms = ("chosen", "plain", "fifo") if args.rate == 2000.0 else ("chosen", "plain")
+    for form in forms:
         reqs, ops = S.make_request_stream(args.rate, args.requests, seed=5, device="cuda:0")
         if cpu is None and not args.no_cpu_baseline:
             cpu = _serving_cpu_baseline(reqs, ops)
@@ -830,6 +870,9 @@ def bench_serving(args):
                   "kernel": "whole request loop (QR panel + trailing kernels dominate)"},
                  cpu,
                  {"p50_ms": c["p50_ms"], "p99_ms": c["p99_ms"], "chosen": c, "plain_form": p,
+                  "plain_form_kind": "static batching: FP32 kernels once per request kind over each batch of "
+                                     "arrived requests, no refinement, no residency",
+                  "fifo_form": out.get("fifo"),
                   "speedup_vs_plain": c["solves_per_s"] / p["solves_per_s"] if p["solves_per_s"] else None})
 
 

@@ -26,9 +26,12 @@ The chosen form (``SolveServer(form="chosen")``) runs in ticks:
    ``resident_budget`` bytes, least recently used evicted) for reuse
    requests.
 
-The plain form runs the same float32 kernels one request at a time in arrival
-order (no batching, no reuse: a reuse request factors its operator again; no
-refinement, so its answers are float32).
+The plain form (``form="plain"``) is static batching: when idle it takes every
+request that has arrived as one batch, runs the float32 kernels once per
+request kind over the batch (uneven QR factor + apply + trsv, uneven blocked
+Cholesky), without refinement or residency, and takes the next batch when this
+one is done.  ``form="fifo"`` runs the same kernels one request at a time in
+arrival order.  Their answers are float32.
 
 Semantically each request is ``lx.linear_solve(lx.MatrixLinearOperator(A),
 b, lx.QR())`` (lineax/_solver/qr.py:55-95; least squares for m >= n) or
@@ -49,6 +52,7 @@ import torch
 from . import _kernels as K
 
 QR_ELEM = 4  # float32 bytes
+SMALL_CHOL = 256  # Cholesky orders served by the uneven tile-class launch
 SOLVE_STREAMS = 2  # with the factor stream and the default stream: the 4 hardware queues of a process
 
 
@@ -134,8 +138,8 @@ class _Resident:
 class SolveServer:
     def __init__(self, device, *, form: str = "chosen", hbm_budget: int = 8 << 30,
                  resident_budget: int = 2 << 30, refine_steps: int = 2):
-        if form not in ("chosen", "plain"):
-            raise ValueError("form must be 'chosen' or 'plain'")
+        if form not in ("chosen", "plain", "fifo"):
+            raise ValueError("form must be 'chosen', 'plain' or 'fifo'")
         if refine_steps < 0:
             raise ValueError("refine_steps must be >= 0")
         self.device = torch.device(device)
@@ -148,6 +152,7 @@ class SolveServer:
         self.resident: dict[int, _Resident] = {}
         self.resident_elems = 0
         self.nb = K.qr_block(torch.float32)
+        self._uneven_ws = torch.zeros(256, dtype=torch.uint8, device=self.device)
         self.ticks = 0
         self.tick_log: list[dict] = []
 
@@ -344,8 +349,10 @@ class SolveServer:
                 raise MemoryError(f"request {r.rid} ({r.kind} {r.m}x{r.n}) exceeds the HBM arena")
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
-        if self.form == "plain":
+        if self.form == "fifo":
             self._run_plain(reqs, t0, time_limit)
+        elif self.form == "plain":
+            self._run_static(reqs, t0, time_limit)
         else:
             self._run_chosen(reqs, t0, time_limit)
         makespan = time.perf_counter() - t0
@@ -398,6 +405,54 @@ class SolveServer:
             torch.cuda.synchronize(self.device)
             r.done = self._now(t0)
             self._free_blocks(blocks)
+            self.ticks += 1
+
+    def _run_static(self, reqs, t0, time_limit):
+        """The plain form: static batching.  When idle, every request that has
+        arrived (as many as fit in the arena) becomes one batch: the least-
+        squares requests are factored in one uneven lxa_qr_factor launch
+        sequence and solved in one apply + trsv pair, the Cholesky requests in
+        one lxa_chol_blocked factor + solve pair (FP32, one launch per size
+        bucket = request kind, no refinement, no residency: a reuse request
+        factors its operator again); the next batch starts when this one is
+        done."""
+        i = 0
+        buf = self.arena.buf
+        while i < len(reqs):
+            now = self._now(t0)
+            if now > time_limit:
+                break
+            if reqs[i].arrival > now:
+                continue
+            qr_items, ch_items, taken = [], [], []
+            while i < len(reqs) and reqs[i].arrival <= now:
+                r = reqs[i]
+                fp = qr_footprint(r.m, r.n, self.nb) if r.kind == "qr" else chol_footprint(r.n)
+                blocks = self._alloc_blocks(fp)
+                if blocks is None:
+                    if not taken:
+                        raise MemoryError("request does not fit in the arena")
+                    break
+                a = r.a if r.a is not None else self._operator_data[r.op_id]
+                self.arena.view(*blocks["a"]).copy_(a.reshape(-1))
+                self.arena.view(*blocks["b"]).copy_(r.b)
+                r.admitted = now
+                (qr_items if r.kind == "qr" else ch_items).append((r.m, r.n, r.n, blocks))
+                taken.append((r, blocks))
+                i += 1
+            if qr_items:
+                dd = self._descs(qr_items)
+                max_m = max(it[0] for it in qr_items)
+                max_n = max(it[1] for it in qr_items)
+                K.qr_factor_raw(dd, len(qr_items), max_m, max_n, min(max_m, max_n), buf, buf, buf, buf)
+                self._qr_solve_batch(qr_items)
+            self._chol_batch(ch_items, factor=True)
+            xs = [self.arena.view(blocks["b"][0], r.n).clone() for r, blocks in taken]
+            torch.cuda.synchronize(self.device)
+            t_done = self._now(t0)
+            for (r, blocks), x in zip(taken, xs):
+                r.x, r.done = x, t_done
+                self._free_blocks(blocks)
             self.ticks += 1
 
     def _run_chosen(self, reqs, t0, time_limit):
@@ -456,13 +511,20 @@ class SolveServer:
                 if inflight:
                     finish_batches(False)
                 continue
-            qr_done, chol_new, solve_reuse = [], [], []
+            qr_done, chol_new, solve_reuse, chol_small = [], [], [], []
             still, blocked = [], False
             sstream = sstreams[self.ticks % len(sstreams)]
             with torch.cuda.stream(fstream):
                 for r in waiting:
                     if blocked:
                         still.append(r)
+                        continue
+                    if r.kind == "chol" and r.n <= SMALL_CHOL and self.refine_steps > 0:
+                        # factored, solved and refined in one persistent launch
+                        # straight from the operator's storage (no arena copy);
+                        # unrefined serving keeps the float32 potrf batches
+                        r.admitted = now
+                        chol_small.append(r)
                         continue
                     if r.reuse and r.op_id in self.resident:
                         res = self.resident[r.op_id]
@@ -501,7 +563,7 @@ class SolveServer:
                             self.arena.view(*blocks["b"]).copy_(r.b)
                         chol_new.append((r, (r.n, r.n, r.n, blocks), None))
                 waiting = still
-                if blocked and not (factoring or qr_done or chol_new or solve_reuse or inflight):
+                if blocked and not (factoring or qr_done or chol_new or solve_reuse or chol_small or inflight):
                     if not self._evict_one():
                         raise MemoryError("admission blocked with nothing in flight")
                     continue
@@ -544,6 +606,9 @@ class SolveServer:
                 if batch:
                     xs = self._solutions(batch)
                     batch = [(r, x, item, rb) for (r, item, rb), x in zip(batch, xs)]
+                if chol_small:
+                    batch += [(r, x, None, None) for r, x in zip(chol_small, self._chol_small_solve(chol_small))]
+                if batch:
                     ev = torch.cuda.Event()
                     ev.record(sstream)
                     inflight.append((ev, batch))
@@ -555,9 +620,40 @@ class SolveServer:
         while inflight:
             finish_batches(True)
 
+    def _chol_small_solve(self, reqs):
+        """Cholesky requests of order <= SMALL_CHOL: one lxa_chol_uneven_solve
+        (persistent tile-class launches on the matrix cores, FP64 refinement to
+        eta <= 1e-12) reading each operator where it lives.  Returns float64
+        solutions, on the current stream."""
+        dev = self.device
+        ops = [r.a if r.a is not None else self._operator_data[r.op_id] for r in reqs]
+        ptr = np.array([a.data_ptr() for a in ops], dtype=np.int64)
+        base = int(ptr.min())
+        base_t = next(a for a in ops if a.data_ptr() == base)
+        n = np.array([r.n for r in reqs], dtype=np.int64)
+        b_off = np.concatenate(([0], np.cumsum(n)[:-1]))
+        order = np.lexsort((np.arange(len(reqs)), -n))
+        # one host-to-device copy: a offsets, b offsets, orders, work order
+        meta = np.concatenate(((ptr - base) // 4, b_off, n, order)).astype(np.int64)
+        meta_d = K.host_to_device(torch.from_numpy(meta), dev)
+        cnt = len(reqs)
+        a_off_d, b_off_d = meta_d[:cnt], meta_d[cnt: 2 * cnt]
+        n_d, order_d = meta_d[2 * cnt: 3 * cnt].to(torch.int32), meta_d[3 * cnt:].to(torch.int32)
+        b = torch.cat([r.b for r in reqs])
+        x = torch.empty(b.numel(), dtype=torch.float64, device=dev)
+        info = torch.empty(cnt, dtype=torch.int32, device=dev)
+        a_all = torch.as_strided(base_t, (1,), (1,))  # base pointer; offsets reach every operator
+        K.chol_uneven_solve(n_d, a_off_d, a_all, b_off_d, b, x, order_d, max_n=int(n.max()), persistent=True,
+                            max_refine=10, tol=1e-12, info=info, workspace=self._uneven_ws, sorted_order=True)
+        return [x[int(o): int(o) + int(k)] for o, k in zip(b_off, n)]
+
     def _retire(self, r, item, rhs_blocks, owner_pending):
         """Returns a finished request's blocks to the arena or keeps its
         factorisation resident for reuse requests."""
+        if item is None:  # a small Cholesky request: nothing in the arena
+            if owner_pending.get(r.op_id) is r:
+                owner_pending.pop(r.op_id, None)
+            return
         m, n, lda, blk = item
         if rhs_blocks is not None:
             self._free_blocks(rhs_blocks)  # a reuse solve: the resident factor stays

@@ -220,3 +220,21 @@ def test_request_loop_admission_respects_budget(device):
     assert stats["arena_high_water_bytes"] <= budget
     for r in reqs:
         _check_answer(r, ops)
+
+
+def test_static_and_fifo_plain_forms_agree(device):
+    """The static-batching plain form runs the FIFO form's float32 kernels over
+    batches: the answers agree to float32 rounding, and every request of an
+    overloaded stream completes."""
+    out = {}
+    for form in ("plain", "fifo"):
+        reqs, ops = S.make_request_stream(20000.0, 24, seed=15, device=device, qr_m=(64, 600), chol_n=(16, 300))
+        stats = S.SolveServer(device, form=form, hbm_budget=1 << 29).run(reqs, ops)
+        assert stats["completed"] == 24
+        for r in reqs:
+            assert r.x.dtype == torch.float32
+            _check_answer(r, ops)
+        out[form] = [r.x.clone() for r in reqs]
+    for xs, xf in zip(out["plain"], out["fifo"]):
+        scale = float(torch.max(torch.abs(xf)))
+        assert float(torch.max(torch.abs(xs - xf))) <= 1e-5 * max(1.0, scale)
