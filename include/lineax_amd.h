@@ -551,7 +551,8 @@ int lxa_lu_large_factor(void* stream, int mode, int64_t n, float* a, int64_t lda
                         int32_t* piv, int32_t* info);
 /* The factorisation's row permutation: perm[i] = the original row at
  * position i once the swap sequence piv has been applied in order (getrf's
- * row interchanges, laswp, composed once); lxa_lu_large_permute then
+ * row interchanges, laswp, composed once).  piv must be windowed as
+ * lxa_lu_large_factor writes it: piv[j] in [j, j + 256).  lxa_lu_large_permute then
  * applies it to nrhs vectors x[r * ldx + i]: inverse = 0 gives y = P x
  * (before the L solve), inverse = 1 gives y = P^T x (after the L^T solve of
  * a transposed system); y must not alias x. */

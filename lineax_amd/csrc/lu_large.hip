@@ -32,6 +32,8 @@
 // appear, multiplies by the stored inverse of its diagonal block, and
 // publishes its flag.
 
+#include <mutex>
+
 #include "lxa_common.h"
 
 namespace lxa {
@@ -39,7 +41,6 @@ namespace {
 
 constexpr int kB = 128;          // diagonal / triangular-solve block
 constexpr int kOuter = 512;      // outer panel width
-constexpr int kDiagThreads = 1024;
 constexpr int kTrsvThreads = 256;
 constexpr int kMaxRhs = 8;
 constexpr int kSpinLimit = 1 << 26;
@@ -97,225 +98,488 @@ __global__ void rbt_vec_kernel(int n, int s, int nrhs, T* __restrict__ x, int64_
 }
 
 // ---------------------------------------------------------------------------
-// LU with partial pivoting restricted to the nb x nb diagonal block at (k, k)
-// (nb <= 128), in place, and the inverses of its unit-lower and upper
-// triangles written as dense 128 x 128 blocks (zeros outside the triangle).
-// Pivoting inside the block keeps its multipliers <= 1 in magnitude, so the
-// block's L is well conditioned and its explicit inverse accurate; the
-// butterflies keep the growth across blocks in check.  piv[k + j] receives
-// the 0-based global row swapped with row k + j (always inside the block);
-// the caller applies the swaps to the rest of the block's rows.  info[0] is
-// set to 1 + the global index of the first exactly-zero pivot.
+// LU with partial pivoting of the pivot window at (k, k): the nb x nb
+// diagonal block (nb <= 128) and the rows below it, win rows in all
+// (win <= 256), in place, plus the inverses of the block's unit-lower and
+// upper triangles as dense 128 x 128 blocks (zeros outside the triangle;
+// identity beyond nb when nb < 128).  Pivoting over the window keeps every
+// multiplier of the block column (the diagonal block's L and the next
+// block's L21) bounded by 1; over the diagonal block alone, the multipliers
+// below it are unbounded and the growth reaches ~1e3 on Gaussian operators
+// (~1e-5 backward error instead of ~1e-7).  piv[k + j] receives the 0-based
+// global row swapped with row k + j (always inside the window); the caller
+// applies the swaps to the columns outside the block.  info[0] is set to
+// 1 + the global index of the first exactly-zero pivot.
+//
+// One workgroup of 256 threads; the window lives in LDS, thread r owning
+// window row r (a "physical" row that never moves: each thread tracks the
+// position its row has reached, and phys_at[] maps positions back, as the
+// row interchanges of getrf would move them).  Blocked right-looking over
+// 16-column leaves:
+//   * leaf columns are factored one column per barrier: every wave finds
+//     its arg-max |a(i, j)| (ties: lowest position) with DPP moves, the four
+//     wave winners meet in LDS, each thread reads the pivot row's leaf
+//     entries from LDS and updates its own row's leaf entries in registers,
+//     then writes them back for the next column's pivot;
+//   * the leaf's U rows over the trailing columns: U = L_leaf^-1 A(pivot
+//     rows), one thread per column;
+//   * the trailing columns of the rows still below the leaf: A -= L U, each
+//     thread its own row, the U rows read from LDS as broadcasts.
+// Finally the factors are written in position order, and the triangles of
+// the diagonal block are inverted by recursive doubling in LDS:
+//   inv [A 0; C B] = [A^-1 0; -B^-1 C A^-1  B^-1]
+// from 16 x 16 diagonal blocks (forward / backward substitution, one thread
+// per column) up to 128 x 128.
 // ---------------------------------------------------------------------------
-// 1024 threads as a 32 x 32 grid; thread (ty, tx) keeps the 4 x 4 elements
-// (ty + 32 a, tx + 32 b) in registers.  Per column: the owners of column j
-// publish it, every wave finds the pivot from it redundantly, the owners of
-// rows j and p publish those rows, and each thread swaps and updates its own
-// elements -- two barriers per column, LDS buffers alternating by column.
-// The inverses run the same way on register tiles with the factors staged
-// in LDS, one barrier per column.
-// One step of a wave arg-max (value, lowest index on ties) with a DPP
-// partner: a register-crossbar move, no LDS round trip.
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kPivWin = 2 * kB;     // rows of the pivot window
+constexpr int kPanelThreads = 256;  // one window row per thread
+constexpr int kLeaf = 16;
+constexpr int kPLd = kB + 4;  // LDS row stride (16-byte rows)
+// dynamic LDS: the window [256][kPLd] and the leaf's U rows by column [128][16]
+constexpr size_t kPanelLds = sizeof(float) * ((size_t)kPivWin * kPLd + (size_t)kB * kLeaf);
+
+// One step of a wave maximum of 64-bit keys (hi, lo) with a DPP partner.
 template <int kCtrl>
-__device__ __forceinline__ void amax_dpp(float& v, int& idx) {
-  const float ov = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kCtrl, 0xF, 0xF, false));
-  const int oi = __builtin_amdgcn_mov_dpp(idx, kCtrl, 0xF, 0xF, false);
-  if (ov > v || (ov == v && oi < idx)) {
-    v = ov;
-    idx = oi;
-  }
+__device__ __forceinline__ void key_max_dpp(uint32_t& hi, uint32_t& lo) {
+  const uint32_t oh = (uint32_t)__builtin_amdgcn_mov_dpp((int)hi, kCtrl, 0xF, 0xF, false);
+  const uint32_t ol = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo, kCtrl, 0xF, 0xF, false);
+  const bool gt = oh > hi || (oh == hi && ol > lo);
+  hi = gt ? oh : hi;
+  lo = gt ? ol : lo;
 }
 
-// Arg-max over the wave: quad_perm [1,0,3,2], quad_perm [2,3,0,1],
-// row_half_mirror and row_mirror leave each 16-lane row's arg-max in all of
-// its lanes; the four rows are combined from scalar reads.  The result is
-// wave-uniform.
-__device__ __forceinline__ int wave_argmax(float v, int idx) {
-  amax_dpp<0xB1>(v, idx);
-  amax_dpp<0x4E>(v, idx);
-  amax_dpp<0x141>(v, idx);
-  amax_dpp<0x140>(v, idx);
-  float bv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-  int bi = __builtin_amdgcn_readlane(idx, 0);
+// Wave maximum of (hi, lo): quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror and row_mirror leave each 16-lane row's maximum in all of
+// its lanes; the four rows are combined from scalar reads (wave-uniform).
+__device__ __forceinline__ uint64_t wave_key_max(uint32_t hi, uint32_t lo) {
+  key_max_dpp<0xB1>(hi, lo);
+  key_max_dpp<0x4E>(hi, lo);
+  key_max_dpp<0x141>(hi, lo);
+  key_max_dpp<0x140>(hi, lo);
+  uint64_t best = 0;
 #pragma unroll
-  for (int r = 1; r < 4; ++r) {
-    const float rv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16 * r));
-    const int ri = __builtin_amdgcn_readlane(idx, 16 * r);
-    if (rv > bv || (rv == bv && ri < bi)) {
-      bv = rv;
-      bi = ri;
-    }
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t h = (uint32_t)__builtin_amdgcn_readlane((int)hi, 16 * q);
+    const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)lo, 16 * q);
+    const uint64_t v = ((uint64_t)h << 32) | l;
+    best = v > best ? v : best;
   }
-  return bi;
+  return best;
 }
 
-// Rows of the pivot window: the diagonal block and the block below it.
-// Pivoting over 256 rows keeps every multiplier of the block column (the
-// diagonal block's L and the next block's L21) bounded by 1; over the
-// diagonal block alone, the multipliers below it are unbounded and the
-// growth reaches ~1e3 on Gaussian operators (~1e-5 backward error instead
-// of ~1e-7).
-constexpr int kPivWin = 2 * kB;
+__global__ __launch_bounds__(kPanelThreads) void panel_lu_kernel(int k, int nb, int win, float* __restrict__ a,
+                                                                 int64_t lda, float* __restrict__ linv,
+                                                                 float* __restrict__ uinv, int32_t* __restrict__ piv,
+                                                                 int32_t* __restrict__ info, int s_lo, int s_hi) {
+  extern __shared__ __align__(16) float psm[];
+  float* P = psm;                              // [256][kPLd]: the window, by physical row
+  float* UB = psm + (size_t)kPivWin * kPLd;    // [128][16]: the leaf's U rows, by column
+  __shared__ uint32_t keys[2][4][2];           // per column parity: the four wave winners
+  __shared__ int phys_at[kPivWin];             // physical row at each position
+  __shared__ int piv_s[kB];                    // the interchanges, written out at the end
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = tid;
 
-__global__ __launch_bounds__(kDiagThreads) void diag_lu_kernel(int k, int nb, int win, float* __restrict__ a,
-                                                               int64_t lda, float* __restrict__ linv,
-                                                               float* __restrict__ uinv, int32_t* __restrict__ piv,
-                                                               int32_t* __restrict__ info) {
-  extern __shared__ __align__(16) float dsm[];
-  float* S = dsm;  // [128][129]: the factors, for the inverse sweeps
-  constexpr int LD = kB + 1;
-  constexpr int kRowT = kPivWin / 32;  // register rows per thread
-  __shared__ float colv[2][kPivWin];
-  __shared__ float rowj[2][kB];
-  __shared__ float rowp[2][kB];
-  const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63;
-  // R[ai][bi]: window row ty + 32 ai, column tx + 32 bi.  Rows of the
-  // diagonal block beyond nb are identity padding; window rows beyond win
-  // are zero and never chosen.
-  float R[kRowT][4];
+  // the window: real rows r < win (columns >= nb zero); when nb < 128 (then
+  // win = nb) rows nb..127 are identity padding, fixed at their positions
+  constexpr int kBatch = 16;  // loads in flight per thread
+  // float4 rows when the matrix allows (lda and the base 16-byte aligned;
+  // k is a multiple of 128)
+  const bool vec = (lda & 3) == 0 && (reinterpret_cast<uintptr_t>(a) & 15) == 0;
+  if (vec) {
+    constexpr int kQuads = kB / 4;
+    for (int base = 0; base < kPivWin * kQuads; base += 8 * kPanelThreads) {
+      float4 v[8];
 #pragma unroll
-  for (int ai = 0; ai < kRowT; ++ai)
-#pragma unroll
-    for (int bi = 0; bi < 4; ++bi) {
-      const int r = ty + 32 * ai, c = tx + 32 * bi;
-      if (r < kB)
-        R[ai][bi] = (r < nb && c < nb) ? a[(int64_t)(k + r) * lda + k + c] : (r == c ? 1.f : 0.f);
-      else
-        R[ai][bi] = (r < win && c < nb) ? a[(int64_t)(k + r) * lda + k + c] : 0.f;
-    }
-  for (int j = 0; j < kB; ++j) {
-    const int buf = j & 1, jb = j >> 5, jt = j & 31;
-    if (tx == jt) {
-#pragma unroll
-      for (int ai = 0; ai < kRowT; ++ai)
-#pragma unroll
-        for (int bi = 0; bi < 4; ++bi)
-          if (bi == jb) colv[buf][ty + 32 * ai] = R[ai][bi];
-    }
-    __syncthreads();
-    // arg-max of |column j| over window rows >= j (NaN counts as largest;
-    // ties: lowest row), computed by every wave
-    float v = -1.f;
-    int idx = lane;
-#pragma unroll
-    for (int q = 0; q < kPivWin / kWave; ++q) {
-      const int row = lane + kWave * q;
-      float w = -1.f;
-      if (row >= j && (row < kB || row < win)) {
-        w = fabsf(colv[buf][row]);
-        if (w != w) w = INFINITY;
+      for (int u = 0; u < 8; ++u) {
+        const int idx = base + tid + u * kPanelThreads;
+        const int i = idx / kQuads, c = 4 * (idx % kQuads);
+        v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < win) {
+          const float* src = a + (int64_t)(k + i) * lda + k + c;
+          if (c + 4 <= nb) {
+            v[u] = *reinterpret_cast<const float4*>(src);
+          } else if (c < nb) {
+            v[u].x = src[0];
+            if (c + 1 < nb) v[u].y = src[1];
+            if (c + 2 < nb) v[u].z = src[2];
+          }
+        } else if (i < kB) {
+          v[u].x = (i == c) ? 1.f : 0.f;
+          v[u].y = (i == c + 1) ? 1.f : 0.f;
+          v[u].z = (i == c + 2) ? 1.f : 0.f;
+          v[u].w = (i == c + 3) ? 1.f : 0.f;
+        }
       }
-      if (w > v) {
-        v = w;
-        idx = row;
-      }
-    }
-    const int p = wave_argmax(v, idx);
-    const float pv = colv[buf][p];
-    // rows j and p to LDS (before the swap)
-    if (ty == jt) {
 #pragma unroll
-      for (int ai = 0; ai < 4; ++ai)
-        if (ai == jb) {
-#pragma unroll
-          for (int bi = 0; bi < 4; ++bi) rowj[buf][tx + 32 * bi] = R[ai][bi];
-        }
-    }
-    if (ty == (p & 31)) {
-#pragma unroll
-      for (int ai = 0; ai < kRowT; ++ai)
-        if (ai == (p >> 5)) {
-#pragma unroll
-          for (int bi = 0; bi < 4; ++bi) rowp[buf][tx + 32 * bi] = R[ai][bi];
-        }
-    }
-    if (tid == 0 && j < nb) {
-      piv[k + j] = k + p;
-      if (pv == 0.f && info[0] == 0) info[0] = k + j + 1;
-    }
-    __syncthreads();
-    // swap rows j and p, multipliers into column j, rank-1 update
-#pragma unroll
-    for (int ai = 0; ai < kRowT; ++ai) {
-      const int r = ty + 32 * ai;
-      if (r == j) {
-#pragma unroll
-        for (int bi = 0; bi < 4; ++bi) R[ai][bi] = rowp[buf][tx + 32 * bi];
-      } else if (r > j) {
-        const bool isp = r == p;
-        const float l = (isp ? colv[buf][j] : colv[buf][r]) / pv;
-#pragma unroll
-        for (int bi = 0; bi < 4; ++bi) {
-          const int c = tx + 32 * bi;
-          const float old = isp ? rowj[buf][c] : R[ai][bi];
-          R[ai][bi] = c == j ? l : (c > j ? old - l * rowp[buf][c] : old);
-        }
+      for (int u = 0; u < 8; ++u) {
+        const int idx = base + tid + u * kPanelThreads;
+        *reinterpret_cast<float4*>(P + (idx / kQuads) * kPLd + 4 * (idx % kQuads)) = v[u];
       }
     }
   }
-  // the factors to global memory and to LDS; the window rows below the
-  // block leave with their final multipliers (their L21 block)
+  for (int base = 0; !vec && base < kPivWin * kB; base += kBatch * kPanelThreads) {
+    float v[kBatch];
 #pragma unroll
-  for (int ai = 0; ai < kRowT; ++ai)
-#pragma unroll
-    for (int bi = 0; bi < 4; ++bi) {
-      const int r = ty + 32 * ai, c = tx + 32 * bi;
-      if (r < kB) {
-        S[r * LD + c] = R[ai][bi];
-        if (r < nb && c < nb) a[(int64_t)(k + r) * lda + k + c] = R[ai][bi];
-      } else if (r < win && c < nb) {
-        a[(int64_t)(k + r) * lda + k + c] = R[ai][bi];
+    for (int u = 0; u < kBatch; ++u) {
+      const int idx = base + tid + u * kPanelThreads;
+      const int i = idx / kB, c = idx % kB;
+      v[u] = 0.f;
+      if (i < win) {
+        if (c < nb) v[u] = a[(int64_t)(k + i) * lda + k + c];
+      } else if (i < kB) {
+        v[u] = (i == c) ? 1.f : 0.f;
       }
     }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int idx = base + tid + u * kPanelThreads;
+      P[(idx / kB) * kPLd + idx % kB] = v[u];
+    }
+  }
+  phys_at[tid] = tid;
+  int zero_at = 0;  // 1 + the first column with a zero pivot (thread 0)
+  int pos = r;
+  bool done = r >= win;  // padding and absent rows never pivot
   __syncthreads();
-  // X = L^-1 (unit lower): for j: X[i][:] -= L[i][j] X[j][:], i > j.
-  // Y = (U^T)^-1 (lower): for j: Y[j][:] /= U[j][j]; Y[i][:] -= U[j][i] Y[j][:].
-  // uinv = Y^T.  Row j of each is final when column j is processed.
-  float X[4][4], Y[4][4];
+
+  const int nleaf = (nb + kLeaf - 1) / kLeaf;
+  for (int J = 0; J < nleaf; ++J) {
+    const int c0 = J * kLeaf;
+    float* prow_mine = P + r * kPLd + c0;
+    float x[kLeaf];
 #pragma unroll
-  for (int ai = 0; ai < 4; ++ai)
-#pragma unroll
-    for (int bi = 0; bi < 4; ++bi) X[ai][bi] = Y[ai][bi] = (ty + 32 * ai == tx + 32 * bi) ? 1.f : 0.f;
-  for (int j = 0; j < kB; ++j) {
-    const int buf = j & 1, jb = j >> 5, jt = j & 31;
-    if (ty == jt) {
-      const float d = S[j * LD + j];
-#pragma unroll
-      for (int ai = 0; ai < 4; ++ai)
-        if (ai == jb) {
-#pragma unroll
-          for (int bi = 0; bi < 4; ++bi) {
-            Y[ai][bi] /= d;
-            rowj[buf][tx + 32 * bi] = X[ai][bi];
-            rowp[buf][tx + 32 * bi] = Y[ai][bi];
-          }
-        }
+    for (int q = 0; q < kLeaf / 4; ++q) {
+      const float4 v = *reinterpret_cast<const float4*>(prow_mine + 4 * q);
+      x[4 * q] = v.x;
+      x[4 * q + 1] = v.y;
+      x[4 * q + 2] = v.z;
+      x[4 * q + 3] = v.w;
     }
-    __syncthreads();
 #pragma unroll
-    for (int ai = 0; ai < 4; ++ai) {
-      const int i = ty + 32 * ai;
-      if (i > j) {
-        const float lij = S[i * LD + j], uji = S[j * LD + i];
+    for (int j = 0; j < kLeaf; ++j) {
+      const int c = c0 + j;
+      if (c >= nb) continue;  // (uniform; a guard rather than a break keeps the loop unrolled)
+      const int buf = j & 1;
+      // key: |a(r, c)| (NaN above every number), then the lowest position,
+      // then the physical row; rows already pivoted have key 0
+      uint32_t hi = 0, lo = 0;
+      if (!done) {
+        hi = __float_as_uint(x[j]) & 0x7fffffffu;
+        lo = 0x80000000u | ((uint32_t)(kPivWin - 1 - pos) << 8) | (uint32_t)r;
+      }
+      const uint64_t wbest = wave_key_max(hi, lo);
+      if (lane == 0) {
+        keys[buf][wave][0] = (uint32_t)(wbest >> 32);
+        keys[buf][wave][1] = (uint32_t)wbest;
+      }
+      __syncthreads();
+      uint64_t best = 0;
 #pragma unroll
-        for (int bi = 0; bi < 4; ++bi) {
-          const int c = tx + 32 * bi;
-          if (c <= j) {
-            X[ai][bi] -= lij * rowj[buf][c];
-            Y[ai][bi] -= uji * rowp[buf][c];
-          }
+      for (int w = 0; w < 4; ++w) {
+        const uint64_t v = ((uint64_t)keys[buf][w][0] << 32) | keys[buf][w][1];
+        best = v > best ? v : best;
+      }
+      const int p = (int)(best & 0xffu);                              // pivot's physical row
+      const int cp = kPivWin - 1 - (int)((best >> 8) & 0xffu);        // and its position
+      const float* pr = P + p * kPLd + c0;
+      float prow[kLeaf];
+#pragma unroll
+      for (int q = j / 4; q < kLeaf / 4; ++q) {
+        const float4 v = *reinterpret_cast<const float4*>(pr + 4 * q);
+        prow[4 * q] = v.x;
+        prow[4 * q + 1] = v.y;
+        prow[4 * q + 2] = v.z;
+        prow[4 * q + 3] = v.w;
+      }
+      const float pv = prow[j];
+      if (tid == 0) {
+        piv_s[c] = k + cp;
+        if (pv == 0.f && zero_at == 0) zero_at = k + c + 1;
+      }
+      if (r == p) {
+        pos = c;
+        done = true;
+        phys_at[c] = r;
+      } else if (!done) {
+        if (pos == c) {  // the row at position c takes the pivot's old position
+          pos = cp;
+          phys_at[cp] = r;
         }
+        const float l = (pv != 0.f) ? x[j] / pv : x[j];
+        x[j] = l;
+#pragma unroll
+        for (int jj = j + 1; jj < kLeaf; ++jj) x[jj] -= l * prow[jj];
+        // the updated leaf entries, for the next column's pivot row
+#pragma unroll
+        for (int q = j / 4; q < kLeaf / 4; ++q)
+          *reinterpret_cast<float4*>(prow_mine + 4 * q) =
+              make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
       }
     }
-  }
+    __syncthreads();  // the leaf's factors and phys_at are final
+    const int t0 = c0 + kLeaf;
+    if (t0 >= nb) break;
+    // U rows of the leaf over the trailing columns (columns beyond nb are
+    // zero in every real row and come out zero)
+    if (tid >= t0 && tid < kB) {
+      const int col = tid;
+      float s[kLeaf];
+      int q_at[kLeaf];
 #pragma unroll
-  for (int ai = 0; ai < 4; ++ai)
+      for (int i = 0; i < kLeaf; ++i) {
+        q_at[i] = phys_at[c0 + i];
+        s[i] = P[q_at[i] * kPLd + col];
+      }
 #pragma unroll
-    for (int bi = 0; bi < 4; ++bi) {
-      const int r = ty + 32 * ai, c = tx + 32 * bi;
-      linv[r * kB + c] = X[ai][bi];
-      uinv[c * kB + r] = Y[ai][bi];
+      for (int m = 0; m < kLeaf; ++m) {
+        // s[m] is final: u_m; remove it from the rows below
+        const float um = s[m];
+        P[q_at[m] * kPLd + col] = um;
+        UB[col * kLeaf + m] = um;
+#pragma unroll
+        for (int i = m + 1; i < kLeaf; ++i) s[i] -= P[q_at[i] * kPLd + c0 + m] * um;
+      }
     }
+    __syncthreads();
+    // the trailing columns of the rows still below the leaf
+    if (!done) {
+      const int cend = (nb + 3) & ~3;
+      float* row = P + r * kPLd;
+#pragma unroll 2
+      for (int col = t0; col < cend; col += 4) {
+        float4 v = *reinterpret_cast<const float4*>(row + col);
+        const float* u = UB + col * kLeaf;
+#pragma unroll
+        for (int m4 = 0; m4 < kLeaf / 4; ++m4) {
+          const float4 u0 = *reinterpret_cast<const float4*>(u + 4 * m4);
+          const float4 u1 = *reinterpret_cast<const float4*>(u + kLeaf + 4 * m4);
+          const float4 u2 = *reinterpret_cast<const float4*>(u + 2 * kLeaf + 4 * m4);
+          const float4 u3 = *reinterpret_cast<const float4*>(u + 3 * kLeaf + 4 * m4);
+          const float l0 = x[4 * m4], l1 = x[4 * m4 + 1], l2 = x[4 * m4 + 2], l3 = x[4 * m4 + 3];
+          v.x -= l0 * u0.x + l1 * u0.y + l2 * u0.z + l3 * u0.w;
+          v.y -= l0 * u1.x + l1 * u1.y + l2 * u1.z + l3 * u1.w;
+          v.z -= l0 * u2.x + l1 * u2.y + l2 * u2.z + l3 * u2.w;
+          v.w -= l0 * u3.x + l1 * u3.y + l2 * u3.z + l3 * u3.w;
+        }
+        *reinterpret_cast<float4*>(row + col) = v;
+      }
+    }
+    __syncthreads();
+  }
+
+  // the factors to global memory in position order; the window rows below
+  // the block leave with their final multipliers (their L21 block)
+  if (vec) {
+    constexpr int kQuads = kB / 4;
+    const int nq = (nb + 3) / 4;
+#pragma unroll 4
+    for (int idx = tid; idx < win * nq; idx += kPanelThreads) {
+      const int i = idx / nq, c = 4 * (idx % nq);
+      const float4 v = *reinterpret_cast<const float4*>(P + phys_at[i] * kPLd + c);
+      float* dst = a + (int64_t)(k + i) * lda + k + c;
+      if (c + 4 <= nb) {
+        *reinterpret_cast<float4*>(dst) = v;
+      } else {
+        dst[0] = v.x;
+        if (c + 1 < nb) dst[1] = v.y;
+        if (c + 2 < nb) dst[2] = v.z;
+      }
+    }
+    (void)kQuads;
+  } else {
+#pragma unroll 8
+    for (int idx = tid; idx < win * nb; idx += kPanelThreads) {
+      const int i = idx / nb, c = idx % nb;
+      a[(int64_t)(k + i) * lda + k + c] = P[phys_at[i] * kPLd + c];
+    }
+  }
+  if (tid < nb) piv[k + tid] = piv_s[tid];
+  if (tid == 0 && zero_at != 0 && info[0] == 0) info[0] = zero_at;
+  // the diagonal block in position order: S = P rows 0..127 (through
+  // registers, as the two overlap), X (the inverse being formed) after it
+  constexpr int kSv = kB * kB / kPanelThreads;
+  float stage[kSv];
+#pragma unroll
+  for (int u = 0; u < kSv; ++u) {
+    const int idx = tid + u * kPanelThreads;
+    const int i = idx / kB, c = idx % kB;
+    stage[u] = P[phys_at[i] * kPLd + c];
+  }
+  __syncthreads();
+  float* S = psm;
+  float* X = psm + kB * kPLd;
+#pragma unroll
+  for (int u = 0; u < kSv; ++u) {
+    const int idx = tid + u * kPanelThreads;
+    S[(idx / kB) * kPLd + idx % kB] = stage[u];
+  }
+  __syncthreads();
+
+  // inverses: lower (unit) first, then upper, each formed in X and written out
+  for (int tri = 0; tri < 2; ++tri) {
+    const bool lower = tri == 0;
+    for (int idx = tid; idx < kB * kB; idx += kPanelThreads) X[(idx / kB) * kPLd + idx % kB] = 0.f;
+    __syncthreads();
+    // 16 x 16 diagonal blocks: thread t < 128 forms column t % 16 of block t / 16
+    if (tid < kB) {
+      const int d0 = (tid / kLeaf) * kLeaf, cc = tid % kLeaf;
+      float y[kLeaf];
+      if (lower) {
+#pragma unroll
+        for (int i = 0; i < kLeaf; ++i) {
+          float s = (i == cc) ? 1.f : 0.f;
+#pragma unroll
+          for (int m = 0; m < i; ++m) s -= S[(d0 + i) * kPLd + d0 + m] * y[m];
+          y[i] = s;
+        }
+      } else {
+#pragma unroll
+        for (int i = kLeaf - 1; i >= 0; --i) {
+          float s = (i == cc) ? 1.f : 0.f;
+#pragma unroll
+          for (int m = i + 1; m < kLeaf; ++m) s -= S[(d0 + i) * kPLd + d0 + m] * y[m];
+          y[i] = s / S[(d0 + i) * kPLd + d0 + i];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kLeaf; ++i) X[(d0 + i) * kPLd + d0 + cc] = y[i];
+    }
+    __syncthreads();
+    // doubling: pairs of s x s blocks (A first, B second along the diagonal)
+    //   lower: X_BA = -X_BB (L_BA X_AA)      upper: X_AB = -X_AA (U_AB X_BB)
+    // on the matrix cores (v_mfma_f32_16x16x4f32): the s x s off-diagonal
+    // blocks of all pairs split into 16 x 16 tiles over the four waves; first
+    // T = (the middle product) into the target block, then X = -(.) T.
+    // Operand k index of lane group g at step j of a 16-wide chunk: 4 g + j,
+    // so a lane's four A entries are one 16-byte LDS read.
+    {
+      const int g = lane >> 4, q = lane & 15;
+#pragma unroll
+      for (int lev = 0; lev < 3; ++lev) {
+        const int sz = kLeaf << lev;              // 16, 32, 64
+        const int tdim = sz / 16;                 // tiles per block side
+        constexpr int kTilesPerWave = 4;          // at most: 16 tiles / 4 waves
+        const int per_wave = (kB / (2 * sz)) * tdim * tdim / 4;  // 1, 2, 4
+        f4 acc[kTilesPerWave];
+        int tr0[kTilesPerWave], tc0[kTilesPerWave], mb[kTilesPerWave], ob[kTilesPerWave];
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+          if (t >= per_wave) break;
+          const int tile = wave + 4 * t;
+          const int pair = tile / (tdim * tdim), ti = (tile / tdim) % tdim, tj = tile % tdim;
+          const int A0 = 2 * sz * pair, B0 = A0 + sz;
+          tr0[t] = (lower ? B0 : A0) + 16 * ti;
+          tc0[t] = (lower ? A0 : B0) + 16 * tj;
+          mb[t] = lower ? A0 : B0;  // step 1: k runs over these rows of Xd / columns of M
+          ob[t] = lower ? B0 : A0;  // step 2: k runs over these rows of T / columns of Xo
+        }
+        // step 1: T = M Xd, M from the factors S
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+          if (t >= per_wave) break;
+          acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int m = 0; m < sz; m += 16) {
+            const float4 av = *reinterpret_cast<const float4*>(S + (tr0[t] + q) * kPLd + mb[t] + m + 4 * g);
+            const float* bp = X + (mb[t] + m + 4 * g) * kPLd + tc0[t] + q;
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bp[0], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bp[kPLd], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bp[2 * kPLd], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bp[3 * kPLd], acc[t], 0, 0, 0);
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+          if (t >= per_wave) break;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) X[(tr0[t] + 4 * g + i) * kPLd + tc0[t] + q] = acc[t][i];
+        }
+        __syncthreads();
+        // step 2: X_target = -Xo T, T held in the target block
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+          if (t >= per_wave) break;
+          acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int m = 0; m < sz; m += 16) {
+            const float4 av = *reinterpret_cast<const float4*>(X + (tr0[t] + q) * kPLd + ob[t] + m + 4 * g);
+            const float* bp = X + (ob[t] + m + 4 * g) * kPLd + tc0[t] + q;
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bp[0], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bp[kPLd], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bp[2 * kPLd], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bp[3 * kPLd], acc[t], 0, 0, 0);
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+          if (t >= per_wave) break;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) X[(tr0[t] + 4 * g + i) * kPLd + tc0[t] + q] = -acc[t][i];
+        }
+        __syncthreads();
+      }
+    }
+    float* out = lower ? linv : uinv;
+#pragma unroll 8
+    for (int idx = tid; idx < kB * kB / 4; idx += kPanelThreads)
+      reinterpret_cast<float4*>(out)[idx] = *reinterpret_cast<const float4*>(X + (idx / (kB / 4)) * kPLd + 4 * (idx % (kB / 4)));
+    __syncthreads();
+  }
+
+  // the window's interchanges over the other columns [s_lo, s_hi) of the
+  // caller's panel (128-column chunks aligned with this block, whose own
+  // chunk is skipped), staged through LDS: row position i takes window row
+  // phys_at[i]; rows that stay are not rewritten
+  float* W = psm;  // [256][kPLd]
+  for (int cb = s_lo; cb < s_hi; cb += kB) {
+    if (cb == k) continue;
+    const int cw = min(kB, s_hi - cb);
+    const bool full = vec && cw == kB;
+    if (full) {
+      constexpr int kQuads = kB / 4;
+      for (int base = 0; base < win * kQuads; base += 8 * kPanelThreads) {
+        float4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int idx = base + tid + u * kPanelThreads;
+          v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (idx < win * kQuads)
+            v[u] = *reinterpret_cast<const float4*>(a + (int64_t)(k + idx / kQuads) * lda + cb + 4 * (idx % kQuads));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int idx = base + tid + u * kPanelThreads;
+          if (idx < win * kQuads) *reinterpret_cast<float4*>(W + (idx / kQuads) * kPLd + 4 * (idx % kQuads)) = v[u];
+        }
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int idx = tid; idx < win * kQuads; idx += kPanelThreads) {
+        const int i = idx / kQuads, q = idx % kQuads;
+        const int src = phys_at[i];
+        if (src != i)
+          *reinterpret_cast<float4*>(a + (int64_t)(k + i) * lda + cb + 4 * q) =
+              *reinterpret_cast<const float4*>(W + src * kPLd + 4 * q);
+      }
+    } else {
+#pragma unroll 8
+      for (int idx = tid; idx < win * cw; idx += kPanelThreads)
+        W[(idx / cw) * kPLd + idx % cw] = a[(int64_t)(k + idx / cw) * lda + cb + idx % cw];
+      __syncthreads();
+#pragma unroll 8
+      for (int idx = tid; idx < win * cw; idx += kPanelThreads) {
+        const int i = idx / cw, c = idx % cw;
+        const int src = phys_at[i];
+        if (src != i) a[(int64_t)(k + i) * lda + cb + c] = W[src * kPLd + c];
+      }
+    }
+    __syncthreads();
+  }
 }
 
 __global__ void zero_i32_kernel(int32_t* __restrict__ p, int count) {
@@ -323,59 +587,89 @@ __global__ void zero_i32_kernel(int32_t* __restrict__ p, int count) {
   if (i < count) p[i] = 0;
 }
 
-// The row swaps of block column k (pivot steps k .. k + nb - 1, in order,
-// each within the window rows k .. k + win - 1) applied to every column
-// outside the block column: one thread per column, the window's rows staged
-// in LDS.
-constexpr int kSwapCols = 128;
+// The interchanges of a whole outer panel [K0, K1) (its 128-column blocks in
+// order, each within its 256-row window) applied to the columns outside the
+// panel, [0, K0) and [K1, n): a workgroup stages the rows the panel's windows
+// span (K0 .. K1 + 127, at most 640) of 32 columns in LDS, then, block by
+// block, each window row finds where the block's interchanges take it (one
+// thread per row following its own position through the swap sequence) and
+// moves there, and the rows are written back.
+constexpr int kOsCols = 32;
+constexpr int kOsRows = kOuter + kB;  // rows a panel's windows span
+constexpr int kOsLd = kOsCols + 1;
+constexpr size_t kOsLds = sizeof(float) * (size_t)kOsRows * kOsLd;
 
-__global__ __launch_bounds__(kSwapCols) void block_row_swaps_kernel(int n, int k, int nb, int win,
-                                                                    float* __restrict__ a, int64_t lda,
-                                                                    const int32_t* __restrict__ piv) {
-  extern __shared__ __align__(16) float col[];  // [win][kSwapCols]
-  const int t = threadIdx.x;
-  const int c = blockIdx.x * kSwapCols + t;
-  const bool live = c < n && (c < k || c >= k + nb);
-  if (live)
-    for (int r = 0; r < win; ++r) col[r * kSwapCols + t] = a[(int64_t)(k + r) * lda + c];
-  if (!live) return;
-  for (int j = 0; j < nb; ++j) {
-    const int p = piv[k + j] - k;
-    if (p != j) {
-      const float v = col[j * kSwapCols + t];
-      col[j * kSwapCols + t] = col[p * kSwapCols + t];
-      col[p * kSwapCols + t] = v;
-    }
+__global__ __launch_bounds__(256) void outer_swaps_kernel(int n, int K0, int K1, float* __restrict__ a, int64_t lda,
+                                                          const int32_t* __restrict__ piv) {
+  extern __shared__ __align__(16) float T[];  // [kOsRows][kOsLd]
+  __shared__ int pv[kOuter];
+  const int tid = threadIdx.x;
+  const int rend = min(n, K1 + kB), rows = rend - K0;
+  const int c0 = blockIdx.x * kOsCols;  // in the column sequence [0, K0) ++ [K1, n)
+  for (int j = tid; j < K1 - K0; j += 256) pv[j] = piv[K0 + j] - K0;
+  // stage: row r, column q (32 consecutive global columns, remapped past the panel)
+  for (int idx = tid; idx < rows * kOsCols; idx += 256) {
+    const int r = idx / kOsCols, q = idx % kOsCols;
+    int c = c0 + q;
+    if (c >= K0) c += K1 - K0;
+    T[r * kOsLd + q] = c < n ? a[(int64_t)(K0 + r) * lda + c] : 0.f;
   }
-  for (int r = 0; r < win; ++r) a[(int64_t)(k + r) * lda + c] = col[r * kSwapCols + t];
+  __syncthreads();
+  for (int k = K0; k < K1; k += kB) {
+    const int nb = min(K1, k + kB) - k;
+    const int win = nb < kB ? nb : min(kPivWin, n - k);
+    const int w0 = k - K0;  // window start, relative to K0
+    int dest = tid;
+    float v[kOsCols];
+    if (tid < win) {
+      // the position window row tid reaches (steps k .. k + nb - 1)
+      for (int j = 0; j < nb; ++j) {
+        const int p = pv[w0 + j] - w0;
+        dest = (dest == j) ? p : (dest == p ? j : dest);
+      }
+#pragma unroll
+      for (int q = 0; q < kOsCols; ++q) v[q] = T[(w0 + tid) * kOsLd + q];
+    }
+    __syncthreads();
+    if (tid < win && dest != tid) {
+#pragma unroll
+      for (int q = 0; q < kOsCols; ++q) T[(w0 + dest) * kOsLd + q] = v[q];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < rows * kOsCols; idx += 256) {
+    const int r = idx / kOsCols, q = idx % kOsCols;
+    int c = c0 + q;
+    if (c >= K0) c += K1 - K0;
+    if (c < n) a[(int64_t)(K0 + r) * lda + c] = T[r * kOsLd + q];
+  }
 }
 
 // The row permutation of the whole factorisation: perm[i] = the original
 // row now at position i (the swap sequence piv applied in order to the
-// identity).  One workgroup; the sequence is inherently sequential, so one
-// thread walks it over an LDS copy (n <= kPermLds) or over global memory.
-constexpr int kPermLds = 32768;
+// identity).  The sequence is windowed (step j exchanges positions j and
+// piv[j] in [j, j + 256), as the factorisation writes it), so each row
+// follows its own position: step j moves a row at position j to piv[j] and
+// one at piv[j] to j, steps before pos - 255 cannot reach position pos, and
+// a row whose position is below the current step is final.  One thread per
+// row, the swap steps its workgroup's rows meet staged in LDS.
+constexpr int kPermBefore = kPivWin;       // steps staged before the first row
+constexpr int kPermSpan = kPermBefore + 4 * 256;
 
 __global__ __launch_bounds__(256) void perm_from_swaps_kernel(int n, const int32_t* __restrict__ piv,
                                                               int32_t* __restrict__ perm) {
-  extern __shared__ int32_t pl[];
-  const bool in_lds = n <= kPermLds;
-  int32_t* p = in_lds ? pl : perm;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = i;
+  __shared__ int32_t ps[kPermSpan];
+  const int r0 = blockIdx.x * 256, base = max(0, r0 - kPermBefore);
+  for (int i = threadIdx.x; i < kPermSpan; i += 256) ps[i] = base + i < n ? piv[base + i] : 0;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int j = 0; j < n; ++j) {
-      const int q = piv[j];
-      if (q != j) {
-        const int32_t t = p[j];
-        p[j] = p[q];
-        p[q] = t;
-      }
-    }
+  const int r = r0 + threadIdx.x;
+  if (r >= n) return;
+  int pos = r;
+  for (int j = max(0, r - (kPivWin - 1)); j <= pos && j < n; ++j) {
+    const int q = j - base < kPermSpan ? ps[j - base] : piv[j];
+    pos = (pos == j) ? q : (pos == q ? j : pos);
   }
-  __syncthreads();
-  if (in_lds)
-    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = pl[i];
+  perm[pos] = r;
 }
 
 // y = P x (y[i] = x[perm[i]], inverse = 0) or y = P^T x (y[perm[i]] = x[i]),
@@ -662,15 +956,62 @@ int gemv_f32a_launch(hipStream_t st, int m, int k, int nrhs, const float* a, int
   return LXA_OK;
 }
 
+// Per device: the side stream that runs the far part of each outer Schur
+// update, and the two events that order it against the main stream.  Made
+// once, kept for the process; the mutex serialises factorisations that are
+// enqueued from several host threads (they would share these).
+struct LookAhead {
+  hipStream_t side = nullptr;
+  hipEvent_t u12_done = nullptr, far_done = nullptr;
+};
+constexpr int kMaxDevices = 64;
+constexpr int kFarChunk = 1024;  // columns per launch of the far update
+std::mutex g_look_mu[kMaxDevices];
+LookAhead g_look[kMaxDevices];
+
+int look_ahead_for(int dev, LookAhead** out) {
+  LookAhead& la = g_look[dev];
+  if (!la.side) {
+    // non-blocking: the caller's stream may be the null stream
+    hipError_t e = hipStreamCreateWithFlags(&la.side, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      la.side = nullptr;
+      return hip_error(e, "hipStreamCreateWithFlags");
+    }
+    e = hipEventCreateWithFlags(&la.u12_done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&la.far_done, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_error(e, "hipEventCreateWithFlags");
+  }
+  *out = &la;
+  return LXA_OK;
+}
+
+// Blocked right-looking LU with look-ahead.  Outer panels of 512 columns
+// [K0, K1), each factored by 128-column inner steps that touch only the
+// panel's columns (row interchanges included).  The outer Schur update of
+// the trailing matrix is split: its first 512 columns (the next panel,
+// "near") on the caller's stream, the rest ("far") on a side stream, where
+// it runs while the next panel is factored.  A panel's interchanges reach
+// the columns outside it only after the previous far update is done (that
+// update reads and writes those rows); as the far update acts on whole rows
+// of the trailing matrix and of its multipliers alike, applying the
+// interchanges after it gives the same matrix as applying them before.
 int lu_large_factor_impl(hipStream_t st, int mode, int n, float* a, int64_t lda, float* linv, float* uinv,
                          int32_t* piv, int32_t* info) {
-  const size_t dsmem = sizeof(float) * kB * (kB + 1);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&diag_lu_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)dsmem);
-  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(diag_lu_kernel)");
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_row_swaps_kernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float) * kPivWin * kSwapCols));
-  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(block_row_swaps_kernel)");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_error(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return set_error(LXA_EINVAL, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_look_mu[dev]);
+  LookAhead* la = nullptr;
+  if (int rc = look_ahead_for(dev, &la)) return rc;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_lu_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPanelLds);
+  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(panel_lu_kernel)");
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&outer_swaps_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOsLds);
+  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(outer_swaps_kernel)");
+  bool far_pending = false;
   for (int K0 = 0; K0 < n; K0 += kOuter) {
     const int K1 = min(n, K0 + kOuter);
     for (int k = K0; k < K1; k += kB) {
@@ -680,14 +1021,13 @@ int lu_large_factor_impl(hipStream_t st, int mode, int n, float* a, int64_t lda,
       const int win = nb < kB ? nb : min(kPivWin, n - k);
       float* Li = linv + (int64_t)(k / kB) * kB * kB;
       float* Ui = uinv + (int64_t)(k / kB) * kB * kB;
-      diag_lu_kernel<<<1, kDiagThreads, dsmem, st>>>(k, nb, win, a, lda, Li, Ui, piv, info);
-      LXA_LAUNCH_CHECK("diag_lu_kernel");
-      block_row_swaps_kernel<<<(n + kSwapCols - 1) / kSwapCols, kSwapCols, sizeof(float) * win * kSwapCols, st>>>(
-          n, k, nb, win, a, lda, piv);
-      LXA_LAUNCH_CHECK("block_row_swaps_kernel");
+      // (the kernel also applies the block's interchanges to the rest of
+      // the panel's columns)
+      panel_lu_kernel<<<1, kPanelThreads, kPanelLds, st>>>(k, nb, win, a, lda, Li, Ui, piv, info, K0, K1);
+      LXA_LAUNCH_CHECK("panel_lu_kernel");
       float* A11 = a + (int64_t)k * lda + k;
       // L21 = A21 U11^-1 (in place, N = nb <= 128) for the rows below the
-      // window (the window's rows hold theirs from diag_lu_kernel)
+      // window (the window's rows hold theirs from panel_lu_kernel)
       if (n - k - win > 0) {
         if (int rc = gemm_update(st, LXA_GEMM_F32, n - k - win, nb, nb, A11 + (int64_t)win * lda, lda, Ui, kB,
                                  A11 + (int64_t)win * lda, lda, 0, 1.f))
@@ -704,6 +1044,17 @@ int lu_large_factor_impl(hipStream_t st, int mode, int n, float* a, int64_t lda,
         }
       }
     }
+    // the panel's interchanges outside its columns, once the previous far
+    // update is done with those rows
+    if (far_pending) {
+      e = hipStreamWaitEvent(st, la->far_done, 0);
+      if (e != hipSuccess) return hip_error(e, "hipStreamWaitEvent");
+      far_pending = false;
+    }
+    if (const int cols = n - (K1 - K0); cols > 0) {
+      outer_swaps_kernel<<<(cols + kOsCols - 1) / kOsCols, 256, kOsLds, st>>>(n, K0, K1, a, lda, piv);
+      LXA_LAUNCH_CHECK("outer_swaps_kernel");
+    }
     if (n - K1 <= 0) continue;
     // outer U12 = L11^-1 A12 over the panel's 128-row blocks
     for (int k = K0; k < K1; k += kB) {
@@ -718,10 +1069,31 @@ int lu_large_factor_impl(hipStream_t st, int mode, int n, float* a, int64_t lda,
                                rowblk, lda, 0, 1.f))
         return rc;
     }
-    // outer Schur update
-    if (int rc = gemm_update(st, mode, n - K1, n - K1, K1 - K0, a + (int64_t)K1 * lda + K0, lda,
-                             a + (int64_t)K0 * lda + K1, lda, a + (int64_t)K1 * lda + K1, lda, 1, -1.f))
-      return rc;
+    // outer Schur update: near columns here, far columns on the side stream
+    const float* L21 = a + (int64_t)K1 * lda + K0;
+    const float* U12 = a + (int64_t)K0 * lda + K1;
+    float* A22 = a + (int64_t)K1 * lda + K1;
+    const int near = min(kOuter, n - K1);
+    if (int rc = gemm_update(st, mode, n - K1, near, K1 - K0, L21, lda, U12, lda, A22, lda, 1, -1.f)) return rc;
+    if (n - K1 - near > 0) {
+      e = hipEventRecord(la->u12_done, st);
+      if (e == hipSuccess) e = hipStreamWaitEvent(la->side, la->u12_done, 0);
+      if (e != hipSuccess) return hip_error(e, "look-ahead event");
+      // in column chunks, so that the next panel's kernels find compute
+      // units between them
+      for (int c = near; c < n - K1; c += kFarChunk) {
+        const int w = min(kFarChunk, n - K1 - c);
+        if (int rc = gemm_update(la->side, mode, n - K1, w, K1 - K0, L21, lda, U12 + c, lda, A22 + c, lda, 1, -1.f))
+          return rc;
+      }
+      e = hipEventRecord(la->far_done, la->side);
+      if (e != hipSuccess) return hip_error(e, "hipEventRecord");
+      far_pending = true;
+    }
+  }
+  if (far_pending) {
+    e = hipStreamWaitEvent(st, la->far_done, 0);
+    if (e != hipSuccess) return hip_error(e, "hipStreamWaitEvent");
   }
   return LXA_OK;
 }
@@ -748,11 +1120,8 @@ extern "C" int lxa_lu_large_perm(void* stream, int64_t n, const int32_t* piv, in
   LXA_REQUIRE(n >= 0 && n <= (1 << 20), "invalid n");
   if (n == 0) return LXA_OK;
   LXA_REQUIRE(piv && perm, "piv and perm must be non-NULL");
-  const size_t smem = n <= kPermLds ? sizeof(int32_t) * (size_t)n : 0;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&perm_from_swaps_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(int32_t) * kPermLds));
-  if (e != hipSuccess) return hip_error(e, "hipFuncSetAttribute(perm_from_swaps_kernel)");
-  perm_from_swaps_kernel<<<1, 256, smem, static_cast<hipStream_t>(stream)>>>((int)n, piv, perm);
+  perm_from_swaps_kernel<<<(unsigned)((n + 255) / 256), 256, 0, static_cast<hipStream_t>(stream)>>>((int)n, piv,
+                                                                                                      perm);
   LXA_LAUNCH_CHECK("perm_from_swaps_kernel");
   return LXA_OK;
 }
