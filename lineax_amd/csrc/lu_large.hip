@@ -607,12 +607,41 @@ __global__ __launch_bounds__(256) void outer_swaps_kernel(int n, int K0, int K1,
   const int rend = min(n, K1 + kB), rows = rend - K0;
   const int c0 = blockIdx.x * kOsCols;  // in the column sequence [0, K0) ++ [K1, n)
   for (int j = tid; j < K1 - K0; j += 256) pv[j] = piv[K0 + j] - K0;
-  // stage: row r, column q (32 consecutive global columns, remapped past the panel)
-  for (int idx = tid; idx < rows * kOsCols; idx += 256) {
-    const int r = idx / kOsCols, q = idx % kOsCols;
-    int c = c0 + q;
-    if (c >= K0) c += K1 - K0;
-    T[r * kOsLd + q] = c < n ? a[(int64_t)(K0 + r) * lda + c] : 0.f;
+  // stage: 32 consecutive global columns (remapped past the panel; the
+  // group never straddles it, K0 and K1 - K0 being multiples of 32) as
+  // 8 column quads per row, four loads in flight per thread
+  int cg = c0;
+  if (cg >= K0) cg += K1 - K0;
+  const bool vec = (lda & 3) == 0 && (reinterpret_cast<uintptr_t>(a) & 15) == 0 && cg + kOsCols <= n;
+  for (int base = 0; base < rows * 8; base += 4 * 256) {
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = base + tid + 256 * u, r = idx >> 3, c = cg + 4 * (idx & 7);
+      v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (r < rows) {
+        const float* src = a + (int64_t)(K0 + r) * lda + c;
+        if (vec) {
+          v[u] = *reinterpret_cast<const float4*>(src);
+        } else {
+          if (c < n) v[u].x = src[0];
+          if (c + 1 < n) v[u].y = src[1];
+          if (c + 2 < n) v[u].z = src[2];
+          if (c + 3 < n) v[u].w = src[3];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = base + tid + 256 * u, r = idx >> 3, q = 4 * (idx & 7);
+      if (r < rows) {
+        float* t = T + r * kOsLd + q;
+        t[0] = v[u].x;
+        t[1] = v[u].y;
+        t[2] = v[u].z;
+        t[3] = v[u].w;
+      }
+    }
   }
   __syncthreads();
   for (int k = K0; k < K1; k += kB) {
@@ -637,11 +666,19 @@ __global__ __launch_bounds__(256) void outer_swaps_kernel(int n, int K0, int K1,
     }
     __syncthreads();
   }
-  for (int idx = tid; idx < rows * kOsCols; idx += 256) {
-    const int r = idx / kOsCols, q = idx % kOsCols;
-    int c = c0 + q;
-    if (c >= K0) c += K1 - K0;
-    if (c < n) a[(int64_t)(K0 + r) * lda + c] = T[r * kOsLd + q];
+#pragma unroll 4
+  for (int idx = tid; idx < rows * 8; idx += 256) {
+    const int r = idx >> 3, q = 4 * (idx & 7), c = cg + q;
+    const float* t = T + r * kOsLd + q;
+    float* dst = a + (int64_t)(K0 + r) * lda + c;
+    if (vec) {
+      *reinterpret_cast<float4*>(dst) = make_float4(t[0], t[1], t[2], t[3]);
+    } else {
+      if (c < n) dst[0] = t[0];
+      if (c + 1 < n) dst[1] = t[1];
+      if (c + 2 < n) dst[2] = t[2];
+      if (c + 3 < n) dst[3] = t[3];
+    }
   }
 }
 
