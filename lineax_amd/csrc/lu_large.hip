@@ -806,15 +806,23 @@ __device__ __forceinline__ void tile_product(const float* tile, const float (*ve
 }
 
 // R: right-hand sides computed per product (4 or kMaxRhs; nrhs <= R).
+//
+// Block row bi needs every solved block before it; only the last of them
+// (bj = the previous block) is on the chain from block to block.  So
+// before it waits, the workgroup forms M = D^-1 A(bi, last) (D the diagonal
+// block; D^-1 is stored) and z = D^-1 (x_bi - sum over the other solved
+// blocks of A(bi, bj) y_bj); once the last block's flag is up, its y is one
+// product away: y_bi = z - M y_last.
 template <int R>
 __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nrhs, const float* __restrict__ lu,
                                                                    int64_t lda, const float* __restrict__ inv,
                                                                    float* __restrict__ x, int64_t ldx, int kind,
                                                                    int32_t* __restrict__ sync) {
   extern __shared__ __align__(16) float tsm[];
-  float* tile = tsm;                                                     // [128][132]
-  float(*xs)[kB] = reinterpret_cast<float(*)[kB]>(tsm + kB * kTileLd);   // [8][128] solved block
-  float(*acc)[kB] = reinterpret_cast<float(*)[kB]>(tsm + kB * kTileLd + kMaxRhs * kB);  // [8][128]
+  float* tile = tsm;                                                        // [128][132]: streamed block, D^-1
+  float* Mt = tsm + kB * kTileLd;                                           // [128][132]: M
+  float(*xs)[kB] = reinterpret_cast<float(*)[kB]>(tsm + 2 * kB * kTileLd);  // [8][128] a solved block
+  float(*acc)[kB] = reinterpret_cast<float(*)[kB]>(tsm + 2 * kB * kTileLd + kMaxRhs * kB);  // [8][128]
   __shared__ int s_ticket;
   int32_t* ticket = sync;
   int32_t* flags = sync + 1;
@@ -835,8 +843,6 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
   // row i of the block handled by threads 2i, 2i + 1 (64 columns each)
   const int row = tid >> 1, hcol = (tid & 1) * 64;
   const bool tr = kind >= 2;
-  // operator blocks do not depend on the solution: block `step + 1` is
-  // loaded while block `step` waits for its x and is applied
   float4 pre[kStageVec];
   auto block_col = [&](int step) { return forward ? step : nblk - 1 - step; };
   // the inverse of the diagonal block, staged like an operator block
@@ -848,22 +854,7 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
       pre[u] = *reinterpret_cast<const float4*>(D + (idx >> 5) * kB + 4 * (idx & 31));
     }
   };
-  if (t > 0) {
-    const int bj = block_col(0);
-    tile_load(pre, lu, lda, kind, bi, bj, rows, min(kB, n - bj * kB), tid);
-  } else {
-    load_inv();
-  }
-  for (int step = 0; step < t; ++step) {
-    const int bj = block_col(step);
-    const int c0 = bj * kB, cols = min(kB, n - c0);
-    tile_store(tile, pre, tr, tid);
-    if (step + 1 < t) {
-      const int bn = block_col(step + 1);
-      tile_load(pre, lu, lda, kind, bi, bn, rows, min(kB, n - bn * kB), tid);
-    } else {
-      load_inv();  // in flight while the last dependency is awaited
-    }
+  auto wait_flag = [&](int bj) {
     if (tid == 0) {
       int spin = 0;
       while (__hip_atomic_load(&flags[bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
@@ -876,10 +867,61 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
     }
     __syncthreads();
     (void)__hip_atomic_load(&flags[bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  auto load_xs = [&](int bj) {
+    const int c0 = bj * kB, cols = min(kB, n - c0);
     for (int idx = tid; idx < kMaxRhs * kB; idx += kTrsvThreads) {
       const int rr = idx / kB, i = idx % kB;
       xs[rr][i] = (rr < nrhs && i < cols) ? x[(int64_t)rr * ldx + c0 + i] : 0.f;
     }
+  };
+  if (t > 0) {
+    // M = D^-1 A(bi, last): D^-1 into tile, A(bi, last) into Mt, then each
+    // thread its 64 entries of M in registers, written over Mt
+    const int bl = block_col(t - 1);
+    load_inv();
+    tile_store(tile, pre, tr, tid);
+    tile_load(pre, lu, lda, kind, bi, bl, rows, min(kB, n - bl * kB), tid);
+    tile_store(Mt, pre, tr, tid);
+    __syncthreads();
+    float m[64];
+#pragma unroll
+    for (int c = 0; c < 64; ++c) m[c] = 0.f;
+    const float* drow = tile + row * kTileLd;
+    for (int kk = 0; kk < kB; kk += 4) {
+      const float4 d = *reinterpret_cast<const float4*>(drow + kk);
+      const float dv[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float* lr = Mt + (kk + j) * kTileLd + hcol;
+#pragma unroll
+        for (int c = 0; c < 64; c += 4) {
+          const float4 v = *reinterpret_cast<const float4*>(lr + c);
+          m[c] += dv[j] * v.x;
+          m[c + 1] += dv[j] * v.y;
+          m[c + 2] += dv[j] * v.z;
+          m[c + 3] += dv[j] * v.w;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 64; c += 4)
+      *reinterpret_cast<float4*>(Mt + row * kTileLd + hcol + c) = make_float4(m[c], m[c + 1], m[c + 2], m[c + 3]);
+  }
+  // the solved blocks before the last: operator blocks do not depend on the
+  // solution, so block `step + 1` is loaded while block `step` waits for its
+  // x and is applied
+  if (t > 1) tile_load(pre, lu, lda, kind, bi, block_col(0), rows, min(kB, n - block_col(0) * kB), tid);
+  for (int step = 0; step + 1 < t; ++step) {
+    const int bj = block_col(step);
+    tile_store(tile, pre, tr, tid);
+    if (step + 2 < t) {
+      const int bn = block_col(step + 1);
+      tile_load(pre, lu, lda, kind, bi, bn, rows, min(kB, n - bn * kB), tid);
+    }
+    wait_flag(bj);
+    load_xs(bj);
     __syncthreads();
     float part[R];
     tile_product<R>(tile, xs, row, hcol, part);
@@ -890,16 +932,26 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
     }
     __syncthreads();
   }
-  // y = inv(D) acc, with D staged like an operator block (transposed kinds
-  // use inv(D)^T); its loads were issued before the last wait
+  // z = D^-1 acc (D^-1 staged like an operator block; transposed kinds use
+  // its transpose), kept in registers by the even thread of each row pair
+  load_inv();
   tile_store(tile, pre, tr, tid);
   __syncthreads();
-  float part[R];
-  tile_product<R>(tile, acc, row, hcol, part);
+  float z[R];
+  tile_product<R>(tile, acc, row, hcol, z);
+  if (t > 0) {
+    wait_flag(block_col(t - 1));
+    load_xs(block_col(t - 1));
+    __syncthreads();
+    float part[R];
+    tile_product<R>(Mt, xs, row, hcol, part);
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) z[rr] -= part[rr];
+  }
   if ((tid & 1) == 0 && row < rows) {
 #pragma unroll
     for (int rr = 0; rr < R; ++rr)
-      if (rr < nrhs) x[(int64_t)rr * ldx + r0 + row] = part[rr];
+      if (rr < nrhs) x[(int64_t)rr * ldx + r0 + row] = z[rr];
   }
   // Every wave makes its own stores visible device-wide before the flag is
   // raised (a fence orders only the issuing wave's memory operations).
@@ -1187,7 +1239,7 @@ extern "C" int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, co
   // a fill kernel rather than a memset node, so the call can sit in a captured graph
   zero_i32_kernel<<<(nblk + 2 + 255) / 256, 256, 0, st>>>(sync, nblk + 2);
   LXA_LAUNCH_CHECK("zero_i32_kernel");
-  const size_t smem = sizeof(float) * (kB * kTileLd + 2 * kMaxRhs * kB);
+  const size_t smem = sizeof(float) * (2 * kB * kTileLd + 2 * kMaxRhs * kB);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&trsv_blocks_kernel<4>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   if (e == hipSuccess)

@@ -44,13 +44,37 @@ __device__ __forceinline__ T lane_value(T v, int lane) {
   }
 }
 
+// One step of a wave maximum of 64-bit keys with a DPP partner (register
+// crossbar moves: no LDS round trip, unlike a shuffle).
+template <int kCtrl>
+__device__ __forceinline__ void key_max_dpp(uint32_t& hi, uint32_t& lo) {
+  const uint32_t oh = (uint32_t)__builtin_amdgcn_mov_dpp((int)hi, kCtrl, 0xF, 0xF, false);
+  const uint32_t ol = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo, kCtrl, 0xF, 0xF, false);
+  const bool gt = oh > hi || (oh == hi && ol > lo);
+  hi = gt ? oh : hi;
+  lo = gt ? ol : lo;
+}
+
+// Wave maximum: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror
+// and row_mirror leave each 16-lane row's maximum in all of its lanes; the
+// rows (the first N / 16 hold every real row) are combined from scalar reads.
+// The result is wave-uniform.
+template <int N>
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+  uint32_t hi = static_cast<uint32_t>(v >> 32), lo = static_cast<uint32_t>(v);
+  key_max_dpp<0xB1>(hi, lo);
+  key_max_dpp<0x4E>(hi, lo);
+  if constexpr (N > 4) key_max_dpp<0x141>(hi, lo);
+  if constexpr (N > 8) key_max_dpp<0x140>(hi, lo);
+  uint64_t best = 0;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint64_t o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
+  for (int q = 0; q < (N + 15) / 16; ++q) {
+    const uint32_t h = (uint32_t)__builtin_amdgcn_readlane((int)hi, 16 * q);
+    const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)lo, 16 * q);
+    const uint64_t k = (static_cast<uint64_t>(h) << 32) | l;
+    best = k > best ? k : best;
   }
-  return v;
+  return best;
 }
 
 // x = A^-1 b for `batch` systems of order n <= N.  lu/piv may be null.
@@ -87,13 +111,13 @@ __global__ __launch_bounds__(64) void lu_small_fused_kernel(int64_t batch, int n
     const uint32_t hi = done ? 0u : mag_key(row[k]);
     const uint32_t lo_pos = done ? 0u : static_cast<uint32_t>(63 - pos);
     uint64_t key = (static_cast<uint64_t>(hi) << 32) | lo_pos;
-    key = wave_max_u64(key);
+    key = wave_max_u64<N>(key);
     if constexpr (sizeof(T) == 8) {
       // doubles: break ties of the top 32 magnitude bits on the low 32 bits
       const uint32_t top = static_cast<uint32_t>(key >> 32);
       const bool cand = !done && mag_key(row[k]) == top;
       uint64_t k2 = cand ? ((static_cast<uint64_t>(mag_lo(row[k])) << 32) | static_cast<uint32_t>(63 - pos)) : 0;
-      k2 = wave_max_u64(k2);
+      k2 = wave_max_u64<N>(k2);
       key = (static_cast<uint64_t>(top) << 32) | static_cast<uint32_t>(k2);
     }
     const int cp = 63 - static_cast<int>(static_cast<uint32_t>(key));
