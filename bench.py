@@ -182,6 +182,7 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: arrivals per second")
     ap.add_argument("--requests", type=int, default=400, help="serving: stream length")
     ap.add_argument("--refine-steps", type=int, default=2, help="serving: FP64 refinement steps (chosen form)")
+    ap.add_argument("--serving-forms", default=None, help="serving: comma-separated forms to run (default: all)")
     ap.add_argument("--max-iters", type=int, default=None, help="krylov_cg: cap on CG iterations (default: lineax's)")
     ap.add_argument("--configs", default="all", choices=["all", "none"],
                     help="with the default workload: also run every other configuration and carry it under "
@@ -244,12 +245,16 @@ CONFIG_LEGS = [
     ("serving_2k_per_s", "serving", {"rate": 2000.0}),
     ("serving_10k_per_s", "serving", {"rate": 10000.0}),
     ("serving_50k_per_s", "serving", {"rate": 50000.0}),
+    # a rate below the loop's capacity, arrivals over 5 s: the latency of a
+    # stable queue rather than of a backlog
+    ("serving_sustained_600_per_s", "serving", {"rate": 600.0, "requests": 3000, "serving_forms": "chosen"}),
 ]
 
 _SUMMARY_KEYS = ("metric", "value", "unit", "ms_per_step", "p50_ms", "p99_ms", "dtype", "config", "roofline",
                  "cpu_baseline", "eta_max", "eta_met", "factor_ms", "refine_ms", "refine_stats", "plain_form",
                  "speedup_vs_plain", "speedup_vs_plain_time", "max_backward_error_sample", "iterations_max",
-                 "ms_per_iteration", "results", "max_scaled_residual")
+                 "ms_per_iteration", "results", "max_scaled_residual", "arrival_span_s", "p50_first_quarter_ms",
+                 "p50_last_quarter_ms")
 
 
 def run_configs(args):
@@ -844,6 +849,8 @@ def bench_serving(args):
         torch.cuda.synchronize()
         _SERVING_WARM.append(True)
     forms = ("chosen", "plain", "fifo") if args.rate == 2000.0 else ("chosen", "plain")
+    if getattr(args, "serving_forms", None):
+        forms = tuple(args.serving_forms.split(","))
     for form in forms:
         reqs, ops = S.make_request_stream(args.rate, args.requests, seed=5, device="cuda:0")
         if cpu is None and not args.no_cpu_baseline:
@@ -856,10 +863,18 @@ def bench_serving(args):
         stats["backward_error"] = _serving_errors(reqs, ops)
         if form == "chosen":
             stats["refine_steps"] = args.refine_steps
+            # a stable queue keeps the latency of late arrivals near that of
+            # early ones; a backlog makes it grow through the stream
+            done = sorted((r for r in reqs if r.done is not None), key=lambda r: r.arrival)
+            q = max(1, len(done) // 4)
+            lat = [1e3 * (r.done - r.arrival) for r in done]
+            stats["p50_first_quarter_ms"] = float(np.median(lat[:q])) if lat else None
+            stats["p50_last_quarter_ms"] = float(np.median(lat[-q:])) if lat else None
+            stats["arrival_span_s"] = float(max(r.arrival for r in reqs))
         out[form] = stats
         del reqs, ops, server
         torch.cuda.empty_cache()
-    c, p = out["chosen"], out["plain"]
+    c, p = out["chosen"], out.get("plain") or {}
     return _line("solves/s + p50/p99 latency at equal FP64-oracle error, chosen vs plain form", c["solves_per_s"],
                  "solves/s", c["makespan_s"] * 1e3, args,
                  {"workload": "serving", "rate_per_s": args.rate, "requests": args.requests, "seed": 5,
@@ -873,7 +888,10 @@ def bench_serving(args):
                   "plain_form_kind": "static batching: FP32 kernels once per request kind over each batch of "
                                      "arrived requests, no refinement, no residency",
                   "fifo_form": out.get("fifo"),
-                  "speedup_vs_plain": c["solves_per_s"] / p["solves_per_s"] if p["solves_per_s"] else None})
+                  "arrival_span_s": c.get("arrival_span_s"),
+                  "p50_first_quarter_ms": c.get("p50_first_quarter_ms"),
+                  "p50_last_quarter_ms": c.get("p50_last_quarter_ms"),
+                  "speedup_vs_plain": c["solves_per_s"] / p["solves_per_s"] if p.get("solves_per_s") else None})
 
 
 def _blas_threads():
