@@ -7,9 +7,10 @@ The chosen form (``SolveServer(form="chosen")``) runs in ticks:
 1. admission: arrived requests are admitted first-come first-served while
    their working memory fits in the HBM arena (``hbm_budget`` bytes; the
    operator, its factors, scratch and right-hand side are allocated there);
-2. one launch of ``lxa_qr_factor_step`` advances every admitted QR system by
-   one 16-column panel, each at its own column, so systems admitted in
-   different ticks share the launches (continuous batching);
+2. ``panels_per_tick`` launches of ``lxa_qr_factor_step`` advance every
+   admitted QR system by that many 16-column panels, each at its own column,
+   so systems admitted in different ticks share the launches (continuous
+   batching);
 3. systems whose factorisation completed, and requests that reuse a resident
    factorisation with a new right-hand side, are solved together (one
    ``lxa_qr_apply`` + ``lxa_trsv`` launch pair);
@@ -53,6 +54,7 @@ from . import _kernels as K
 
 QR_ELEM = 4  # float32 bytes
 SMALL_CHOL = 256  # Cholesky orders served by the uneven tile-class launch
+PANELS_PER_TICK = 4  # QR panel steps per tick of the factor stream
 SOLVE_STREAMS = 2  # with the factor stream and the default stream: the 4 hardware queues of a process
 
 
@@ -137,7 +139,7 @@ class _Resident:
 
 class SolveServer:
     def __init__(self, device, *, form: str = "chosen", hbm_budget: int = 8 << 30,
-                 resident_budget: int = 2 << 30, refine_steps: int = 2):
+                 resident_budget: int = 2 << 30, refine_steps: int = 2, panels_per_tick: int = PANELS_PER_TICK):
         if form not in ("chosen", "plain", "fifo"):
             raise ValueError("form must be 'chosen', 'plain' or 'fifo'")
         if refine_steps < 0:
@@ -152,6 +154,9 @@ class SolveServer:
         self.resident: dict[int, _Resident] = {}
         self.resident_elems = 0
         self.nb = K.qr_block(torch.float32)
+        if panels_per_tick < 1:
+            raise ValueError("panels_per_tick must be >= 1")
+        self.panels_per_tick = int(panels_per_tick)
         self._uneven_ws = torch.zeros(256, dtype=torch.uint8, device=self.device)
         self.ticks = 0
         self.tick_log: list[dict] = []
@@ -574,10 +579,17 @@ class SolveServer:
                     kcol = torch.tensor([k for _, _, k in factoring], dtype=torch.int32).to(dev)
                     max_rows = max(r.m - k for r, _, k in factoring)
                     max_tc = max(r.n - k for r, _, k in factoring)
-                    K.qr_factor_step_raw(dd, len(factoring), kcol, max_rows, max_tc, buf, buf, buf, buf)
+                    # several panel steps per tick: the host work of a tick
+                    # (descriptors, events, polling) is paid once for them;
+                    # systems past their last column are skipped by the kernels
+                    steps = min(self.panels_per_tick, -(-max(r.n - k for r, _, k in factoring) // self.nb))
+                    for j in range(steps):
+                        if j:
+                            kcol += self.nb
+                        K.qr_factor_step_raw(dd, len(factoring), kcol, max_rows, max_tc, buf, buf, buf, buf)
                     nxt = []
                     for f in factoring:
-                        f[2] += self.nb
+                        f[2] += self.nb * steps
                         (qr_done if f[2] >= f[0].n else nxt).append(f)
                     factoring = nxt
                 ev_f = torch.cuda.Event()
