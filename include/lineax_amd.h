@@ -536,8 +536,11 @@ int lxa_gemm_batched(void* stream, int mode, int64_t batch, int64_t m, int64_t n
  *   1 + the index of the first exactly-zero pivot.
  * lxa_lu_large_trsv: kind 0: L y = x, 1: U y = x, 2: U^T y = x, 3: L^T y = x
  *   for nrhs <= 8 vectors x[r * ldx + i] in place; inv is linv (kinds 0, 3)
- *   or uinv (kinds 1, 2); sync is caller scratch of ceil(n/128) + 2 int32
- *   (cleared by the call).  One persistent launch.
+ *   or uinv (kinds 1, 2); sync is 16-byte aligned caller scratch of
+ *   lxa_lu_large_trsv_workspace(n) int32 words (cleared by the call; word 1
+ *   is set if a block gave up waiting for the block it depends on).  One
+ *   persistent launch; the solved blocks pass between workgroups through
+ *   the workspace as (tag, value) words.
  * lxa_rbt_matrix: A <- U^T A V with du, dv of 2n diagonal entries each
  *   (outer level d[0..n), inner level d[n..2n)).
  * lxa_rbt_vectors: form 1 applies U^T (or V^T) to nrhs vectors, form 0
@@ -559,6 +562,7 @@ int lxa_lu_large_factor(void* stream, int mode, int64_t n, float* a, int64_t lda
 int lxa_lu_large_perm(void* stream, int64_t n, const int32_t* piv, int32_t* perm);
 int lxa_lu_large_permute(void* stream, int64_t n, int nrhs, const float* x, int64_t ldx, float* y, int64_t ldy,
                          const int32_t* perm, int inverse);
+int lxa_lu_large_trsv_workspace(int64_t n, int64_t* words);
 int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, const float* lu, int64_t lda, const float* inv,
                       float* x, int64_t ldx, int32_t* sync);
 int lxa_rbt_matrix(void* stream, int64_t n, float* a, int64_t lda, const float* du, const float* dv);

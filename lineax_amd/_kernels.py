@@ -104,6 +104,7 @@ _SIGNATURES = {
     "lxa_lu_large_perm": [_vp, _i64, _vp, _vp],
     "lxa_lu_large_permute": [_vp, _i64, _int, _vp, _i64, _vp, _i64, _vp, _int],
     "lxa_lu_large_trsv": [_vp, _int, _i64, _int, _vp, _i64, _vp, _vp, _i64, _vp],
+    "lxa_lu_large_trsv_workspace": [_i64, _vp],
     "lxa_rbt_matrix": [_vp, _i64, _vp, _i64, _vp, _vp],
     "lxa_rbt_vectors": [_vp, _int, _i64, _int, _vp, _i64, _vp, _int],
     "lxa_gemv_f32a": [_vp, _i64, _i64, _int, _vp, _i64, _int, _vp, _i64, _vp, _i64, _int, _vp, _i64],
@@ -1038,6 +1039,16 @@ def lu_large_permute(x: torch.Tensor, perm: torch.Tensor, inverse: bool) -> torc
     _check(load().lxa_lu_large_permute(_stream(x), x.shape[1], x.shape[0], _ptr(x), x.stride(0), _ptr(y), y.stride(0),
                                        _ptr(perm), int(bool(inverse))), "lxa_lu_large_permute")
     return y
+
+
+LU_TRSV_FAILED_WORD = 1  # the workspace word a timed-out block wait sets
+
+
+def lu_large_trsv_workspace(n: int, device) -> torch.Tensor:
+    """Zeroed int32 workspace of lxa_lu_large_trsv for order n (16-byte aligned)."""
+    words = ctypes.c_int64(0)
+    _check(load().lxa_lu_large_trsv_workspace(int(n), ctypes.byref(words)), "lxa_lu_large_trsv_workspace")
+    return torch.zeros(int(words.value), dtype=torch.int32, device=device)
 
 
 def lu_large_trsv(kind: int, lu: torch.Tensor, inv: torch.Tensor, x: torch.Tensor, sync: torch.Tensor) -> None:

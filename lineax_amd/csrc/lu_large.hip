@@ -582,11 +582,6 @@ __global__ __launch_bounds__(kPanelThreads) void panel_lu_kernel(int k, int nb, 
   }
 }
 
-__global__ void zero_i32_kernel(int32_t* __restrict__ p, int count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = 0;
-}
-
 // The interchanges of a whole outer panel [K0, K1) (its 128-column blocks in
 // order, each within its 256-row window) applied to the columns outside the
 // panel, [0, K0) and [K1, n): a workgroup stages the rows the panel's windows
@@ -731,6 +726,21 @@ __global__ void permute_rows_kernel(int n, int nrhs, const float* __restrict__ x
 // (kinds 1, 2), 128 x 128 each; transposed kinds read blocks transposed.
 // ---------------------------------------------------------------------------
 constexpr int kTileLd = kB + 4;  // LDS row stride of a staged 128 x 128 block (16-byte rows)
+// Workspace of a block triangular solve (int32 words): [0] the ticket
+// counter, [1] the failure word, [2..3] padding, then the channel: one
+// 64-bit (tag, value) word per solution entry, nblk x 128 x kMaxRhs.
+constexpr int kTrsvHeader = 4;
+constexpr uint32_t kChanTag = 1;
+constexpr int kChanPer = kMaxRhs * kB / kTrsvThreads;  // channel entries per thread and block
+
+// Clears the ticket, the failure word and the channel's tags (a fill kernel,
+// not a memset node, so that a solve can sit in a captured graph).
+__global__ void trsv_arm_kernel(int32_t* __restrict__ ws, int64_t chan_words) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < kTrsvHeader) ws[i] = 0;
+  uint64_t* chan = reinterpret_cast<uint64_t*>(ws + kTrsvHeader);
+  for (int64_t j = i; j < chan_words; j += static_cast<int64_t>(gridDim.x) * blockDim.x) chan[j] = 0;
+}
 
 // Stages block (bi, bj) of the operator of `kind` into tile[r][c] (r, c <
 // 128; zeros outside rows < rows, cols < cols).  Loads are float4 along the
@@ -825,8 +835,8 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
   float(*acc)[kB] = reinterpret_cast<float(*)[kB]>(tsm + 2 * kB * kTileLd + kMaxRhs * kB);  // [8][128]
   __shared__ int s_ticket;
   int32_t* ticket = sync;
-  int32_t* flags = sync + 1;
-  int32_t* failed = sync + 1 + ((n + kB - 1) / kB);
+  int32_t* failed = sync + 1;
+  uint64_t* chan = reinterpret_cast<uint64_t*>(sync + kTrsvHeader);
   const int nblk = (n + kB - 1) / kB;
   const int tid = threadIdx.x;
   if (tid == 0) s_ticket = atomicAdd(ticket, 1);
@@ -854,25 +864,36 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
       pre[u] = *reinterpret_cast<const float4*>(D + (idx >> 5) * kB + 4 * (idx & 31));
     }
   };
-  auto wait_flag = [&](int bj) {
-    if (tid == 0) {
+  // A solved block arrives through the channel: each value is published
+  // with its tag in one 64-bit store, so a reader needs no flag and the
+  // writer no fence; every thread polls its own entries until their tags
+  // are up (bounded: past the spin limit the failure word is set).
+  auto recv_xs = [&](int bj) {
+    const int cols = min(kB, n - bj * kB);
+    uint64_t v[kChanPer];
+    bool need[kChanPer];
+#pragma unroll
+    for (int u = 0; u < kChanPer; ++u) {
+      const int idx = tid + u * kTrsvThreads, rr = idx / kB, i = idx % kB;
+      need[u] = rr < nrhs && i < cols;
+      v[u] = need[u] ? __hip_atomic_load(&chan[((int64_t)bj * kB + i) * kMaxRhs + rr], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT)
+                     : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kChanPer; ++u) {
+      const int idx = tid + u * kTrsvThreads, rr = idx / kB, i = idx % kB;
       int spin = 0;
-      while (__hip_atomic_load(&flags[bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      while (need[u] && (v[u] >> 32) != kChanTag) {
         if (++spin > kSpinLimit) {
           __hip_atomic_store(failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
+        v[u] = __hip_atomic_load(&chan[((int64_t)bj * kB + i) * kMaxRhs + rr], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
       }
-    }
-    __syncthreads();
-    (void)__hip_atomic_load(&flags[bj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-  };
-  auto load_xs = [&](int bj) {
-    const int c0 = bj * kB, cols = min(kB, n - c0);
-    for (int idx = tid; idx < kMaxRhs * kB; idx += kTrsvThreads) {
-      const int rr = idx / kB, i = idx % kB;
-      xs[rr][i] = (rr < nrhs && i < cols) ? x[(int64_t)rr * ldx + c0 + i] : 0.f;
+      xs[rr][i] = need[u] ? __uint_as_float(static_cast<uint32_t>(v[u])) : 0.f;
     }
   };
   if (t > 0) {
@@ -920,8 +941,7 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
       const int bn = block_col(step + 1);
       tile_load(pre, lu, lda, kind, bi, bn, rows, min(kB, n - bn * kB), tid);
     }
-    wait_flag(bj);
-    load_xs(bj);
+    recv_xs(bj);
     __syncthreads();
     float part[R];
     tile_product<R>(tile, xs, row, hcol, part);
@@ -940,8 +960,7 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
   float z[R];
   tile_product<R>(tile, acc, row, hcol, z);
   if (t > 0) {
-    wait_flag(block_col(t - 1));
-    load_xs(block_col(t - 1));
+    recv_xs(block_col(t - 1));
     __syncthreads();
     float part[R];
     tile_product<R>(Mt, xs, row, hcol, part);
@@ -951,13 +970,13 @@ __global__ __launch_bounds__(kTrsvThreads) void trsv_blocks_kernel(int n, int nr
   if ((tid & 1) == 0 && row < rows) {
 #pragma unroll
     for (int rr = 0; rr < R; ++rr)
-      if (rr < nrhs) x[(int64_t)rr * ldx + r0 + row] = z[rr];
+      if (rr < nrhs) {
+        x[(int64_t)rr * ldx + r0 + row] = z[rr];
+        const uint64_t word = (static_cast<uint64_t>(kChanTag) << 32) | __float_as_uint(z[rr]);
+        __hip_atomic_store(&chan[((int64_t)bi * kB + row) * kMaxRhs + rr], word, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
   }
-  // Every wave makes its own stores visible device-wide before the flag is
-  // raised (a fence orders only the issuing wave's memory operations).
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) __hip_atomic_store(&flags[bi], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -1227,6 +1246,13 @@ extern "C" int lxa_lu_large_permute(void* stream, int64_t n, int nrhs, const flo
   return LXA_OK;
 }
 
+extern "C" int lxa_lu_large_trsv_workspace(int64_t n, int64_t* words) {
+  LXA_REQUIRE(n >= 0 && n <= (1 << 20) && words, "lxa_lu_large_trsv_workspace: bad argument");
+  const int64_t nblk = (n + kB - 1) / kB;
+  *words = kTrsvHeader + 2 * nblk * kB * kMaxRhs;
+  return LXA_OK;
+}
+
 extern "C" int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, const float* lu, int64_t lda,
                                  const float* inv, float* x, int64_t ldx, int32_t* sync) {
   LXA_REQUIRE(kind >= 0 && kind <= 3, "kind must be 0..3 (got %d)", kind);
@@ -1234,11 +1260,13 @@ extern "C" int lxa_lu_large_trsv(void* stream, int kind, int64_t n, int nrhs, co
   LXA_REQUIRE(nrhs >= 0 && nrhs <= kMaxRhs, "nrhs must be in [0, %d] (got %d)", kMaxRhs, nrhs);
   if (n == 0 || nrhs == 0) return LXA_OK;
   LXA_REQUIRE(lu && inv && x && sync && ldx >= n, "invalid pointers or ldx");
+  LXA_REQUIRE((reinterpret_cast<uintptr_t>(sync) & 15) == 0, "the workspace must be 16-byte aligned");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int nblk = (int)((n + kB - 1) / kB);
-  // a fill kernel rather than a memset node, so the call can sit in a captured graph
-  zero_i32_kernel<<<(nblk + 2 + 255) / 256, 256, 0, st>>>(sync, nblk + 2);
-  LXA_LAUNCH_CHECK("zero_i32_kernel");
+  const int64_t chan_words = (int64_t)nblk * kB * kMaxRhs;
+  const int64_t arm_blocks = (chan_words + 255) / 256 < 1024 ? (chan_words + 255) / 256 : 1024;
+  trsv_arm_kernel<<<(unsigned)arm_blocks, 256, 0, st>>>(sync, chan_words);
+  LXA_LAUNCH_CHECK("trsv_arm_kernel");
   const size_t smem = sizeof(float) * (2 * kB * kTileLd + 2 * kMaxRhs * kB);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&trsv_blocks_kernel<4>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);

@@ -106,7 +106,7 @@ class LargeLU:
         K.lu_large_factor(lu, precision, self.linv, self.uinv, self.piv, self.info)
         self.perm = K.lu_large_perm(self.piv)
         self.lu = lu
-        self.sync = torch.empty(self.nblk + 2, dtype=torch.int32, device=dev)
+        self.sync = K.lu_large_trsv_workspace(np_, dev)
         # set on the device when a block triangular solve gave up waiting for a
         # dependency (the kernel's bounded spin); checked once per solve
         self.trsv_failed = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -125,7 +125,8 @@ class LargeLU:
             chunk = w[r0: r0 + MAX_RHS_PER_SOLVE]
             for kind, inv in (((2, self.uinv), (3, self.linv)) if trans else ((0, self.linv), (1, self.uinv))):
                 K.lu_large_trsv(kind, self.lu, inv, chunk, self.sync)
-                torch.maximum(self.trsv_failed, self.sync[self.nblk + 1:], out=self.trsv_failed)
+                fw = K.LU_TRSV_FAILED_WORD
+                torch.maximum(self.trsv_failed, self.sync[fw: fw + 1], out=self.trsv_failed)
         if trans:
             w = K.lu_large_permute(w, self.perm, inverse=True)
         K.rbt_vectors(w, last, form=0)
