@@ -576,7 +576,8 @@ class SolveServer:
                 if factoring:
                     items = [(r.m, r.n + 1, r.n + 1, blk) for r, blk, _ in factoring]
                     dd = self._descs(items)
-                    kcol = torch.tensor([k for _, _, k in factoring], dtype=torch.int32).to(dev)
+                    # pinned, non-blocking (a pageable copy would wait for the stream)
+                    kcol = K.host_to_device(torch.tensor([k for _, _, k in factoring], dtype=torch.int32), dev)
                     max_rows = max(r.m - k for r, _, k in factoring)
                     max_tc = max(r.n - k for r, _, k in factoring)
                     # several panel steps per tick: the host work of a tick
