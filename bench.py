@@ -885,8 +885,9 @@ def bench_serving(args):
                   "kernel": "whole request loop (QR panel + trailing kernels dominate)"},
                  cpu,
                  {"p50_ms": c["p50_ms"], "p99_ms": c["p99_ms"], "chosen": c, "plain_form": p,
-                  "plain_form_kind": "static batching: FP32 kernels once per request kind over each batch of "
-                                     "arrived requests, no refinement, no residency",
+                  "plain_form_kind": "static batching: each batch of arrived requests grouped by kind and size "
+                                     "bucket (QR by m in 256-row buckets, Cholesky by n in 64s), FP32 kernels "
+                                     "once per bucket, no refinement, no residency",
                   "fifo_form": out.get("fifo"),
                   "arrival_span_s": c.get("arrival_span_s"),
                   "p50_first_quarter_ms": c.get("p50_first_quarter_ms"),

@@ -27,11 +27,11 @@ The chosen form (``SolveServer(form="chosen")``) runs in ticks:
    ``resident_budget`` bytes, least recently used evicted) for reuse
    requests.
 
-The plain form (``form="plain"``) is static batching: when idle it takes every
-request that has arrived as one batch, runs the float32 kernels once per
-request kind over the batch (uneven QR factor + apply + trsv, uneven blocked
-Cholesky), without refinement or residency, and takes the next batch when this
-one is done.  ``form="fifo"`` runs the same kernels one request at a time in
+The plain form (``form="plain"``) is static batching with one launch per
+size bucket: when idle it takes every request that has arrived as one batch,
+groups it by kind and size bucket, runs the float32 kernels once per bucket
+(uneven QR factor + apply + trsv, uneven blocked Cholesky), without refinement
+or residency, and takes the next batch when this one is done.  ``form="fifo"`` runs the same kernels one request at a time in
 arrival order.  Their answers are float32.
 
 Semantically each request is ``lx.linear_solve(lx.MatrixLinearOperator(A),
@@ -55,6 +55,8 @@ from . import _kernels as K
 QR_ELEM = 4  # float32 bytes
 SMALL_CHOL = 256  # Cholesky orders served by the uneven tile-class launch
 PANELS_PER_TICK = 4  # QR panel steps per tick of the factor stream
+PLAIN_QR_BUCKET = 256  # rows per size bucket of the plain form's least-squares launches
+PLAIN_CHOL_BUCKET = 64  # orders per size bucket of the plain form's Cholesky launches
 SOLVE_STREAMS = 2  # with the factor stream and the default stream: the 4 hardware queues of a process
 
 
@@ -413,14 +415,15 @@ class SolveServer:
             self.ticks += 1
 
     def _run_static(self, reqs, t0, time_limit):
-        """The plain form: static batching.  When idle, every request that has
-        arrived (as many as fit in the arena) becomes one batch: the least-
-        squares requests are factored in one uneven lxa_qr_factor launch
-        sequence and solved in one apply + trsv pair, the Cholesky requests in
-        one lxa_chol_blocked factor + solve pair (FP32, one launch per size
-        bucket = request kind, no refinement, no residency: a reuse request
-        factors its operator again); the next batch starts when this one is
-        done."""
+        """The plain form: static batching with one launch per size bucket.
+        When idle, every request that has arrived (as many as fit in the
+        arena) becomes one batch.  Its requests are grouped by kind and size
+        bucket (least squares by m in PLAIN_QR_BUCKET-row buckets, Cholesky by
+        n in PLAIN_CHOL_BUCKET buckets), and each bucket runs the FP32 kernels
+        once: an uneven lxa_qr_factor launch sequence and one apply + trsv
+        pair, or one lxa_chol_blocked factor + solve pair.  No refinement and
+        no residency (a reuse request factors its operator again); the next
+        batch starts when this one is done."""
         i = 0
         buf = self.arena.buf
         while i < len(reqs):
@@ -445,13 +448,20 @@ class SolveServer:
                 (qr_items if r.kind == "qr" else ch_items).append((r.m, r.n, r.n, blocks))
                 taken.append((r, blocks))
                 i += 1
-            if qr_items:
-                dd = self._descs(qr_items)
-                max_m = max(it[0] for it in qr_items)
-                max_n = max(it[1] for it in qr_items)
-                K.qr_factor_raw(dd, len(qr_items), max_m, max_n, min(max_m, max_n), buf, buf, buf, buf)
-                self._qr_solve_batch(qr_items)
-            self._chol_batch(ch_items, factor=True)
+            qr_buckets: dict[int, list] = {}
+            for it in qr_items:
+                qr_buckets.setdefault((it[0] - 1) // PLAIN_QR_BUCKET, []).append(it)
+            for _, items in sorted(qr_buckets.items()):
+                dd = self._descs(items)
+                max_m = max(it[0] for it in items)
+                max_n = max(it[1] for it in items)
+                K.qr_factor_raw(dd, len(items), max_m, max_n, min(max_m, max_n), buf, buf, buf, buf)
+                self._qr_solve_batch(items)
+            ch_buckets: dict[int, list] = {}
+            for it in ch_items:
+                ch_buckets.setdefault((it[0] - 1) // PLAIN_CHOL_BUCKET, []).append(it)
+            for _, items in sorted(ch_buckets.items()):
+                self._chol_batch(items, factor=True)
             xs = [self.arena.view(blocks["b"][0], r.n).clone() for r, blocks in taken]
             torch.cuda.synchronize(self.device)
             t_done = self._now(t0)
