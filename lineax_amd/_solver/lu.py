@@ -11,6 +11,18 @@ from .._solve import AbstractLinearSolver
 from .misc import pack_structures, ravel_vector, transpose_packed_structures, unravel_solution
 
 
+# Unbatched float32 operators from this order up take the large path: one
+# workgroup factoring a whole matrix in global memory (the batched tiers)
+# measures 47 ms at n = 512 and 13.4 s at n = 3000 on one system, the large
+# path 4.9 ms and 17 ms (refined to float32 epsilon).
+LARGE_THRESHOLD = 256
+# Default refinement target of the large path: float32 epsilon (getrf's
+# level in float32); float64 operators, factored in float32 and refined
+# against themselves, to 4 u_f64 (getrf measures 4e-16 .. 2e-15 at
+# n = 512 .. 3000).
+LARGE_TOL = {torch.float32: float(torch.finfo(torch.float32).eps), torch.float64: 4 * float(torch.finfo(torch.float64).eps)}
+
+
 class _PendingFactors:
     """Factors of a batch of small operators, formed by the first solve.
 
@@ -41,17 +53,20 @@ class LU(AbstractLinearSolver):
     order <= 64 ``("pending", factors)`` whose factors the first compute
     forms together with the solve (one fused launch).
 
-    One unbatched float32 operator of order >= ``large_threshold`` takes the
-    large path instead (``lineax_amd.large_lu``): butterfly randomisation,
-    unpivoted blocked LU with ``large_precision`` Schur products, and
-    refinement against A to a normwise backward error <= ``large_tol``
-    (default: float32 epsilon), i.e. at least getrf's accuracy.  Its state is
-    ``(("large", factorisation), packed_structures, transposed)``.
-    (The constructor arguments are not in lineax; their defaults keep
-    lineax's behaviour.)
+    One unbatched float32 or float64 operator of order >= ``large_threshold``
+    takes the large path instead (``lineax_amd.large_lu``): butterfly
+    randomisation, blocked LU pivoting within 256-row windows with
+    ``large_precision`` Schur products (float32 factors; a float64 operator
+    is factored in float32), and refinement against A with FP64 residuals to
+    a normwise backward error <= ``large_tol`` (default LARGE_TOL: getrf's
+    level for the dtype).  Its state is ``(("large", factorisation),
+    packed_structures, transposed)``.  (The constructor arguments are not in
+    lineax; with their defaults the answers are getrf's to that tolerance,
+    the factors are not.)
     """
 
-    def __init__(self, large_threshold: int = 4096, large_precision: str = "bf16x3", large_tol: float | None = None):
+    def __init__(self, large_threshold: int = LARGE_THRESHOLD, large_precision: str = "bf16x3",
+                 large_tol: float | None = None):
         self.large_threshold = int(large_threshold)
         self.large_precision = large_precision
         self.large_tol = large_tol
@@ -62,7 +77,7 @@ class LU(AbstractLinearSolver):
         packed = pack_structures(operator)
         matrix = operator.as_matrix()
         n = operator.in_size()
-        if (matrix.dim() == 2 and n >= self.large_threshold and matrix.dtype == torch.float32
+        if (matrix.dim() == 2 and n >= self.large_threshold and matrix.dtype in (torch.float32, torch.float64)
                 and not is_diagonal(operator)):
             from ..large_lu import LargeLU
 
@@ -93,7 +108,7 @@ class LU(AbstractLinearSolver):
             return unravel_solution(x, packed, tuple(x.shape[:-1])), RESULTS.successful, {}
         if isinstance(lu, str):
             fac = piv
-            tol = self.large_tol if self.large_tol is not None else float(torch.finfo(torch.float32).eps)
+            tol = self.large_tol if self.large_tol is not None else LARGE_TOL[fac.a.dtype]
             rhs = flat.reshape(-1, fac.n)
             x, _eta, _stats = fac.solve(rhs, tol=tol, trans=transpose)
             x = x.to(flat.dtype).reshape(*flat.shape[:-1], fac.n)

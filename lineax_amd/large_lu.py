@@ -1,6 +1,8 @@
-"""Large dense LU: butterfly-randomised, unpivoted, refined (float32 data).
+"""Large dense LU: butterfly-randomised, window-pivoted, refined.
 
-``LargeLU(a)`` factors one float32 matrix of order n (thousands):
+``LargeLU(a)`` factors one float32 (or float64, factored in float32 and
+refined against the float64 operator) matrix of order n (hundreds to
+thousands):
 
 1. A' = U^T A V with two-sided depth-2 random butterflies (seeded), padded
    to a multiple of 4 with an identity block;
@@ -54,6 +56,14 @@ class _F32Matrix(FunctionLinearOperator):
         return K.gemv_f32a(self.a, x.reshape(-1, self.a.shape[1]), out_dtype).reshape(*x.shape[:-1], self.a.shape[0])
 
 
+class _F64Matrix(_F32Matrix):
+    """x -> A x for a float64 matrix (lxa_gemv in FP64), in the dtype asked for."""
+
+    def mv_as(self, x: torch.Tensor, out_dtype: torch.dtype) -> torch.Tensor:
+        y = K.gemv(self.a, x.reshape(-1, self.a.shape[1]).to(torch.float64))
+        return y.to(out_dtype).reshape(*x.shape[:-1], self.a.shape[0])
+
+
 class _Preconditioner(FunctionLinearOperator):
     """x -> V (L U)^-1 U^T x (or its transpose), float32 arithmetic."""
 
@@ -78,8 +88,8 @@ class LargeLU:
         K._gpu(a, "matrix")
         if a.dim() != 2 or a.shape[0] != a.shape[1]:
             raise ValueError(f"LargeLU needs one square matrix, got {tuple(a.shape)}")
-        if a.dtype != torch.float32:
-            raise TypeError("LargeLU factors float32 matrices")
+        if a.dtype not in (torch.float32, torch.float64):
+            raise TypeError("LargeLU factors float32 or float64 matrices")
         if precision not in K.GEMM_MODES:
             raise ValueError(f"precision must be one of {sorted(K.GEMM_MODES)}")
         self.precision = precision
@@ -87,6 +97,8 @@ class LargeLU:
         self.n = n = a.shape[0]
         self.np = np_ = -(-n // 4) * 4
         dev = a.device
+        # a float64 operator is factored in float32 and refined against
+        # itself (FP64 residuals and products)
         lu = torch.zeros((np_, np_), dtype=torch.float32, device=dev)
         lu[:n, :n] = self.a
         if np_ > n:
@@ -134,6 +146,8 @@ class LargeLU:
 
     def residual(self, x: torch.Tensor, b: torch.Tensor, trans: bool = False) -> torch.Tensor:
         """b - A x (or b - A^T x) in FP64."""
+        if self.a.dtype == torch.float64:
+            return b.reshape(-1, self.n) - K.gemv(self.a, x.reshape(-1, self.n), trans=trans)
         a = self.a.T.contiguous() if trans else self.a
         return K.gemv_f32a(a, x, torch.float64, b=b)
 
@@ -158,7 +172,8 @@ class LargeLU:
 
         b64 = b.reshape(-1, self.n).to(torch.float64).contiguous()
         x = self.apply_inverse(b64.to(torch.float32), trans=trans).to(torch.float64)
-        op = _F32Matrix(self.a.T.contiguous() if trans else self.a, torch.float64)
+        mat = _F64Matrix if self.a.dtype == torch.float64 else _F32Matrix
+        op = mat(self.a.T.contiguous() if trans else self.a, torch.float64)
         prec = _Preconditioner(self, torch.float64, trans)
         residuals = 0
         gmres_steps = 0

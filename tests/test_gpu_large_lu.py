@@ -320,3 +320,30 @@ def test_large_lu_config_results_through_linear_solve(device):
     ref = np.linalg.solve(a.double().numpy(), b[0].double().numpy())
     x = sol.value.cpu().numpy().astype(np.float64)
     assert np.max(np.abs(x - ref)) <= 2e-6 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("n", [300, 1001])
+def test_lu_solver_large_path_float64(device, rng, n):
+    # an unbatched float64 operator at or above LU's default threshold is
+    # factored in float32 and refined against itself with FP64 residuals: the
+    # answer at getrf's float64 level (normwise backward error <= 4 u_f64
+    # after refinement, checked here in float64 on the CPU within 16 u_f64),
+    # forward and transposed
+    a = rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    op = lx.MatrixLinearOperator(torch.as_tensor(a, device=device))
+    solver = lx.LU()
+    state = solver.init(op, {})
+    assert isinstance(state[0][0], str) and state[0][0] == "large"
+    x = lx.linear_solve(op, torch.as_tensor(b, device=device), solver).value
+    assert x.dtype == torch.float64
+    eps = np.finfo(np.float64).eps
+    for mat, sol in ((a, x), (a.T, None)):
+        if sol is None:
+            st_t, _ = solver.transpose(state, {})
+            sol, _, _ = solver.compute(st_t, torch.as_tensor(b, device=device), {})
+        xs = sol.cpu().numpy()
+        eta = np.max(np.abs(b - mat @ xs)) / (np.abs(mat).sum(1).max() * np.abs(xs).max() + np.abs(b).max())
+        assert eta <= 16 * eps, eta
+        ref = np.linalg.solve(mat, b)
+        assert np.max(np.abs(xs - ref)) <= 1e-10 * np.max(np.abs(ref))
