@@ -2,6 +2,8 @@
 
 from __future__ import annotations
 
+import math
+
 import torch
 
 from .. import _kernels as K
@@ -16,6 +18,10 @@ from .misc import pack_structures, ravel_vector, transpose_packed_structures, un
 # measures 47 ms at n = 512 and 13.4 s at n = 3000 on one system, the large
 # path 4.9 ms and 17 ms (refined to float32 epsilon).
 LARGE_THRESHOLD = 256
+# Batches of up to this many operators at or above the threshold take the
+# large path one operator at a time (the batched tiers factor each operator
+# on one workgroup).
+LARGE_BATCH_MAX = 16
 # Default refinement target of the large path: float32 epsilon (getrf's
 # level in float32); float64 operators, factored in float32 and refined
 # against themselves, to 4 u_f64 (getrf measures 4e-16 .. 2e-15 at
@@ -60,7 +66,9 @@ class LU(AbstractLinearSolver):
     is factored in float32), and refinement against A with FP64 residuals to
     a normwise backward error <= ``large_tol`` (default LARGE_TOL: getrf's
     level for the dtype).  Its state is ``(("large", factorisation),
-    packed_structures, transposed)``.  (The constructor arguments are not in
+    packed_structures, transposed)``; a batch of up to LARGE_BATCH_MAX such
+    operators takes the large path one operator at a time
+    (``("large_batch", (factorisations, batch_shape))``).  (The constructor arguments are not in
     lineax; with their defaults the answers are getrf's to that tolerance,
     the factors are not.)
     """
@@ -77,11 +85,18 @@ class LU(AbstractLinearSolver):
         packed = pack_structures(operator)
         matrix = operator.as_matrix()
         n = operator.in_size()
-        if (matrix.dim() == 2 and n >= self.large_threshold and matrix.dtype in (torch.float32, torch.float64)
+        if (n >= self.large_threshold and matrix.dtype in (torch.float32, torch.float64)
                 and not is_diagonal(operator)):
             from ..large_lu import LargeLU
 
-            return ("large", LargeLU(matrix, precision=self.large_precision)), packed, False
+            if matrix.dim() == 2:
+                return ("large", LargeLU(matrix, precision=self.large_precision)), packed, False
+            count = math.prod(matrix.shape[:-2])
+            if 0 < count <= LARGE_BATCH_MAX:
+                # a small batch of large operators: one large-path factorisation each
+                flat = matrix.reshape(count, n, n)
+                facs = [LargeLU(flat[i], precision=self.large_precision) for i in range(count)]
+                return ("large_batch", (facs, tuple(matrix.shape[:-2]))), packed, False
         if is_diagonal(operator):
             # lineax/_solver/lu.py:50-51: the diagonal is its own LU factor.
             n = operator.in_size()
@@ -106,6 +121,17 @@ class LU(AbstractLinearSolver):
                 lu_mat, pv = holder.factors()
                 x = K.lu_solve(lu_mat, pv, vec, trans=1 if transpose else 0)
             return unravel_solution(x, packed, tuple(x.shape[:-1])), RESULTS.successful, {}
+        if isinstance(lu, str) and lu == "large_batch":
+            facs, op_batch = piv
+            tol = self.large_tol if self.large_tol is not None else LARGE_TOL[facs[0].a.dtype]
+            n = facs[0].n
+            vec = flat.expand(*torch.broadcast_shapes(flat.shape[:-1], op_batch), n)
+            if tuple(vec.shape[:-1]) != op_batch:
+                raise ValueError(f"vector batch {tuple(flat.shape[:-1])} does not match the operators' {op_batch}")
+            rows = vec.reshape(len(facs), n)
+            xs = [fac.solve(rows[i: i + 1], tol=tol, trans=transpose)[0] for i, fac in enumerate(facs)]
+            x = torch.cat(xs).to(flat.dtype).reshape(*op_batch, n)
+            return unravel_solution(x, packed, op_batch), RESULTS.successful, {}
         if isinstance(lu, str):
             fac = piv
             tol = self.large_tol if self.large_tol is not None else LARGE_TOL[fac.a.dtype]

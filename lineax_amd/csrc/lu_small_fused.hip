@@ -109,20 +109,23 @@ __global__ __launch_bounds__(64) void lu_small_fused_kernel(int64_t batch, int n
   for (int k = 0; k < N; ++k) {
     // pivot: largest |a(i,k)| among rows not yet used, smallest position on ties
     const uint32_t hi = done ? 0u : mag_key(row[k]);
-    const uint32_t lo_pos = done ? 0u : static_cast<uint32_t>(63 - pos);
+    // low word: position (smallest wins ties), then the lane (a pure label)
+    const uint32_t lo_pos = done ? 0u : ((static_cast<uint32_t>(63 - pos) << 8) | static_cast<uint32_t>(r));
     uint64_t key = (static_cast<uint64_t>(hi) << 32) | lo_pos;
     key = wave_max_u64<N>(key);
     if constexpr (sizeof(T) == 8) {
       // doubles: break ties of the top 32 magnitude bits on the low 32 bits
       const uint32_t top = static_cast<uint32_t>(key >> 32);
       const bool cand = !done && mag_key(row[k]) == top;
-      uint64_t k2 = cand ? ((static_cast<uint64_t>(mag_lo(row[k])) << 32) | static_cast<uint32_t>(63 - pos)) : 0;
+      uint64_t k2 = cand ? ((static_cast<uint64_t>(mag_lo(row[k])) << 32) |
+                            ((static_cast<uint32_t>(63 - pos) << 8) | static_cast<uint32_t>(r)))
+                         : 0;
       k2 = wave_max_u64<N>(k2);
       key = (static_cast<uint64_t>(top) << 32) | static_cast<uint32_t>(k2);
     }
-    const int cp = 63 - static_cast<int>(static_cast<uint32_t>(key));
-    const bool is_piv = !done && pos == cp;
-    const int p = __builtin_amdgcn_readfirstlane(__ffsll(static_cast<long long>(__ballot(is_piv))) - 1);
+    const int cp = 63 - static_cast<int>(static_cast<uint32_t>(key) >> 8);
+    const int p = static_cast<int>(static_cast<uint32_t>(key) & 0xffu);  // the pivot's lane
+    const bool is_piv = r == p;
     const T pval = lane_value(row[k], p);
     const T pb = lane_value(bv, p);
     if (r == k) mypiv = cp;
@@ -134,13 +137,16 @@ __global__ __launch_bounds__(64) void lu_small_fused_kernel(int64_t batch, int n
     } else if (!done && pos == k) {
       pos = cp;  // the row at position k moves to the pivot's old position
     }
-    if (!done) {
-      const T l = (pval != T(0)) ? row[k] / pval : row[k];
-      row[k] = l;
+    // every lane computes the update (the pivot row's entries are scalars);
+    // rows already pivoted keep their values by selection, not by a branch
+    const T l = (pval != T(0)) ? row[k] / pval : row[k];
 #pragma unroll
-      for (int j = k + 1; j < N; ++j) row[j] -= l * lane_value(row[j], p);
-      bv -= l * pb;
+    for (int j = k + 1; j < N; ++j) {
+      const T upd = row[j] - l * lane_value(row[j], p);
+      row[j] = done ? row[j] : upd;
     }
+    bv = done ? bv : bv - l * pb;
+    row[k] = done ? row[k] : l;
   }
   __syncthreads();  // lane_at (one wave)
   // U x = y over positions N-1..0; lane_at[k] holds row k of U and y_k

@@ -347,3 +347,32 @@ def test_lu_solver_large_path_float64(device, rng, n):
         assert eta <= 16 * eps, eta
         ref = np.linalg.solve(mat, b)
         assert np.max(np.abs(xs - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+def test_lu_solver_large_path_small_batch(device, rng):
+    # a small batch of operators above the threshold: one large-path
+    # factorisation each, refined to float32 epsilon in normwise backward
+    # error (checked in float64 within 4 eps, the forward error against
+    # numpy within cond(A) times that); forward and transposed solves, and a
+    # vector without batch dimensions broadcast over the operators
+    n = 260
+    a = rng.standard_normal((3, n, n)).astype(np.float32)
+    b = rng.standard_normal((3, n)).astype(np.float32)
+    op = lx.MatrixLinearOperator(torch.as_tensor(a, device=device))
+    solver = lx.LU()
+    state = solver.init(op, {})
+    assert state[0][0] == "large_batch"
+    x = lx.linear_solve(op, torch.as_tensor(b, device=device), solver).value.cpu().numpy()
+    st_t, _ = solver.transpose(state, {})
+    xt, _, _ = solver.compute(st_t, torch.as_tensor(b, device=device), {})
+    xb, _, _ = solver.compute(state, torch.as_tensor(b[0], device=device), {})
+    for i in range(3):
+        a64 = a[i].astype(np.float64)
+        eps = np.finfo(np.float32).eps
+        for sol, mat, rhs in ((x[i], a64, b[i]), (xt[i].cpu().numpy(), a64.T, b[i]), (xb[i].cpu().numpy(), a64, b[0])):
+            sol = sol.astype(np.float64)
+            rhs = rhs.astype(np.float64)
+            eta = np.max(np.abs(rhs - mat @ sol)) / (np.abs(mat).sum(1).max() * np.abs(sol).max() + np.abs(rhs).max())
+            assert eta <= 4 * eps, eta
+            ref = np.linalg.solve(mat, rhs)
+            assert np.max(np.abs(sol - ref)) <= 4 * eps * np.linalg.cond(mat, np.inf) * np.max(np.abs(ref))
