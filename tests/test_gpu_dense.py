@@ -274,6 +274,14 @@ def test_lu_small_fused_bit_equal(device, rng, dtype, n):
     x2, lu2, piv2, _ = K.lu_small_solve(A, B)
     assert lu2 is None and piv2 is None and torch.equal(torch.nan_to_num(x2, nan=7.0, posinf=8.0, neginf=9.0),
                                                          torch.nan_to_num(x, nan=7.0, posinf=8.0, neginf=9.0))
+    # operators one element off 16-byte alignment take the row-per-lane loads
+    base = torch.empty(batch * n * n + 1, dtype=dtype, device=device)
+    A_off = base[1:].view(batch, n, n)
+    A_off.copy_(A)
+    x3, lu3, piv3, _ = K.lu_small_solve(A_off, B, want_factors=True)
+    assert torch.equal(piv3, piv) and torch.equal(torch.nan_to_num(lu3, nan=7.0), torch.nan_to_num(lu, nan=7.0))
+    assert torch.equal(torch.nan_to_num(x3, nan=7.0, posinf=8.0, neginf=9.0),
+                       torch.nan_to_num(x, nan=7.0, posinf=8.0, neginf=9.0))
     # and the solves are backward stable (random Gaussian operators: judged by
     # the normwise backward error, not the forward error)
     good = [i for i in range(batch) if i not in (3, 4)]
