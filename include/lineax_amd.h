@@ -508,7 +508,13 @@ int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_t k, const 
  * broadcasts an operand), same tiles and modes.  Replaces the materialised
  * product of lineax's Normal solver (lineax/_solver/normal.py:105-120 via
  * ComposedLinearOperator.as_matrix, _operator.py:1125-1192) for float32
- * operators: the Gram matrix A^T A (A A^T when wide). */
+ * operators: the Gram matrix A^T A (A A^T when wide).  With LXA_GEMM_F32,
+ * mode may add LXA_GEMM_TRANS_A (A_s stored k x m: element (i, l) at
+ * a[l * lda + i], lda >= m) and / or LXA_GEMM_TRANS_B (B_s stored n x k:
+ * element (l, j) at b[j * ldb + l], ldb >= k), so a transposed operand is
+ * read in place, without a copy. */
+#define LXA_GEMM_TRANS_A 256
+#define LXA_GEMM_TRANS_B 512
 int lxa_gemm_batched(void* stream, int mode, int64_t batch, int64_t m, int64_t n, int64_t k,
                      const float* a, int64_t lda, int64_t a_stride, const float* b,
                      int64_t ldb, int64_t b_stride, float* c, int64_t ldc,

@@ -936,6 +936,7 @@ GEMM_F32 = 0
 GEMM_BF16 = 1
 GEMM_F16 = 2
 GEMM_BF16X3 = 3
+GEMM_TRANS_A, GEMM_TRANS_B = 256, 512  # LXA_GEMM_TRANS_A / _B: operand stored transposed
 GEMM_MODES = {"fp32": GEMM_F32, "bf16": GEMM_BF16, "fp16": GEMM_F16, "bf16x3": GEMM_BF16X3}
 
 
@@ -982,28 +983,35 @@ def gemm(a: torch.Tensor, b: torch.Tensor, mode: str = "fp32") -> torch.Tensor:
         return c.zero_()
 
     def flat(t):
-        # (count, rows, cols) with unit column stride; a broadcast operand
-        # keeps batch stride 0
+        # (count, rows, cols) with one unit stride among the last two (a
+        # transposed layout is read in place in float32 mode); a broadcast
+        # operand keeps batch stride 0.  Returns (view, batch stride, leading
+        # dimension, transposed).
         if t.shape[:-2] == batch_shape or not batch_shape:
             t = t.reshape(-1, *t.shape[-2:]) if t.dim() > 2 else t.unsqueeze(0)
-            if t.stride(-1) != 1 or t.stride(-2) < t.shape[-1]:
-                t = t.contiguous()
-            return t, (t.stride(0) if t.shape[0] > 1 else 0)
-        if t.dim() == 2 or all(d == 1 for d in t.shape[:-2]):
-            t = t.reshape(t.shape[-2:]).contiguous()
-            return t.unsqueeze(0), 0
-        t = t.expand(*batch_shape, *t.shape[-2:]).contiguous().reshape(-1, *t.shape[-2:])
-        return t, t.stride(0)
+        elif t.dim() == 2 or all(d == 1 for d in t.shape[:-2]):
+            t = t.reshape(1, *t.shape[-2:])
+        else:
+            t = t.expand(*batch_shape, *t.shape[-2:]).contiguous().reshape(-1, *t.shape[-2:])
+        rows, cols = t.shape[-2:]
+        if t.stride(-1) == 1 and t.stride(-2) >= cols:
+            trans, ld = False, t.stride(-2)
+        elif mode == "fp32" and t.stride(-2) == 1 and t.stride(-1) >= rows:
+            trans, ld = True, t.stride(-1)
+        else:
+            t = t.contiguous()
+            trans, ld = False, cols
+        return t, (t.stride(0) if t.shape[0] > 1 else 0), ld, trans
 
-    af, sa = flat(a)
-    bf, sb = flat(b)
+    af, sa, lda, ta = flat(a)
+    bf, sb, ldb, tb = flat(b)
     cf = c.reshape(-1, m, n)
+    code = GEMM_MODES[mode] | (GEMM_TRANS_A if ta else 0) | (GEMM_TRANS_B if tb else 0)
     lib = load()
     for s0 in range(0, count, GEMM_MAX_BATCH):
         cnt = min(GEMM_MAX_BATCH, count - s0)
-        _check(lib.lxa_gemm_batched(_stream(c), GEMM_MODES[mode], cnt, m, n, k, _ptr(af[s0 if sa else 0]),
-                                    af.stride(1), sa, _ptr(bf[s0 if sb else 0]), bf.stride(1), sb, _ptr(cf[s0]), n,
-                                    m * n), "lxa_gemm_batched")
+        _check(lib.lxa_gemm_batched(_stream(c), code, cnt, m, n, k, _ptr(af[s0 if sa else 0]), lda, sa,
+                                    _ptr(bf[s0 if sb else 0]), ldb, sb, _ptr(cf[s0]), n, m * n), "lxa_gemm_batched")
     return c
 
 

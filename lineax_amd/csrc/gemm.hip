@@ -87,7 +87,9 @@ __device__ __forceinline__ void mfma_block(floatx16& acc, const typename GemmMod
   }
 }
 
-template <int MODE>
+// TA: A is stored transposed (K x M, element (i, k) at A[k * lda + i]);
+// TB: B is stored transposed (N x K, element (k, j) at B[j * ldb + k]).
+template <int MODE, bool TA = false, bool TB = false>
 __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(int M, int N, int K, const float* A, int64_t lda,
                                                             const float* B, int64_t ldb, float* C, int64_t ldc,
                                                             int beta, float sign, int64_t sA, int64_t sB,
@@ -121,44 +123,49 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(int M, int N, int K,
       }
     }
 
-  // global -> register staging: A tile 128 x 32 and B tile 32 x 128, 4 float4 each per thread
+  // global -> register staging: A tile 128 x 32 and B tile 32 x 128, 4 float4 each per thread.
+  // An operand whose k runs along its rows in memory is read 4 consecutive k
+  // of one row per float4 (row idx / 8, k 4 (idx % 8)); one whose k runs down
+  // its columns is read 4 consecutive rows at one k (k idx / 32, row 4 (idx % 32)).
   float4 ra[4], rb[4];
+  auto load_k_along = [&](const float* P, int64_t ld, int r0, int rows, int k0, int idx) {
+    const int row = r0 + (idx >> 3), k = k0 + 4 * (idx & 7);
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (row < rows) {
+      const float* p = P + (int64_t)row * ld + k;
+      if (k + 3 < K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+        v = *reinterpret_cast<const float4*>(p);
+      } else {
+        v.x = k < K ? p[0] : 0.f;
+        v.y = k + 1 < K ? p[1] : 0.f;
+        v.z = k + 2 < K ? p[2] : 0.f;
+        v.w = k + 3 < K ? p[3] : 0.f;
+      }
+    }
+    return v;
+  };
+  auto load_k_down = [&](const float* P, int64_t ld, int r0, int rows, int k0, int idx) {
+    const int k = k0 + (idx >> 5), row = r0 + 4 * (idx & 31);
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (k < K) {
+      const float* p = P + (int64_t)k * ld + row;
+      if (row + 3 < rows && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+        v = *reinterpret_cast<const float4*>(p);
+      } else {
+        v.x = row < rows ? p[0] : 0.f;
+        v.y = row + 1 < rows ? p[1] : 0.f;
+        v.z = row + 2 < rows ? p[2] : 0.f;
+        v.w = row + 3 < rows ? p[3] : 0.f;
+      }
+    }
+    return v;
+  };
   auto load_tiles = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int idx = tid + kGemmThreads * q;
-      {  // A: row idx / 8, k 4 (idx % 8)
-        const int row = m0 + (idx >> 3), k = k0 + 4 * (idx & 7);
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (row < M) {
-          const float* p = A + (int64_t)row * lda + k;
-          if (k + 3 < K && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
-            v = *reinterpret_cast<const float4*>(p);
-          } else {
-            v.x = k < K ? p[0] : 0.f;
-            v.y = k + 1 < K ? p[1] : 0.f;
-            v.z = k + 2 < K ? p[2] : 0.f;
-            v.w = k + 3 < K ? p[3] : 0.f;
-          }
-        }
-        ra[q] = v;
-      }
-      {  // B: k idx / 32, col 4 (idx % 32)
-        const int k = k0 + (idx >> 5), col = n0 + 4 * (idx & 31);
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (k < K) {
-          const float* p = B + (int64_t)k * ldb + col;
-          if (col + 3 < N && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
-            v = *reinterpret_cast<const float4*>(p);
-          } else {
-            v.x = col < N ? p[0] : 0.f;
-            v.y = col + 1 < N ? p[1] : 0.f;
-            v.z = col + 2 < N ? p[2] : 0.f;
-            v.w = col + 3 < N ? p[3] : 0.f;
-          }
-        }
-        rb[q] = v;
-      }
+      ra[q] = TA ? load_k_down(A, lda, m0, M, k0, idx) : load_k_along(A, lda, m0, M, k0, idx);
+      rb[q] = TB ? load_k_along(B, ldb, n0, N, k0, idx) : load_k_down(B, ldb, n0, N, k0, idx);
     }
   };
   auto put = [&](E (*dst)[2][kBM][LDK], int buf, int r, int k, float v) {
@@ -166,20 +173,28 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(int M, int N, int K,
     dst[0][buf][r][k] = hi;
     if constexpr (kSplit) dst[1][buf][r][k] = (E)(v - (float)hi);
   };
+  // both tiles live in LDS as [row or column][k]
+  auto put4 = [&](E (*dst)[2][kBM][LDK], int buf, bool k_along, int idx, float4 v, float sc) {
+    if (k_along) {
+      const int r = idx >> 3, k = 4 * (idx & 7);
+      put(dst, buf, r, k + 0, sc * v.x);
+      put(dst, buf, r, k + 1, sc * v.y);
+      put(dst, buf, r, k + 2, sc * v.z);
+      put(dst, buf, r, k + 3, sc * v.w);
+    } else {
+      const int k = idx >> 5, r = 4 * (idx & 31);
+      put(dst, buf, r + 0, k, sc * v.x);
+      put(dst, buf, r + 1, k, sc * v.y);
+      put(dst, buf, r + 2, k, sc * v.z);
+      put(dst, buf, r + 3, k, sc * v.w);
+    }
+  };
   auto store_tiles = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int idx = tid + kGemmThreads * q;
-      const int ar = idx >> 3, ak = 4 * (idx & 7);
-      put(sa, buf, ar, ak + 0, sign * ra[q].x);
-      put(sa, buf, ar, ak + 1, sign * ra[q].y);
-      put(sa, buf, ar, ak + 2, sign * ra[q].z);
-      put(sa, buf, ar, ak + 3, sign * ra[q].w);
-      const int bk = idx >> 5, bc = 4 * (idx & 31);
-      put(sb, buf, bc + 0, bk, rb[q].x);
-      put(sb, buf, bc + 1, bk, rb[q].y);
-      put(sb, buf, bc + 2, bk, rb[q].z);
-      put(sb, buf, bc + 3, bk, rb[q].w);
+      put4(sa, buf, !TA, idx, ra[q], sign);
+      put4(sb, buf, TB, idx, rb[q], 1.f);
     }
   };
 
@@ -241,6 +256,18 @@ int gemm_update_batched(hipStream_t st, int mode, int batch, int M, int N, int K
       gemm_kernel<LXA_GEMM_F32><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign, sA, sB,
                                                               sC);
       break;
+    case LXA_GEMM_F32 | LXA_GEMM_TRANS_A:
+      gemm_kernel<LXA_GEMM_F32, true, false><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta,
+                                                                           sign, sA, sB, sC);
+      break;
+    case LXA_GEMM_F32 | LXA_GEMM_TRANS_B:
+      gemm_kernel<LXA_GEMM_F32, false, true><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta,
+                                                                           sign, sA, sB, sC);
+      break;
+    case LXA_GEMM_F32 | LXA_GEMM_TRANS_A | LXA_GEMM_TRANS_B:
+      gemm_kernel<LXA_GEMM_F32, true, true><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta,
+                                                                          sign, sA, sB, sC);
+      break;
     case LXA_GEMM_BF16X3:
       gemm_kernel<LXA_GEMM_BF16X3><<<grid, kGemmThreads, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, sign, sA,
                                                                  sB, sC);
@@ -265,6 +292,7 @@ extern "C" int lxa_gemm_sub(void* stream, int mode, int64_t m, int64_t n, int64_
               "invalid sizes m=%lld n=%lld k=%lld", (long long)m, (long long)n, (long long)k);
   if (m == 0 || n == 0 || k == 0) return LXA_OK;
   LXA_REQUIRE(a && b && c, "a, b and c must be non-NULL");
+  LXA_REQUIRE(mode >= LXA_GEMM_F32 && mode <= LXA_GEMM_BF16X3, "lxa_gemm_sub: unknown mode %d", mode);
   LXA_REQUIRE(lda >= k && ldb >= n && ldc >= n, "leading dimensions too small (lda=%lld ldb=%lld ldc=%lld)",
               (long long)lda, (long long)ldb, (long long)ldc);
   return lxa::gemm_update(static_cast<hipStream_t>(stream), mode, (int)m, (int)n, (int)k, a, lda, b, ldb, c, ldc, 1,
@@ -280,8 +308,13 @@ extern "C" int lxa_gemm_batched(void* stream, int mode, int64_t batch, int64_t m
               (long long)k);
   if (batch == 0 || m == 0 || n == 0) return LXA_OK;
   LXA_REQUIRE(a && b && c, "a, b and c must be non-NULL");
-  LXA_REQUIRE(lda >= k && ldb >= n && ldc >= n, "leading dimensions too small (lda=%lld ldb=%lld ldc=%lld)",
-              (long long)lda, (long long)ldb, (long long)ldc);
+  LXA_REQUIRE((mode & ~(LXA_GEMM_TRANS_A | LXA_GEMM_TRANS_B)) == LXA_GEMM_F32 ||
+                  (mode & (LXA_GEMM_TRANS_A | LXA_GEMM_TRANS_B)) == 0,
+              "transposed operands are supported in float32 mode only (mode %d)", mode);
+  const int64_t a_ld_min = (mode & LXA_GEMM_TRANS_A) ? m : k, b_ld_min = (mode & LXA_GEMM_TRANS_B) ? k : n;
+  LXA_REQUIRE(lda >= a_ld_min && ldb >= b_ld_min && ldc >= n,
+              "leading dimensions too small (lda=%lld ldb=%lld ldc=%lld)", (long long)lda, (long long)ldb,
+              (long long)ldc);
   LXA_REQUIRE(a_stride >= 0 && b_stride >= 0 && c_stride >= m * ldc, "bad batch strides");
   return lxa::gemm_update_batched(static_cast<hipStream_t>(stream), mode, (int)batch, (int)m, (int)n, (int)k, a, lda,
                                   a_stride, b, ldb, b_stride, c, ldc, c_stride, 0, 1.f);

@@ -143,6 +143,26 @@ def test_batched_gemm_vs_float64(device, rng, shape_a, shape_b):
     assert np.all(np.abs(c - exact) <= bound + 1e-30)
 
 
+@pytest.mark.parametrize("ta,tb", [(True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("shape_a,shape_b", [((37, 200), (200, 37)), ((130, 129), (129, 257)),
+                                             ((4, 64, 33), (4, 33, 70)), ((3, 20, 9), (9, 20)), ((1, 3), (3, 1))])
+def test_batched_gemm_transposed_operands(device, rng, shape_a, shape_b, ta, tb):
+    # transposed views (.mT of contiguous storage) go to the kernel as
+    # LXA_GEMM_TRANS_A / _B and are read in place; same bound as above
+    a = rng.standard_normal(shape_a)
+    b = rng.standard_normal(shape_b)
+    A = _t(np.swapaxes(a, -1, -2).copy(), device, torch.float32).mT if ta else _t(a, device, torch.float32)
+    B = _t(np.swapaxes(b, -1, -2).copy(), device, torch.float32).mT if tb else _t(b, device, torch.float32)
+    assert A.shape == a.shape and B.shape == b.shape
+    c = K.gemm(A, B).double().cpu().numpy()
+    a32 = a.astype(np.float32).astype(np.float64)
+    b32 = b.astype(np.float32).astype(np.float64)
+    exact = np.matmul(a32, b32)
+    bound = 4 * shape_a[-1] * np.finfo(np.float32).eps * np.matmul(np.abs(a32), np.abs(b32))
+    assert c.shape == exact.shape
+    assert np.all(np.abs(c - exact) <= bound + 1e-30)
+
+
 def test_normal_float32_gram_on_gemm_batched(device, rng):
     # tall float32 problems: the Gram matrix comes from the matrix-core GEMM,
     # the solve from the Cholesky kernels
