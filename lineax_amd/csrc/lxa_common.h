@@ -81,14 +81,17 @@ struct Log2<1> {
   static constexpr int value = 0;
 };
 
-// Sums K values (K a power of two <= 64) over the 64 lanes of a wave.  On
-// return, every lane holds the complete wave sum of value number
-// lane >> (6 - log2 K).  v is destroyed.
-template <int K, typename R>
-__device__ __forceinline__ R wave_reduce_scatter(R (&v)[K], int lane) {
-  int off = 32;
-#pragma unroll
-  for (int half = K / 2; half >= 1; half >>= 1) {
+// One halving level of wave_reduce_scatter, and the levels after it: lanes
+// with bit `off` set keep the upper half of v and add their partner's upper
+// half; the others keep the lower half.  The level count is a template
+// parameter so that every register index is a compile-time constant (a
+// runtime loop bound here turns each v[i + half] into a select chain over
+// all of v).
+template <int L, int K, typename R>
+__device__ __forceinline__ void reduce_scatter_levels(R (&v)[K], int lane) {
+  constexpr int half = K >> (L + 1);
+  if constexpr (half >= 1) {
+    constexpr int off = 32 >> L;
     const bool upper = (lane & off) != 0;
 #pragma unroll
     for (int i = 0; i < half; ++i) {
@@ -96,8 +99,17 @@ __device__ __forceinline__ R wave_reduce_scatter(R (&v)[K], int lane) {
       const R send = upper ? v[i] : v[i + half];
       v[i] = keep + __shfl_xor(send, off, kWave);
     }
-    off >>= 1;
+    reduce_scatter_levels<L + 1, K>(v, lane);
   }
+}
+
+// Sums K values (K a power of two <= 64) over the 64 lanes of a wave.  On
+// return, every lane holds the complete wave sum of value number
+// lane >> (6 - log2 K).  v is destroyed.
+template <int K, typename R>
+__device__ __forceinline__ R wave_reduce_scatter(R (&v)[K], int lane) {
+  static_assert(K >= 1 && K <= 64 && (K & (K - 1)) == 0, "K must be a power of two <= 64");
+  reduce_scatter_levels<0, K>(v, lane);
   R x = v[0];
 #pragma unroll
   for (int o = 32 / K; o >= 1; o >>= 1) x += __shfl_xor(x, o, kWave);

@@ -5,12 +5,13 @@
 // each panel step factors nb columns of every system and then updates the
 // systems' trailing columns:
 //
-//   panel   one 1024-thread workgroup per system.  The panel (rows k..m-1,
+//   panel   one 512-thread workgroup per system.  The panel (rows k..m-1,
 //           nb columns) lives in registers, up to eight rows x nb columns
-//           per thread of a 512-thread workgroup, i.e. 4096 rows (a
-//           global-memory variant covers taller panels).  Per column: one block reduction for the norm
-//           below the diagonal (larfg), one for v^T A over the remaining
-//           panel columns, which also carries v_a^T v_j for the T factor.
+//           per thread, i.e. 4096 rows (a global-memory variant covers
+//           taller panels).  Per column one block reduction carries the
+//           norm below the diagonal (larfg) and the raw products of that
+//           column with the other panel columns, from which v^T A (larf)
+//           and v_a^T v_j (for the T factor) follow.
 //           The workgroup ends by forming T (larft) and writing tau, T and
 //           the panel back.
 //   trail   C <- (I - V T V^T)^T C = C - V (T^T (V^T C)) on the columns
@@ -73,6 +74,46 @@ __device__ __forceinline__ void block_sum(R (&v)[K], R* stage, R* out, int tid) 
   for (int i = 0; i < K; ++i) v[i] = out[i];
 }
 
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Two block sums (K1 values of type R1, K2 of type R2 per thread) behind one
+// pair of barriers: the second array's cross-wave pass runs on another wave.
+template <int K1, int K2, int WAVES, typename R1, typename R2>
+__device__ __forceinline__ void block_sum2(R1 (&u)[K1], R1* stage1, R1* out1, R2 (&v)[K2], R2* stage2, R2* out2,
+                                           int tid) {
+  static_assert(WAVES >= 2, "the two cross-wave passes run on waves 0 and 1");
+  const int lane = tid & (kWave - 1);
+  const int wv = tid >> 6;
+  const R1 x1 = wave_reduce_scatter<K1>(u, lane);
+  const R2 x2 = wave_reduce_scatter<K2>(v, lane);
+  constexpr int kShift1 = 6 - Log2<K1>::value, kShift2 = 6 - Log2<K2>::value;
+  if ((lane & ((1 << kShift1) - 1)) == 0) stage1[wv * K1 + (lane >> kShift1)] = x1;
+  if ((lane & ((1 << kShift2) - 1)) == 0) stage2[wv * K2 + (lane >> kShift2)] = x2;
+  __syncthreads();
+  if (tid < K1) {
+    R1 t = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) t += stage1[w * K1 + tid];
+    out1[tid] = t;
+  } else if (tid >= kWave && tid < kWave + K2) {
+    R2 t = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) t += stage2[w * K2 + tid - kWave];
+    out2[tid - kWave] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < K1; ++i) u[i] = out1[i];
+#pragma unroll
+  for (int i = 0; i < K2; ++i) v[i] = out2[i];
+}
+
 // Householder vector entry j of panel-relative row (row - k): 1 on the
 // diagonal, the stored value below it, 0 above.
 template <typename T>
@@ -91,11 +132,12 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_de
                                                                  T* __restrict__ tblk) {
   constexpr int NB = QrCfg<T>::NB;
   constexpr int KW = NB;  // width of the per-column dot reduction (>= NB - 1)
-  __shared__ double s_stage2[kPanelWaves * 2];
-  __shared__ double s_out2[2];
+  __shared__ double s_stage2[kPanelWaves];
+  __shared__ double s_out2[1];
   __shared__ T s_stage[kPanelWaves * KW];
   __shared__ T s_out[KW];
-  __shared__ T s_g[NB][NB];  // s_g[a][j] = v_a^T v_j, a < j
+  __shared__ T s_diag[2][NB];  // the diagonal row of the current column
+  __shared__ T s_g[NB][NB];    // s_g[a][j] = v_a^T v_j, a < j
 
   const lxa_qr_desc d = descs[blockIdx.x];
   const int m = d.m, n = d.n, lda = d.lda;
@@ -126,24 +168,41 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_de
   }
 
   T taus[NB];
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
+  // the column loop at compile time (every register index a constant)
+  static_for<0, NB>([&](auto J_) __attribute__((always_inline)) {
+    constexpr int j = decltype(J_)::value;
     taus[j] = T(0);
-    if (j >= nb) continue;  // uniform
+    if (j >= nb) return;  // uniform
     const int diag = k + j;
-    // ---- larfg: norm of the column below the diagonal, and alpha
-    double nv[2] = {0.0, 0.0};
+    // ---- one block reduction per column.  With x = column j below the
+    // diagonal, it sums ||x||^2 and, for every other panel column c, the raw
+    // product x^T A(:, c) over the same rows; the diagonal row goes to LDS
+    // (thread j holds it).  v = [1; x / (alpha - beta)] then gives
+    //   v^T A(:, c)  = A(diag, c) + scale * x^T A(:, c)    (c > j, larf)
+    //   v_c^T v      = v_c(diag)  + scale * v_c^T x        (c < j, larft)
+    // so the norm and the products share one pair of barriers.
+    T* sd = s_diag[j & 1];  // double-buffered: readers of column j - 1 may lag
+    double nv[1] = {0.0};
+    T dots[KW];
+#pragma unroll
+    for (int c = 0; c < KW; ++c) dots[c] = T(0);
 #pragma unroll
     for (int r = 0; r < nr; ++r) {
       const int row = k + tid + r * kPanelThreads;
-      if (row < m) {
-        const double x = (double)P(r, j);
-        if (row > diag) nv[0] += x * x;
-        if (row == diag) nv[1] = x;
+      if (row < m && row > diag) {
+        const T x = P(r, j);
+        nv[0] += (double)x * (double)x;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c != j && c < nb) dots[c] += x * P(r, c);
       }
     }
-    block_sum<2, kPanelWaves>(nv, s_stage2, s_out2, tid);
-    const double ss = nv[0], alpha = nv[1];
+    if (tid == j) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) sd[c] = c < nb ? P(0, c) : T(0);
+    }
+    block_sum2<1, KW, kPanelWaves>(nv, s_stage2, s_out2, dots, s_stage, s_out, tid);
+    const double ss = nv[0], alpha = (double)sd[j];
     T tj = T(0), beta = (T)alpha, scale = T(1);
     if (ss != 0.0) {
       const double bd = -copysign(sqrt(alpha * alpha + ss), alpha);
@@ -152,47 +211,36 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_de
       scale = (T)(1.0 / (alpha - bd));
     }
     taus[j] = tj;
-#pragma unroll
-    for (int r = 0; r < nr; ++r) {
-      const int row = k + tid + r * kPanelThreads;
-      if (row < m) {
-        if (row > diag && ss != 0.0) P(r, j) = P(r, j) * scale;
-        if (row == diag) P(r, j) = beta;
-      }
-    }
-    // ---- w_c = v^T A(:, c) for c > j, and g_a = v_a^T v_j for a < j
     T acc[KW];
 #pragma unroll
-    for (int c = 0; c < KW; ++c) acc[c] = T(0);
-#pragma unroll
-    for (int r = 0; r < nr; ++r) {
-      const int row = k + tid + r * kPanelThreads;
-      if (row < m && row >= k) {
-        const T vj = vpick(row, diag, P(r, j));
-#pragma unroll
-        for (int c = 0; c < NB; ++c) {
-          if (c > j && c < nb) acc[c] += vj * P(r, c);
-          if (c < j) acc[c] += vj * vpick(row, k + c, P(r, c));
-        }
-      }
-    }
-    block_sum<KW, kPanelWaves>(acc, s_stage, s_out, tid);
+    for (int c = 0; c < KW; ++c) acc[c] = (c != j && c < nb) ? sd[c] + scale * dots[c] : T(0);
     if (tid == 0) {
 #pragma unroll
       for (int c = 0; c < NB; ++c)
         if (c < j) s_g[c][j] = acc[c];
     }
+    // ---- scale x into v, put beta on the diagonal, and apply
+    // (I - tau v v^T) to the panel columns right of j
 #pragma unroll
     for (int r = 0; r < nr; ++r) {
       const int row = k + tid + r * kPanelThreads;
-      if (row < m && row >= diag && tj != T(0)) {
-        const T tv = tj * vpick(row, diag, P(r, j));
+      if (row < m && row >= diag) {
+        T vj = T(1);
+        if (row > diag) {
+          vj = P(r, j) * scale;
+          if (ss != 0.0) P(r, j) = vj;
+        } else {
+          P(r, j) = beta;
+        }
+        if (tj != T(0)) {
+          const T tv = tj * vj;
 #pragma unroll
-        for (int c = 0; c < NB; ++c)
-          if (c > j && c < nb) P(r, c) -= tv * acc[c];
+          for (int c = 0; c < NB; ++c)
+            if (c > j && c < nb) P(r, c) -= tv * acc[c];
+        }
       }
     }
-  }
+  });
   __syncthreads();  // s_g complete
 
   // ---- T (larft, forward, columnwise): lane a owns row a of T.
@@ -241,19 +289,24 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_de
 template <typename T, int NB>
 __device__ __forceinline__ void load_v_chunk(T (*sv)[NB], const T* A, int lda, int m, int k, int nb, int r0,
                                              int tid) {
-  for (int i = tid; i < kTrailRows; i += kTrailThreads) {
-    const int row = r0 + i;
+  // NB consecutive lanes read one row's NB entries (contiguous in memory)
+  static_assert(kTrailThreads % NB == 0 && kTrailRows % (kTrailThreads / NB) == 0, "tile shape");
+  const int j = tid % NB;
+  const int diag = k + j;
+  constexpr int kPer = kTrailRows / (kTrailThreads / NB);
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      T v = T(0);
-      if (row < m && j < nb) {
-        const int diag = k + j;
-        v = row > diag ? A[(int64_t)row * lda + k + j] : (row == diag ? T(1) : T(0));
-      }
-      sv[i][j] = v;
-    }
+  for (int t = 0; t < kPer; ++t) {
+    const int i = tid / NB + t * (kTrailThreads / NB);
+    const int row = r0 + i;
+    T v = T(0);
+    if (row < m && j < nb) v = row > diag ? A[(int64_t)row * lda + k + j] : (row == diag ? T(1) : T(0));
+    sv[i][j] = v;
   }
 }
+
+// Rows of a trailing chunk handled per wave between waits: each wave issues
+// this many independent loads of C before using any of them.
+constexpr int kTrailBatch = 8;
 
 // (1) partial W = V^T C over one 256-row chunk and one 64-column block.
 template <typename T>
@@ -282,10 +335,23 @@ __global__ __launch_bounds__(kTrailThreads) void qr_trail_partial_kernel(const l
 #pragma unroll
   for (int j = 0; j < NB; ++j) acc[j] = T(0);
   const int rows = min(kTrailRows, m - r0);
-  for (int i = wv; i < rows; i += kTrailThreads / kWave) {
-    const T c = cv ? A[(int64_t)(r0 + i) * lda + col] : T(0);
+  constexpr int kStep = kTrailThreads / kWave;
+  for (int i0 = wv; i0 < rows; i0 += kTrailBatch * kStep) {
+    T cb[kTrailBatch];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) acc[j] += sv[i][j] * c;
+    for (int u = 0; u < kTrailBatch; ++u) {
+      const int i = i0 + u * kStep;
+      cb[u] = (cv && i < rows) ? A[(int64_t)(r0 + i) * lda + col] : T(0);
+    }
+    // rows in the same order as one at a time: the sums are unchanged
+#pragma unroll
+    for (int u = 0; u < kTrailBatch; ++u) {
+      const int i = i0 + u * kStep;
+      if (i < rows) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[j] += sv[i][j] * cb[u];
+      }
+    }
   }
 #pragma unroll
   for (int j = 0; j < NB; ++j) sp[wv][j][lane] = acc[j];
@@ -368,11 +434,24 @@ __global__ __launch_bounds__(kTrailThreads) void qr_trail_update_kernel(const lx
   __syncthreads();
   if (!cv) return;
   const int rows = min(kTrailRows, m - r0);
-  for (int i = wv; i < rows; i += kTrailThreads / kWave) {
-    T s = T(0);
+  constexpr int kStep = kTrailThreads / kWave;
+  for (int i0 = wv; i0 < rows; i0 += kTrailBatch * kStep) {
+    T cb[kTrailBatch];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) s += sv[i][j] * w[j];
-    A[(int64_t)(r0 + i) * lda + col] -= s;
+    for (int u = 0; u < kTrailBatch; ++u) {
+      const int i = i0 + u * kStep;
+      cb[u] = i < rows ? A[(int64_t)(r0 + i) * lda + col] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kTrailBatch; ++u) {
+      const int i = i0 + u * kStep;
+      if (i < rows) {
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) s += sv[i][j] * w[j];
+        A[(int64_t)(r0 + i) * lda + col] = cb[u] - s;
+      }
+    }
   }
 }
 
