@@ -137,6 +137,7 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_de
   __shared__ T s_stage[kPanelWaves * KW];
   __shared__ T s_out[KW];
   __shared__ T s_diag[2][NB];  // the diagonal row of the current column
+  __shared__ T s_rows[RPT > 0 ? kPanelThreads : 1][NB + 1];  // staging of 512 panel rows
   __shared__ T s_g[NB][NB];    // s_g[a][j] = v_a^T v_j, a < j
 
   const lxa_qr_desc d = descs[blockIdx.x];
@@ -159,11 +160,26 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_de
   };
 
   if constexpr (RPT > 0) {
+    // rows k + r * 512 + [0, 512) pass through LDS: NB consecutive lanes read
+    // one row's NB contiguous entries, then each thread takes its own row
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-      const int row = k + tid + r * kPanelThreads;
+      const int base = k + r * kPanelThreads;
+      if (base < m) {  // uniform
 #pragma unroll
-      for (int c = 0; c < NB; ++c) regs[r][c] = (row < m && c < nb) ? A[(int64_t)row * lda + k + c] : T(0);
+        for (int u = 0; u < NB; ++u) {
+          const int e = tid + kPanelThreads * u, rr = e / NB, c = e % NB;
+          const int row = base + rr;
+          s_rows[rr][c] = (row < m && c < nb) ? A[(int64_t)row * lda + k + c] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NB; ++c) regs[r][c] = s_rows[tid][c];
+        __syncthreads();
+      } else {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) regs[r][c] = T(0);
+      }
     }
   }
 
@@ -272,13 +288,21 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_de
   }
 
   if constexpr (RPT > 0) {
+    // the panel back through LDS, in the same coalesced order as the load
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-      const int row = k + tid + r * kPanelThreads;
-      if (row < m) {
+      const int base = k + r * kPanelThreads;
+      if (base < m) {  // uniform
+        __syncthreads();  // the previous chunk's readers are done
 #pragma unroll
-        for (int c = 0; c < NB; ++c)
-          if (c < nb) A[(int64_t)row * lda + k + c] = regs[r][c];
+        for (int c = 0; c < NB; ++c) s_rows[tid][c] = regs[r][c];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const int e = tid + kPanelThreads * u, rr = e / NB, c = e % NB;
+          const int row = base + rr;
+          if (row < m && c < nb) A[(int64_t)row * lda + k + c] = s_rows[rr][c];
+        }
       }
     }
   }
