@@ -54,7 +54,7 @@ from . import _kernels as K
 
 QR_ELEM = 4  # float32 bytes
 SMALL_CHOL = 256  # Cholesky orders served by the uneven tile-class launch
-PANELS_PER_TICK = 4  # QR panel steps per tick of the factor stream
+PANELS_PER_TICK = 16  # QR panel steps per tick of the factor stream (swept: 4, 8, 16)
 PLAIN_QR_BUCKET = 256  # rows per size bucket of the plain form's least-squares launches
 PLAIN_CHOL_BUCKET = 64  # orders per size bucket of the plain form's Cholesky launches
 SOLVE_STREAMS = 2  # with the factor stream and the default stream: the 4 hardware queues of a process
