@@ -28,6 +28,8 @@
 // reflector convention here is LAPACK's (larfg: beta = -sign(alpha) ||x||,
 // tau = (beta - alpha) / beta, v = x / (alpha - beta), tau = 0 when x = 0).
 
+#include <type_traits>
+
 #include "lxa_common.h"
 
 namespace lxa {
@@ -479,6 +481,116 @@ __global__ __launch_bounds__(kTrailThreads) void qr_trail_update_kernel(const lx
   }
 }
 
+// float32 forms of (1) and (3) on the matrix cores (v_mfma_f32_16x16x4f32,
+// float32 products and accumulation).  Operand layouts (lane l):
+//   A operand: A[l % 16][l / 16];  B operand: B[l / 16][l % 16];
+//   result:    rows 4 (l / 16) + i (i < 4), column l % 16.
+// Each wave of the 256-thread workgroup owns 16 of the block's 64 columns.
+typedef float qr_f4 __attribute__((ext_vector_type(4)));
+
+// (1) W(16 x 16 per wave) = V^T C over one 256-row chunk: A = V^T from LDS,
+// B = C streamed from HBM, four rows per MFMA.  No cross-wave reduction.
+__global__ __launch_bounds__(kTrailThreads) void qr_trail_partial_mfma_kernel(const lxa_qr_desc* __restrict__ descs,
+                                                                            int k_all,
+                                                                            const int32_t* __restrict__ kcol,
+                                                                            const float* __restrict__ a,
+                                                                            float* __restrict__ work) {
+  constexpr int NB = QrCfg<float>::NB;
+  static_assert(NB == 16 && kTrailCols == 64 && kTrailThreads == 256, "one 16-column slab per wave");
+  __shared__ float sv[kTrailRows][NB];
+  const lxa_qr_desc d = descs[blockIdx.z];
+  const int m = d.m, n = d.n, lda = d.lda;
+  const int kmax = min(m, n);
+  const int k = kcol ? kcol[blockIdx.z] : k_all;
+  if (k >= kmax) return;
+  const int nb = min(NB, kmax - k);
+  const int c0 = k + nb + kTrailCols * (int)blockIdx.x;
+  const int r0 = k + kTrailRows * (int)blockIdx.y;
+  if (c0 >= n || r0 >= m) return;
+  const int tid = threadIdx.x, l = tid & (kWave - 1), wv = tid >> 6;
+  const float* A = a + d.a_offset;
+  load_v_chunk<float, NB>(sv, A, lda, m, k, nb, r0, tid);
+  __syncthreads();
+  const int col = c0 + 16 * wv + (l & 15);
+  const bool cv = col < n;
+  const int rows = min(kTrailRows, m - r0);
+  qr_f4 acc = {0.f, 0.f, 0.f, 0.f};
+  // rows kk + l / 16 of the chunk; eight MFMA row groups (32 rows) per batch
+  constexpr int kGroups = 8;
+  for (int kb = 0; kb < rows; kb += 4 * kGroups) {
+    float cb[kGroups];
+#pragma unroll
+    for (int u = 0; u < kGroups; ++u) {
+      const int i = kb + 4 * u + (l >> 4);
+      cb[u] = (cv && i < rows) ? A[(int64_t)(r0 + i) * lda + col] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kGroups; ++u) {
+      const int i = kb + 4 * u + (l >> 4);
+      const float av = i < kTrailRows ? sv[i][l & 15] : 0.f;  // zero past the chunk (and past m)
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, cb[u], acc, 0, 0, 0);
+    }
+  }
+  float* W = work + d.w_offset + (int64_t)blockIdx.y * NB * n;
+  if (cv) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) W[(int64_t)(4 * (l >> 4) + i) * n + col] = acc[i];
+  }
+}
+
+// (3) C -= V W: each wave takes its 16 columns down the chunk in 16 x 16
+// tiles; the tile is loaded into the accumulator, and V (from LDS) times -W
+// (registers) is added by four MFMAs.
+__global__ __launch_bounds__(kTrailThreads) void qr_trail_update_mfma_kernel(const lxa_qr_desc* __restrict__ descs,
+                                                                           int k_all,
+                                                                           const int32_t* __restrict__ kcol,
+                                                                           float* __restrict__ a,
+                                                                           const float* __restrict__ work) {
+  constexpr int NB = QrCfg<float>::NB;
+  __shared__ float sv[kTrailRows][NB];
+  const lxa_qr_desc d = descs[blockIdx.z];
+  const int m = d.m, n = d.n, lda = d.lda;
+  const int kmax = min(m, n);
+  const int k = kcol ? kcol[blockIdx.z] : k_all;
+  if (k >= kmax) return;
+  const int nb = min(NB, kmax - k);
+  const int c0 = k + nb + kTrailCols * (int)blockIdx.x;
+  const int r0 = k + kTrailRows * (int)blockIdx.y;
+  if (c0 >= n || r0 >= m) return;
+  const int tid = threadIdx.x, l = tid & (kWave - 1), wv = tid >> 6;
+  float* A = a + d.a_offset;
+  load_v_chunk<float, NB>(sv, A, lda, m, k, nb, r0, tid);
+  const int col = c0 + 16 * wv + (l & 15);
+  const bool cv = col < n;
+  // -W as the B operand: rows j = 4 s + l / 16 of this wave's columns
+  const float* W = work + d.w_offset;
+  float wb[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) wb[q] = cv ? -W[(int64_t)(4 * q + (l >> 4)) * n + col] : 0.f;
+  __syncthreads();
+  const int rows = min(kTrailRows, m - r0);
+  for (int rt = 0; rt < rows; rt += 16) {
+    qr_f4 acc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = rt + 4 * (l >> 4) + i;
+      acc[i] = (cv && row < rows) ? A[(int64_t)(r0 + row) * lda + col] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float av = sv[rt + (l & 15)][4 * q + (l >> 4)];
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, wb[q], acc, 0, 0, 0);
+    }
+    if (cv) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = rt + 4 * (l >> 4) + i;
+        if (row < rows) A[(int64_t)(r0 + row) * lda + col] = acc[i];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // b <- Q^T b (trans) or Q b: one workgroup per (system, right-hand side),
 // panel by panel: w = V^T b; w = T^T w (or T w); b -= V w.
@@ -861,11 +973,21 @@ int qr_step_impl(hipStream_t st, int64_t count, const lxa_qr_desc* desc, int k_a
   if (max_trail_cols <= 0) return LXA_OK;
   const unsigned cb = (unsigned)((max_trail_cols + kTrailCols - 1) / kTrailCols);
   const unsigned rc = (unsigned)((max_rows + kTrailRows - 1) / kTrailRows);
-  qr_trail_partial_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, a, work);
+  if constexpr (std::is_same_v<T, float>) {
+    qr_trail_partial_mfma_kernel<<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, a,
+                                                                                          work);
+  } else {
+    qr_trail_partial_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, a, work);
+  }
   LXA_LAUNCH_CHECK("qr_trail_partial_kernel");
   qr_trail_reduce_kernel<T><<<dim3(cb, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, tblk, work);
   LXA_LAUNCH_CHECK("qr_trail_reduce_kernel");
-  qr_trail_update_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, a, work);
+  if constexpr (std::is_same_v<T, float>) {
+    qr_trail_update_mfma_kernel<<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, a,
+                                                                                         work);
+  } else {
+    qr_trail_update_kernel<T><<<dim3(cb, rc, (unsigned)count), kTrailThreads, 0, st>>>(desc, k_all, kcol, a, work);
+  }
   LXA_LAUNCH_CHECK("qr_trail_update_kernel");
   return LXA_OK;
 }
