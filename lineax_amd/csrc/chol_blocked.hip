@@ -8,7 +8,8 @@
 //       floats), solved column-parallel in place, written back to A and kept
 //       in LDS for the update;
 //   (3) the trailing upper triangle takes the rank-kb update
-//       A22 -= U12^T U12 in 4 x 4 register micro-tiles reading U12 from LDS.
+//       A22 -= U12^T U12 in 16 x 16 tiles on the matrix cores, U12 read
+//       from LDS as both MFMA operands.
 // The trailing matrix is read and written once per panel (from L2 / HBM).
 // A non-positive pivot fills the factor with NaN and sets info = k + 1, as
 // lxa_chol_factor does.  Only the upper triangle is read; the strictly lower
@@ -25,8 +26,10 @@ namespace {
 constexpr int kP = 32;           // panel width
 constexpr int kThreads = 256;
 constexpr int kMaxN = 1024;
-constexpr int kPanelLd = kMaxN;  // LDS row stride of the U12 panel
+constexpr int kPanelLd = kMaxN + 16;  // LDS row stride of the U12 panel (rows start 16 banks apart)
 constexpr int kDLd = kP + 4;     // LDS row stride of the diagonal block (16-byte rows)
+
+typedef float cb_f4 __attribute__((ext_vector_type(4)));
 
 struct Geometry {
   int n;
@@ -125,36 +128,33 @@ __global__ __launch_bounds__(kThreads) void chol_blocked_factor_kernel(const lxa
       const int i = idx / rest, c = idx % rest;
       A[(int64_t)(k0 + i) * lda + k0 + kb + c] = P[i * kPanelLd + c];
     }
-    // (3) A22 -= U12^T U12 on the upper triangle, 4 x 4 micro-tiles
-    const int T4 = (rest + 3) / 4;
-    for (int idx = tid; idx < T4 * T4; idx += kThreads) {
-      const int bi = idx / T4, bj = idx % T4;
-      if (bj < bi) continue;
-      float acc[4][4];
+    // (3) A22 -= U12^T U12 on the upper triangle in 16 x 16 tiles on the
+    // matrix cores (v_mfma_f32_16x16x4f32, float32 products and sums; lane l:
+    // A operand [l % 16][l / 16], B operand [l / 16][l % 16], result rows
+    // 4 (l / 16) + i, column l % 16).  Tiles go to the waves in turn; the K
+    // dimension is the panel's 32 rows of U12 (rows past kb are zero).  Only
+    // entries on or above the diagonal are read or written.
+    const int T16 = (rest + 15) / 16;
+    const int g = lane >> 4, q = lane & 15;
+    for (int tix = wv; tix < T16 * T16; tix += kThreads / 64) {
+      const int bi = tix / T16, bj = tix % T16;
+      if (bj < bi) continue;  // uniform per wave
+      const int j = 16 * bj + q;
+      cb_f4 acc;
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = 0.f;
-      for (int t = 0; t < kb; ++t) {
-        const float4 u = *reinterpret_cast<const float4*>(&P[t * kPanelLd + 4 * bi]);
-        const float4 v = *reinterpret_cast<const float4*>(&P[t * kPanelLd + 4 * bj]);
-        const float ur[4] = {u.x, u.y, u.z, u.w};
-        const float vc[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) acc[r][c] += ur[r] * vc[c];
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * bi + 4 * g + i;
+        acc[i] = (j < rest && j >= r) ? A[(int64_t)(k0 + kb + r) * lda + k0 + kb + j] : 0.f;
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 4 * bi + r;
-        if (i >= rest) continue;
-        float* Ar = A + (int64_t)(k0 + kb + i) * lda + k0 + kb;
+      for (int t0 = 0; t0 < kP; t0 += 4) {
+        const float* Pt = P + (t0 + g) * kPanelLd;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-Pt[16 * bi + q], Pt[j], acc, 0, 0, 0);
+      }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int j = 4 * bj + c;
-          if (j >= i && j < rest) Ar[j] -= acc[r][c];
-        }
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * bi + 4 * g + i;
+        if (j < rest && j >= r) A[(int64_t)(k0 + kb + r) * lda + k0 + kb + j] = acc[i];
       }
     }
     __syncthreads();

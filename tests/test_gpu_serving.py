@@ -48,7 +48,13 @@ def test_chol_blocked_uneven_vs_oracle(device, rng):
     b = torch.as_tensor(np.concatenate(rhs), device=device)
     info = torch.full((len(ns),), 7, dtype=torch.int32, device=device)
     dd = K.descs_to_device(d, device)
+    a0 = a.clone()
     K.chol_blocked_factor_raw(dd, len(ns), a, info)
+    # the strictly lower triangles are not touched
+    for i, n in enumerate(ns):
+        o = int(d[i]["a_offset"])
+        low = torch.tril(torch.ones(n, n, dtype=torch.bool, device=device), -1)
+        assert torch.equal(a[o: o + n * n].view(n, n)[low], a0[o: o + n * n].view(n, n)[low])
     K.chol_blocked_solve_raw(dd, len(ns), a, b)
     assert info.cpu().tolist() == [0] * len(ns)
     x = b.cpu().numpy().astype(np.float64)
