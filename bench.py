@@ -248,6 +248,9 @@ CONFIG_LEGS = [
     # a rate below the loop's capacity, arrivals over 5 s: the latency of a
     # stable queue rather than of a backlog
     ("serving_sustained_600_per_s", "serving", {"rate": 600.0, "requests": 3000, "serving_forms": "chosen"}),
+    # the configuration's 2k/s for 5 s: the rate the loop serves under that
+    # load and the latency of its queue
+    ("serving_sustained_2k_per_s", "serving", {"rate": 2000.0, "requests": 10000, "serving_forms": "chosen"}),
 ]
 
 _SUMMARY_KEYS = ("metric", "value", "unit", "ms_per_step", "p50_ms", "p99_ms", "dtype", "config", "roofline",

@@ -159,7 +159,10 @@ class SolveServer:
         if panels_per_tick < 1:
             raise ValueError("panels_per_tick must be >= 1")
         self.panels_per_tick = int(panels_per_tick)
-        self._uneven_ws = torch.zeros(256, dtype=torch.uint8, device=self.device)
+        # the tile-class launch keeps its work-queue counters in this scratch:
+        # one per solve stream, since launches on different streams overlap
+        # (a shared one lets one launch take the other's queue entries)
+        self._uneven_ws = [torch.zeros(256, dtype=torch.uint8, device=self.device) for _ in range(SOLVE_STREAMS)]
         self.ticks = 0
         self.tick_log: list[dict] = []
 
@@ -528,7 +531,8 @@ class SolveServer:
                 continue
             qr_done, chol_new, solve_reuse, chol_small = [], [], [], []
             still, blocked = [], False
-            sstream = sstreams[self.ticks % len(sstreams)]
+            si = self.ticks % len(sstreams)
+            sstream = sstreams[si]
             with torch.cuda.stream(fstream):
                 for r in waiting:
                     if blocked:
@@ -630,7 +634,7 @@ class SolveServer:
                     xs = self._solutions(batch)
                     batch = [(r, x, item, rb) for (r, item, rb), x in zip(batch, xs)]
                 if chol_small:
-                    batch += [(r, x, None, None) for r, x in zip(chol_small, self._chol_small_solve(chol_small))]
+                    batch += [(r, x, None, None) for r, x in zip(chol_small, self._chol_small_solve(chol_small, si))]
                 if batch:
                     ev = torch.cuda.Event()
                     ev.record(sstream)
@@ -643,11 +647,13 @@ class SolveServer:
         while inflight:
             finish_batches(True)
 
-    def _chol_small_solve(self, reqs):
+    def _chol_small_solve(self, reqs, stream_slot: int = 0):
         """Cholesky requests of order <= SMALL_CHOL: one lxa_chol_uneven_solve
         (persistent tile-class launches on the matrix cores, FP64 refinement to
         eta <= 1e-12) reading each operator where it lives.  Returns float64
-        solutions, on the current stream."""
+        solutions, on the current stream.  stream_slot picks the work-queue
+        scratch of the solve stream the call runs on: calls on different
+        streams must not share one."""
         dev = self.device
         ops = [r.a if r.a is not None else self._operator_data[r.op_id] for r in reqs]
         ptr = np.array([a.data_ptr() for a in ops], dtype=np.int64)
@@ -667,7 +673,8 @@ class SolveServer:
         info = torch.empty(cnt, dtype=torch.int32, device=dev)
         a_all = torch.as_strided(base_t, (1,), (1,))  # base pointer; offsets reach every operator
         K.chol_uneven_solve(n_d, a_off_d, a_all, b_off_d, b, x, order_d, max_n=int(n.max()), persistent=True,
-                            max_refine=10, tol=1e-12, info=info, workspace=self._uneven_ws, sorted_order=True)
+                            max_refine=10, tol=1e-12, info=info, workspace=self._uneven_ws[stream_slot],
+                            sorted_order=True)
         return [x[int(o): int(o) + int(k)] for o, k in zip(b_off, n)]
 
     def _retire(self, r, item, rhs_blocks, owner_pending):

@@ -244,3 +244,38 @@ def test_static_and_fifo_plain_forms_agree(device):
     for xs, xf in zip(out["plain"], out["fifo"]):
         scale = float(torch.max(torch.abs(xf)))
         assert float(torch.max(torch.abs(xs - xf))) <= 1e-5 * max(1.0, scale)
+
+
+def test_small_cholesky_launches_on_two_streams(device):
+    # The loop alternates its solve streams, so the tile-class launches of
+    # consecutive ticks overlap on the GPU; each stream has its own
+    # work-queue scratch (one shared scratch let a launch take the other's
+    # queue entries and leave those systems unsolved).
+    srv = S.SolveServer(device, form="chosen", hbm_budget=1 << 28)
+
+    def batch(seed, count):
+        g = np.random.default_rng(seed)
+        reqs = []
+        for i in range(count):
+            n = int(g.integers(129, 257))
+            m = g.standard_normal((n, n))
+            a = torch.as_tensor((m @ m.T / n + 0.1 * np.eye(n)).astype(np.float32), device=device)
+            b = torch.as_tensor(g.standard_normal(n).astype(np.float32), device=device)
+            reqs.append(S.Request(i, "chol", 0.0, n, n, i, False, a, b))
+        return reqs
+
+    r1, r2 = batch(1, 600), batch(2, 600)
+    s1, s2 = torch.cuda.Stream(device), torch.cuda.Stream(device)
+    torch.cuda.synchronize(device)
+    with torch.cuda.stream(s1):
+        x1 = srv._chol_small_solve(r1, 0)
+    with torch.cuda.stream(s2):
+        x2 = srv._chol_small_solve(r2, 1)
+    torch.cuda.synchronize(device)
+    worst = 0.0
+    for reqs, xs in ((r1, x1), (r2, x2)):
+        for r, x in zip(reqs, xs):
+            a, b, xv = r.a.double(), r.b.double(), x.double()
+            den = a.abs().sum(1).max() * xv.abs().max() + b.abs().max()
+            worst = max(worst, float(((b - a @ xv).abs().max() / den).item()))
+    assert worst <= 1e-12
