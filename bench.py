@@ -878,6 +878,8 @@ def bench_serving(args):
         del reqs, ops, server
         torch.cuda.empty_cache()
     c, p = out["chosen"], out.get("plain") or {}
+    errs = [e["max"] for e in (c.get("backward_error") or {}).values()]
+    eta_max = max(errs) if errs else None
     return _line("solves/s + p50/p99 latency at equal FP64-oracle error, chosen vs plain form", c["solves_per_s"],
                  "solves/s", c["makespan_s"] * 1e3, args,
                  {"workload": "serving", "rate_per_s": args.rate, "requests": args.requests, "seed": 5,
@@ -895,6 +897,8 @@ def bench_serving(args):
                   "arrival_span_s": c.get("arrival_span_s"),
                   "p50_first_quarter_ms": c.get("p50_first_quarter_ms"),
                   "p50_last_quarter_ms": c.get("p50_last_quarter_ms"),
+                  # every refined answer against the float64 data (eta_LS for least squares)
+                  "eta_max": eta_max, "eta_met": eta_max is not None and eta_max <= 1e-12,
                   "speedup_vs_plain": c["solves_per_s"] / p["solves_per_s"] if p.get("solves_per_s") else None})
 
 
