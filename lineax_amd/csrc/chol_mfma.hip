@@ -288,6 +288,10 @@ __device__ __forceinline__ void load_tiles(Tiles<NB, WAVES>& T, const float* __r
   using L = Lay<NB, WAVES>;
   const __amdgpu_buffer_rsrc_t rs = matrix_rsrc(A, n);
   const int w = wave_id(), l = threadIdx.x & 63, g = l >> 4, q = l & 15;
+  // Every load is issued first, straight into the tiles (blocks past the
+  // matrix read the out-of-range offset, which returns 0), and the identity
+  // padding is added afterwards: with the add next to each load the
+  // compiler waited for every load before issuing the next.
 #pragma unroll
   for (int t = 0; t < L::kPPW; ++t) {
     const int p = pair_of<WAVES>(w, t);
@@ -295,16 +299,29 @@ __device__ __forceinline__ void load_tiles(Tiles<NB, WAVES>& T, const float* __r
     for (int k = 0; k < L::kTPP; ++k) {
       int I, J;
       slot_block<NB>(p, k, I, J);
-      if (J < nk) {
+      const int j = 16 * J + q;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) T[t][k][r] = upper_at(rs, n, 16 * I + 4 * g + r, j);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < L::kPPW; ++t) {
+    const int p = pair_of<WAVES>(w, t);
+#pragma unroll
+    for (int k = 0; k < L::kTPP; ++k) {
+      int I, J;
+      slot_block<NB>(p, k, I, J);
+      if (I == J && 16 * J + 16 > n) {  // uniform: the diagonal block holding the padding
         const int j = 16 * J + q;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int i = 16 * I + 4 * g + r;
-          T[t][k][r] = upper_at(rs, n, i, j) + ((i == j && i >= n) ? 1.0f : 0.0f);  // identity padding
+          if (i == j && i >= n) T[t][k][r] += 1.0f;
         }
       }
     }
   }
+  (void)nk;
 }
 
 // The right-looking factorisation (see the file comment).  Returns 0 or the
