@@ -236,22 +236,27 @@ void chol_factor_regs_kernel(int n, float* __restrict__ a, int64_t a_stride, int
   const int N16 = round16(n);
   const int nk = N16 >> 4;
   float R[KR][KR];
+  // every load unconditional (outside the upper triangle from element 0, then
+  // discarded), so that all of them are in flight before the first use; in
+  // branches the compiler waited for each one
 #pragma unroll
   for (int aa = 0; aa < KR; ++aa) {
 #pragma unroll
     for (int bb = aa; bb < KR; ++bb) {
-      R[aa][bb] = 0.0f;
-      if (aa < nk && bb < nk) {
-        const int i = ty + 16 * aa, j = tx + 16 * bb;
-        if (i < n && j < n) {
-          if (j >= i) {
-            const float v = A[static_cast<int64_t>(i) * n + j];
-            R[aa][bb] = negate ? -v : v;
-          }
-        } else if (i == j) {
-          R[aa][bb] = 1.0f;
-        }
-      }
+      const int i = ty + 16 * aa, j = tx + 16 * bb;
+      const bool in = aa < nk && bb < nk && i < n && j < n && j >= i;
+      R[aa][bb] = A[in ? static_cast<int64_t>(i) * n + j : 0];
+    }
+  }
+#pragma unroll
+  for (int aa = 0; aa < KR; ++aa) {
+#pragma unroll
+    for (int bb = aa; bb < KR; ++bb) {
+      const int i = ty + 16 * aa, j = tx + 16 * bb;
+      const bool tile = aa < nk && bb < nk;
+      const bool in = tile && i < n && j < n && j >= i;
+      const float pad = (tile && (i >= n || j >= n) && i == j) ? 1.0f : 0.0f;  // identity padding
+      R[aa][bb] = in ? (negate ? -R[aa][bb] : R[aa][bb]) : pad;
     }
   }
   const int fail = factor_registers<KR>(R, n, s);

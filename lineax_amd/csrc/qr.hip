@@ -168,11 +168,18 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(const lxa_qr_de
     for (int r = 0; r < RPT; ++r) {
       const int base = k + r * kPanelThreads;
       if (base < m) {  // uniform
+        // all NB loads in flight before any LDS write
+        T v[NB];
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
           const int e = tid + kPanelThreads * u, rr = e / NB, c = e % NB;
           const int row = base + rr;
-          s_rows[rr][c] = (row < m && c < nb) ? A[(int64_t)row * lda + k + c] : T(0);
+          v[u] = (row < m && c < nb) ? A[(int64_t)row * lda + k + c] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const int e = tid + kPanelThreads * u;
+          s_rows[e / NB][e % NB] = v[u];
         }
         __syncthreads();
 #pragma unroll
