@@ -80,12 +80,13 @@ __global__ __launch_bounds__(256) void stencil5_rows4_kernel(int nx, int ny, Coe
   if (j0 >= ny) return;
   const TI* X = x + row * xs;
   TO* Y = y + row * ys;
+  // loads are unconditional (clamped rows and columns, then zeroed outside
+  // the grid): in branches the compiler waited for each load in turn
   auto ldrow = [&](int i, double (&o)[4]) {
-    if (i < 0 || i >= nx) {
-      o[0] = o[1] = o[2] = o[3] = 0.0;
-    } else {
-      load4(X + static_cast<int64_t>(i) * ny + j0, o);
-    }
+    const bool in = i >= 0 && i < nx;
+    load4(X + static_cast<int64_t>(in ? i : 0) * ny + j0, o);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) o[e] = in ? o[e] : 0.0;
   };
   double up[4], cu[4];
   ldrow(i0 - 1, up);
@@ -97,8 +98,10 @@ __global__ __launch_bounds__(256) void stencil5_rows4_kernel(int nx, int ny, Coe
       ldrow(ib + 1 + u, q[u]);
       const int i = ib + u;
       const TI* Xi = X + static_cast<int64_t>(i < i1 ? i : i1 - 1) * ny;
-      we[u][0] = j0 > 0 ? static_cast<double>(Xi[j0 - 1]) : 0.0;
-      we[u][1] = j0 + 4 < ny ? static_cast<double>(Xi[j0 + 4]) : 0.0;
+      const double wl = static_cast<double>(Xi[j0 > 0 ? j0 - 1 : 0]);
+      const double wr = static_cast<double>(Xi[j0 + 4 < ny ? j0 + 4 : ny - 1]);
+      we[u][0] = j0 > 0 ? wl : 0.0;
+      we[u][1] = j0 + 4 < ny ? wr : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kAhead; ++u) {
