@@ -470,6 +470,8 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
       mine = mine || p == K || NB - 1 - p == K;
     }
     if (mine) {
+      // every slot reads its x_J (J < NB: inside C) without a branch, so the
+      // LDS reads are in flight together; slots outside row K contribute 0
       float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int t = 0; t < L::kPPW; ++t) {
@@ -478,11 +480,10 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
         for (int k = 0; k < L::kTPP; ++k) {
           int Ib, J;
           slot_block<NB>(p, k, Ib, J);
-          if (Ib == K && J > K && J < nk) {
-            const float xj = C[16 * J + col_l];
+          const float xr = C[16 * J + col_l];
+          const float xj = (Ib == K && J > K && J < nk) ? xr : 0.0f;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[r] = fmaf(T[t][k][r], xj, acc[r]);
-          }
+          for (int r = 0; r < 4; ++r) acc[r] = fmaf(T[t][k][r], xj, acc[r]);
         }
       }
 #pragma unroll
