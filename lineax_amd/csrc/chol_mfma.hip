@@ -96,17 +96,37 @@ __device__ __forceinline__ double row_sum16(double v) {
 }
 
 // Sum over the four rows of 16 (lanes q, q+16, q+32, q+48), in every lane.
-__device__ __forceinline__ float group_sum4(float v) {
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
-  return v;
+// v_permlane16_swap / v_permlane32_swap of a register with itself leave
+// (value of lane l & ~16, value of lane l | 16) (resp. bit 32) in lane l: the
+// lane's own value and its partner's, so their sum is v + partner, exactly as
+// an exchange through LDS would give, in two VALU instructions.
+__device__ __forceinline__ float pair_sum16(float v) {
+  const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(s[0]) + __uint_as_float(s[1]);
+}
+__device__ __forceinline__ float pair_sum32(float v) {
+  const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(s[0]) + __uint_as_float(s[1]);
+}
+__device__ __forceinline__ double pair_sum16(double v) {
+  const auto lo = __builtin_amdgcn_permlane16_swap(static_cast<unsigned>(__double2loint(v)),
+                                                   static_cast<unsigned>(__double2loint(v)), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(static_cast<unsigned>(__double2hiint(v)),
+                                                   static_cast<unsigned>(__double2hiint(v)), false, false);
+  return __hiloint2double(static_cast<int>(hi[0]), static_cast<int>(lo[0])) +
+         __hiloint2double(static_cast<int>(hi[1]), static_cast<int>(lo[1]));
+}
+__device__ __forceinline__ double pair_sum32(double v) {
+  const auto lo = __builtin_amdgcn_permlane32_swap(static_cast<unsigned>(__double2loint(v)),
+                                                   static_cast<unsigned>(__double2loint(v)), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(static_cast<unsigned>(__double2hiint(v)),
+                                                   static_cast<unsigned>(__double2hiint(v)), false, false);
+  return __hiloint2double(static_cast<int>(hi[0]), static_cast<int>(lo[0])) +
+         __hiloint2double(static_cast<int>(hi[1]), static_cast<int>(lo[1]));
 }
 
-__device__ __forceinline__ double group_sum4(double v) {
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
-  return v;
-}
+__device__ __forceinline__ float group_sum4(float v) { return pair_sum32(pair_sum16(v)); }
+__device__ __forceinline__ double group_sum4(double v) { return pair_sum32(pair_sum16(v)); }
 
 __host__ __device__ constexpr int round16(int n) { return (n + 15) & ~15; }
 
