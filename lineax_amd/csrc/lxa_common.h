@@ -81,6 +81,48 @@ struct Log2<1> {
   static constexpr int value = 0;
 };
 
+// The value of lane (this lane xor OFF), for a power-of-two OFF < 64, by
+// register crossbar moves instead of an LDS exchange: v_permlane32_swap /
+// v_permlane16_swap for 32 and 16 (swapping a register with itself leaves
+// the values of lanes l & ~OFF and l | OFF), DPP row_ror:8 for 8, row_shr:4 /
+// row_shl:4 for 4, quad_perm for 2 and 1.  Every lane of the wave must be
+// active (the moves read source lanes whatever EXEC says).
+template <int OFF>
+__device__ __forceinline__ unsigned lane_xor_u32(unsigned v) {
+  static_assert(OFF == 32 || OFF == 16 || OFF == 8 || OFF == 4 || OFF == 2 || OFF == 1, "power of two < 64");
+  const int lane = __lane_id();
+  if constexpr (OFF == 32) {
+    const auto s = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32) ? s[0] : s[1];
+  } else if constexpr (OFF == 16) {
+    const auto s = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16) ? s[0] : s[1];
+  } else if constexpr (OFF == 8) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  } else if constexpr (OFF == 4) {
+    const unsigned from_below = __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    const unsigned from_above = __builtin_amdgcn_update_dpp(0u, v, 0x104, 0xF, 0xF, false);  // row_shl:4
+    return (lane & 4) ? from_below : from_above;
+  } else if constexpr (OFF == 2) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else {
+    return __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  }
+}
+
+template <int OFF, typename T>
+__device__ __forceinline__ T lane_xor(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, lane_xor_u32<OFF>(__builtin_bit_cast(unsigned, v)));
+  } else {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = lane_xor_u32<OFF>(static_cast<unsigned>(u));
+    const unsigned hi = lane_xor_u32<OFF>(static_cast<unsigned>(u >> 32));
+    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
+  }
+}
+
 // One halving level of wave_reduce_scatter, and the levels after it: lanes
 // with bit `off` set keep the upper half of v and add their partner's upper
 // half; the others keep the lower half.  The level count is a template
@@ -97,7 +139,7 @@ __device__ __forceinline__ void reduce_scatter_levels(R (&v)[K], int lane) {
     for (int i = 0; i < half; ++i) {
       const R keep = upper ? v[i + half] : v[i];
       const R send = upper ? v[i] : v[i + half];
-      v[i] = keep + __shfl_xor(send, off, kWave);
+      v[i] = keep + lane_xor<off>(send);
     }
     reduce_scatter_levels<L + 1, K>(v, lane);
   }
@@ -105,14 +147,19 @@ __device__ __forceinline__ void reduce_scatter_levels(R (&v)[K], int lane) {
 
 // Sums K values (K a power of two <= 64) over the 64 lanes of a wave.  On
 // return, every lane holds the complete wave sum of value number
-// lane >> (6 - log2 K).  v is destroyed.
+// lane >> (6 - log2 K).  v is destroyed.  Every lane must be active.
 template <int K, typename R>
 __device__ __forceinline__ R wave_reduce_scatter(R (&v)[K], int lane) {
   static_assert(K >= 1 && K <= 64 && (K & (K - 1)) == 0, "K must be a power of two <= 64");
   reduce_scatter_levels<0, K>(v, lane);
   R x = v[0];
-#pragma unroll
-  for (int o = 32 / K; o >= 1; o >>= 1) x += __shfl_xor(x, o, kWave);
+  // the remaining butterfly levels, offsets 32 / K down to 1
+  if constexpr (32 / K >= 32) x += lane_xor<32>(x);
+  if constexpr (32 / K >= 16) x += lane_xor<16>(x);
+  if constexpr (32 / K >= 8) x += lane_xor<8>(x);
+  if constexpr (32 / K >= 4) x += lane_xor<4>(x);
+  if constexpr (32 / K >= 2) x += lane_xor<2>(x);
+  if constexpr (32 / K >= 1) x += lane_xor<1>(x);
   return x;
 }
 
