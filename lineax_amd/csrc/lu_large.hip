@@ -1033,10 +1033,87 @@ __global__ __launch_bounds__(256) void gemv_f32a_kernel(int m, int k, const floa
   }
 }
 
+// The same product through 16-byte loads, for 16-byte aligned rows of A and
+// x with k a multiple of four: each lane takes four consecutive columns of
+// the wave's four rows per step (4 KB of A per wave per step, as against
+// 1 KB with a float per lane).
+__device__ __forceinline__ void gemv_x4(const float* p, double (&o)[4]) {
+  const float4 q = *reinterpret_cast<const float4*>(p);
+  o[0] = q.x;
+  o[1] = q.y;
+  o[2] = q.z;
+  o[3] = q.w;
+}
+__device__ __forceinline__ void gemv_x4(const double* p, double (&o)[4]) {
+  const double2 q0 = *reinterpret_cast<const double2*>(p);
+  const double2 q1 = *reinterpret_cast<const double2*>(p + 2);
+  o[0] = q0.x;
+  o[1] = q0.y;
+  o[2] = q1.x;
+  o[3] = q1.y;
+}
+
+template <typename TX, typename TY, int NR>
+__global__ __launch_bounds__(256) void gemv_f32a4_kernel(int m, int k, const float* __restrict__ a, int64_t lda,
+                                                         const TX* __restrict__ x, int64_t ldx,
+                                                         const TY* __restrict__ b, int64_t ldb, TY* __restrict__ y,
+                                                         int64_t ldy) {
+  const int lane = threadIdx.x & 63;
+  const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * kGemvRows;
+  if (row0 >= m) return;
+  double s[kGemvRows][NR];
+#pragma unroll
+  for (int i = 0; i < kGemvRows; ++i)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[i][r] = 0.0;
+  // rows past m read the last row; their sums are never written
+  const float* arow[kGemvRows];
+#pragma unroll
+  for (int i = 0; i < kGemvRows; ++i) arow[i] = a + (int64_t)(row0 + i < m ? row0 + i : m - 1) * lda;
+  for (int c = 4 * lane; c < k; c += 4 * kWave) {
+    float4 av[kGemvRows];
+#pragma unroll
+    for (int i = 0; i < kGemvRows; ++i) av[i] = *reinterpret_cast<const float4*>(arow[i] + c);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      double xv[4];
+      gemv_x4(x + (int64_t)r * ldx + c, xv);
+#pragma unroll
+      for (int i = 0; i < kGemvRows; ++i) {
+        s[i][r] = fma((double)av[i].x, xv[0], s[i][r]);
+        s[i][r] = fma((double)av[i].y, xv[1], s[i][r]);
+        s[i][r] = fma((double)av[i].z, xv[2], s[i][r]);
+        s[i][r] = fma((double)av[i].w, xv[3], s[i][r]);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kGemvRows; ++i)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[i][r] = wave_sum(s[i][r]);
+  if (lane < kGemvRows * NR) {
+    const int i = lane / NR, r = lane % NR;
+    double v = 0.0;
+#pragma unroll
+    for (int ii = 0; ii < kGemvRows; ++ii)
+#pragma unroll
+      for (int rr = 0; rr < NR; ++rr)
+        if (ii == i && rr == r) v = s[ii][rr];
+    const int row = row0 + i;
+    if (row < m) {
+      const double out = b ? (double)b[(int64_t)r * ldb + row] - v : v;
+      y[(int64_t)r * ldy + row] = (TY)out;
+    }
+  }
+}
+
 template <typename TX, typename TY>
 int gemv_f32a_launch(hipStream_t st, int m, int k, int nrhs, const float* a, int64_t lda, const TX* x, int64_t ldx,
                      const TY* b, int64_t ldb, TY* y, int64_t ldy) {
   const unsigned blocks = (unsigned)((m + 4 * kGemvRows - 1) / (4 * kGemvRows));
+  // 16-byte loads when every row of A and of x starts 16-byte aligned
+  const bool vec4 = k % 4 == 0 && lda % 4 == 0 && ldx % 4 == 0 && (reinterpret_cast<uintptr_t>(a) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(x) & 15) == 0;
   // split into launches of 8, 4, 2, 1 right-hand sides
   int done = 0;
   while (done < nrhs) {
@@ -1045,7 +1122,21 @@ int gemv_f32a_launch(hipStream_t st, int m, int k, int nrhs, const float* a, int
     const TY* bp = b ? b + (int64_t)done * ldb : nullptr;
     TY* yp = y + (int64_t)done * ldy;
     int take;
-    if (left >= 8) {
+    if (vec4) {
+      if (left >= 8) {
+        take = 8;
+        gemv_f32a4_kernel<TX, TY, 8><<<blocks, 256, 0, st>>>(m, k, a, lda, xp, ldx, bp, ldb, yp, ldy);
+      } else if (left >= 4) {
+        take = 4;
+        gemv_f32a4_kernel<TX, TY, 4><<<blocks, 256, 0, st>>>(m, k, a, lda, xp, ldx, bp, ldb, yp, ldy);
+      } else if (left >= 2) {
+        take = 2;
+        gemv_f32a4_kernel<TX, TY, 2><<<blocks, 256, 0, st>>>(m, k, a, lda, xp, ldx, bp, ldb, yp, ldy);
+      } else {
+        take = 1;
+        gemv_f32a4_kernel<TX, TY, 1><<<blocks, 256, 0, st>>>(m, k, a, lda, xp, ldx, bp, ldb, yp, ldy);
+      }
+    } else if (left >= 8) {
       take = 8;
       gemv_f32a_kernel<TX, TY, 8><<<blocks, 256, 0, st>>>(m, k, a, lda, xp, ldx, bp, ldb, yp, ldy);
     } else if (left >= 4) {
