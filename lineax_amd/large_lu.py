@@ -15,9 +15,11 @@ thousands):
    ill-conditioned operators (GMRES then needs hundreds of steps, and fp16
    can overflow); bf16x3 keeps float32-level factors at bf16 MFMA rates;
 3. ``solve`` refines against the original A: FP64 residuals r = b - A x
-   (float32 A, FP64 accumulation), each correction from GMRES on A
-   preconditioned by x -> V (L U)^-1 P U^T x (float32 basis, FP64 inner
-   products), until the normwise backward error
+   (float32 A, FP64 accumulation), each correction a plain refinement
+   step x += V (L U)^-1 P U^T r while those divide the backward error by
+   at least 10 (bf16x3 and float32 factors), then from GMRES on A
+   preconditioned by the same map
+   (float32 basis, FP64 inner products), until the normwise backward error
    eta = |r|_inf / (|A|_inf |x|_inf + |b|_inf) <= tol or ``max_refine``
    residuals.
 
@@ -158,15 +160,28 @@ class LargeLU:
         return torch.where(num == 0, torch.zeros_like(num), num / den)
 
     def solve(self, b: torch.Tensor, *, tol: float = 1e-12, max_refine: int = 10, trans: bool = False,
-              inner_rtol: float = 1e-4, restart: int = 4, inner_max_steps: int = 50):
+              inner_rtol: float = 1e-4, restart: int = 4, inner_max_steps: int = 50, plain_first: bool | None = None,
+              plain_gain: float = 0.1):
         """Refined solution x (float64, [nrhs, n]) of A x = b (A^T x = b if trans).
 
+        With plain_first (the default for bf16x3 and float32 factors),
+        corrections start as plain iterative refinement, x += (LU)^-1 r: one
+        preconditioner application each (bf16x3 factors of Gaussian
+        operators of order 8192 divide the backward error by about 10 per
+        step; bf16 and fp16 factors by less than 2, so they start with
+        GMRES).  Once a plain step divides it by less than 1 / plain_gain on
+        a right-hand side that has not converged (or plain_first is off),
+        the remaining corrections are GMRES-IR (Carson &
+        Higham 2017): GMRES in float32 on the preconditioned FP64 residual
+        equation, restart cycles of ``restart`` Arnoldi iterations (with
+        factors this close to A a few iterations per correction suffice, so
+        short cycles avoid preconditioner applications that lineax's
+        cycle-end convergence test would otherwise spend).  A right-hand side
+        whose last plain step raised its backward error goes back to the
+        iterate before that step.
+
         Returns (x, eta, stats) with eta the last backward error per
-        right-hand side; stats counts residuals and GMRES steps (restart
-        cycles of ``restart`` Arnoldi iterations: with factors this close to
-        A, a few iterations per correction suffice, so short cycles avoid
-        preconditioner applications that lineax's cycle-end convergence test
-        would otherwise spend).
+        right-hand side; stats counts residuals, plain steps and GMRES steps.
         """
         from . import _solve, _solver
 
@@ -176,16 +191,33 @@ class LargeLU:
         op = mat(self.a.T.contiguous() if trans else self.a, torch.float64)
         prec = _Preconditioner(self, torch.float64, trans)
         residuals = 0
+        plain_steps = 0
         gmres_steps = 0
         eta = None
+        plain = self.precision in ("bf16x3", "fp32") if plain_first is None else bool(plain_first)
+        before = None  # (x, r, eta) before the last plain step
         for _ in range(max(1, max_refine)):
             r = self.residual(x, b64, trans)
             residuals += 1
             eta = self.backward_error(x, b64, r, trans)
+            if before is not None:
+                x0, r0, eta0 = before
+                before = None
+                if not bool(((eta <= plain_gain * eta0) | (eta <= tol)).all()):
+                    plain = False
+                    worse = ~(eta <= eta0)  # NaN counts as worse
+                    x = torch.where(worse[:, None], x0, x)
+                    r = torch.where(worse[:, None], r0, r)
+                    eta = torch.where(worse, eta0, eta)
             # stop when converged, out of residuals, or non-finite (a zero
             # pivot: the output stays non-finite, reported as singular)
             if bool((eta <= tol).all()) or residuals >= max_refine or not bool(torch.isfinite(eta).all()):
                 break
+            if plain:
+                before = (x, r, eta)
+                x = x + prec.mv_as(r, torch.float64)
+                plain_steps += 1
+                continue
             d0 = prec.mv_as(r, torch.float64)
             atol = inner_rtol * float(d0.abs().max())
             sol = _solve.linear_solve(
@@ -196,7 +228,8 @@ class LargeLU:
             gmres_steps += int(sol.stats["num_steps"].max())
             x = x + sol.value
         self.check_trsv()
-        return x, eta, {"residuals": residuals, "gmres_steps": gmres_steps, "zero_pivot": int(self.info.item())}
+        return x, eta, {"residuals": residuals, "plain_steps": plain_steps, "gmres_steps": gmres_steps,
+                        "zero_pivot": int(self.info.item())}
 
     def check_trsv(self) -> None:
         """Raises if any block triangular solve since construction timed out

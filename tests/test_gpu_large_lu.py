@@ -9,7 +9,9 @@
   multipliers stay small); the factors alone reach float32-level backward
   error on Gaussian operators (the pivot growth stays small); the
   refined solve reaches normwise backward error <= 1e-12 against the float64
-  oracle matrix and matches numpy's float64 solve; the transposed solve and
+  oracle matrix and matches numpy's float64 solve, by plain refinement steps
+  handing over to GMRES-IR once a step gains too little, and by GMRES-IR
+  alone; the transposed solve and
   the LU solver's large path agree with float64 solves; an all-zero matrix
   gives non-finite output (RESULTS.singular).
 """
@@ -166,13 +168,21 @@ def test_large_lu_block_inverses_and_solves_on_gaussian(device, rng):
         assert np.max(np.abs(y - ref)) <= 1e-2 * np.max(np.abs(ref)), trans
 
 
+@pytest.mark.parametrize("plain_first", [True, False])
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
 @pytest.mark.parametrize("n", [1024, 2000])
-def test_large_lu_refined_solve(device, rng, precision, n):
+def test_large_lu_refined_solve(device, rng, precision, n, plain_first):
+    # float32-level factors of a Gaussian operator: plain refinement steps
+    # first (GMRES-IR takes over on operators conditioned badly enough that
+    # a step gains less than a digit); with them off, GMRES-IR throughout
     a = rng.standard_normal((n, n)).astype(np.float32)
     b = rng.standard_normal((4, n)).astype(np.float32)
     fac = LargeLU(torch.as_tensor(a, device=device), precision=precision)
-    x, eta, stats = fac.solve(torch.as_tensor(b, device=device), tol=1e-12, max_refine=10)
+    x, eta, stats = fac.solve(torch.as_tensor(b, device=device), tol=1e-12, max_refine=10, plain_first=plain_first)
+    if plain_first:
+        assert stats["plain_steps"] >= 1, stats
+    else:
+        assert stats["plain_steps"] == 0 and stats["gmres_steps"] >= 1, stats
     x = x.cpu().numpy()
     a64 = a.astype(np.float64)
     for i in range(4):
@@ -194,6 +204,29 @@ def test_large_lu_single_pass_16bit_converges_when_well_conditioned(device, rng,
     fac = LargeLU(torch.as_tensor(a, device=device), precision=precision)
     x, eta, stats = fac.solve(torch.as_tensor(b, device=device), tol=1e-12, max_refine=10)
     assert float(eta.max()) <= 1e-12, stats
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16"])
+def test_large_lu_plain_steps_hand_over_to_gmres(device, rng, precision):
+    # plain steps forced on (fp16 factors start with GMRES by default) with
+    # a gain no plain step reaches: one plain step, then GMRES-IR from the
+    # better of the two iterates per right-hand side, to the same accuracy
+    # (operators on which one plain step stays above 1e-12: Gaussian for
+    # bf16x3 factors, diagonally dominant for fp16 ones)
+    n = 1024
+    a = rng.standard_normal((n, n))
+    if precision == "fp16":
+        a = a / np.sqrt(n) + 3 * np.eye(n)
+    a = a.astype(np.float32)
+    b = rng.standard_normal((3, n)).astype(np.float32)
+    fac = LargeLU(torch.as_tensor(a, device=device), precision=precision)
+    x, eta, stats = fac.solve(torch.as_tensor(b, device=device), tol=1e-12, max_refine=10, plain_first=True,
+                              plain_gain=1e-30)
+    assert stats["plain_steps"] == 1 and stats["gmres_steps"] >= 1, stats
+    assert float(eta.max()) <= 1e-12, stats
+    a64 = a.astype(np.float64)
+    ref = np.linalg.solve(a64, b.astype(np.float64).T).T
+    assert np.max(np.abs(x.cpu().numpy() - ref)) <= 1e-9 * np.max(np.abs(ref))
 
 
 def test_large_lu_transposed_solve(device, rng):
