@@ -446,12 +446,17 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
     int col_l = q;  // opaque per step, as in factor_tiles
     asm volatile("" : "+v"(col_l));
     bool mine = false;
+    int tsel = 0;  // the pair slot holding block row I (wave-uniform)
 #pragma unroll
     for (int t = 0; t < L::kPPW; ++t) {
       const int p = pair_of<WAVES>(w, t);
-      mine = mine || p == I || NB - 1 - p == I;
+      if (p == I || NB - 1 - p == I) {
+        mine = true;
+        tsel = t;
+      }
     }
     if (mine) {
+      const int ps = pair_of<WAVES>(w, tsel);
       // y_I[q] = sum_p V[p][q] c[p] (rows 4g..4g+3 here, then across groups)
       const f4 cv = *reinterpret_cast<const f4*>(C + 16 * I + 4 * g);
       const float* V = s.Vall + 256 * I + (4 * g) * 16 + col_l;
@@ -461,19 +466,27 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
       if (l < 16) C[16 * I + q] = part;
       wave_lds_sync();
       const f4 yv = *reinterpret_cast<const f4*>(C + 16 * I + 4 * g);
+      // push c_J -= U_IJ^T y_I over the pair's slots: every slot reads its
+      // c_J (J < NB: inside C) without a branch, so the reads are in flight
+      // together; only row I's blocks right of the diagonal write back
+      // (distinct J, so no slot reads another's write)
+      float cj[L::kTPP];
 #pragma unroll
-      for (int t = 0; t < L::kPPW; ++t) {
-        const int p = pair_of<WAVES>(w, t);
+      for (int k = 0; k < L::kTPP; ++k) {
+        int Ib, J;
+        slot_block<NB>(ps, k, Ib, J);
+        cj[k] = C[16 * J + col_l];
+      }
 #pragma unroll
-        for (int k = 0; k < L::kTPP; ++k) {
-          int Ib, J;
-          slot_block<NB>(p, k, Ib, J);
-          if (Ib == I && J > I && J < nk) {
-            float pt = T[t][k][0] * yv[0] + T[t][k][1] * yv[1] + T[t][k][2] * yv[2] + T[t][k][3] * yv[3];
-            pt = group_sum4(pt);
-            if (l < 16) C[16 * J + col_l] -= pt;
-          }
-        }
+      for (int k = 0; k < L::kTPP; ++k) {
+        int Ib, J;
+        slot_block<NB>(ps, k, Ib, J);
+        f4 u = T[0][k];
+#pragma unroll
+        for (int t = 1; t < L::kPPW; ++t) u = tsel == t ? T[t][k] : u;
+        float pt = u[0] * yv[0] + u[1] * yv[1] + u[2] * yv[2] + u[3] * yv[3];
+        pt = group_sum4(pt);
+        if (Ib == I && J > I && J < nk && l < 16) C[16 * J + col_l] = cj[k] - pt;
       }
     }
     wg_barrier<WAVES>();
@@ -484,27 +497,32 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
     int col_l = q;
     asm volatile("" : "+v"(col_l));
     bool mine = false;
+    int tsel = 0;
 #pragma unroll
     for (int t = 0; t < L::kPPW; ++t) {
       const int p = pair_of<WAVES>(w, t);
-      mine = mine || p == K || NB - 1 - p == K;
+      if (p == K || NB - 1 - p == K) {
+        mine = true;
+        tsel = t;
+      }
     }
     if (mine) {
-      // every slot reads its x_J (J < NB: inside C) without a branch, so the
-      // LDS reads are in flight together; slots outside row K contribute 0
+      // every slot of the pair holding row K reads its x_J (J < NB: inside
+      // C) without a branch, so the LDS reads are in flight together; slots
+      // outside row K contribute 0
+      const int ps = pair_of<WAVES>(w, tsel);
       float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int t = 0; t < L::kPPW; ++t) {
-        const int p = pair_of<WAVES>(w, t);
+      for (int k = 0; k < L::kTPP; ++k) {
+        int Ib, J;
+        slot_block<NB>(ps, k, Ib, J);
+        const float xr = C[16 * J + col_l];
+        const float xj = (Ib == K && J > K && J < nk) ? xr : 0.0f;
+        f4 u = T[0][k];
 #pragma unroll
-        for (int k = 0; k < L::kTPP; ++k) {
-          int Ib, J;
-          slot_block<NB>(p, k, Ib, J);
-          const float xr = C[16 * J + col_l];
-          const float xj = (Ib == K && J > K && J < nk) ? xr : 0.0f;
+        for (int t = 1; t < L::kPPW; ++t) u = tsel == t ? T[t][k] : u;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) acc[r] = fmaf(T[t][k][r], xj, acc[r]);
-        }
+        for (int r = 0; r < 4; ++r) acc[r] = fmaf(u[r], xj, acc[r]);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = row_sum16(acc[r]);  // s[4g + r]
