@@ -461,15 +461,12 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
       const f4 cv = *reinterpret_cast<const f4*>(C + 16 * I + 4 * g);
       const float* V = s.Vall + 256 * I + (4 * g) * 16 + col_l;
       float part = V[0] * cv[0] + V[16] * cv[1] + V[32] * cv[2] + V[48] * cv[3];
-      part = group_sum4(part);
-      wave_lds_sync();
-      if (l < 16) C[16 * I + q] = part;
-      wave_lds_sync();
-      const f4 yv = *reinterpret_cast<const f4*>(C + 16 * I + 4 * g);
+      part = group_sum4(part);  // y_I[q], the same in all four groups
       // push c_J -= U_IJ^T y_I over the pair's slots: every slot reads its
-      // c_J (J < NB: inside C) without a branch, so the reads are in flight
-      // together; only row I's blocks right of the diagonal write back
-      // (distinct J, so no slot reads another's write)
+      // c_J (J < NB: inside C) before any product, so the reads are in
+      // flight together; only row I's blocks right of the diagonal are
+      // computed and written back (distinct J, so no slot reads another's
+      // write, and none reads c_I)
       float cj[L::kTPP];
 #pragma unroll
       for (int k = 0; k < L::kTPP; ++k) {
@@ -477,16 +474,25 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
         slot_block<NB>(ps, k, Ib, J);
         cj[k] = C[16 * J + col_l];
       }
+      if (l < 16) C[16 * I + q] = part;
+      // y_I[4g + r] from lane 4g + r (group 0) instead of through LDS
+      f4 yv;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) yv[r] = __shfl(part, 4 * g + r, 64);
+#pragma unroll
+      for (int k = 0; k < L::kTPP; ++k) asm volatile("" : "+v"(cj[k]));
 #pragma unroll
       for (int k = 0; k < L::kTPP; ++k) {
         int Ib, J;
         slot_block<NB>(ps, k, Ib, J);
-        f4 u = T[0][k];
+        if (Ib == I && J > I && J < nk) {
+          f4 u = T[0][k];
 #pragma unroll
-        for (int t = 1; t < L::kPPW; ++t) u = tsel == t ? T[t][k] : u;
-        float pt = u[0] * yv[0] + u[1] * yv[1] + u[2] * yv[2] + u[3] * yv[3];
-        pt = group_sum4(pt);
-        if (Ib == I && J > I && J < nk && l < 16) C[16 * J + col_l] = cj[k] - pt;
+          for (int t = 1; t < L::kPPW; ++t) u = tsel == t ? T[t][k] : u;
+          float pt = u[0] * yv[0] + u[1] * yv[1] + u[2] * yv[2] + u[3] * yv[3];
+          pt = group_sum4(pt);
+          if (l < 16) C[16 * J + col_l] = cj[k] - pt;
+        }
       }
     }
     wg_barrier<WAVES>();
