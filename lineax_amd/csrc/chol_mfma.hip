@@ -514,32 +514,49 @@ __device__ __forceinline__ void solve_tiles(const Tiles<NB, WAVES>& T, int nk, c
     }
     if (mine) {
       // every slot of the pair holding row K reads its x_J (J < NB: inside
-      // C) without a branch, so the LDS reads are in flight together; slots
-      // outside row K contribute 0
+      // C), and y_K and V_K are read, before any product, so the LDS reads
+      // are in flight together; only row K's blocks right of the diagonal
+      // are multiplied
       const int ps = pair_of<WAVES>(w, tsel);
+      const float yq = C[16 * K + q];
+      const float* V = s.Vall + 256 * K + (4 * g) * 16 + col_l;
+      float vr[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) vr[r] = V[16 * r];
+      float xj[L::kTPP];
+#pragma unroll
+      for (int k = 0; k < L::kTPP; ++k) {
+        int Ib, J;
+        slot_block<NB>(ps, k, Ib, J);
+        xj[k] = C[16 * J + col_l];
+      }
+#pragma unroll
+      for (int k = 0; k < L::kTPP; ++k) asm volatile("" : "+v"(xj[k]));
       float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int k = 0; k < L::kTPP; ++k) {
         int Ib, J;
         slot_block<NB>(ps, k, Ib, J);
-        const float xr = C[16 * J + col_l];
-        const float xj = (Ib == K && J > K && J < nk) ? xr : 0.0f;
-        f4 u = T[0][k];
+        if (Ib == K && J > K && J < nk) {
+          f4 u = T[0][k];
 #pragma unroll
-        for (int t = 1; t < L::kPPW; ++t) u = tsel == t ? T[t][k] : u;
+          for (int t = 1; t < L::kPPW; ++t) u = tsel == t ? T[t][k] : u;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = fmaf(u[r], xj, acc[r]);
+          for (int r = 0; r < 4; ++r) acc[r] = fmaf(u[r], xj[k], acc[r]);
+        }
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] = row_sum16(acc[r]);  // s[4g + r]
-      float* S = s.S + w * 16;
-      if (q == 0) *reinterpret_cast<f4*>(S + 4 * g) = f4{acc[0], acc[1], acc[2], acc[3]};
-      wave_lds_sync();
-      const float tq = C[16 * K + q] - S[q];
-      const float* V = s.Vall + 256 * K + (4 * g) * 16 + col_l;
+      for (int r = 0; r < 4; ++r) acc[r] = row_sum16(acc[r]);  // s[4g + r], the same in the row's 16 lanes
+      // s[q] from lane 16 (q / 4), entry q % 4, instead of through LDS
+      float sv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sv[r] = __shfl(acc[r], 16 * (q >> 2), 64);
+      const int qr = q & 3;
+      const float sq = qr == 0 ? sv[0] : qr == 1 ? sv[1] : qr == 2 ? sv[2] : sv[3];
+      const float tq = yq - sq;
       float xr[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) xr[r] = row_sum16(V[16 * r] * tq);
+      for (int r = 0; r < 4; ++r) xr[r] = row_sum16(vr[r] * tq);
       wave_lds_sync();
       if (q < 4) C[16 * K + 4 * g + q] = q == 0 ? xr[0] : q == 1 ? xr[1] : q == 2 ? xr[2] : xr[3];
     }
