@@ -422,11 +422,12 @@ __device__ __forceinline__ int factor_tiles(Tiles<NB, WAVES>& T, int nk, const S
             acc = mfma4(fi[2], fj[2], acc);
             acc = mfma4(fi[3], fj[3], acc);
             T[t][k] = acc;
-            // one block's operands at a time: issued all at once, the
-            // fragment loads of every block would be live together
-            __builtin_amdgcn_sched_barrier(0);
           }
         }
+        // two blocks' operands at a time (their loads and MFMA chains
+        // interleave): issued all at once, the fragment loads of every
+        // block would be live together
+        if (k & 1) __builtin_amdgcn_sched_barrier(0);
       }
     }
   }
