@@ -31,6 +31,14 @@ constexpr int kDLd = kP + 4;     // LDS row stride of the diagonal block (16-byt
 
 typedef float cb_f4 __attribute__((ext_vector_type(4)));
 
+template <int B, int E, typename F>
+__device__ __forceinline__ void cb_static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    cb_static_for<B + 1, E>(f);
+  }
+}
+
 struct Geometry {
   int n;
   int64_t a_off, b_off;
@@ -74,31 +82,57 @@ __global__ __launch_bounds__(kThreads) void chol_blocked_factor_kernel(const lxa
   for (int k0 = 0; k0 < n; k0 += kP) {
     const int kb = min(kP, n - k0);
     const int rest = n - k0 - kb;
-    for (int idx = tid; idx < kP * kP; idx += kThreads) {
-      const int i = idx / kP, j = idx % kP;
-      D[i * kDLd + j] =
-          (i < kb && j < kb) ? (j >= i ? A[(int64_t)(k0 + i) * lda + k0 + j] : 0.f) : (i == j ? 1.f : 0.f);
+    {
+      // every load issued before any store (indices clamped onto the upper
+      // triangle of the block; the values outside it are replaced below)
+      constexpr int kPer = kP * kP / kThreads;
+      float v[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int idx = tid + u * kThreads, i = idx / kP, j = idx % kP;
+        const int jc = min(j, kb - 1), ic = min(min(i, kb - 1), jc);
+        v[u] = A[(int64_t)(k0 + ic) * lda + k0 + jc];
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int idx = tid + u * kThreads, i = idx / kP, j = idx % kP;
+        D[i * kDLd + j] = (i < kb && j < kb) ? (j >= i ? v[u] : 0.f) : (i == j ? 1.f : 0.f);
+      }
     }
     __syncthreads();
     if (wv == 0) {
-      // right-looking on the upper triangle; lane j owns column j
+      // right-looking on the upper triangle in registers: lane j holds
+      // column j (c[i] = D[i][j]); the pivot and row t come by readlane.
+      // Entries below the diagonal are updated too and never read.  The
+      // padding (rows and columns past kb) is the identity, so its pivots
+      // are 1 and its rows stay 0 off the diagonal.
+      float c[kP];
+      const int jl = lane < kP ? lane : kP - 1;
+#pragma unroll
+      for (int i = 0; i < kP; ++i) c[i] = D[i * kDLd + jl];
       int fail = 0;
-      for (int t = 0; t < kb; ++t) {
-        const float dtt = D[t * kDLd + t];
-        if (!(dtt > 0.f)) {
-          fail = k0 + t + 1;
-          break;
+      cb_static_for<0, kP>([&](auto T_) {
+        constexpr int t = decltype(T_)::value;
+        if (fail == 0 && t < kb) {
+          const float dtt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c[t]), t));
+          if (!(dtt > 0.f)) {
+            fail = k0 + t + 1;
+          } else {
+            const float utt = sqrtf(dtt);
+            c[t] = lane > t ? c[t] / utt : (lane == t ? utt : c[t]);
+            cb_static_for<t + 1, kP>([&](auto I_) {
+              constexpr int i = decltype(I_)::value;
+              const float uti = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c[t]), i));
+              c[i] -= uti * c[t];
+            });
+          }
         }
-        const float utt = sqrtf(dtt);
-        wave_lds_sync();
-        if (lane > t && lane < kP) D[t * kDLd + lane] /= utt;
-        if (lane == t) D[t * kDLd + t] = utt;
-        wave_lds_sync();
-        if (lane > t && lane < kP) {
-          const float utj = D[t * kDLd + lane];
-          for (int i = t + 1; i <= lane; ++i) D[i * kDLd + lane] -= D[t * kDLd + i] * utj;
-        }
-        wave_lds_sync();
+        __builtin_amdgcn_sched_barrier(0);
+      });
+      if (lane < kP) {
+#pragma unroll
+        for (int i = 0; i < kP; ++i)
+          if (i <= lane) D[i * kDLd + lane] = c[i];
       }
       if (lane == 0 && fail) s_fail = fail;
     }
@@ -111,40 +145,87 @@ __global__ __launch_bounds__(kThreads) void chol_blocked_factor_kernel(const lxa
     if (rest <= 0) break;
     // (2) U12 = U11^-T A12: the row block is staged in the LDS panel
     // (coalesced), swept in place one column per thread, and written back.
-    for (int idx = tid; idx < kP * rest; idx += kThreads) {
-      const int i = idx / rest, c = idx % rest;
-      P[i * kPanelLd + c] = i < kb ? A[(int64_t)(k0 + i) * lda + k0 + kb + c] : 0.f;
-    }
-    __syncthreads();
-    for (int c = tid; c < rest; c += kThreads) {
-      for (int t = 0; t < kb; ++t) {
-        const float xt = P[t * kPanelLd + c] / D[t * kDLd + t];
-        P[t * kPanelLd + c] = xt;
-        for (int i = t + 1; i < kb; ++i) P[i * kPanelLd + c] -= D[t * kDLd + i] * xt;
+    // (thread tid takes column c = tid + kThreads m: its kP loads go out
+    // together, coalesced along the rows; the forward substitution with
+    // U11^T runs in registers, D read from LDS by broadcast; rows past kb
+    // are zero and the identity padding of D keeps them zero)
+    for (int c0 = 0; c0 < rest; c0 += kThreads) {
+      const int c = c0 + tid, cc = min(c, rest - 1);
+      // D's offset, opaque per column pass: otherwise D's 528 entries are
+      // loaded once before the loop and held in registers
+      int doff = 0;
+      asm volatile("" : "+v"(doff));
+      const float* Dc = D + doff;
+      float v[kP];
+#pragma unroll
+      for (int i = 0; i < kP; ++i) v[i] = A[(int64_t)(k0 + min(i, kb - 1)) * lda + k0 + kb + cc];
+#pragma unroll
+      for (int i = 0; i < kP; ++i) v[i] = i < kb ? v[i] : 0.f;
+      cb_static_for<0, kP>([&](auto T_) {
+        constexpr int t = decltype(T_)::value;
+        const float xt = v[t] / Dc[t * kDLd + t];
+        v[t] = xt;
+        cb_static_for<t + 1, kP>([&](auto I_) {
+          constexpr int i = decltype(I_)::value;
+          v[i] -= Dc[t * kDLd + i] * xt;
+        });
+        // one row of D in flight at a time (hoisted, every row's broadcast
+        // reads would be live together)
+        __builtin_amdgcn_sched_barrier(0);
+      });
+      if (c < rest) {
+#pragma unroll
+        for (int i = 0; i < kP; ++i) {
+          P[i * kPanelLd + c] = v[i];
+          if (i < kb) A[(int64_t)(k0 + i) * lda + k0 + kb + c] = v[i];
+        }
       }
     }
-    __syncthreads();
-    for (int idx = tid; idx < kb * rest; idx += kThreads) {
-      const int i = idx / rest, c = idx % rest;
-      A[(int64_t)(k0 + i) * lda + k0 + kb + c] = P[i * kPanelLd + c];
-    }
+    __syncthreads();  // the update reads every thread's columns of P
     // (3) A22 -= U12^T U12 on the upper triangle in 16 x 16 tiles on the
     // matrix cores (v_mfma_f32_16x16x4f32, float32 products and sums; lane l:
     // A operand [l % 16][l / 16], B operand [l / 16][l % 16], result rows
-    // 4 (l / 16) + i, column l % 16).  Tiles go to the waves in turn; the K
-    // dimension is the panel's 32 rows of U12 (rows past kb are zero).  Only
-    // entries on or above the diagonal are read or written.
+    // 4 (l / 16) + i, column l % 16).  The upper tiles (bi <= bj), row by
+    // row, go to the waves in turn; the K dimension is the panel's 32 rows
+    // of U12 (rows past kb are zero).  Only entries on or above the diagonal
+    // are read or written.  A tile's four loads are issued before the
+    // previous tile's products (indices clamped onto the upper triangle;
+    // entries outside it are masked when the tile is used), so a wave does
+    // not wait for memory once per tile.
     const int T16 = (rest + 15) / 16;
     const int g = lane >> 4, q = lane & 15;
-    for (int tix = wv; tix < T16 * T16; tix += kThreads / 64) {
-      const int bi = tix / T16, bj = tix % T16;
-      if (bj < bi) continue;  // uniform per wave
+    const int base = k0 + kb;
+    constexpr int kWaves = kThreads / 64;
+    auto advance = [&](int& ti, int& tj, int step) {  // the step-th upper tile after (ti, tj)
+      tj += step;
+      while (ti < T16 && tj >= T16) {
+        ++ti;
+        tj = ti + (tj - T16);
+      }
+    };
+    auto load_raw = [&](int ti, int tj, float (&v)[4]) {
+      const int j = min(16 * tj + q, rest - 1);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = min(min(16 * ti + 4 * g + i, rest - 1), j);
+        v[i] = A[(int64_t)(base + r) * lda + base + j];
+      }
+    };
+    int bi = 0, bj = 0;
+    advance(bi, bj, wv);
+    float cur[4] = {0.f, 0.f, 0.f, 0.f};
+    if (bi < T16) load_raw(bi, bj, cur);
+    while (bi < T16) {
+      int ni = bi, nj = bj;
+      advance(ni, nj, kWaves);
+      float nxt[4] = {0.f, 0.f, 0.f, 0.f};
+      if (ni < T16) load_raw(ni, nj, nxt);
       const int j = 16 * bj + q;
       cb_f4 acc;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = 16 * bi + 4 * g + i;
-        acc[i] = (j < rest && j >= r) ? A[(int64_t)(k0 + kb + r) * lda + k0 + kb + j] : 0.f;
+        acc[i] = (j < rest && j >= r) ? cur[i] : 0.f;
       }
 #pragma unroll
       for (int t0 = 0; t0 < kP; t0 += 4) {
@@ -154,8 +235,12 @@ __global__ __launch_bounds__(kThreads) void chol_blocked_factor_kernel(const lxa
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = 16 * bi + 4 * g + i;
-        if (j < rest && j >= r) A[(int64_t)(k0 + kb + r) * lda + k0 + kb + j] = acc[i];
+        if (j < rest && j >= r) A[(int64_t)(base + r) * lda + base + j] = acc[i];
       }
+      bi = ni;
+      bj = nj;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
     }
     __syncthreads();
   }
