@@ -282,6 +282,13 @@ __global__ __launch_bounds__(kThreads) void chol_blocked_solve_kernel(const lxa_
     }
     __syncthreads();
     if (wv == 0) {
+      // lane l holds column l of the diagonal block (rows t <= l; the
+      // indices are clamped onto the upper triangle), its loads issued
+      // together instead of one per step
+      const int lc = min(lane, kb - 1);
+      float col[kP];
+#pragma unroll
+      for (int t = 0; t < kP; ++t) col[t] = U[(int64_t)(k0 + min(min(t, kb - 1), lc)) * lda + k0 + lc];
       float z = 0.f;
       if (lane < kb) {
         float s = 0.f;
@@ -289,10 +296,13 @@ __global__ __launch_bounds__(kThreads) void chol_blocked_solve_kernel(const lxa_
         for (int gp = 0; gp < 8; ++gp) s += part[gp][lane];
         z = y[k0 + lane] - s;
       }
-      for (int t = 0; t < kb; ++t) {
-        if (lane == t) z = z / U[(int64_t)(k0 + t) * lda + k0 + t];
-        const float yt = __shfl(z, t, kWave);
-        if (lane > t && lane < kb) z -= U[(int64_t)(k0 + t) * lda + k0 + lane] * yt;
+#pragma unroll
+      for (int t = 0; t < kP; ++t) {
+        if (t < kb) {
+          if (lane == t) z = z / col[t];
+          const float yt = __shfl(z, t, kWave);
+          if (lane > t && lane < kb) z -= col[t] * yt;
+        }
       }
       if (lane < kb) y[k0 + lane] = z;
     }
@@ -310,11 +320,20 @@ __global__ __launch_bounds__(kThreads) void chol_blocked_solve_kernel(const lxa_
     }
     __syncthreads();
     if (wv == 0) {
+      // lane l holds row l of the diagonal block (columns t >= l), loaded
+      // together as above
+      const int lr = min(lane, kb - 1);
+      float row[kP];
+#pragma unroll
+      for (int t = 0; t < kP; ++t) row[t] = U[(int64_t)(k0 + lr) * lda + k0 + max(min(t, kb - 1), lr)];
       float z = lane < kb ? y[k0 + lane] - red[0][lane] : 0.f;
-      for (int t = kb - 1; t >= 0; --t) {
-        if (lane == t) z = z / U[(int64_t)(k0 + t) * lda + k0 + t];
-        const float xt = __shfl(z, t, kWave);
-        if (lane < t) z -= U[(int64_t)(k0 + lane) * lda + k0 + t] * xt;
+#pragma unroll
+      for (int t = kP - 1; t >= 0; --t) {
+        if (t < kb) {
+          if (lane == t) z = z / row[t];
+          const float xt = __shfl(z, t, kWave);
+          if (lane < t) z -= row[t] * xt;
+        }
       }
       if (lane < kb) y[k0 + lane] = z;
     }
