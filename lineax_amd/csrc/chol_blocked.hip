@@ -276,8 +276,10 @@ __global__ __launch_bounds__(kThreads) void chol_blocked_solve_kernel(const lxa_
     {  // thread (column i = tid % 32, group gp = tid / 32) over rows t < k0
       const int i = tid & (kP - 1), gp = tid >> 5;
       float s = 0.f;
-      if (i < kb)
+      if (i < kb) {
+#pragma unroll 8
         for (int t = gp; t < k0; t += 8) s += U[(int64_t)t * lda + k0 + i] * y[t];
+      }
       part[gp][i] = s;
     }
     __syncthreads();
@@ -314,6 +316,7 @@ __global__ __launch_bounds__(kThreads) void chol_blocked_solve_kernel(const lxa_
     for (int i = wv; i < kb; i += kThreads / kWave) {
       float s = 0.f;
       const float* Ur = U + (int64_t)(k0 + i) * lda;
+#pragma unroll 8
       for (int t = k1 + lane; t < n; t += kWave) s += Ur[t] * y[t];
       s = wave_sum(s);
       if (lane == 0) red[0][i] = s;
