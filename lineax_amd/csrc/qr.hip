@@ -706,6 +706,15 @@ __global__ __launch_bounds__(kBigThreads) void qr_apply_rows_kernel(const lxa_qr
     const int nb = min(NB, kmax - k);
     const int jc = min(j, nb - 1);  // clamped column: loads stay inside the panel
     const T* Ak = A + k + jc;
+    // wave 0's row (trans) or column of the panel's T block, loaded now so
+    // the loads overlap the reduction below instead of one wait per term
+    T tv[NB];
+    if (wv == 0) {
+      const T* Tp = tblk + d.t_offset + (int64_t)p * NB * NB;
+      const int lt = min(lane, NB - 1);
+#pragma unroll
+      for (int i = 0; i < NB; ++i) tv[i] = trans ? Tp[i * NB + lt] : Tp[lt * NB + i];
+    }
     T acc = T(0);
     if constexpr (RPT > 0) {
 #pragma unroll
@@ -742,16 +751,15 @@ __global__ __launch_bounds__(kBigThreads) void qr_apply_rows_kernel(const lxa_qr
 #pragma unroll
         for (int w = 0; w < kBigWaves; ++w) wsum += s_part[w][lane];
       }
-      const T* Tp = tblk + d.t_offset + (int64_t)p * NB * NB;
       T s = T(0);
 #pragma unroll
       for (int i = 0; i < NB; ++i) {
         const T wi = __shfl(wsum, i, kWave);
         if (lane < NB && i < nb && lane < nb) {
           if (trans) {  // (T^T w)_lane = sum_{i <= lane} T[i][lane] w_i
-            if (i <= lane) s += Tp[i * NB + lane] * wi;
+            if (i <= lane) s += tv[i] * wi;
           } else {  // (T w)_lane = sum_{i >= lane} T[lane][i] w_i
-            if (i >= lane) s += Tp[lane * NB + i] * wi;
+            if (i >= lane) s += tv[i] * wi;
           }
         }
       }
